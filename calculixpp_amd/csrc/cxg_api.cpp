@@ -1,0 +1,459 @@
+// C ABI of libcxg.so (include/cxg.h): argument checks, object lifetime, the mesh
+// set-up (node incidence + element colouring) and the translation of internal
+// errors into status codes + last-error text.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "cxg_internal.hpp"
+
+using cxg::DevBuf;
+using cxg::Error;
+
+namespace cxg {
+
+static hipEvent_t take_event(cxg_context* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  CXG_HIP(hipEventCreate(&e));
+  return e;
+}
+
+LaunchTimer::LaunchTimer(cxg_context* c, std::string name, int64_t elements)
+    : ctx(c), on(c->timing) {
+  if (!on) return;
+  p.name = std::move(name);
+  p.elements = elements;
+  p.a = take_event(ctx);
+  p.b = take_event(ctx);
+  (void)hipEventRecord(p.a, ctx->stream);
+}
+
+LaunchTimer::~LaunchTimer() {
+  if (!on) return;
+  (void)hipEventRecord(p.b, ctx->stream);
+  ctx->pending.push_back(p);
+}
+
+void collect_timing(cxg_context* ctx) {
+  for (const auto& p : ctx->pending) {
+    float ms = 0.f;
+    CXG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto it = std::find_if(ctx->stats.begin(), ctx->stats.end(),
+                           [&](const cxg_context::Stat& s) { return s.name == p.name; });
+    if (it == ctx->stats.end()) {
+      ctx->stats.push_back({p.name, 0, 0, 0.0});
+      it = ctx->stats.end() - 1;
+    }
+    it->launches += 1;
+    it->elements += p.elements;
+    it->ms += double(ms);
+    ctx->event_pool.push_back(p.a);
+    ctx->event_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+}
+
+void check_flags(cxg_context* ctx) {
+  CXG_HIP(hipMemcpyAsync(ctx->flag_host, ctx->flag_dev, sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  CXG_HIP(hipStreamSynchronize(ctx->stream));
+  if (!ctx->pending.empty()) collect_timing(ctx);
+  const int f = *ctx->flag_host;
+  if (f == 0) return;
+  *ctx->flag_host = 0;
+  CXG_HIP(hipMemcpyAsync(ctx->flag_dev, ctx->flag_host, sizeof(int), hipMemcpyHostToDevice,
+                         ctx->stream));
+  CXG_HIP(hipStreamSynchronize(ctx->stream));
+  if (f & kFlagJacobian) throw Error(CXG_ERR_JACOBIAN, kBadJacobian);
+  if (f & kFlagValence)
+    throw Error(CXG_ERR_UNSUPPORTED, "node valence exceeds the pattern builder capacity");
+  throw Error(CXG_ERR_STATE, "element node pair missing from the CSR pattern");
+}
+
+namespace {
+
+// Greedy first-fit colouring in ascending element order: element e takes the
+// smallest colour not used by an earlier element sharing a node with it. On a
+// structured hex grid this yields the 8-colouring (i%2, j%2, k%2).
+std::vector<int32_t> colour_elements(int64_t n_elem, const std::vector<int64_t>& eptr,
+                                     const std::vector<int32_t>& conn,
+                                     const std::vector<int64_t>& inc_ptr,
+                                     const std::vector<int32_t>& inc_elem, int32_t& n_colors) {
+  constexpr int kMaxColours = 64;
+  std::vector<int32_t> col(size_t(n_elem), -1);
+  n_colors = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    uint64_t used = 0;
+    for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
+      const int32_t nd = conn[size_t(k)];
+      for (int64_t i = inc_ptr[size_t(nd)]; i < inc_ptr[size_t(nd) + 1]; ++i) {
+        const int32_t c = col[size_t(inc_elem[size_t(i)])];
+        if (c >= 0) used |= (1ull << c);
+      }
+    }
+    int c = 0;
+    while (c < kMaxColours && (used >> c) & 1ull) ++c;
+    if (c == kMaxColours) throw Error(CXG_ERR_UNSUPPORTED, "element colouring needs more than 64 colours");
+    col[size_t(e)] = c;
+    n_colors = std::max(n_colors, c + 1);
+  }
+  return col;
+}
+
+void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_block* blocks,
+                 cxg_mesh* M) {
+  if (n_nodes < 0 || n_blocks < 0 || (n_blocks > 0 && !blocks))
+    throw Error(CXG_ERR_ARG, "bad mesh arguments");
+  hipStream_t s = ctx->stream;
+  M->ctx = ctx;
+  M->n_nodes = n_nodes;
+  int64_t n_elem = 0, n_points = 0, n_conn = 0;
+  for (int32_t b = 0; b < n_blocks; ++b) {
+    const cxg_block& B = blocks[b];
+    const int nn = nodes_per_element(B.type);
+    if (nn == 0) throw Error(CXG_ERR_UNSUPPORTED, "element type outside the GPU path");
+    if (B.n_elem < 0 || (B.n_elem > 0 && !B.conn)) throw Error(CXG_ERR_ARG, "bad block");
+    if (B.elem_offset != n_elem || B.gp_offset != n_points)
+      throw Error(CXG_ERR_ARG, "blocks must cover consecutive elements and Gauss points");
+    cxg_block_dev D;
+    D.type = B.type;
+    D.nn = nn;
+    D.ngp = gauss_points(B.type);
+    D.n_elem = B.n_elem;
+    D.elem_offset = B.elem_offset;
+    D.gp_offset = B.gp_offset;
+    D.conn_offset = n_conn;
+    M->blocks.push_back(std::move(D));
+    n_elem += B.n_elem;
+    n_points += B.n_elem * gauss_points(B.type);
+    n_conn += B.n_elem * nn;
+  }
+  if (n_elem > INT32_MAX) throw Error(CXG_ERR_UNSUPPORTED, "more than 2^31-1 elements");
+  M->n_elem = n_elem;
+  M->n_points = n_points;
+  std::vector<int32_t> conn(static_cast<size_t>(n_conn));
+  std::vector<int64_t> eptr(size_t(n_elem) + 1, 0);
+  for (int32_t b = 0; b < n_blocks; ++b) {
+    const cxg_block_dev& D = M->blocks[size_t(b)];
+    if (D.n_elem)
+      CXG_HIP(hipMemcpyAsync(conn.data() + D.conn_offset, blocks[b].conn,
+                             size_t(D.n_elem * D.nn) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    for (int64_t e = 0; e < D.n_elem; ++e)
+      eptr[size_t(D.elem_offset + e) + 1] = D.conn_offset + (e + 1) * D.nn;
+  }
+  CXG_HIP(hipStreamSynchronize(s));
+  for (int32_t v : conn)
+    if (v < 0 || v >= n_nodes) throw Error(CXG_ERR_ARG, "element references unknown node");
+  // node -> element incidence, ascending element order
+  std::vector<int64_t> inc_ptr(size_t(n_nodes) + 1, 0);
+  for (int32_t v : conn) ++inc_ptr[size_t(v) + 1];
+  for (int32_t a = 0; a < n_nodes; ++a) inc_ptr[size_t(a) + 1] += inc_ptr[size_t(a)];
+  std::vector<int32_t> inc_elem(static_cast<size_t>(n_conn));
+  {
+    std::vector<int64_t> cur(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int64_t e = 0; e < n_elem; ++e)
+      for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
+        const int32_t v = conn[size_t(k)];
+        // a node repeated inside one element is listed once
+        if (cur[size_t(v)] > inc_ptr[size_t(v)] && inc_elem[size_t(cur[size_t(v)] - 1)] == int32_t(e))
+          continue;
+        inc_elem[size_t(cur[size_t(v)]++)] = int32_t(e);
+      }
+    // compact (duplicates left holes at the end of a node's range)
+    std::vector<int64_t> np(size_t(n_nodes) + 1, 0);
+    int64_t w = 0;
+    for (int32_t a = 0; a < n_nodes; ++a) {
+      np[size_t(a)] = w;
+      for (int64_t i = inc_ptr[size_t(a)]; i < cur[size_t(a)]; ++i) inc_elem[size_t(w++)] = inc_elem[size_t(i)];
+    }
+    np[size_t(n_nodes)] = w;
+    inc_ptr.swap(np);
+    inc_elem.resize(size_t(w));
+  }
+  int32_t ncol = 0;
+  const std::vector<int32_t> colour = colour_elements(n_elem, eptr, conn, inc_ptr, inc_elem, ncol);
+  M->n_colors = ncol;
+  for (cxg_block_dev& D : M->blocks) {
+    std::vector<std::vector<int32_t>> lists(static_cast<size_t>(ncol));
+    for (int64_t e = 0; e < D.n_elem; ++e) lists[size_t(colour[size_t(D.elem_offset + e)])].push_back(int32_t(e));
+    for (int c = 0; c < ncol; ++c) {
+      DevBuf<int32_t> buf(lists[size_t(c)].size());
+      if (!lists[size_t(c)].empty())
+        CXG_HIP(hipMemcpyAsync(buf.p, lists[size_t(c)].data(), lists[size_t(c)].size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, s));
+      D.color_count.push_back(int64_t(lists[size_t(c)].size()));
+      D.color.push_back(std::move(buf));
+    }
+    CXG_HIP(hipStreamSynchronize(s));  // host lists go out of scope
+  }
+  M->conn.alloc(size_t(std::max<int64_t>(n_conn, 1)));
+  M->eptr.alloc(eptr.size());
+  M->inc_ptr.alloc(inc_ptr.size());
+  M->inc_elem.alloc(std::max<size_t>(inc_elem.size(), 1));
+  if (n_conn)
+    CXG_HIP(hipMemcpyAsync(M->conn.p, conn.data(), conn.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  CXG_HIP(hipMemcpyAsync(M->eptr.p, eptr.data(), eptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  CXG_HIP(hipMemcpyAsync(M->inc_ptr.p, inc_ptr.data(), inc_ptr.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+  if (!inc_elem.empty())
+    CXG_HIP(hipMemcpyAsync(M->inc_elem.p, inc_elem.data(), inc_elem.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+  CXG_HIP(hipStreamSynchronize(s));
+}
+
+cxg_status fail(cxg_context* ctx, cxg_status st, const char* msg) {
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+template <class F>
+cxg_status guarded(cxg_context* ctx, F&& f) {
+  if (!ctx) return CXG_ERR_ARG;
+  try {
+    CXG_HIP(hipSetDevice(ctx->device));
+    f();
+    return CXG_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(ctx, CXG_ERR_HIP, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ctx, CXG_ERR_HIP, e.what());
+  }
+}
+
+}  // namespace
+}  // namespace cxg
+
+extern "C" {
+
+const char* cxg_version(void) { return "cxg 0.1 gfx950"; }
+
+int32_t cxg_nodes_per_element(int32_t type) { return cxg::nodes_per_element(type); }
+int32_t cxg_gauss_points(int32_t type) { return cxg::gauss_points(type); }
+
+cxg_status cxg_create(int device, void* stream, cxg_context** out) {
+  if (!out) return CXG_ERR_ARG;
+  *out = nullptr;
+  auto* ctx = new (std::nothrow) cxg_context();
+  if (!ctx) return CXG_ERR_HIP;
+  ctx->device = device;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  const cxg_status st = cxg::guarded(ctx, [&] {
+    CXG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->flag_dev), sizeof(int)));
+    CXG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->flag_host), sizeof(int), hipHostMallocDefault));
+    *ctx->flag_host = 0;
+    CXG_HIP(hipMemcpy(ctx->flag_dev, ctx->flag_host, sizeof(int), hipMemcpyHostToDevice));
+    for (int t : {CXG_C3D8, CXG_C3D20, CXG_C3D20R, CXG_C3D6}) {
+      std::vector<double> dN, w;
+      cxg::host_tables(t, dN, w);
+      ctx->tab_dN[t].alloc(dN.size());
+      ctx->tab_w[t].alloc(w.size());
+      CXG_HIP(hipMemcpy(ctx->tab_dN[t].p, dN.data(), dN.size() * sizeof(double), hipMemcpyHostToDevice));
+      CXG_HIP(hipMemcpy(ctx->tab_w[t].p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+      ctx->tables[t] = {cxg::nodes_per_element(t), cxg::gauss_points(t), ctx->tab_dN[t].p, ctx->tab_w[t].p};
+    }
+  });
+  if (st != CXG_OK) {
+    cxg_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return CXG_OK;
+}
+
+void cxg_destroy(cxg_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->pending.empty()) (void)hipStreamSynchronize(ctx->stream);
+  for (const auto& p : ctx->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->flag_dev) (void)hipFree(ctx->flag_dev);
+  if (ctx->flag_host) (void)hipHostFree(ctx->flag_host);
+  delete ctx;
+}
+
+cxg_status cxg_set_stream(cxg_context* ctx, void* stream) {
+  if (!ctx) return CXG_ERR_ARG;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return CXG_OK;
+}
+
+const char* cxg_last_error(const cxg_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+cxg_status cxg_profile_enable(cxg_context* ctx, int enable) {
+  return cxg::guarded(ctx, [&] {
+    if (!enable && !ctx->pending.empty()) {
+      CXG_HIP(hipStreamSynchronize(ctx->stream));
+      cxg::collect_timing(ctx);
+    }
+    ctx->timing = enable != 0;
+  });
+}
+
+cxg_status cxg_profile_reset(cxg_context* ctx) {
+  return cxg::guarded(ctx, [&] {
+    if (!ctx->pending.empty()) {
+      CXG_HIP(hipStreamSynchronize(ctx->stream));
+      cxg::collect_timing(ctx);
+    }
+    ctx->stats.clear();
+  });
+}
+
+int32_t cxg_profile_count(const cxg_context* ctx) { return ctx ? int32_t(ctx->stats.size()) : 0; }
+
+cxg_status cxg_profile_get(const cxg_context* ctx, int32_t i, const char** name,
+                           int64_t* launches, double* total_ms, int64_t* elements) {
+  if (!ctx || i < 0 || i >= int32_t(ctx->stats.size())) return CXG_ERR_ARG;
+  const auto& s = ctx->stats[size_t(i)];
+  if (name) *name = s.name.c_str();
+  if (launches) *launches = s.launches;
+  if (total_ms) *total_ms = s.ms;
+  if (elements) *elements = s.elements;
+  return CXG_OK;
+}
+
+cxg_status cxg_mesh_create(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks,
+                           const cxg_block* blocks, cxg_mesh** out) {
+  if (!out) return CXG_ERR_ARG;
+  *out = nullptr;
+  auto* M = new (std::nothrow) cxg_mesh();
+  if (!M) return CXG_ERR_HIP;
+  const cxg_status st = cxg::guarded(ctx, [&] { cxg::create_mesh(ctx, n_nodes, n_blocks, blocks, M); });
+  if (st != CXG_OK) {
+    delete M;
+    return st;
+  }
+  *out = M;
+  return CXG_OK;
+}
+
+void cxg_mesh_destroy(cxg_mesh* mesh) {
+  if (!mesh) return;
+  (void)hipSetDevice(mesh->ctx->device);
+  delete mesh;
+}
+
+cxg_status cxg_mesh_info(const cxg_mesh* m, int64_t* n_elem, int64_t* n_points, int32_t* n_colors) {
+  if (!m) return CXG_ERR_ARG;
+  if (n_elem) *n_elem = m->n_elem;
+  if (n_points) *n_points = m->n_points;
+  if (n_colors) *n_colors = m->n_colors;
+  return CXG_OK;
+}
+
+cxg_status cxg_pattern_build(cxg_context* ctx, const cxg_mesh* mesh, const int32_t* dof_eq,
+                             int32_t n_free, cxg_pattern** out) {
+  if (!out) return CXG_ERR_ARG;
+  *out = nullptr;
+  if (!mesh || (!dof_eq && mesh->n_nodes > 0) || n_free < 0) return cxg::fail(ctx, CXG_ERR_ARG, "bad pattern arguments");
+  auto* P = new (std::nothrow) cxg_pattern();
+  if (!P) return CXG_ERR_HIP;
+  const cxg_status st = cxg::guarded(ctx, [&] { cxg::build_pattern(ctx, mesh, dof_eq, n_free, P); });
+  if (st != CXG_OK) {
+    delete P;
+    return st;
+  }
+  *out = P;
+  return CXG_OK;
+}
+
+void cxg_pattern_destroy(cxg_pattern* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->mesh->ctx->device);
+  delete p;
+}
+
+cxg_status cxg_pattern_info(const cxg_pattern* p, int32_t* n_free, int64_t* nnz) {
+  if (!p) return CXG_ERR_ARG;
+  if (n_free) *n_free = p->n_free;
+  if (nnz) *nnz = p->nnz;
+  return CXG_OK;
+}
+
+const int64_t* cxg_pattern_row_ptr(const cxg_pattern* p) { return p ? p->row_ptr.p : nullptr; }
+const int32_t* cxg_pattern_col_idx(const cxg_pattern* p) { return p ? p->col_idx.p : nullptr; }
+
+cxg_status cxg_pattern_copy(cxg_context* ctx, const cxg_pattern* p, int64_t* row_ptr,
+                            int32_t* col_idx) {
+  if (!p) return cxg::fail(ctx, CXG_ERR_ARG, "null pattern");
+  return cxg::guarded(ctx, [&] {
+    if (row_ptr)
+      CXG_HIP(hipMemcpyAsync(row_ptr, p->row_ptr.p, size_t(p->n_free + 1) * sizeof(int64_t),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    if (col_idx && p->nnz)
+      CXG_HIP(hipMemcpyAsync(col_idx, p->col_idx.p, size_t(p->nnz) * sizeof(int32_t),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                               const double* E, const double* nu, double* vals, const double* u,
+                               double* f_int) {
+  if (!p || !coords || !E || !nu || !vals) return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  const bool with_f = u && f_int;
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
+                                with_f ? f_int : nullptr);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_internal_force(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                              const double* E, const double* nu, const double* u, double* f_int) {
+  if (!mesh || !coords || !E || !nu || !u || !f_int) return cxg::fail(ctx, CXG_ERR_ARG, "bad internal_force arguments");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_internal_force(ctx, mesh, coords, E, nu, u, f_int);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
+                           const double* f_ext, const double* f_c, double* rhs) {
+  if (!dof_eq || !rhs || n_nodes < 0) return cxg::fail(ctx, CXG_ERR_ARG, "bad reduce_free arguments");
+  return cxg::guarded(ctx, [&] { cxg::launch_reduce_free(ctx, n_nodes, dof_eq, f_ext, f_c, rhs); });
+}
+
+cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                           const double* E, const double* nu, const int32_t* mat_id,
+                           const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                           const double* committed, double* trial, double* vals,
+                           double* f_int) {
+  if (!p || !coords || !E || !nu || !u || !committed || !trial || !vals || !f_int ||
+      (n_mats > 0 && (!mats || !mat_id)) || n_mats < 0)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_j2 arguments");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_j2(ctx, p, coords, E, nu, n_mats > 0 ? mat_id : nullptr, mats, n_mats, u,
+                            committed, trial, vals, f_int);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial) {
+  if (!committed || !trial || n_points < 0) return cxg::fail(ctx, CXG_ERR_ARG, "bad commit arguments");
+  return cxg::guarded(ctx, [&] { cxg::launch_commit(ctx, n_points, committed, trial); });
+}
+
+cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords, const double* E,
+                   const double* nu, const double* u, const double* w, double* dx, double* dE,
+                   double* du) {
+  if (!mesh || !coords || !E || !nu || !u || !w || !dx || !dE || !du)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad vjp arguments");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_vjp(ctx, mesh, coords, E, nu, u, w, dx, dE, du);
+    cxg::check_flags(ctx);
+  });
+}
+
+}  // extern "C"

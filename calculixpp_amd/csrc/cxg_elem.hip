@@ -1,0 +1,847 @@
+// Element kernels of libcxg.so for gfx950 (CDNA4, 64-lane wavefronts).
+//
+// One element per 64-lane wavefront (one wave per workgroup). The element's node
+// coordinates, DOF map, per-Gauss-point J^-1 and the physical gradients g[q][k] of
+// every Gauss point are staged in LDS; the element stiffness never exists in
+// memory: each lane owns node-pair blocks K_ab (3x3, nine FP64 accumulators in
+// registers), integrates them over the Gauss points and adds them straight into
+// the CSR value array through the element -> CSR slot map.
+//
+// Determinism: kernels are launched colour by colour; no two elements of one
+// colour share a node, so every CSR value, f_int, dx and du entry has exactly one
+// writer per launch and is updated by a plain read-modify-write. The summation
+// order is fixed (colour order, then Gauss-point order), independent of scheduling;
+// no floating-point atomics are used anywhere.
+//
+// Reference algorithm (restated, not translated):
+//   Jacobian / gradients   src/fem/element.cpp:80-102, 361-380
+//   B, D, B^T D B          src/fem/element.cpp:382-462 (isotropic D exploited:
+//                          K_ab = lam g_a g_b^T + mu g_b g_a^T + mu (g_a.g_b) I)
+//   internal force         src/fem/stress.cpp:125-145 (as sum_q s B^T D B u)
+//   J2 return map/tangent  src/fem/plasticity.cpp:90-236
+//   scatter                src/fem/assembly.cpp:76-110 (SPC-only transform)
+#include <cstring>
+
+#include "cxg_internal.hpp"
+
+namespace cxg {
+namespace {
+
+constexpr int kWave = 64;
+
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// ---- LDS staging shared by all element kernels ------------------------------
+template <int NN, int NGP>
+struct Geo {
+  double x[NN * 3];
+  double Ai[NGP * 9];      // J^-1 (row-major) per Gauss point
+  double sc[NGP];          // det J * w
+  double g[NGP * NN * 3];  // physical gradients, [q][k][i]
+};
+
+template <int NN>
+struct DofLds {
+  int64_t row[NN * 3];  // CSR row start of each element DOF (-1: constrained row)
+  int32_t col[NN * 3];  // offset of the DOF inside its node's column block (-1: constrained)
+};
+
+template <int NN, int NGP>
+__device__ __forceinline__ void stage_coords(Geo<NN, NGP>& G, int32_t* node,
+                                             const int32_t* __restrict__ ce,
+                                             const double* __restrict__ coords, int lane) {
+  if (lane < NN) {
+    const int32_t nd = ce[lane];
+    node[lane] = nd;
+    const double* xp = coords + 3 * int64_t(nd);
+    G.x[3 * lane + 0] = xp[0];
+    G.x[3 * lane + 1] = xp[1];
+    G.x[3 * lane + 2] = xp[2];
+  }
+}
+
+template <int NN>
+__device__ __forceinline__ void stage_vec(double* dst, const int32_t* __restrict__ ce,
+                                          const double* __restrict__ v, int lane) {
+  if (lane < NN) {
+    const double* p = v + 3 * int64_t(ce[lane]);
+    dst[3 * lane + 0] = p[0];
+    dst[3 * lane + 1] = p[1];
+    dst[3 * lane + 2] = p[2];
+  }
+}
+
+template <int NN>
+__device__ __forceinline__ void stage_dofs(DofLds<NN>& D, const int32_t* __restrict__ ce,
+                                           const int32_t* __restrict__ dof_eq,
+                                           const int64_t* __restrict__ row_ptr, int lane) {
+  if (lane < 3 * NN) {
+    const int k = lane / 3, c = lane % 3;
+    const int64_t base = 3 * int64_t(ce[k]);
+    const int32_t eq = dof_eq[base + c];
+    int before = 0;
+    for (int cc = 0; cc < c; ++cc) before += dof_eq[base + cc] >= 0 ? 1 : 0;
+    D.row[lane] = eq >= 0 ? row_ptr[eq] : -1;
+    D.col[lane] = eq >= 0 ? before : -1;
+  }
+}
+
+// J, det J and J^-1 per Gauss point; one lane per Gauss point.
+template <int NN, int NGP>
+__device__ __forceinline__ void jacobians(Geo<NN, NGP>& G, const double* __restrict__ dN,
+                                          const double* __restrict__ wq, int lane,
+                                          int* __restrict__ flag) {
+  for (int q = lane; q < NGP; q += kWave) {
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double* d = dN + q * NN * 3;
+#pragma unroll
+    for (int k = 0; k < NN; ++k)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) J[i][j] += G.x[3 * k + i] * d[3 * k + j];
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
+    const double r = 1.0 / det;
+    double* A = G.Ai + 9 * q;
+    A[0] = c00 * r;
+    A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+    A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+    A[3] = c01 * r;
+    A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+    A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+    A[6] = c02 * r;
+    A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+    A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+    G.sc[q] = det * wq[q];
+  }
+}
+
+// g_k = J^-T dN_k for every (Gauss point, node).
+template <int NN, int NGP>
+__device__ __forceinline__ void gradients(Geo<NN, NGP>& G, const double* __restrict__ dN,
+                                          int lane) {
+  for (int idx = lane; idx < NGP * NN; idx += kWave) {
+    const int q = idx / NN;
+    const double* A = G.Ai + 9 * q;
+    const double d0 = dN[3 * idx], d1 = dN[3 * idx + 1], d2 = dN[3 * idx + 2];
+    G.g[3 * idx + 0] = A[0] * d0 + A[3] * d1 + A[6] * d2;
+    G.g[3 * idx + 1] = A[1] * d0 + A[4] * d1 + A[7] * d2;
+    G.g[3 * idx + 2] = A[2] * d0 + A[5] * d1 + A[8] * d2;
+  }
+}
+
+// Displacement gradient H[i][j] = sum_k v_k[i] g_k[j] at Gauss point q.
+template <int NN, int NGP>
+__device__ __forceinline__ void grad_field(const Geo<NN, NGP>& G, const double* v, int q,
+                                           double H[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) H[i][j] = 0.0;
+  const double* g = G.g + q * NN * 3;
+#pragma unroll
+  for (int k = 0; k < NN; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) H[i][j] += v[3 * k + i] * g[3 * k + j];
+}
+
+__device__ __forceinline__ void voigt_strain(const double H[3][3], double e[6]) {
+  e[0] = H[0][0];
+  e[1] = H[1][1];
+  e[2] = H[2][2];
+  e[3] = H[0][1] + H[1][0];
+  e[4] = H[0][2] + H[2][0];
+  e[5] = H[1][2] + H[2][1];
+}
+
+// sigma = D_iso eps (engineering-shear strain).
+__device__ __forceinline__ void iso_stress(double lam, double mu, const double e[6], double s[6]) {
+  const double tr = lam * (e[0] + e[1] + e[2]);
+  s[0] = tr + 2.0 * mu * e[0];
+  s[1] = tr + 2.0 * mu * e[1];
+  s[2] = tr + 2.0 * mu * e[2];
+  s[3] = mu * e[3];
+  s[4] = mu * e[4];
+  s[5] = mu * e[5];
+}
+
+__device__ __forceinline__ void lame(double E, double nu, double& lam, double& mu) {
+  const double two_mu = E / (1.0 + nu);
+  lam = nu * two_mu / (1.0 - 2.0 * nu);
+  mu = two_mu / 2.0;
+}
+
+// f_k = sum_q B_k^T S_q where S_q already carries det J * w. Written (RMW) into the
+// full-DOF vector at the element's nodes; one lane per node.
+template <int NN, int NGP>
+__device__ __forceinline__ void node_forces(const Geo<NN, NGP>& G, const double* S,
+                                            const int32_t* node, double* __restrict__ f,
+                                            int lane) {
+  for (int k = lane; k < NN; k += kWave) {
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      const double* g = G.g + (q * NN + k) * 3;
+      const double* s = S + 6 * q;
+      f0 += g[0] * s[0] + g[1] * s[3] + g[2] * s[4];
+      f1 += g[1] * s[1] + g[0] * s[3] + g[2] * s[5];
+      f2 += g[2] * s[2] + g[0] * s[4] + g[1] * s[5];
+    }
+    double* fp = f + 3 * int64_t(node[k]);
+    fp[0] += f0;
+    fp[1] += f1;
+    fp[2] += f2;
+  }
+}
+
+// Node pair p -> (a, b). Upper numbering walks a-major over b >= a.
+template <int NN, bool UPPER>
+__device__ __forceinline__ void decode_pair(int p, int& a, int& b) {
+  if (UPPER) {
+    int aa = 0, rem = p;
+    while (rem >= NN - aa) {
+      rem -= NN - aa;
+      ++aa;
+    }
+    a = aa;
+    b = aa + rem;
+  } else {
+    a = p / NN;
+    b = p % NN;
+  }
+}
+
+// Add the 3x3 block K (rows: DOFs of node a, cols: DOFs of node b) into the CSR
+// values; with `mirror` also its transpose into the rows of node b.
+template <int NN>
+__device__ __forceinline__ void scatter_pair(const DofLds<NN>& D, const int32_t* __restrict__ se,
+                                             int a, int b, const double K[9], bool mirror,
+                                             double* __restrict__ vals) {
+  const int32_t cab = se[a * NN + b];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int64_t r = D.row[3 * a + i];
+    if (r < 0) continue;
+    double* rp = vals + r + cab;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int32_t c = D.col[3 * b + j];
+      if (c >= 0) rp[c] += K[3 * i + j];
+    }
+  }
+  if (mirror) {
+    const int32_t cba = se[b * NN + a];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int64_t r = D.row[3 * b + j];
+      if (r < 0) continue;
+      double* rp = vals + r + cba;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int32_t c = D.col[3 * a + i];
+        if (c >= 0) rp[c] += K[3 * i + j];
+      }
+    }
+  }
+}
+
+// ---- K1+K4(+K5): isotropic elastic stiffness straight into CSR ---------------
+template <int NN, int NGP, bool UPPER>
+struct AsmLds {
+  Geo<NN, NGP> G;
+  DofLds<NN> D;
+  double u[NN * 3];
+  double S[NGP * 6];
+  int32_t node[NN];
+};
+
+template <int NN, int NGP, bool UPPER>
+__global__ __launch_bounds__(kWave) void k_assemble_iso(
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const int32_t* __restrict__ dof_eq,
+    const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ slot,
+    double* __restrict__ vals, const double* __restrict__ u, double* __restrict__ f_int,
+    int* __restrict__ flag) {
+  __shared__ AsmLds<NN, NGP, UPPER> L;
+  const int lane = threadIdx.x;
+  const int64_t le = elist[blockIdx.x];
+  const int32_t* ce = conn + le * NN;
+  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
+  stage_dofs<NN>(L.D, ce, dof_eq, row_ptr, lane);
+  if (f_int) stage_vec<NN>(L.u, ce, u, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  gradients<NN, NGP>(L.G, dN, lane);
+  __syncthreads();
+  double lam, mu;
+  lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
+  if (f_int) {
+    for (int q = lane; q < NGP; q += kWave) {
+      double H[3][3], e[6], s[6];
+      grad_field<NN, NGP>(L.G, L.u, q, H);
+      voigt_strain(H, e);
+      iso_stress(lam, mu, e, s);
+      for (int i = 0; i < 6; ++i) L.S[6 * q + i] = L.G.sc[q] * s[i];
+    }
+    __syncthreads();
+    node_forces<NN, NGP>(L.G, L.S, L.node, f_int, lane);
+  }
+  const int32_t* se = slot + le * NN * NN;
+  constexpr int NP = UPPER ? NN * (NN + 1) / 2 : NN * NN;
+  for (int p = lane; p < NP; p += kWave) {
+    int a, b;
+    decode_pair<NN, UPPER>(p, a, b);
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < NGP; ++q) {
+      const double* ga = L.G.g + (q * NN + a) * 3;
+      const double* gb = L.G.g + (q * NN + b) * 3;
+      const double ls = lam * L.G.sc[q], ms = mu * L.G.sc[q];
+      const double la0 = ls * ga[0], la1 = ls * ga[1], la2 = ls * ga[2];
+      const double ma0 = ms * ga[0], ma1 = ms * ga[1], ma2 = ms * ga[2];
+      const double gb0 = gb[0], gb1 = gb[1], gb2 = gb[2];
+      const double dd = ma0 * gb0 + ma1 * gb1 + ma2 * gb2;
+      K[0] += la0 * gb0 + ma0 * gb0 + dd;
+      K[1] += la0 * gb1 + ma1 * gb0;
+      K[2] += la0 * gb2 + ma2 * gb0;
+      K[3] += la1 * gb0 + ma0 * gb1;
+      K[4] += la1 * gb1 + ma1 * gb1 + dd;
+      K[5] += la1 * gb2 + ma2 * gb1;
+      K[6] += la2 * gb0 + ma0 * gb2;
+      K[7] += la2 * gb1 + ma1 * gb2;
+      K[8] += la2 * gb2 + ma2 * gb2 + dd;
+    }
+    scatter_pair<NN>(L.D, se, a, b, K, UPPER && a != b, vals);
+  }
+}
+
+// ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
+template <int NN, int NGP>
+struct ForceLds {
+  Geo<NN, NGP> G;
+  double u[NN * 3];
+  double S[NGP * 6];
+  int32_t node[NN];
+};
+
+template <int NN, int NGP>
+__global__ __launch_bounds__(kWave) void k_internal_force(
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const double* __restrict__ u, double* __restrict__ f_int,
+    int* __restrict__ flag) {
+  __shared__ ForceLds<NN, NGP> L;
+  const int lane = threadIdx.x;
+  const int64_t le = elist[blockIdx.x];
+  const int32_t* ce = conn + le * NN;
+  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
+  stage_vec<NN>(L.u, ce, u, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  gradients<NN, NGP>(L.G, dN, lane);
+  __syncthreads();
+  double lam, mu;
+  lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
+  for (int q = lane; q < NGP; q += kWave) {
+    double H[3][3], e[6], s[6];
+    grad_field<NN, NGP>(L.G, L.u, q, H);
+    voigt_strain(H, e);
+    iso_stress(lam, mu, e, s);
+    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = L.G.sc[q] * s[i];
+  }
+  __syncthreads();
+  node_forces<NN, NGP>(L.G, L.S, L.node, f_int, lane);
+}
+
+// ---- K2: J2 radial return + consistent tangent + internal force --------------
+// Material constants prepared on the host (J2PlasticMaterial ctor,
+// src/fem/plasticity.cpp:42-65): kinematic modulus derived from the table slope
+// for KINEMATIC/COMBINED, yield table flattened for pure KINEMATIC.
+struct J2Dev {
+  double E, nu, mu, hkin;
+  int32_t n;
+  double ep[CXG_MAX_TABLE], sy[CXG_MAX_TABLE];
+};
+
+constexpr double kSqrt32 = 1.224744871391589;  // sqrt(3/2)
+
+// Piecewise-linear yield curve with flat ends (plasticity.cpp:67-88).
+__device__ __forceinline__ void yield_at(const J2Dev& m, double ep, double& sy, double& h) {
+  const int n = m.n;
+  if (n == 1 || ep <= m.ep[0]) {
+    sy = m.sy[0];
+    h = 0.0;
+    return;
+  }
+  if (ep >= m.ep[n - 1]) {
+    sy = m.sy[n - 1];
+    h = 0.0;
+    return;
+  }
+  for (int i = 0; i + 1 < n; ++i)
+    if (ep >= m.ep[i] && ep <= m.ep[i + 1]) {
+      h = (m.sy[i + 1] - m.sy[i]) / (m.ep[i + 1] - m.ep[i]);
+      sy = m.sy[i] + h * (ep - m.ep[i]);
+      return;
+    }
+  sy = m.sy[n - 1];
+  h = 0.0;
+}
+
+// Radial return of one Gauss point: reads the committed history hc[13], writes the
+// trial history ht[13], the stress and the algorithmic tangent D (6x6 row-major).
+__device__ void j2_point(const J2Dev& m, const double eps[6], const double hc[13],
+                         double ht[13], double sig[6], double D[36]) {
+  double lam, mu;
+  lame(m.E, m.nu, lam, mu);
+  double ee[6];
+  for (int i = 0; i < 6; ++i) ee[i] = eps[i] - hc[i];
+  double st[6];
+  iso_stress(lam, mu, ee, st);
+  const double mean = (st[0] + st[1] + st[2]) / 3.0;
+  const double dev[6] = {st[0] - mean, st[1] - mean, st[2] - mean, st[3], st[4], st[5]};
+  double xi[6];
+  for (int i = 0; i < 6; ++i) xi[i] = dev[i] - hc[7 + i];
+  const double nrm = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] +
+                          2.0 * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
+  double sy0, h0;
+  yield_at(m, hc[6], sy0, h0);
+  const double qtr = kSqrt32 * nrm;
+  // Elastic tangent De first; the plastic branch subtracts from it.
+  for (int i = 0; i < 36; ++i) D[i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D[6 * i + j] = (i == j) ? lam + 2.0 * mu : lam;
+  for (int i = 3; i < 6; ++i) D[7 * i] = mu;
+  if (qtr - sy0 <= 0.0 || nrm == 0.0) {
+    for (int i = 0; i < 6; ++i) sig[i] = st[i];
+    for (int i = 0; i < 13; ++i) ht[i] = hc[i];
+    return;
+  }
+  const double ep0 = hc[6];
+  double dg = 0.0;
+  for (int it = 0; it < 50; ++it) {
+    double sy, h;
+    yield_at(m, ep0 + dg, sy, h);
+    const double res = qtr - 3.0 * mu * dg - m.hkin * dg - sy;
+    const double step = res / (3.0 * mu + m.hkin + h);
+    dg += step;
+    if (dg < 0.0) dg = 0.0;
+    if (fabs(step) <= 1e-14 * (1.0 + fabs(dg))) break;
+  }
+  const double inv = 1.0 / nrm;
+  double n[6];
+  for (int i = 0; i < 6; ++i) n[i] = xi[i] * inv;
+  const double cdev = 2.0 * mu * dg * kSqrt32;
+  const double calpha = (2.0 / 3.0) * m.hkin * dg * kSqrt32;
+  const double cep = dg * kSqrt32;
+  for (int i = 0; i < 6; ++i) {
+    sig[i] = dev[i] - cdev * n[i] + (i < 3 ? mean : 0.0);
+    ht[i] = hc[i] + cep * n[i] * (i < 3 ? 1.0 : 2.0);
+    ht[7 + i] = hc[7 + i] + calpha * n[i];
+  }
+  ht[6] = ep0 + dg;
+  double sy, h;
+  yield_at(m, ep0 + dg, sy, h);
+  const double theta = 1.0 - cdev / nrm;
+  const double theta_bar = 1.0 / (1.0 + (h + m.hkin) / (3.0 * mu)) - (1.0 - theta);
+  const double soft = 1.0 - theta, c_nn = 2.0 * mu * theta_bar;
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double Pij;
+      if (i < 3 && j < 3) Pij = 2.0 * mu * ((i == j ? 1.0 : 0.0) - 1.0 / 3.0);
+      else Pij = (i == j) ? mu : 0.0;
+      D[6 * i + j] -= soft * Pij + c_nn * n[i] * n[j];
+    }
+}
+
+// (D_q B_b)[r][j] for the sparse B column pattern of node b (gradient gb).
+__device__ __forceinline__ void d_times_b(const double* D, const double* gb, double DB[18]) {
+  // Nonzero rows of B column j: j=0: (0,gx) (3,gy) (4,gz); j=1: (1,gy) (3,gx) (5,gz);
+  // j=2: (2,gz) (4,gx) (5,gy).
+  for (int r = 0; r < 6; ++r) {
+    const double* Dr = D + 6 * r;
+    DB[3 * r + 0] = Dr[0] * gb[0] + Dr[3] * gb[1] + Dr[4] * gb[2];
+    DB[3 * r + 1] = Dr[1] * gb[1] + Dr[3] * gb[0] + Dr[5] * gb[2];
+    DB[3 * r + 2] = Dr[2] * gb[2] + Dr[4] * gb[0] + Dr[5] * gb[1];
+  }
+}
+
+// K_ab[i][j] += s * sum_r B_a[r][i] (D B_b)[r][j]
+__device__ __forceinline__ void bt_db(const double* ga, const double* DB, double s,
+                                      double K[9]) {
+  for (int j = 0; j < 3; ++j) {
+    K[0 + j] += s * (ga[0] * DB[0 + j] + ga[1] * DB[9 + j] + ga[2] * DB[12 + j]);
+    K[3 + j] += s * (ga[1] * DB[3 + j] + ga[0] * DB[9 + j] + ga[2] * DB[15 + j]);
+    K[6 + j] += s * (ga[2] * DB[6 + j] + ga[0] * DB[12 + j] + ga[1] * DB[15 + j]);
+  }
+}
+
+template <int NN, int NGP, bool UPPER>
+struct J2Lds {
+  Geo<NN, NGP> G;
+  DofLds<NN> D;
+  double u[NN * 3];
+  double S[NGP * 6];
+  double Dt[NGP * 36];
+  int32_t node[NN];
+};
+
+template <int NN, int NGP, bool UPPER>
+__global__ __launch_bounds__(kWave) void k_assemble_j2(
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    int64_t gp_offset, int64_t n_points, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu,
+    const int32_t* __restrict__ mat_id, const J2Dev* __restrict__ mats,
+    const double* __restrict__ u, const double* __restrict__ committed,
+    double* __restrict__ trial, const int32_t* __restrict__ dof_eq,
+    const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ slot,
+    double* __restrict__ vals, double* __restrict__ f_int, int* __restrict__ flag) {
+  __shared__ J2Lds<NN, NGP, UPPER> L;
+  const int lane = threadIdx.x;
+  const int64_t le = elist[blockIdx.x];
+  const int64_t ge = elem_offset + le;
+  const int32_t* ce = conn + le * NN;
+  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
+  stage_dofs<NN>(L.D, ce, dof_eq, row_ptr, lane);
+  stage_vec<NN>(L.u, ce, u, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  gradients<NN, NGP>(L.G, dN, lane);
+  __syncthreads();
+  const int32_t mid = mat_id ? mat_id[ge] : -1;
+  for (int q = lane; q < NGP; q += kWave) {
+    double H[3][3], e[6], s[6];
+    grad_field<NN, NGP>(L.G, L.u, q, H);
+    voigt_strain(H, e);
+    const int64_t pt = gp_offset + le * NGP + q;
+    double* Dq = L.Dt + 36 * q;
+    if (mid >= 0) {
+      double hc[13], ht[13];
+      for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
+      j2_point(mats[mid], e, hc, ht, s, Dq);
+      for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = ht[c];
+    } else {
+      double lam, mu;
+      lame(E[ge], nu[ge], lam, mu);
+      iso_stress(lam, mu, e, s);
+      for (int i = 0; i < 36; ++i) Dq[i] = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Dq[6 * i + j] = (i == j) ? lam + 2.0 * mu : lam;
+      for (int i = 3; i < 6; ++i) Dq[7 * i] = mu;
+      for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = committed[c * n_points + pt];
+    }
+    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = L.G.sc[q] * s[i];
+  }
+  __syncthreads();
+  node_forces<NN, NGP>(L.G, L.S, L.node, f_int, lane);
+  const int32_t* se = slot + le * NN * NN;
+  constexpr int NP = UPPER ? NN * (NN + 1) / 2 : NN * NN;
+  for (int p = lane; p < NP; p += kWave) {
+    int a, b;
+    decode_pair<NN, UPPER>(p, a, b);
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < NGP; ++q) {
+      double DB[18];
+      d_times_b(L.Dt + 36 * q, L.G.g + (q * NN + b) * 3, DB);
+      bt_db(L.G.g + (q * NN + a) * 3, DB, L.G.sc[q], K);
+    }
+    scatter_pair<NN>(L.D, se, a, b, K, UPPER && a != b, vals);
+  }
+}
+
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+// rhs[eq(g)] += f_ext[g] - f_c[g] on free DOFs.
+__global__ void k_reduce_free(int64_t n_dof, const int32_t* __restrict__ dof_eq,
+                              const double* __restrict__ f_ext, const double* __restrict__ f_c,
+                              double* __restrict__ rhs) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < n_dof; g += stride) {
+    const int32_t eq = dof_eq[g];
+    if (eq < 0) continue;
+    double v = 0.0;
+    if (f_ext) v += f_ext[g];
+    if (f_c) v -= f_c[g];
+    rhs[eq] += v;
+  }
+}
+
+// ---- K6: adjoint of r = K(x, E) u -------------------------------------------
+// Per Gauss point, with s = det J * w, sigma = D eps:
+//   phi_q = s sigma_u : H_w             (w . r restricted to the element)
+//   dphi/dx_m = sum_q P_q g_m,  P_q = phi_q I - s (sigma_u H_w + sigma_w H_u)^T
+//   dphi/dE   = phi / E,   dphi/du_m = sum_q s B_m^T sigma_w
+// B is recomputed at the Gauss points (never stored), exactly as the forward pass.
+template <int NN, int NGP>
+struct VjpLds {
+  Geo<NN, NGP> G;
+  double u[NN * 3];
+  double w[NN * 3];
+  double P[NGP * 9];
+  double S[NGP * 6];
+  double phi[NGP];
+  int32_t node[NN];
+};
+
+template <int NN, int NGP>
+__global__ __launch_bounds__(kWave) void k_vjp(
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const double* __restrict__ u, const double* __restrict__ w,
+    double* __restrict__ dx, double* __restrict__ dE, double* __restrict__ du,
+    int* __restrict__ flag) {
+  __shared__ VjpLds<NN, NGP> L;
+  const int lane = threadIdx.x;
+  const int64_t le = elist[blockIdx.x];
+  const int64_t ge = elem_offset + le;
+  const int32_t* ce = conn + le * NN;
+  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
+  stage_vec<NN>(L.u, ce, u, lane);
+  stage_vec<NN>(L.w, ce, w, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  gradients<NN, NGP>(L.G, dN, lane);
+  __syncthreads();
+  const double Ee = E[ge];
+  double lam, mu;
+  lame(Ee, nu[ge], lam, mu);
+  for (int q = lane; q < NGP; q += kWave) {
+    double Hu[3][3], Hw[3][3], eu[6], ew[6], su[6], sw[6];
+    grad_field<NN, NGP>(L.G, L.u, q, Hu);
+    grad_field<NN, NGP>(L.G, L.w, q, Hw);
+    voigt_strain(Hu, eu);
+    voigt_strain(Hw, ew);
+    iso_stress(lam, mu, eu, su);
+    iso_stress(lam, mu, ew, sw);
+    const double s = L.G.sc[q];
+    double pq = 0.0;
+    for (int i = 0; i < 6; ++i) pq += ew[i] * su[i];
+    pq *= s;
+    const double Su[3][3] = {{su[0], su[3], su[4]}, {su[3], su[1], su[5]}, {su[4], su[5], su[2]}};
+    const double Sw[3][3] = {{sw[0], sw[3], sw[4]}, {sw[3], sw[1], sw[5]}, {sw[4], sw[5], sw[2]}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double m = 0.0;
+        for (int l = 0; l < 3; ++l) m += Su[i][l] * Hw[l][j] + Sw[i][l] * Hu[l][j];
+        // P[j][i] = pq delta_ij - s M[i][j]
+        L.P[9 * q + 3 * j + i] = (i == j ? pq : 0.0) - s * m;
+      }
+    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = s * sw[i];
+    L.phi[q] = pq;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double phi = 0.0;
+    for (int q = 0; q < NGP; ++q) phi += L.phi[q];
+    dE[ge] = phi / Ee;
+  }
+  for (int k = lane; k < NN; k += kWave) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      const double* g = L.G.g + (q * NN + k) * 3;
+      const double* P = L.P + 9 * q;
+      a0 += P[0] * g[0] + P[1] * g[1] + P[2] * g[2];
+      a1 += P[3] * g[0] + P[4] * g[1] + P[5] * g[2];
+      a2 += P[6] * g[0] + P[7] * g[1] + P[8] * g[2];
+    }
+    double* xp = dx + 3 * int64_t(L.node[k]);
+    xp[0] += a0;
+    xp[1] += a1;
+    xp[2] += a2;
+  }
+  node_forces<NN, NGP>(L.G, L.S, L.node, du, lane);
+}
+
+// ---- host-side dispatch -------------------------------------------------------
+inline int grid_for(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return int(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+const char* type_name(int t) {
+  switch (t) {
+    case CXG_C3D8: return "C3D8";
+    case CXG_C3D20: return "C3D20";
+    case CXG_C3D20R: return "C3D20R";
+    case CXG_C3D6: return "C3D6";
+    default: return "other";
+  }
+}
+
+// Launch f once per (colour, block) in colour order; each launch is timed as
+// "<what>_<family>" when the context's timing is on.
+template <class F>
+void for_each_colour_block(cxg_context* ctx, const char* what, const cxg_mesh* m, F&& f) {
+  for (int c = 0; c < m->n_colors; ++c)
+    for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+      const cxg_block_dev& B = m->blocks[bi];
+      if (c < int(B.color_count.size()) && B.color_count[size_t(c)] > 0) {
+        LaunchTimer timer(ctx, std::string(what) + "_" + type_name(B.type), B.color_count[size_t(c)]);
+        f(bi, B, B.color[size_t(c)].p, B.color_count[size_t(c)]);
+      }
+    }
+}
+
+#define CXG_DISPATCH(TYPE, MACRO)                 \
+  switch (TYPE) {                                 \
+    case CXG_C3D8: MACRO(8, 8, false); break;     \
+    case CXG_C3D20: MACRO(20, 27, true); break;   \
+    case CXG_C3D20R: MACRO(20, 8, true); break;   \
+    case CXG_C3D6: MACRO(6, 2, false); break;     \
+    default: throw Error(CXG_ERR_UNSUPPORTED, "element type outside the GPU path"); \
+  }
+
+}  // namespace
+
+void fill_f64(hipStream_t s, double* p, int64_t n, double v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+  CXG_HIP(hipGetLastError());
+}
+
+void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                            const double* E, const double* nu, double* vals, const double* u,
+                            double* f_int) {
+  const cxg_mesh* m = p->mesh;
+  fill_f64(ctx->stream, vals, p->nnz, 0.0);
+  if (f_int) fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
+  for_each_colour_block(ctx, "assemble_iso", m, [&](size_t bi, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+#define LAUNCH(NN, NGP, UP)                                                                     \
+  hipLaunchKernelGGL((k_assemble_iso<NN, NGP, UP>), dim3(unsigned(cnt)), dim3(kWave), 0,        \
+                     ctx->stream, conn, el, B.elem_offset, T.dN, T.w, coords, E, nu,           \
+                     p->dof_eq.p, p->row_ptr.p, p->slot[bi].p, vals, u, f_int, ctx->flag_dev)
+    CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+    CXG_HIP(hipGetLastError());
+  });
+}
+
+void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
+                           const double* E, const double* nu, const double* u, double* f_int) {
+  fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
+  for_each_colour_block(ctx, "internal_force", m, [&](size_t, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+#define LAUNCH(NN, NGP, UP)                                                                   \
+  hipLaunchKernelGGL((k_internal_force<NN, NGP>), dim3(unsigned(cnt)), dim3(kWave), 0,        \
+                     ctx->stream, conn, el, B.elem_offset, T.dN, T.w, coords, E, nu, u,      \
+                     f_int, ctx->flag_dev)
+    CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+    CXG_HIP(hipGetLastError());
+  });
+}
+
+void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
+                        const double* f_ext, const double* f_c, double* rhs) {
+  const int64_t n = int64_t(n_nodes) * 3;
+  hipLaunchKernelGGL(k_reduce_free, dim3(grid_for(n)), dim3(256), 0, ctx->stream, n, dof_eq,
+                     f_ext, f_c, rhs);
+  CXG_HIP(hipGetLastError());
+}
+
+static J2Dev prepare_j2(const cxg_j2_material& p) {
+  if (p.n_table < 1 || p.n_table > CXG_MAX_TABLE)
+    throw Error(CXG_ERR_ARG, "J2PlasticMaterial needs a non-empty *PLASTIC table");
+  J2Dev d{};
+  d.E = p.E;
+  d.nu = p.nu;
+  d.mu = p.E / (2.0 * (1.0 + p.nu));
+  d.hkin = p.kinematic_modulus;
+  d.n = p.n_table;
+  for (int i = 0; i < p.n_table; ++i) {
+    d.ep[i] = p.eqplastic[i];
+    d.sy[i] = p.yield[i];
+  }
+  if (p.hardening == CXG_HARD_KINEMATIC || p.hardening == CXG_HARD_COMBINED) {
+    if (d.hkin == 0.0 && d.n >= 2 && d.ep[1] > d.ep[0])
+      d.hkin = (d.sy[1] - d.sy[0]) / (d.ep[1] - d.ep[0]);
+    if (p.hardening == CXG_HARD_KINEMATIC)
+      for (int i = 1; i < d.n; ++i) d.sy[i] = d.sy[0];
+  }
+  return d;
+}
+
+void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                        const double* E, const double* nu, const int32_t* mat_id,
+                        const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                        const double* committed, double* trial, double* vals, double* f_int) {
+  // The device copy of the material table is re-uploaded only when it changes, so
+  // the steady state of a Newton loop enqueues kernels only.
+  std::vector<J2Dev> h(size_t(n_mats > 0 ? n_mats : 0));
+  for (int32_t i = 0; i < n_mats; ++i) h[size_t(i)] = prepare_j2(mats[i]);
+  const size_t bytes = h.size() * sizeof(J2Dev);
+  const bool same = bytes == ctx->j2_host.size() && ctx->j2_dev.p &&
+                    std::memcmp(h.data(), ctx->j2_host.data(), bytes) == 0;
+  if (!same && bytes) {
+    ctx->j2_dev.alloc(bytes);
+    CXG_HIP(hipStreamSynchronize(ctx->stream));
+    CXG_HIP(hipMemcpy(ctx->j2_dev.p, h.data(), bytes, hipMemcpyHostToDevice));
+    ctx->j2_host.assign(reinterpret_cast<const unsigned char*>(h.data()),
+                        reinterpret_cast<const unsigned char*>(h.data()) + bytes);
+  }
+  const J2Dev* dmats = bytes ? reinterpret_cast<const J2Dev*>(ctx->j2_dev.p) : nullptr;
+  const cxg_mesh* m = p->mesh;
+  fill_f64(ctx->stream, vals, p->nnz, 0.0);
+  fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
+  for_each_colour_block(ctx, "assemble_j2", m, [&](size_t bi, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+#define LAUNCH(NN, NGP, UP)                                                                     \
+  hipLaunchKernelGGL((k_assemble_j2<NN, NGP, UP>), dim3(unsigned(cnt)), dim3(kWave), 0,         \
+                     ctx->stream, conn, el, B.elem_offset, B.gp_offset, m->n_points, T.dN,     \
+                     T.w, coords, E, nu, mat_id, dmats, u, committed, trial, p->dof_eq.p,      \
+                     p->row_ptr.p, p->slot[bi].p, vals, f_int, ctx->flag_dev)
+    CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+    CXG_HIP(hipGetLastError());
+  });
+}
+
+void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial) {
+  const int64_t n = n_points * CXG_STATE_COMPONENTS;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, committed, trial, n);
+  CXG_HIP(hipGetLastError());
+}
+
+void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
+                const double* nu, const double* u, const double* w, double* dx, double* dE,
+                double* du) {
+  fill_f64(ctx->stream, dx, int64_t(m->n_nodes) * 3, 0.0);
+  fill_f64(ctx->stream, du, int64_t(m->n_nodes) * 3, 0.0);
+  for_each_colour_block(ctx, "vjp", m, [&](size_t, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+#define LAUNCH(NN, NGP, UP)                                                                   \
+  hipLaunchKernelGGL((k_vjp<NN, NGP>), dim3(unsigned(cnt)), dim3(kWave), 0, ctx->stream,      \
+                     conn, el, B.elem_offset, T.dN, T.w, coords, E, nu, u, w, dx, dE, du,     \
+                     ctx->flag_dev)
+    CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+    CXG_HIP(hipGetLastError());
+  });
+}
+
+}  // namespace cxg

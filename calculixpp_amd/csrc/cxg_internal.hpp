@@ -1,0 +1,171 @@
+// Internal declarations of libcxg.so (not installed; the public ABI is include/cxg.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cxg.h"
+
+namespace cxg {
+
+// Error carried from the internals to the C ABI, where it becomes a status code and
+// the context's last-error text. Never crosses the ABI as an exception.
+struct Error : std::runtime_error {
+  cxg_status status;
+  Error(cxg_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define CXG_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::cxg::Error(CXG_ERR_HIP, std::string(#call " failed: ") +                 \
+                                          hipGetErrorString(e_));                      \
+  } while (0)
+
+// The reference's message for a non-positive Jacobian (src/fem/element.cpp:87).
+inline const char* kBadJacobian = "non-positive element Jacobian (bad connectivity?)";
+
+// Bits of the device status word every kernel may raise.
+enum : int { kFlagJacobian = 1, kFlagValence = 2, kFlagPattern = 4 };
+
+// Shape-derivative tables of one element family, resident in device memory:
+// dN[q][k][j] = dN_k/dxi_j at Gauss point q, and the Gauss weights w[q].
+struct ElemTable {
+  int nn = 0, ngp = 0;
+  const double* dN = nullptr;  // device, ngp*nn*3
+  const double* w = nullptr;   // device, ngp
+};
+
+int nodes_per_element(int type);  // 0 for types outside the GPU path
+int gauss_points(int type);       // 0 for types outside the GPU path
+// Host tables (natural coordinates of the Gauss points and derivatives).
+void host_tables(int type, std::vector<double>& dN, std::vector<double>& w);
+
+// Device buffer owning helper (hipMalloc/hipFree), move-only.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void alloc(size_t count) {
+    reset();
+    n = count;
+    if (count) CXG_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace cxg
+
+struct cxg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int* flag_dev = nullptr;   // device status word (cxg::kFlag*)
+  int* flag_host = nullptr;  // pinned mirror
+  cxg::DevBuf<double> tab_dN[8], tab_w[8];
+  cxg::ElemTable tables[8];
+  std::vector<unsigned char> j2_host;  // last uploaded J2 material table (bytes)
+  cxg::DevBuf<unsigned char> j2_dev;
+  // Kernel timing (cxg_profile_*): events bracket each launch while enabled.
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+    int64_t elements;
+  };
+  struct Stat {
+    std::string name;
+    int64_t launches = 0, elements = 0;
+    double ms = 0.0;
+  };
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<Pending> pending;
+  std::vector<Stat> stats;
+};
+
+namespace cxg {
+// Records a start event on construction and an end event on destruction when the
+// context's timing is on; results are collected by collect_timing().
+struct LaunchTimer {
+  cxg_context* ctx;
+  cxg_context::Pending p{};
+  bool on;
+  LaunchTimer(cxg_context* c, std::string name, int64_t elements);
+  ~LaunchTimer();
+};
+void collect_timing(cxg_context* ctx);  // after a stream synchronization
+}  // namespace cxg
+
+// One element family run inside a mesh.
+struct cxg_block_dev {
+  int32_t type = 0, nn = 0, ngp = 0;
+  int64_t n_elem = 0, elem_offset = 0, gp_offset = 0;
+  int64_t conn_offset = 0;                     // into cxg_mesh::conn (entries)
+  std::vector<cxg::DevBuf<int32_t>> color;     // per colour: block-local element ids
+  std::vector<int64_t> color_count;
+};
+
+struct cxg_mesh {
+  cxg_context* ctx = nullptr;
+  int32_t n_nodes = 0;
+  int64_t n_elem = 0, n_points = 0;
+  int32_t n_colors = 0;
+  std::vector<cxg_block_dev> blocks;
+  cxg::DevBuf<int32_t> conn;      // all blocks, concatenated
+  cxg::DevBuf<int64_t> eptr;      // model element -> offset into conn (n_elem+1)
+  cxg::DevBuf<int64_t> inc_ptr;   // node -> incident elements (n_nodes+1)
+  cxg::DevBuf<int32_t> inc_elem;  // incident model element numbers, ascending
+  int32_t max_candidates = 0;     // max over nodes of sum of incident element sizes
+};
+
+struct cxg_pattern {
+  const cxg_mesh* mesh = nullptr;
+  int32_t n_free = 0;
+  int64_t nnz = 0;
+  cxg::DevBuf<int32_t> dof_eq;                // 3*n_nodes
+  cxg::DevBuf<int64_t> row_ptr;               // n_free+1
+  cxg::DevBuf<int32_t> col_idx;               // nnz
+  std::vector<cxg::DevBuf<int32_t>> slot;     // per block: n_elem*nn*nn column offsets
+};
+
+namespace cxg {
+// Kernels launchers (cxg_elem.hip / cxg_pattern.hip).
+void fill_f64(hipStream_t s, double* p, int64_t n, double v);
+void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                            const double* E, const double* nu, double* vals, const double* u,
+                            double* f_int);
+void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
+                           const double* E, const double* nu, const double* u, double* f_int);
+void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
+                        const double* f_ext, const double* f_c, double* rhs);
+void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                        const double* E, const double* nu, const int32_t* mat_id,
+                        const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                        const double* committed, double* trial, double* vals, double* f_int);
+void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial);
+void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
+                const double* nu, const double* u, const double* w, double* dx, double* dE,
+                double* du);
+void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq, int32_t n_free,
+                   cxg_pattern* out);
+// Read back and clear the device status word; throws on a raised flag.
+void check_flags(cxg_context* ctx);
+}  // namespace cxg

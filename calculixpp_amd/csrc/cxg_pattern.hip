@@ -1,0 +1,272 @@
+// K3: CSR pattern of the free/free stiffness and the element -> CSR slot map.
+//
+// Replaces what the reference spreads over build_dof_map (src/fem/assembly.cpp:
+// 19-42), the int64 COO keys of scatter_block (src/fem/assembly.cpp:80-94), the
+// hash-ordered flush (src/fem/assembly.cpp:211-221) and SciPP's COO->CSR. The
+// pattern is the structural one: free DOF (a,i) couples to free DOF (b,j) when
+// nodes a and b share an element. Rows are free equations in ascending order;
+// columns ascend inside a row (free equations are numbered in ascending global DOF
+// order, constraint_transform.cpp:79-82, so sorting by node then component sorts by
+// equation). Built once per mesh + constraint set and reused by every assembly.
+//
+// Steps:
+//   1. node adjacency: one 64-lane wave per node gathers the nodes of its incident
+//      elements into LDS, bitonic-sorts them and keeps the unique ones (count pass,
+//      host prefix sum, fill pass);
+//   2. per adjacency entry the offset of the neighbour's column block in the row
+//      (prefix of free components), and per node the row length;
+//   3. row_ptr by a host prefix sum over free equations;
+//   4. col_idx, one wave per node;
+//   5. slot[e][a][b] = column offset of node b's block inside the rows of node a,
+//      by binary search of the sorted adjacency (one thread per element node pair).
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+#include "cxg_internal.hpp"
+
+namespace cxg {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kCap = 2048;  // candidate nodes per node (sum of incident element sizes)
+
+template <bool FILL>
+__global__ __launch_bounds__(kWave) void k_adjacency(
+    const int64_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_elem,
+    const int64_t* __restrict__ eptr, const int32_t* __restrict__ conn,
+    int32_t* __restrict__ adj_count, const int64_t* __restrict__ adj_ptr,
+    int32_t* __restrict__ adj, int* __restrict__ flag) {
+  __shared__ int32_t buf[kCap];
+  __shared__ int32_t total;
+  const int32_t a = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
+  if (lane == 0) {
+    int64_t t = 0;
+    for (int64_t i = i0; i < i1; ++i) {
+      const int32_t e = inc_elem[i];
+      t += eptr[e + 1] - eptr[e];
+    }
+    total = t > kCap ? -1 : int32_t(t);
+  }
+  __syncthreads();
+  const int cnt = total;
+  if (cnt < 0) {
+    if (lane == 0) atomicOr(flag, kFlagValence);
+    return;
+  }
+  int off = 0;
+  for (int64_t i = i0; i < i1; ++i) {
+    const int32_t e = inc_elem[i];
+    const int64_t b = eptr[e];
+    const int n = int(eptr[e + 1] - b);
+    if (lane < n) buf[off + lane] = conn[b + lane];
+    off += n;
+  }
+  int N = kWave;
+  while (N < cnt) N <<= 1;
+  for (int i = cnt + lane; i < N; i += kWave) buf[i] = INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= N; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < N; i += kWave) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int32_t x = buf[i], y = buf[ixj];
+          const bool asc = (i & k) == 0;
+          if ((x > y) == asc) {
+            buf[i] = y;
+            buf[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  int out = 0;
+  const int64_t base_out = FILL ? adj_ptr[a] : 0;
+  for (int base = 0; base < cnt; base += kWave) {
+    const int i = base + lane;
+    const bool keep = i < cnt && (i == 0 || buf[i] != buf[i - 1]);
+    const unsigned long long m = __ballot(keep);
+    if (FILL && keep) {
+      const unsigned long long below = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+      adj[base_out + out + __popcll(below)] = buf[i];
+    }
+    out += __popcll(m);
+  }
+  if (!FILL && lane == 0) adj_count[a] = out;
+}
+
+// Column-block offsets inside each node's rows, and the row length of each node.
+__global__ void k_column_offsets(int32_t n_nodes, const int64_t* __restrict__ adj_ptr,
+                                 const int32_t* __restrict__ adj,
+                                 const int32_t* __restrict__ dof_eq, int32_t* __restrict__ cpre,
+                                 int64_t* __restrict__ rowlen) {
+  const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_nodes) return;
+  int64_t run = 0;
+  for (int64_t p = adj_ptr[a]; p < adj_ptr[a + 1]; ++p) {
+    cpre[p] = int32_t(run);
+    const int64_t m = adj[p];
+    run += (dof_eq[3 * m] >= 0) + (dof_eq[3 * m + 1] >= 0) + (dof_eq[3 * m + 2] >= 0);
+  }
+  rowlen[a] = run;
+}
+
+__global__ __launch_bounds__(kWave) void k_col_idx(const int64_t* __restrict__ adj_ptr,
+                                                   const int32_t* __restrict__ adj,
+                                                   const int32_t* __restrict__ cpre,
+                                                   const int32_t* __restrict__ dof_eq,
+                                                   const int64_t* __restrict__ row_ptr,
+                                                   int32_t* __restrict__ col_idx) {
+  const int32_t a = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t p0 = adj_ptr[a], deg = adj_ptr[a + 1] - p0;
+  for (int c = 0; c < 3; ++c) {
+    const int32_t eq = dof_eq[3 * int64_t(a) + c];
+    if (eq < 0) continue;
+    const int64_t rs = row_ptr[eq];
+    for (int64_t p = lane; p < deg; p += kWave) {
+      const int64_t m = adj[p0 + p];
+      int64_t pos = rs + cpre[p0 + p];
+      for (int j = 0; j < 3; ++j) {
+        const int32_t e2 = dof_eq[3 * m + j];
+        if (e2 >= 0) col_idx[pos++] = e2;
+      }
+    }
+  }
+}
+
+__global__ void k_slot(int64_t n_elem, int nn, const int32_t* __restrict__ conn,
+                       const int64_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj,
+                       const int32_t* __restrict__ cpre, int32_t* __restrict__ slot,
+                       int* __restrict__ flag) {
+  const int64_t total = n_elem * nn * nn;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+    const int64_t le = idx / (nn * nn);
+    const int r = int(idx % (nn * nn));
+    const int32_t na = conn[le * nn + r / nn], nb = conn[le * nn + r % nn];
+    int64_t lo = adj_ptr[na], hi = adj_ptr[na + 1] - 1;
+    const int64_t p0 = lo;
+    int64_t found = -1;
+    while (lo <= hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const int32_t v = adj[mid];
+      if (v == nb) {
+        found = mid;
+        break;
+      }
+      if (v < nb) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    if (found < 0) {
+      atomicOr(flag, kFlagPattern);
+      slot[idx] = 0;
+    } else {
+      slot[idx] = cpre[found];
+      (void)p0;
+    }
+  }
+}
+
+}  // namespace
+
+void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_dev,
+                   int32_t n_free, cxg_pattern* P) {
+  hipStream_t s = ctx->stream;
+  const int32_t nn_nodes = m->n_nodes;
+  const int64_t n_dof = int64_t(nn_nodes) * 3;
+  P->mesh = m;
+  P->n_free = n_free;
+  P->dof_eq.alloc(static_cast<size_t>(n_dof));
+  CXG_HIP(hipMemcpyAsync(P->dof_eq.p, dof_eq_dev, size_t(n_dof) * sizeof(int32_t),
+                         hipMemcpyDeviceToDevice, s));
+  std::vector<int32_t> deq(static_cast<size_t>(n_dof));
+  CXG_HIP(hipMemcpyAsync(deq.data(), dof_eq_dev, size_t(n_dof) * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, s));
+  CXG_HIP(hipStreamSynchronize(s));
+  // Free equations must be 0..n_free-1 in ascending global DOF order, the numbering
+  // of build_constraint_transform (constraint_transform.cpp:79-82).
+  int32_t expect = 0;
+  for (int64_t g = 0; g < n_dof; ++g) {
+    if (deq[size_t(g)] < 0) continue;
+    if (deq[size_t(g)] != expect)
+      throw Error(CXG_ERR_ARG, "dof_eq must number free DOFs 0..n_free-1 in ascending DOF order");
+    ++expect;
+  }
+  if (expect != n_free) throw Error(CXG_ERR_ARG, "n_free does not match dof_eq");
+
+  // 1. node adjacency
+  DevBuf<int32_t> adj_count(static_cast<size_t>(nn_nodes));
+  if (nn_nodes > 0)
+    hipLaunchKernelGGL((k_adjacency<false>), dim3(unsigned(nn_nodes)), dim3(kWave), 0, s,
+                       m->inc_ptr.p, m->inc_elem.p, m->eptr.p, m->conn.p, adj_count.p,
+                       nullptr, nullptr, ctx->flag_dev);
+  CXG_HIP(hipGetLastError());
+  std::vector<int32_t> hcount(static_cast<size_t>(nn_nodes));
+  CXG_HIP(hipMemcpyAsync(hcount.data(), adj_count.p, hcount.size() * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, s));
+  check_flags(ctx);
+  std::vector<int64_t> hadj_ptr(size_t(nn_nodes) + 1, 0);
+  for (int32_t a = 0; a < nn_nodes; ++a) hadj_ptr[size_t(a) + 1] = hadj_ptr[size_t(a)] + hcount[size_t(a)];
+  const int64_t n_adj = hadj_ptr.back();
+  DevBuf<int64_t> adj_ptr(hadj_ptr.size());
+  CXG_HIP(hipMemcpyAsync(adj_ptr.p, hadj_ptr.data(), hadj_ptr.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+  DevBuf<int32_t> adj(size_t(std::max<int64_t>(n_adj, 1)));
+  if (nn_nodes > 0)
+    hipLaunchKernelGGL((k_adjacency<true>), dim3(unsigned(nn_nodes)), dim3(kWave), 0, s,
+                       m->inc_ptr.p, m->inc_elem.p, m->eptr.p, m->conn.p, nullptr, adj_ptr.p,
+                       adj.p, ctx->flag_dev);
+  CXG_HIP(hipGetLastError());
+
+  // 2. column-block offsets and row lengths
+  DevBuf<int32_t> cpre(size_t(std::max<int64_t>(n_adj, 1)));
+  DevBuf<int64_t> rowlen(static_cast<size_t>(nn_nodes));
+  if (nn_nodes > 0)
+    hipLaunchKernelGGL(k_column_offsets, dim3(unsigned((nn_nodes + 255) / 256)), dim3(256), 0, s,
+                       nn_nodes, adj_ptr.p, adj.p, P->dof_eq.p, cpre.p, rowlen.p);
+  CXG_HIP(hipGetLastError());
+  std::vector<int64_t> hrowlen(static_cast<size_t>(nn_nodes));
+  CXG_HIP(hipMemcpyAsync(hrowlen.data(), rowlen.p, hrowlen.size() * sizeof(int64_t),
+                         hipMemcpyDeviceToHost, s));
+  CXG_HIP(hipStreamSynchronize(s));
+
+  // 3. row_ptr over free equations
+  std::vector<int64_t> hrow(size_t(n_free) + 1, 0);
+  for (int64_t g = 0; g < n_dof; ++g) {
+    const int32_t eq = deq[size_t(g)];
+    if (eq >= 0) hrow[size_t(eq) + 1] = hrow[size_t(eq)] + hrowlen[size_t(g / 3)];
+  }
+  P->nnz = hrow.back();
+  P->row_ptr.alloc(hrow.size());
+  CXG_HIP(hipMemcpyAsync(P->row_ptr.p, hrow.data(), hrow.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+
+  // 4. col_idx
+  P->col_idx.alloc(size_t(std::max<int64_t>(P->nnz, 1)));
+  if (nn_nodes > 0)
+    hipLaunchKernelGGL(k_col_idx, dim3(unsigned(nn_nodes)), dim3(kWave), 0, s, adj_ptr.p, adj.p,
+                       cpre.p, P->dof_eq.p, P->row_ptr.p, P->col_idx.p);
+  CXG_HIP(hipGetLastError());
+
+  // 5. element -> slot map per block
+  P->slot.clear();
+  for (const cxg_block_dev& B : m->blocks) {
+    DevBuf<int32_t> sl(size_t(std::max<int64_t>(B.n_elem * B.nn * B.nn, 1)));
+    const int64_t total = B.n_elem * B.nn * B.nn;
+    if (total > 0) {
+      const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
+      hipLaunchKernelGGL(k_slot, dim3(unsigned(blocks)), dim3(256), 0, s, B.n_elem, B.nn,
+                         m->conn.p + B.conn_offset, adj_ptr.p, adj.p, cpre.p, sl.p,
+                         ctx->flag_dev);
+      CXG_HIP(hipGetLastError());
+    }
+    P->slot.push_back(std::move(sl));
+  }
+  check_flags(ctx);  // synchronizes: temporaries may be freed afterwards
+}
+
+}  // namespace cxg

@@ -1,0 +1,200 @@
+/*
+ * cxg.h — C ABI of libcxg.so, the MI355X (gfx950) element-integration and
+ * global-assembly library that stands in for CalculiX++'s fem:: hot path.
+ *
+ * Conventions (all from the reference):
+ *  - element type codes are cxpp::ElementType's enumerator values
+ *    (include/calculixpp/core/element.hpp:13-22); the GPU path implements C3D8,
+ *    C3D20, C3D20R and C3D6, anything else returns CXG_ERR_UNSUPPORTED;
+ *  - node order inside an element is CalculiX order (src/fem/element.cpp:160-253);
+ *  - global DOF g = 3*node_index + comp (core/types.hpp:13); dof_eq[g] is the free
+ *    equation of g or -1 when g is constrained (fem/assembly.hpp:25-27);
+ *  - Voigt order xx,yy,zz,xy,xz,yz with engineering shear (core/results.hpp:9);
+ *  - all floating point is FP64 (core/types.hpp:10).
+ *
+ * Memory: every array argument is a DEVICE pointer on the context's device unless
+ * it is marked (host). Arrays are caller-owned; objects created here (mesh,
+ * pattern) own their device memory and free it on destroy. Work is enqueued on the
+ * context's stream; functions that must report a data-dependent error (a
+ * non-positive Jacobian) synchronize that stream before returning.
+ *
+ * Errors: every function returns a cxg_status; the message of the last failure
+ * on a context is cxg_last_error(ctx). Messages for conditions the reference
+ * reports reuse the reference text (for example src/fem/element.cpp:87).
+ * No C++ exception crosses this interface.
+ *
+ * Threading: one context per host thread; objects may be shared read-only.
+ */
+#ifndef CXG_H
+#define CXG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CXG_OK = 0,
+  CXG_ERR_ARG = 1,          /* bad argument (null pointer, size, type)           */
+  CXG_ERR_HIP = 2,          /* HIP runtime failure                               */
+  CXG_ERR_JACOBIAN = 3,     /* det J <= 0 at a Gauss point (element.cpp:86-88)   */
+  CXG_ERR_UNSUPPORTED = 4,  /* element type or mesh shape outside the GPU path   */
+  CXG_ERR_STATE = 5         /* object used with the wrong mesh/pattern           */
+} cxg_status;
+
+enum {
+  CXG_C3D4 = 0, CXG_C3D10 = 1, CXG_C3D8 = 2, CXG_C3D8R = 3,
+  CXG_C3D20 = 4, CXG_C3D20R = 5, CXG_C3D6 = 6, CXG_C3D15 = 7
+};
+
+/* Hardening kinds, cxpp::Plastic::Hardening (core/material.hpp:27-28). */
+enum { CXG_HARD_ISOTROPIC = 0, CXG_HARD_KINEMATIC = 1, CXG_HARD_COMBINED = 2 };
+
+#define CXG_MAX_TABLE 16
+/* Committed (or trial) J2 history per Gauss point, SoA: 13 planes of n_points
+ * doubles: plastic strain [0..5] (engineering shear), equivalent plastic strain
+ * [6], back stress [7..12] (fem/material_model.hpp:38-48). */
+#define CXG_STATE_COMPONENTS 13
+
+typedef struct cxg_context cxg_context;
+typedef struct cxg_mesh cxg_mesh;
+typedef struct cxg_pattern cxg_pattern;
+
+/* A run of elements of one type. Blocks of a mesh cover consecutive element
+ * numbers: block k holds elements elem_offset .. elem_offset+n_elem-1 of the
+ * model, and its Gauss points are gp_offset .. gp_offset + n_elem*ngp - 1. */
+typedef struct {
+  int32_t type;            /* CXG_C3D8 ... */
+  int64_t n_elem;
+  int64_t elem_offset;     /* first model-wide element number of the block */
+  int64_t gp_offset;       /* first model-wide Gauss-point number of the block */
+  const int32_t* conn;     /* device, n_elem * nodes_per_element node indices */
+} cxg_block;
+
+/* One J2 material (core/material.hpp:24-44 + ElasticIso), passed by host pointer. */
+typedef struct {
+  double E, nu;
+  int32_t hardening;       /* CXG_HARD_* */
+  int32_t n_table;         /* 1..CXG_MAX_TABLE points of the *PLASTIC table */
+  double eqplastic[CXG_MAX_TABLE];
+  double yield[CXG_MAX_TABLE];
+  double kinematic_modulus;
+} cxg_j2_material;
+
+/* ---- context ------------------------------------------------------------- */
+/* Create a context on HIP device `device`, enqueueing on `stream` (a
+ * hipStream_t; NULL = the null stream). No counterpart in the reference, which is
+ * single-threaded host code (SURVEY: src/compute/cpu_backend.cpp:108-111). */
+cxg_status cxg_create(int device, void* stream, cxg_context** out);
+void cxg_destroy(cxg_context* ctx);
+cxg_status cxg_set_stream(cxg_context* ctx, void* stream);
+const char* cxg_last_error(const cxg_context* ctx);
+/* Library version string, e.g. "cxg 0.1 gfx950". */
+const char* cxg_version(void);
+int32_t cxg_nodes_per_element(int32_t type);   /* core/element.hpp:24-42 */
+int32_t cxg_gauss_points(int32_t type);        /* fem/element.cpp:106-126 */
+
+/* ---- mesh: element blocks, node incidence and a conflict-free element colouring.
+ * Replaces the per-element node-id gather of fem/assembly.cpp:46-59: connectivity
+ * is copied to library memory once. The colouring (no two elements of one colour
+ * share a node) is what makes every scatter below deterministic without atomics. */
+cxg_status cxg_mesh_create(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks,
+                           const cxg_block* blocks /* host array */, cxg_mesh** out);
+void cxg_mesh_destroy(cxg_mesh* mesh);
+/* n_elem, n_gauss_points, n_colors (any may be NULL). */
+cxg_status cxg_mesh_info(const cxg_mesh* mesh, int64_t* n_elem, int64_t* n_points,
+                         int32_t* n_colors);
+
+/* ---- K3: CSR pattern + element -> CSR slot map ------------------------------
+ * Replaces build_dof_map (fem/assembly.cpp:19-42) + the COO keys of scatter_block
+ * (fem/assembly.cpp:76-100) + SciPP CsrMatrix::from_coo (src/compute/
+ * cpu_backend.cpp:46). Rows are free equations in ascending order, columns
+ * ascending within a row; every structural entry (free DOF pair of two nodes that
+ * share an element) is present, also when its value is zero.
+ * dof_eq: device, 3*n_nodes (free equation or -1). */
+cxg_status cxg_pattern_build(cxg_context* ctx, const cxg_mesh* mesh, const int32_t* dof_eq,
+                             int32_t n_free, cxg_pattern** out);
+void cxg_pattern_destroy(cxg_pattern* pattern);
+cxg_status cxg_pattern_info(const cxg_pattern* p, int32_t* n_free, int64_t* nnz);
+/* Device pointers owned by the pattern: row_ptr[n_free+1] (int64), col_idx[nnz]. */
+const int64_t* cxg_pattern_row_ptr(const cxg_pattern* p);
+const int32_t* cxg_pattern_col_idx(const cxg_pattern* p);
+/* Copy row_ptr / col_idx into caller buffers (device pointers; either may be NULL),
+ * enqueued on the context's stream. */
+cxg_status cxg_pattern_copy(cxg_context* ctx, const cxg_pattern* p, int64_t* row_ptr,
+                            int32_t* col_idx);
+
+/* ---- K1 + K4 (+K5): linear-elastic stiffness into CSR values -----------------
+ * Replaces the element loop of assemble_linear_static (fem/assembly.cpp:237-244):
+ * element_stiffness (fem/element.cpp:614) per element and scatter_tangent /
+ * scatter_block into the free/free matrix. vals[nnz] is overwritten with K_ff.
+ * Optional: when u and f_int are both non-NULL, f_int[3*n_nodes] is overwritten
+ * with the internal force sum_e Ke ue over the full DOF set (fem/stress.cpp:
+ * 125-145) in the same pass. coords: n_nodes*3; E, nu: per model element. */
+cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                               const double* E, const double* nu, double* vals,
+                               const double* u, double* f_int);
+
+/* ---- K5: matrix-free internal force f_int = sum_e Ke(x,E) ue ----------------
+ * fem::internal_force (fem/stress.cpp:125-145) without forming Ke: per Gauss point
+ * B^T D B ue. f_int[3*n_nodes] is overwritten. This is also the forward
+ * r = K(x, E) u of the adjoint pair below. */
+cxg_status cxg_internal_force(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                              const double* E, const double* nu, const double* u,
+                              double* f_int);
+
+/* rhs[eq(g)] += f_ext[g] - f_c[g] for every free g, where f_c = K_full u_c is the
+ * internal force of the prescribed displacement field u_c (zero on free DOFs).
+ * Together with cxg_internal_force this forms the reduced rhs of
+ * assemble_linear_static: loads through add_external_load (fem/assembly.cpp:
+ * 200-208) and the eliminated prescribed columns (fem/assembly.cpp:95-96).
+ * f_ext or f_c may be NULL (treated as zero). rhs[n_free] is accumulated into. */
+cxg_status cxg_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
+                           const double* f_ext, const double* f_c, double* rhs);
+
+/* ---- K2: fused J2 return map + consistent tangent + internal force -----------
+ * Replaces assemble_material_tangent (fem/assembly.cpp:385-423) with
+ * element_tangent_force (fem/element.cpp:708-743) and J2PlasticMaterial::evaluate
+ * (src/fem/plasticity.cpp:90-236) at every Gauss point.
+ * mat_id[n_elem]: index into mats (host array of n_mats), -1 = linear elastic
+ * with E/nu. committed, trial: SoA state, 13 planes of n_points (see above);
+ * committed is read, trial is overwritten. vals[nnz] <- K_t, f_int[3*n_nodes]
+ * <- internal force (full DOF set). */
+cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                           const double* E, const double* nu, const int32_t* mat_id,
+                           const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                           const double* committed, double* trial, double* vals,
+                           double* f_int);
+
+/* MaterialPoints::commit (fem/assembly.hpp:84-87): committed <- trial. */
+cxg_status cxg_commit(cxg_context* ctx, int64_t n_points, double* committed,
+                      const double* trial);
+
+/* ---- K6: adjoint (vector-Jacobian product) of r = K(x, E) u ------------------
+ * For a cotangent w: dx[n_nodes*3] = d(w.r)/dx, dE[n_elem] = d(w.r)/dE_e,
+ * du[n_nodes*3] = K^T w = K w. Outputs are overwritten. The reference's closest
+ * counterpart is the semi-analytic central difference of the reduced operator in
+ * solve_sensitivity (src/numerics/sensitivity.cpp:52-108), which covers
+ * coordinate variables only. */
+cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                   const double* E, const double* nu, const double* u, const double* w,
+                   double* dx, double* dE, double* du);
+
+/* ---- kernel timing ----------------------------------------------------------
+ * When enabled, every kernel launch of the library is bracketed by HIP events on
+ * the context's stream; per kernel name the library accumulates launches, total
+ * milliseconds and the number of elements processed. No counterpart in the
+ * reference (it has no timers, SURVEY: aux subsystems). Timing adds two event
+ * records per launch and a synchronization per call; leave it off when timing
+ * whole calls. */
+cxg_status cxg_profile_enable(cxg_context* ctx, int enable);
+cxg_status cxg_profile_reset(cxg_context* ctx);
+int32_t cxg_profile_count(const cxg_context* ctx);
+/* Entry i: name (owned by the context), launches, total ms, elements processed. */
+cxg_status cxg_profile_get(const cxg_context* ctx, int32_t i, const char** name,
+                           int64_t* launches, double* total_ms, int64_t* elements);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

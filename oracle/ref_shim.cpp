@@ -1,0 +1,290 @@
+// TEST INFRASTRUCTURE ONLY — C-ABI glue from the oracle's array model
+// (orc_model.h) to the reference's own calculixpp_core, compiled together with the
+// reference sources by oracle/Makefile into oracle/_ref/libcxpp_ref.so.
+//
+// It is used to pin the CPU restatement (cxg_oracle.cpp), to write the golden
+// fixtures under tests/golden/, and as the "reference" CPU baseline in bench.py.
+// Only this glue is ours; every number it returns is computed by the reference
+// functions it calls:
+//   fem::element_stiffness           (src/fem/element.cpp:614)
+//   fem::element_tangent_force       (src/fem/element.cpp:708)
+//   fem::J2PlasticMaterial::evaluate (src/fem/plasticity.cpp:90)
+//   fem::assemble_linear_static      (src/fem/assembly.cpp:225)
+//   fem::make_material_points        (src/fem/assembly.cpp:357)
+//   fem::assemble_material_tangent   (src/fem/assembly.cpp:385)
+//   fem::internal_force              (src/fem/stress.cpp:125)
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "calculixpp/core/model.hpp"
+#include "calculixpp/fem/assembly.hpp"
+#include "calculixpp/fem/element.hpp"
+#include "calculixpp/fem/material_model.hpp"
+#include "calculixpp/fem/stress.hpp"
+#include "orc_model.h"
+
+using namespace cxpp;
+
+namespace {
+
+void set_err(char* err, int len, const char* msg) {
+  if (err && len > 0) {
+    std::strncpy(err, msg, size_t(len) - 1);
+    err[len - 1] = 0;
+  }
+}
+
+Plastic to_plastic(const orc_j2& p) {
+  Plastic pl;
+  pl.hardening = p.hardening == ORC_HARD_KIN        ? Plastic::Hardening::Kinematic
+                 : p.hardening == ORC_HARD_COMBINED ? Plastic::Hardening::Combined
+                                                    : Plastic::Hardening::Isotropic;
+  pl.eqplastic.assign(p.eqplastic, p.eqplastic + p.n_table);
+  pl.yield.assign(p.yield, p.yield + p.n_table);
+  pl.kinematic_modulus = p.kinematic_modulus;
+  return pl;
+}
+
+void load_state(const double* st, fem::MaterialState& s) {
+  for (int i = 0; i < 6; ++i) {
+    s.committed_plastic_strain[size_t(i)] = st[i];
+    s.committed_back_stress[size_t(i)] = st[7 + i];
+    s.plastic_strain[size_t(i)] = st[13 + i];
+    s.back_stress[size_t(i)] = st[20 + i];
+  }
+  s.committed_eqplastic = st[6];
+  s.eqplastic = st[19];
+}
+
+void store_state(const fem::MaterialState& s, double* st) {
+  for (int i = 0; i < 6; ++i) {
+    st[i] = s.committed_plastic_strain[size_t(i)];
+    st[7 + i] = s.committed_back_stress[size_t(i)];
+    st[13 + i] = s.plastic_strain[size_t(i)];
+    st[20 + i] = s.back_stress[size_t(i)];
+  }
+  st[6] = s.committed_eqplastic;
+  st[19] = s.eqplastic;
+}
+
+std::vector<Vec3> as_vec3(const double* p, size_t n) {
+  std::vector<Vec3> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+  return v;
+}
+
+// Build a reference Model: node ids = index+1, element ids = index+1, one
+// material + elset per distinct (E, nu, J2 material) triple.
+std::unique_ptr<Model> build_model(const orc_model& m) {
+  auto M = std::make_unique<Model>();
+  for (int32_t i = 0; i < m.n_nodes; ++i)
+    M->mesh.add_node(i + 1, {m.coords[3 * i], m.coords[3 * i + 1], m.coords[3 * i + 2]});
+  std::map<std::tuple<double, double, int32_t>, std::vector<Index>> groups;
+  for (int32_t e = 0; e < m.n_elem; ++e) {
+    std::vector<Index> ids;
+    for (int64_t k = m.eptr[e]; k < m.eptr[e + 1]; ++k) ids.push_back(m.conn[k] + 1);
+    M->mesh.add_element(e + 1, static_cast<ElementType>(m.etype[e]), std::move(ids));
+    const int32_t jid = m.j2_id ? m.j2_id[e] : -1;
+    groups[{m.E[e], m.nu[e], jid}].push_back(e + 1);
+    if (m.active && m.active[e] == 0) M->deactivated_elements.push_back(e + 1);
+  }
+  int g = 0;
+  for (auto& [key, ids] : groups) {
+    const std::string name = "M" + std::to_string(g++);
+    Material mat;
+    mat.name = name;
+    mat.elastic = ElasticIso{std::get<0>(key), std::get<1>(key)};
+    if (std::get<2>(key) >= 0) mat.plastic = to_plastic(m.j2[std::get<2>(key)]);
+    M->materials[name] = mat;
+    M->mesh.add_elset(name, ids);
+    M->sections.push_back(SolidSection{name, name});
+  }
+  for (int32_t i = 0; i < m.n_spc; ++i) {
+    Spc s;
+    s.node_id = m.spc_node[i] + 1;
+    s.comp = m.spc_comp[i];
+    s.value = m.spc_value[i];
+    M->spcs.push_back(s);
+  }
+  for (int32_t i = 0; i < m.n_cload; ++i) {
+    Cload c;
+    c.node_id = m.cload_node[i] + 1;
+    c.comp = m.cload_comp[i];
+    c.value = m.cload_value[i];
+    M->cloads.push_back(c);
+  }
+  return M;
+}
+
+}  // namespace
+
+struct ref_model {
+  std::unique_ptr<Model> model;
+};
+
+struct ref_system {
+  fem::LinearSystem sys;
+};
+
+extern "C" {
+
+int ref_element_stiffness(int type, const double* coords, double E, double nu, double* Ke,
+                          char* err, int errlen) {
+  try {
+    const int n = nodes_per_element(static_cast<ElementType>(type));
+    const std::vector<Vec3> x = as_vec3(coords, size_t(n));
+    const std::vector<Real> K = fem::element_stiffness(static_cast<ElementType>(type), x,
+                                                      ElasticIso{E, nu});
+    std::memcpy(Ke, K.data(), K.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int ref_j2_evaluate(const orc_j2* p, const double* strain, double* state, double* stress,
+                    double* D, char* err, int errlen) {
+  try {
+    fem::J2PlasticMaterial mat(ElasticIso{p->E, p->nu}, to_plastic(*p));
+    fem::MaterialState s;
+    load_state(state, s);
+    Voigt6 eps{};
+    for (int i = 0; i < 6; ++i) eps[size_t(i)] = strain[i];
+    const fem::MaterialResponse r = mat.evaluate(eps, s);
+    store_state(s, state);
+    for (int i = 0; i < 6; ++i) {
+      stress[i] = r.stress[size_t(i)];
+      for (int j = 0; j < 6; ++j) D[6 * i + j] = r.tangent[size_t(i)][size_t(j)];
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int ref_element_tangent_force(int type, const double* coords, const double* ue, double E,
+                              double nu, const orc_j2* p, double* state, double* Ke,
+                              double* fe, char* err, int errlen) {
+  try {
+    const ElementType t = static_cast<ElementType>(type);
+    const int n = nodes_per_element(t);
+    const size_t ng = fem::gauss_rule(t).size();
+    const std::vector<Vec3> x = as_vec3(coords, size_t(n)), u = as_vec3(ue, size_t(n));
+    std::unique_ptr<fem::MaterialModel> mat;
+    if (p)
+      mat = std::make_unique<fem::J2PlasticMaterial>(ElasticIso{E, nu}, to_plastic(*p));
+    else
+      mat = std::make_unique<fem::ElasticIsoMaterial>(ElasticIso{E, nu});
+    std::vector<fem::MaterialState> st(ng);
+    if (state)
+      for (size_t q = 0; q < ng; ++q) load_state(state + q * ORC_STATE, st[q]);
+    const fem::ElementResponse r = fem::element_tangent_force(t, x, u, *mat, st);
+    if (state)
+      for (size_t q = 0; q < ng; ++q) store_state(st[q], state + q * ORC_STATE);
+    std::memcpy(Ke, r.Ke.data(), r.Ke.size() * sizeof(double));
+    std::memcpy(fe, r.fe.data(), r.fe.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int ref_model_create(const orc_model* m, ref_model** out, char* err, int errlen) {
+  try {
+    auto* h = new ref_model();
+    h->model = build_model(*m);
+    *out = h;
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+void ref_model_free(ref_model* h) { delete h; }
+
+int ref_assemble_linear_static(const ref_model* h, ref_system** out, char* err, int errlen) {
+  try {
+    auto* s = new ref_system();
+    s->sys = fem::assemble_linear_static(*h->model);
+    *out = s;
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// state: sum_e ngp(e)*26 doubles, element-major (orc_model.h layout); committed
+// slots are loaded, trial slots are written back. f_int: 3*n_nodes.
+int ref_assemble_material_tangent(const ref_model* h, const double* u, double* state,
+                                  double* f_int, ref_system** out, char* err, int errlen) {
+  try {
+    const Model& M = *h->model;
+    fem::MaterialPoints mp = fem::make_material_points(M);
+    size_t off = 0;
+    for (auto& es : mp.state)
+      for (auto& s : es) {
+        load_state(state + off, s);
+        off += ORC_STATE;
+      }
+    const std::vector<Vec3> uu = as_vec3(u, M.mesh.num_nodes());
+    std::vector<Real> fi;
+    auto* s = new ref_system();
+    s->sys = fem::assemble_material_tangent(M, uu, mp, fi);
+    off = 0;
+    for (auto& es : mp.state)
+      for (auto& st : es) {
+        store_state(st, state + off);
+        off += ORC_STATE;
+      }
+    std::memcpy(f_int, fi.data(), fi.size() * sizeof(double));
+    *out = s;
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int ref_internal_force(const ref_model* h, const double* u, double* f_int, char* err,
+                       int errlen) {
+  try {
+    const std::vector<Vec3> uu = as_vec3(u, h->model->mesh.num_nodes());
+    const std::vector<Real> fi = fem::internal_force(*h->model, uu);
+    std::memcpy(f_int, fi.data(), fi.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int32_t ref_system_n_free(const ref_system* s) { return s->sys.n_free; }
+int64_t ref_system_nnz(const ref_system* s) { return int64_t(s->sys.vals.size()); }
+
+void ref_system_coo(const ref_system* s, int32_t* rows, int32_t* cols, double* vals) {
+  std::memcpy(rows, s->sys.rows.data(), s->sys.rows.size() * sizeof(int32_t));
+  std::memcpy(cols, s->sys.cols.data(), s->sys.cols.size() * sizeof(int32_t));
+  std::memcpy(vals, s->sys.vals.data(), s->sys.vals.size() * sizeof(double));
+}
+
+void ref_system_rhs(const ref_system* s, double* rhs) {
+  std::memcpy(rhs, s->sys.rhs.data(), s->sys.rhs.size() * sizeof(double));
+}
+
+void ref_system_dof_eq(const ref_system* s, int32_t* dof_eq) {
+  std::memcpy(dof_eq, s->sys.dof_eq.data(), s->sys.dof_eq.size() * sizeof(int32_t));
+}
+
+void ref_system_free(ref_system* s) { delete s; }
+
+}  // extern "C"

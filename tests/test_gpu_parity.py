@@ -1,0 +1,178 @@
+"""Parity of the HIP path (libcxg.so through calculixpp_amd) against the oracle.
+
+The oracle's CPU restatement is itself pinned bit-for-bit to the reference core
+(tests/test_oracle.py); here the GPU results are compared with it on identical
+seeded inputs:
+  CSR row_ptr / col_idx   bit-exact
+  K values, rhs, f_int    rel-L2 <= 1e-12
+  J2 trial history        rel-L2 <= 1e-10
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL_K = 1e-12
+REL_STATE = 1e-10
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from calculixpp_amd import fem
+    return fem
+
+
+def _families():
+    from calculixpp_amd import _lib
+    return [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6]
+
+
+def check_linear(fem, port, model):
+    exp = port.assemble_linear_static(model)
+    rp, ci, vals = exp.csr()
+    sys_ = fem.assemble_linear_static(model)
+    assert sys_.n_free == exp.n_free
+    assert np.array_equal(sys_.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(sys_.col_idx.cpu().numpy(), ci)
+    got = sys_.vals.cpu().numpy()
+    assert rel_l2(got, vals) <= REL_K, rel_l2(got, vals)
+    assert rel_l2(sys_.rhs.cpu().numpy(), exp.rhs) <= REL_K
+    return sys_
+
+
+def test_config1_cantilever(gpu, port):
+    from calculixpp_amd import meshgen
+    m = meshgen.cantilever()
+    s = check_linear(gpu, port, m)
+    assert s.n_free == 1500 and s.vals.numel() == 88218
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_jittered_block_each_family(gpu, port, fam_i):
+    from calculixpp_amd import meshgen
+    fam = _families()[fam_i]
+    check_linear(gpu, port, meshgen.block(3, fam))
+
+
+def test_nonuniform_block_prescribed_and_loads(gpu, port):
+    """Ragged box, nonzero prescribed displacements (rhs -= K_fc u_c) and CLOADs."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, conn = meshgen.hex_grid(4, 3, 2, (2.0, 1.5, 1.0), _lib.C3D20)
+    coords = meshgen.jitter(coords, 0.5, 0.1, 11, (2.0, 1.5, 1.0))
+    rng = np.random.default_rng(12)
+    n = len(coords)
+    x0 = np.nonzero(coords[:, 0] < 1e-12)[0]
+    spc_node = np.concatenate([x0, x0[:5]])
+    spc_comp = np.concatenate([np.full(len(x0), 1), np.full(5, 3)])
+    spc_value = rng.uniform(-1e-3, 1e-3, len(spc_node))
+    cl = rng.choice(n, 7, replace=False)
+    m = Model(coords, [(_lib.C3D20, conn)], rng.uniform(1e5, 2e5, conn.shape[0]), 0.28,
+              spc_node=spc_node, spc_comp=spc_comp, spc_value=spc_value, cload_node=cl,
+              cload_comp=rng.integers(1, 4, 7), cload_value=rng.normal(size=7))
+    check_linear(gpu, port, m)
+
+
+def test_mixed_families_one_mesh(gpu, port):
+    """C3D8 + C3D6 sharing nodes: colours and the pattern span both blocks."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, hexes = meshgen.hex_grid(4, 3, 2, (1.0, 0.75, 0.5), _lib.C3D8)
+    coords = meshgen.jitter(coords, 0.25, 0.1, 13, (1.0, 0.75, 0.5))
+    wed = np.concatenate([hexes[12:][:, [0, 1, 2, 4, 5, 6]], hexes[12:][:, [0, 2, 3, 4, 6, 7]]])
+    m = Model(coords, [(_lib.C3D8, hexes[:12]), (_lib.C3D6, wed)],
+              np.linspace(1e5, 2e5, 12 + len(wed)), 0.3)
+    check_linear(gpu, port, m)
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_internal_force_fused_and_matrix_free(gpu, port, fam_i):
+    import torch
+    from calculixpp_amd import meshgen
+    m = meshgen.block(3, _families()[fam_i])
+    u = meshgen.smooth_field(m.coords, 2)
+    exp = port.internal_force(m, u)
+    got = gpu.internal_force(m, u).cpu().numpy()
+    assert rel_l2(got, exp) <= REL_K
+    a = gpu.Assembler(m)
+    vals = a.new_values()
+    f = torch.empty(3 * m.n_nodes, dtype=torch.float64, device=a.ctx.device)
+    a.assemble(vals, torch.from_numpy(u).to(a.ctx.device), f)
+    assert rel_l2(f.cpu().numpy(), exp) <= REL_K
+
+
+def test_j2_tangent_force_state(gpu, port):
+    from calculixpp_amd import meshgen
+    from oracle.oracle import aos_state
+    m = meshgen.j2_block(6, 5, 4, (1.0, 0.8, 0.6))
+    hist = meshgen.seeded_history(m.n_points, 3)
+    u = meshgen.smooth_field(m.coords, 7, 1.5e-4)  # about 40% of the points yield
+    sys_e, f_e, st_e = port.assemble_material_tangent(m, u, aos_state(hist))
+    mp = gpu.MaterialPoints(m.n_points, committed=hist)
+    sys_g, f_g = gpu.assemble_material_tangent(m, u, mp)
+    rp, ci, vals = sys_e.csr()
+    assert np.array_equal(sys_g.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(sys_g.col_idx.cpu().numpy(), ci)
+    assert rel_l2(sys_g.vals.cpu().numpy(), vals) <= REL_K
+    assert rel_l2(f_g.cpu().numpy(), f_e) <= REL_K
+    trial = mp.trial.cpu().numpy()
+    yielded = st_e[:, 19] > st_e[:, 6]
+    assert 0.05 < yielded.mean() < 1.0
+    for c in range(13):
+        assert rel_l2(trial[c], st_e[:, 13 + c]) <= REL_STATE, c
+    mp.commit()
+    assert np.array_equal(mp.committed.cpu().numpy(), trial)
+
+
+def test_vjp_matches_oracle(gpu, port):
+    import ctypes as C
+    import torch
+    from calculixpp_amd import meshgen
+    from calculixpp_amd.fem import DeviceMesh, _ptr
+    m, rho = meshgen.adjoint_block(6, 4, 3, (2.0, 1.0, 0.75))
+    u = meshgen.smooth_field(m.coords, 7)
+    w = np.random.default_rng(8).normal(size=(m.n_nodes, 3))
+    dx_e, dE_e, du_e, r_e = port.vjp(m, u, w)
+    mesh = DeviceMesh(m)
+    dev = mesh.ctx.device
+    tu, tw = torch.from_numpy(u).to(dev), torch.from_numpy(w).to(dev)
+    dx = torch.empty_like(tu)
+    du = torch.empty_like(tu)
+    dE = torch.empty(m.n_elem, dtype=torch.float64, device=dev)
+    mesh.ctx.check(mesh.ctx.lib.cxg_vjp(mesh.ctx.h, mesh.h, _ptr(mesh.coords), _ptr(mesh.E),
+                                        _ptr(mesh.nu), _ptr(tu), _ptr(tw), _ptr(dx), _ptr(dE),
+                                        _ptr(du)))
+    assert rel_l2(dx.cpu().numpy(), dx_e) <= 1e-11
+    assert rel_l2(dE.cpu().numpy(), dE_e) <= 1e-11
+    assert rel_l2(du.cpu().numpy(), du_e) <= REL_K
+
+
+def test_inverted_element_reports_reference_message(gpu):
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, conn = meshgen.hex_grid(2, 1, 1, (1.0, 0.5, 0.5), _lib.C3D8)
+    bad = conn.copy()
+    bad[1] = bad[1][[4, 5, 6, 7, 0, 1, 2, 3]]  # upside-down element: det J < 0
+    m = Model(coords, [(_lib.C3D8, bad)], 210000.0, 0.3)
+    with pytest.raises(RuntimeError, match=r"non-positive element Jacobian \(bad connectivity\?\)"):
+        gpu.assemble_linear_static(m)
+
+
+def test_deterministic_bitwise(gpu):
+    from calculixpp_amd import meshgen
+    m = meshgen.block(4, _families()[1])
+    a = gpu.Assembler(m)
+    v1, v2 = a.new_values(), a.new_values()
+    a.assemble(v1)
+    a.assemble(v2)
+    assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
