@@ -69,6 +69,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=16, help="edge of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
+    ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -100,14 +102,22 @@ def main() -> int:
     model = meshgen.block(args.n, family)
     u_host = meshgen.smooth_field(model.coords, 2)
     t1 = time.perf_counter()
-    asm = fem.Assembler(model)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    print(f"[bench] mesh {t1 - t0:.1f}s, upload+colour+pattern {t2 - t1:.1f}s: "
-          f"{model.n_elem} elements, {model.n_nodes} nodes, nnz {asm.pattern.nnz}, "
-          f"{asm.mesh.n_colors} colours", file=sys.stderr, flush=True)
-    vals = asm.new_values()
     u = torch.from_numpy(u_host).to(dev) if residual else None
+    from calculixpp_amd import autotune
+    if args.variant and args.patch:
+        choice = {"asm_variant": args.variant, "patch_size": args.patch, "trials": []}
+    else:
+        choice = autotune.tune_linear(model, u, cache_path=None, log=lambda m: print(
+            m, file=sys.stderr, flush=True))
+    t2 = time.perf_counter()
+    asm = autotune.make_assembler(model, choice["asm_variant"], choice["patch_size"])
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    print(f"[bench] mesh {t1 - t0:.1f}s, tune {t2 - t1:.1f}s, upload+colour+pattern "
+          f"{t3 - t2:.1f}s: {model.n_elem} elements, {model.n_nodes} nodes, nnz "
+          f"{asm.pattern.nnz}, {asm.mesh.n_colors} colours, variant {choice['asm_variant']}, "
+          f"patch {choice['patch_size']}", file=sys.stderr, flush=True)
+    vals = asm.new_values()
     f_int = torch.empty(3 * model.n_nodes, dtype=torch.float64, device=dev) if residual else None
 
     for _ in range(args.warmup):
@@ -162,7 +172,9 @@ def main() -> int:
                                f"{' + internal force' if residual else ''}",
                    "elements": model.n_elem, "nodes": model.n_nodes,
                    "dof": 3 * model.n_nodes, "nnz": asm.pattern.nnz,
-                   "colours": asm.mesh.n_colors},
+                   "colours": asm.mesh.n_colors, "asm_variant": choice["asm_variant"],
+                   "patch_size": choice["patch_size"]},
+        "autotune": choice["trials"],
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

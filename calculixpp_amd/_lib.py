@@ -58,6 +58,7 @@ _SIGS = {
     "cxg_destroy": (None, [_P]),
     "cxg_set_stream": (C.c_int, [_P, _P]),
     "cxg_last_error": (C.c_char_p, [_P]),
+    "cxg_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "cxg_mesh_create": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(Block), C.POINTER(_P)]),
     "cxg_mesh_destroy": (None, [_P]),
     "cxg_mesh_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),

@@ -1,0 +1,100 @@
+"""Autotuner of the assembly launch configuration.
+
+Two knobs decide how the stiffness assembly maps onto the GPU:
+  asm_variant  1 = per-pair scatter (one element per wave, all Gauss-point
+                   gradients in LDS), 2/3 = M-identity kernel with row-coalesced
+                   scatter at 4/3 waves per SIMD;
+  patch_size   consecutive elements one workgroup walks (patch colouring keeps
+               patches of one launch node-disjoint; longer patches reuse the CSR
+               rows of shared nodes while they are still in cache, shorter ones
+               give more parallelism).
+The tuner times each candidate on the actual mesh (HIP events on the library's
+stream) and caches the winner in a JSON file keyed by element family, element
+count and nonzero count, so later runs on the same workload reuse it.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Iterable, Optional
+
+import torch
+
+from . import fem
+from .model import Model
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "calculixpp_amd", "autotune.json")
+CANDIDATES = [(4, 8), (2, 8), (2, 1), (3, 8), (1, 8)]
+
+
+def _key(model: Model) -> str:
+    fams = "+".join(str(t) for t, _ in model.blocks)
+    return f"linear:{fams}:{model.n_elem}:{model.n_nodes}"
+
+
+def load_cache(path: str = DEFAULT_CACHE) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(cache: dict, path: str = DEFAULT_CACHE) -> None:
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(cache, f, indent=1)
+    os.replace(tmp, path)
+
+
+def make_assembler(model: Model, variant: int, patch: int,
+                   ctx: Optional[fem.Context] = None) -> fem.Assembler:
+    ctx = ctx or fem.context()
+    ctx.set_option("asm_variant", variant)
+    ctx.set_option("patch_size", patch)
+    return fem.Assembler(model, ctx)
+
+
+def time_assembler(asm: fem.Assembler, vals, u=None, f_int=None, steps: int = 3) -> float:
+    """Mean milliseconds per assembly (after one warm-up)."""
+    asm.assemble(vals, u, f_int)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        asm.assemble(vals, u, f_int)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def tune_linear(model: Model, u=None, candidates: Iterable = CANDIDATES, steps: int = 3,
+                cache_path: Optional[str] = DEFAULT_CACHE, force: bool = False, log=None) -> dict:
+    """Pick (asm_variant, patch_size) for this model; returns
+    {"asm_variant", "patch_size", "ms", "trials": [...]}."""
+    key = _key(model)
+    cache = load_cache(cache_path) if cache_path else {}
+    if not force and key in cache:
+        return cache[key]
+    ctx = fem.context()
+    trials = []
+    best = None
+    for variant, patch in candidates:
+        asm = make_assembler(model, variant, patch, ctx)
+        vals = asm.new_values()
+        f_int = None
+        if u is not None:
+            f_int = torch.empty(3 * model.n_nodes, dtype=torch.float64, device=ctx.device)
+        ms = time_assembler(asm, vals, u, f_int, steps)
+        trials.append({"asm_variant": variant, "patch_size": patch, "ms": ms})
+        if log:
+            log(f"[autotune] variant {variant} patch {patch}: {ms:.3f} ms")
+        if best is None or ms < best["ms"]:
+            best = {"asm_variant": variant, "patch_size": patch, "ms": ms}
+        del asm, vals, f_int
+        torch.cuda.empty_cache()
+    best["trials"] = trials
+    if cache_path:
+        cache[key] = best
+        save_cache(cache, cache_path)
+    return best

@@ -78,29 +78,34 @@ void check_flags(cxg_context* ctx) {
 
 namespace {
 
-// Greedy first-fit colouring in ascending element order: element e takes the
-// smallest colour not used by an earlier element sharing a node with it. On a
-// structured hex grid this yields the 8-colouring (i%2, j%2, k%2).
-std::vector<int32_t> colour_elements(int64_t n_elem, const std::vector<int64_t>& eptr,
-                                     const std::vector<int32_t>& conn,
-                                     const std::vector<int64_t>& inc_ptr,
-                                     const std::vector<int32_t>& inc_elem, int32_t& n_colors) {
+// Greedy first-fit colouring of patches in ascending order: patch p takes the
+// smallest colour not used by an earlier patch sharing a node with it. With one
+// element per patch on a structured hex grid this yields the 8-colouring
+// (i%2, j%2, k%2); runs of consecutive elements along x colour the same way.
+std::vector<int32_t> colour_patches(int64_t n_patches, const std::vector<int64_t>& patch_first,
+                                    const std::vector<int32_t>& elem_patch,
+                                    const std::vector<int64_t>& eptr,
+                                    const std::vector<int32_t>& conn,
+                                    const std::vector<int64_t>& inc_ptr,
+                                    const std::vector<int32_t>& inc_elem, int32_t& n_colors) {
   constexpr int kMaxColours = 64;
-  std::vector<int32_t> col(size_t(n_elem), -1);
+  std::vector<int32_t> col(size_t(n_patches), -1);
   n_colors = 0;
-  for (int64_t e = 0; e < n_elem; ++e) {
+  for (int64_t p = 0; p < n_patches; ++p) {
     uint64_t used = 0;
-    for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
-      const int32_t nd = conn[size_t(k)];
-      for (int64_t i = inc_ptr[size_t(nd)]; i < inc_ptr[size_t(nd) + 1]; ++i) {
-        const int32_t c = col[size_t(inc_elem[size_t(i)])];
-        if (c >= 0) used |= (1ull << c);
+    for (int64_t e = patch_first[size_t(p)]; e < patch_first[size_t(p) + 1]; ++e)
+      for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
+        const int32_t nd = conn[size_t(k)];
+        for (int64_t i = inc_ptr[size_t(nd)]; i < inc_ptr[size_t(nd) + 1]; ++i) {
+          const int32_t q = elem_patch[size_t(inc_elem[size_t(i)])];
+          const int32_t c = col[size_t(q)];
+          if (c >= 0 && q != p) used |= (1ull << c);
+        }
       }
-    }
     int c = 0;
     while (c < kMaxColours && (used >> c) & 1ull) ++c;
-    if (c == kMaxColours) throw Error(CXG_ERR_UNSUPPORTED, "element colouring needs more than 64 colours");
-    col[size_t(e)] = c;
+    if (c == kMaxColours) throw Error(CXG_ERR_UNSUPPORTED, "patch colouring needs more than 64 colours");
+    col[size_t(p)] = c;
     n_colors = std::max(n_colors, c + 1);
   }
   return col;
@@ -176,19 +181,43 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
     inc_ptr.swap(np);
     inc_elem.resize(size_t(w));
   }
+  // patches: up to patch_size consecutive elements of one block
+  const int64_t P = std::max(1, ctx->patch_size);
+  std::vector<int64_t> patch_first;
+  std::vector<int32_t> elem_patch(static_cast<size_t>(n_elem));
+  for (const cxg_block_dev& D : M->blocks)
+    for (int64_t e = 0; e < D.n_elem; e += P) {
+      const int64_t end = std::min(D.n_elem, e + P);
+      for (int64_t k = e; k < end; ++k) elem_patch[size_t(D.elem_offset + k)] = int32_t(patch_first.size());
+      patch_first.push_back(D.elem_offset + e);
+    }
+  const int64_t n_patches = int64_t(patch_first.size());
+  patch_first.push_back(n_elem);
   int32_t ncol = 0;
-  const std::vector<int32_t> colour = colour_elements(n_elem, eptr, conn, inc_ptr, inc_elem, ncol);
+  const std::vector<int32_t> colour =
+      colour_patches(n_patches, patch_first, elem_patch, eptr, conn, inc_ptr, inc_elem, ncol);
   M->n_colors = ncol;
   for (cxg_block_dev& D : M->blocks) {
-    std::vector<std::vector<int32_t>> lists(static_cast<size_t>(ncol));
-    for (int64_t e = 0; e < D.n_elem; ++e) lists[size_t(colour[size_t(D.elem_offset + e)])].push_back(int32_t(e));
+    std::vector<std::vector<int32_t>> lists(static_cast<size_t>(ncol)), ptrs(static_cast<size_t>(ncol));
+    for (auto& v : ptrs) v.push_back(0);
+    for (int64_t e = 0; e < D.n_elem; e += P) {
+      const int32_t p = elem_patch[size_t(D.elem_offset + e)];
+      const size_t c = size_t(colour[size_t(p)]);
+      for (int64_t k = e; k < std::min(D.n_elem, e + P); ++k) lists[c].push_back(int32_t(k));
+      ptrs[c].push_back(int32_t(lists[c].size()));
+    }
     for (int c = 0; c < ncol; ++c) {
-      DevBuf<int32_t> buf(lists[size_t(c)].size());
-      if (!lists[size_t(c)].empty())
+      DevBuf<int32_t> buf(lists[size_t(c)].size()), pb(ptrs[size_t(c)].size());
+      if (!lists[size_t(c)].empty()) {
         CXG_HIP(hipMemcpyAsync(buf.p, lists[size_t(c)].data(), lists[size_t(c)].size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, s));
+        CXG_HIP(hipMemcpyAsync(pb.p, ptrs[size_t(c)].data(), ptrs[size_t(c)].size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, s));
+      }
       D.color_count.push_back(int64_t(lists[size_t(c)].size()));
+      D.patch_count.push_back(int64_t(ptrs[size_t(c)].size()) - 1);
       D.color.push_back(std::move(buf));
+      D.patch_ptr.push_back(std::move(pb));
     }
     CXG_HIP(hipStreamSynchronize(s));  // host lists go out of scope
   }
@@ -289,6 +318,20 @@ cxg_status cxg_set_stream(cxg_context* ctx, void* stream) {
 }
 
 const char* cxg_last_error(const cxg_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
+  if (!ctx || !key) return CXG_ERR_ARG;
+  const std::string k(key);
+  if (k == "patch_size" && value >= 1 && value <= 4096) {
+    ctx->patch_size = int(value);
+    return CXG_OK;
+  }
+  if (k == "asm_variant" && value >= 1 && value <= 4) {
+    ctx->asm_variant = int(value);
+    return CXG_OK;
+  }
+  return cxg::fail(ctx, CXG_ERR_ARG, "unknown option or value");
+}
 
 cxg_status cxg_profile_enable(cxg_context* ctx, int enable) {
   return cxg::guarded(ctx, [&] {

@@ -265,7 +265,8 @@ struct AsmLds {
 
 template <int NN, int NGP, bool UPPER>
 __global__ __launch_bounds__(kWave) void k_assemble_iso(
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
+    const int32_t* __restrict__ pptr, int64_t elem_offset,
     const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
     const double* __restrict__ nu, const int32_t* __restrict__ dof_eq,
@@ -274,7 +275,9 @@ __global__ __launch_bounds__(kWave) void k_assemble_iso(
     int* __restrict__ flag) {
   __shared__ AsmLds<NN, NGP, UPPER> L;
   const int lane = threadIdx.x;
-  const int64_t le = elist[blockIdx.x];
+  const int32_t it_end = pptr[blockIdx.x + 1];
+  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
+  const int64_t le = elist[it];
   const int32_t* ce = conn + le * NN;
   stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
   stage_dofs<NN>(L.D, ce, dof_eq, row_ptr, lane);
@@ -323,6 +326,507 @@ __global__ __launch_bounds__(kWave) void k_assemble_iso(
     }
     scatter_pair<NN>(L.D, se, a, b, K, UPPER && a != b, vals);
   }
+  __syncthreads();  // LDS is reused by the next element of the patch
+  }
+}
+
+// ---- K1+K4(+K5), variant 2 ------------------------------------------------------
+// For isotropic D the node-pair block is
+//   K_ab = lam M_ab + mu M_ab^T + mu tr(M_ab) I,   M_ab = sum_q h_a^q (h_b^q)^T,
+// with h_k^q = sqrt(det J_q w_q) g_k^q. Each lane accumulates the 3x3 M of up to PPL
+// upper node pairs (a-major, so consecutive pairs of a lane share node a): 9 FMA
+// per pair and Gauss point. Gauss points are processed QC at a time so only the h
+// of QC points sits in LDS. The scatter is row-coalesced: for each element node a
+// the blocks K_ab of all b are staged in LDS and lane (b, j) adds column (b, j) of
+// the three rows of a, so one wave instruction updates one CSR row segment.
+template <int NN, int NGP, int QC>
+struct Asm2Lds {
+  double x[NN * 3];
+  double u[NN * 3];
+  double Ai[NGP * 9];
+  double rs[NGP];        // sqrt(det J * w)
+  double h[QC * NN * 3]; // h of the current Gauss-point chunk, [qq][k][i]
+  double S[QC * 6];      // D eps(h, u) of the chunk (residual)
+  double kb[NN * 9];     // staged blocks K_ab of the current row node a, [b][i][j]
+  int64_t row[NN * 3];
+  int32_t col[NN * 3];
+  int32_t slot[NN * NN];
+  int32_t node[NN];
+};
+
+template <int NN, int NGP, int QC, int WAVES>
+__global__ __launch_bounds__(kWave, WAVES) void k_assemble_iso2(
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
+    const int32_t* __restrict__ pptr, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu,
+    const int32_t* __restrict__ dof_eq, const int64_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ slot, double* __restrict__ vals, const double* __restrict__ u,
+    double* __restrict__ f_int, int* __restrict__ flag) {
+  constexpr int NP = NN * (NN + 1) / 2;
+  constexpr int PPL = (NP + kWave - 1) / kWave;
+  __shared__ Asm2Lds<NN, NGP, QC> L;
+  const int lane = threadIdx.x;
+  int pa[PPL], pb[PPL];
+#pragma unroll
+  for (int t = 0; t < PPL; ++t) {
+    const int p = lane * PPL + t;
+    if (p < NP) {
+      decode_pair<NN, true>(p, pa[t], pb[t]);
+    } else {
+      pa[t] = -1;
+      pb[t] = -1;
+    }
+  }
+  const int32_t it_end = pptr[blockIdx.x + 1];
+  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
+    const int64_t le = elist[it];
+    const int32_t* ce = conn + le * NN;
+    if (lane < NN) {
+      const int32_t nd = ce[lane];
+      L.node[lane] = nd;
+      const double* xp = coords + 3 * int64_t(nd);
+      L.x[3 * lane + 0] = xp[0];
+      L.x[3 * lane + 1] = xp[1];
+      L.x[3 * lane + 2] = xp[2];
+      if (f_int) {
+        const double* up = u + 3 * int64_t(nd);
+        L.u[3 * lane + 0] = up[0];
+        L.u[3 * lane + 1] = up[1];
+        L.u[3 * lane + 2] = up[2];
+      }
+    }
+    if (lane < 3 * NN) {
+      const int k = lane / 3, c = lane % 3;
+      const int64_t base = 3 * int64_t(ce[k]);
+      const int32_t eq = dof_eq[base + c];
+      int before = 0;
+      for (int cc = 0; cc < c; ++cc) before += dof_eq[base + cc] >= 0 ? 1 : 0;
+      L.row[lane] = eq >= 0 ? row_ptr[eq] : -1;
+      L.col[lane] = eq >= 0 ? before : -1;
+    }
+    const int32_t* se = slot + le * NN * NN;
+    for (int i = lane; i < NN * NN; i += kWave) L.slot[i] = se[i];
+    __syncthreads();
+    for (int q = lane; q < NGP; q += kWave) {
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      const double* d = dN + q * NN * 3;
+      for (int k = 0; k < NN; ++k) {
+        const double x0 = L.x[3 * k], x1 = L.x[3 * k + 1], x2 = L.x[3 * k + 2];
+        const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+        J[0][0] += x0 * d0; J[0][1] += x0 * d1; J[0][2] += x0 * d2;
+        J[1][0] += x1 * d0; J[1][1] += x1 * d1; J[1][2] += x1 * d2;
+        J[2][0] += x2 * d0; J[2][1] += x2 * d1; J[2][2] += x2 * d2;
+      }
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
+      const double r = 1.0 / det;
+      double* A = L.Ai + 9 * q;
+      A[0] = c00 * r;
+      A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+      A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+      A[3] = c01 * r;
+      A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+      A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+      A[6] = c02 * r;
+      A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+      A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+      L.rs[q] = sqrt(det * wq[q]);
+    }
+    double lam, mu;
+    lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
+    double M[PPL][9];
+#pragma unroll
+    for (int t = 0; t < PPL; ++t)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) M[t][i] = 0.0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    __syncthreads();
+    for (int q0 = 0; q0 < NGP; q0 += QC) {
+      for (int idx = lane; idx < QC * NN; idx += kWave) {
+        const int q = q0 + idx / NN;
+        double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+        if (q < NGP) {
+          const double* A = L.Ai + 9 * q;
+          const double* d = dN + (q * NN + idx % NN) * 3;
+          const double s = L.rs[q];
+          h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
+          h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
+          h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
+        }
+        L.h[3 * idx + 0] = h0;
+        L.h[3 * idx + 1] = h1;
+        L.h[3 * idx + 2] = h2;
+      }
+      __syncthreads();
+      if (f_int) {
+        if (lane < QC) {
+          double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+          const double* hq = L.h + lane * NN * 3;
+          for (int k = 0; k < NN; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) H[i][j] += L.u[3 * k + i] * hq[3 * k + j];
+          double e[6], s[6];
+          voigt_strain(H, e);
+          iso_stress(lam, mu, e, s);
+#pragma unroll
+          for (int i = 0; i < 6; ++i) L.S[6 * lane + i] = s[i];
+        }
+        __syncthreads();
+        if (lane < NN) {
+#pragma unroll
+          for (int qq = 0; qq < QC; ++qq) {
+            const double* g = L.h + (qq * NN + lane) * 3;
+            const double* s = L.S + 6 * qq;
+            f0 += g[0] * s[0] + g[1] * s[3] + g[2] * s[4];
+            f1 += g[1] * s[1] + g[0] * s[3] + g[2] * s[5];
+            f2 += g[2] * s[2] + g[0] * s[4] + g[1] * s[5];
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < PPL; ++t) {
+        if (pa[t] < 0) continue;
+#pragma unroll 1
+        for (int qq = 0; qq < QC; ++qq) {
+          const double* ha = L.h + (qq * NN + pa[t]) * 3;
+          const double* hb = L.h + (qq * NN + pb[t]) * 3;
+          const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+          const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
+          M[t][0] += a0 * b0; M[t][1] += a0 * b1; M[t][2] += a0 * b2;
+          M[t][3] += a1 * b0; M[t][4] += a1 * b1; M[t][5] += a1 * b2;
+          M[t][6] += a2 * b0; M[t][7] += a2 * b1; M[t][8] += a2 * b2;
+        }
+      }
+      __syncthreads();
+    }
+    if (f_int && lane < NN) {
+      double* fp = f_int + 3 * int64_t(L.node[lane]);
+      fp[0] += f0;
+      fp[1] += f1;
+      fp[2] += f2;
+    }
+    // K blocks from M, staged per row node a and added row-coalesced.
+    for (int a = 0; a < NN; ++a) {
+#pragma unroll
+      for (int t = 0; t < PPL; ++t) {
+        const bool lead = pa[t] == a, trail = pb[t] == a && pa[t] != a;
+        if (!lead && !trail) continue;
+        const double* m = M[t];
+        const double tr = mu * (m[0] + m[4] + m[8]);
+        double* dst = L.kb + 9 * (lead ? pb[t] : pa[t]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double kij = lam * m[3 * i + j] + mu * m[3 * j + i] + (i == j ? tr : 0.0);
+            if (lead) dst[3 * i + j] = kij;
+            else dst[3 * j + i] = kij;
+          }
+      }
+      __syncthreads();
+      if (lane < 3 * NN) {
+        const int b = lane / 3, j = lane % 3;
+        const int32_t c = L.col[lane];
+        if (c >= 0) {
+          const int64_t off = int64_t(L.slot[a * NN + b]) + c;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const int64_t r = L.row[3 * a + i];
+            if (r >= 0) vals[r + off] += L.kb[9 * b + 3 * i + j];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---- K1, two-phase variant, phase A: element matrices to an arena ------------------
+// Same M-identity arithmetic as k_assemble_iso2. Each lane forms K_ab once for its
+// (up to PPL, a-major consecutive) upper node pairs and stores the block into the
+// rows of a and its transpose into the rows of b of the element's full row-major Ke,
+// so that phase B reads whole contiguous rows. The whole 3n x 3n region is written by
+// one wave within a short time, which the L2 merges into full lines. No colouring,
+// no read-modify-write.
+constexpr int kStageNodes = 4;  // element-matrix node rows staged in LDS per store turn
+
+template <int NN, int NGP, int QC>
+struct ElemLds {
+  double x[NN * 3];
+  double Ai[NGP * 9];
+  double rs[NGP];
+  double h[QC * NN * 3];
+  double stage[kStageNodes * 3 * 3 * NN];
+};
+
+template <int NN, int NGP, int QC, int WAVES>
+__global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
+    int* __restrict__ flag) {
+  constexpr int NP = NN * (NN + 1) / 2;
+  constexpr int PPL = (NP + kWave - 1) / kWave;
+  __shared__ ElemLds<NN, NGP, QC> L;
+  const int lane = threadIdx.x;
+  const int64_t le = blockIdx.x;
+  int pa[PPL], pb[PPL];
+#pragma unroll
+  for (int t = 0; t < PPL; ++t) {
+    const int p = lane * PPL + t;
+    if (p < NP) {
+      decode_pair<NN, true>(p, pa[t], pb[t]);
+    } else {
+      pa[t] = -1;
+      pb[t] = -1;
+    }
+  }
+  const int32_t* ce = conn + le * NN;
+  if (lane < NN) {
+    const double* xp = coords + 3 * int64_t(ce[lane]);
+    L.x[3 * lane + 0] = xp[0];
+    L.x[3 * lane + 1] = xp[1];
+    L.x[3 * lane + 2] = xp[2];
+  }
+  __syncthreads();
+  for (int q = lane; q < NGP; q += kWave) {
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double* d = dN + q * NN * 3;
+    for (int k = 0; k < NN; ++k) {
+      const double x0 = L.x[3 * k], x1 = L.x[3 * k + 1], x2 = L.x[3 * k + 2];
+      const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+      J[0][0] += x0 * d0; J[0][1] += x0 * d1; J[0][2] += x0 * d2;
+      J[1][0] += x1 * d0; J[1][1] += x1 * d1; J[1][2] += x1 * d2;
+      J[2][0] += x2 * d0; J[2][1] += x2 * d1; J[2][2] += x2 * d2;
+    }
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
+    const double r = 1.0 / det;
+    double* A = L.Ai + 9 * q;
+    A[0] = c00 * r;
+    A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+    A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+    A[3] = c01 * r;
+    A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+    A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+    A[6] = c02 * r;
+    A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+    A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+    L.rs[q] = sqrt(det * wq[q]);
+  }
+  double M[PPL][9];
+#pragma unroll
+  for (int t = 0; t < PPL; ++t)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M[t][i] = 0.0;
+  __syncthreads();
+  for (int q0 = 0; q0 < NGP; q0 += QC) {
+    for (int idx = lane; idx < QC * NN; idx += kWave) {
+      const int q = q0 + idx / NN;
+      double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+      if (q < NGP) {
+        const double* A = L.Ai + 9 * q;
+        const double* d = dN + (q * NN + idx % NN) * 3;
+        const double s = L.rs[q];
+        h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
+        h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
+        h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
+      }
+      L.h[3 * idx + 0] = h0;
+      L.h[3 * idx + 1] = h1;
+      L.h[3 * idx + 2] = h2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      if (pa[t] < 0) continue;
+#pragma unroll 1
+      for (int qq = 0; qq < QC; ++qq) {
+        const double* ha = L.h + (qq * NN + pa[t]) * 3;
+        const double* hb = L.h + (qq * NN + pb[t]) * 3;
+        const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+        const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
+        M[t][0] += a0 * b0; M[t][1] += a0 * b1; M[t][2] += a0 * b2;
+        M[t][3] += a1 * b0; M[t][4] += a1 * b1; M[t][5] += a1 * b2;
+        M[t][6] += a2 * b0; M[t][7] += a2 * b1; M[t][8] += a2 * b2;
+      }
+    }
+    __syncthreads();
+  }
+  double lam, mu;
+  lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
+  constexpr int ND = 3 * NN;
+  // M -> K in place: K_ab = lam M + mu M^T + mu tr(M) I
+#pragma unroll
+  for (int t = 0; t < PPL; ++t) {
+    double* m = M[t];
+    const double tr = mu * (m[0] + m[4] + m[8]);
+    double K[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        K[3 * i + j] = lam * m[3 * i + j] + mu * m[3 * j + i] + (i == j ? tr : 0.0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m[i] = K[i];
+  }
+  // Staged store: rows of kStageNodes nodes at a time. Block (a, b) goes to the rows of
+  // a, its transpose to the rows of b; the staged rows are then one contiguous run of
+  // the element's row-major matrix.
+  double* kel = kp + le * (ND * ND);
+  for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      if (pa[t] < 0) continue;
+      const double* K = M[t];
+      if (pa[t] >= g0 && pa[t] < g0 + kStageNodes) {
+        double* ra = L.stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[3 * i + j];
+      }
+      if (pb[t] != pa[t] && pb[t] >= g0 && pb[t] < g0 + kStageNodes) {
+        double* rb = L.stage + (3 * (pb[t] - g0)) * ND + 3 * pa[t];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) rb[j * ND + i] = K[3 * i + j];
+      }
+    }
+    __syncthreads();
+    const int nrow = 3 * (NN - g0 < kStageNodes ? NN - g0 : kStageNodes);
+    double* dst = kel + int64_t(3 * g0) * ND;
+    for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = L.stage[idx];
+    __syncthreads();
+  }
+}
+
+// ---- K4 (+K5), two-phase variant, phase B: CSR rows gathered by their owner node ----
+// One 64-lane wave per node a. Its (up to three) CSR rows share one column layout of
+// length L; they are accumulated in LDS from rows 3la..3la+2 of the element matrices
+// of a's incident elements
+// in ascending element order (the order in which the reference sums its COO
+// duplicates, assembly.cpp:94) and written once, coalesced. Rows longer than the LDS
+// capacity are accumulated in place in the CSR array (one owner, still race-free).
+// With u given, the same pass forms f_int[a] = sum_e (Ke ue)_a over the full DOF
+// set, as fem::internal_force does (stress.cpp:125-145), with a fixed-order wave
+// reduction.
+constexpr int kRowCap = 256;
+constexpr int kBatch = 4;  // incident elements whose loads are issued together
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ __launch_bounds__(kWave) void k_row_gather(
+    const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
+    const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
+    const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
+    const double* __restrict__ u, double* __restrict__ vals, double* __restrict__ f_int) {
+  __shared__ double buf[3 * kRowCap];
+  const int32_t a = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int bl = lane / 3, jl = lane - 3 * (lane / 3);
+  int64_t rs[3];
+  int64_t Lrow = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int32_t eq = dof_eq[3 * int64_t(a) + i];
+    rs[i] = eq >= 0 ? row_ptr[eq] : -1;
+    if (eq >= 0) Lrow = row_ptr[eq + 1] - row_ptr[eq];
+  }
+  const bool in_lds = Lrow <= kRowCap;
+  for (int i = 0; i < 3; ++i) {
+    if (rs[i] < 0) continue;
+    for (int64_t c = lane; c < Lrow; c += kWave) {
+      if (in_lds) buf[i * kRowCap + c] = 0.0;
+      else vals[rs[i] + c] = 0.0;
+    }
+  }
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
+  for (int64_t ib = i0; ib < i1; ib += kBatch) {
+    // issue every load of the batch first ...
+    double v[kBatch][3], uv[kBatch];
+    int32_t colp[kBatch];
+    int fm[kBatch];
+    bool ok[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      ok[k] = false;
+      v[k][0] = v[k][1] = v[k][2] = 0.0;
+      uv[k] = 0.0;
+      colp[k] = 0;
+      fm[k] = 0;
+      if (ib + k < i1) {
+        const IncRecord r = rec[ib + k];
+        if (bl < r.n) {
+          ok[k] = true;
+          const int32_t nd = conn[r.conn + bl];
+          colp[k] = slot_all[r.slot + bl];
+          const double* ke = kp + r.rows + 3 * bl + jl;
+          const int ndof = 3 * r.n;
+          v[k][0] = ke[0];
+          v[k][1] = ke[ndof];
+          v[k][2] = ke[2 * ndof];
+          fm[k] = nfm[nd];
+          if (u) uv[k] = u[3 * int64_t(nd) + jl];
+        }
+      }
+    }
+    // ... then accumulate in ascending element order (one wave: LDS updates of
+    // successive elements are ordered by program order)
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      if (!ok[k]) continue;
+      f0 += v[k][0] * uv[k];
+      f1 += v[k][1] * uv[k];
+      f2 += v[k][2] * uv[k];
+      if ((fm[k] >> jl) & 1) {
+        const int64_t pos = int64_t(colp[k]) + __popc(fm[k] & ((1 << jl) - 1));
+        if (in_lds) {
+          if (rs[0] >= 0) buf[pos] += v[k][0];
+          if (rs[1] >= 0) buf[kRowCap + pos] += v[k][1];
+          if (rs[2] >= 0) buf[2 * kRowCap + pos] += v[k][2];
+        } else {
+          if (rs[0] >= 0) vals[rs[0] + pos] += v[k][0];
+          if (rs[1] >= 0) vals[rs[1] + pos] += v[k][1];
+          if (rs[2] >= 0) vals[rs[2] + pos] += v[k][2];
+        }
+      }
+    }
+    if (!in_lds) __syncthreads();  // global RMWs of the next batch must see these
+  }
+  __syncthreads();
+  if (in_lds)
+    for (int i = 0; i < 3; ++i) {
+      if (rs[i] < 0) continue;
+      for (int64_t c = lane; c < Lrow; c += kWave) vals[rs[i] + c] = buf[i * kRowCap + c];
+    }
+  if (u) {
+    f0 = wave_sum(f0);
+    f1 = wave_sum(f1);
+    f2 = wave_sum(f2);
+    if (lane == 0) {
+      double* fp = f_int + 3 * int64_t(a);
+      fp[0] = f0;
+      fp[1] = f1;
+      fp[2] = f2;
+    }
+  }
 }
 
 // ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
@@ -336,14 +840,17 @@ struct ForceLds {
 
 template <int NN, int NGP>
 __global__ __launch_bounds__(kWave) void k_internal_force(
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
+    const int32_t* __restrict__ pptr, int64_t elem_offset,
     const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
     const double* __restrict__ nu, const double* __restrict__ u, double* __restrict__ f_int,
     int* __restrict__ flag) {
   __shared__ ForceLds<NN, NGP> L;
   const int lane = threadIdx.x;
-  const int64_t le = elist[blockIdx.x];
+  const int32_t it_end = pptr[blockIdx.x + 1];
+  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
+  const int64_t le = elist[it];
   const int32_t* ce = conn + le * NN;
   stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
   stage_vec<NN>(L.u, ce, u, lane);
@@ -363,6 +870,8 @@ __global__ __launch_bounds__(kWave) void k_internal_force(
   }
   __syncthreads();
   node_forces<NN, NGP>(L.G, L.S, L.node, f_int, lane);
+  __syncthreads();
+  }
 }
 
 // ---- K2: J2 radial return + consistent tangent + internal force --------------
@@ -500,7 +1009,8 @@ struct J2Lds {
 
 template <int NN, int NGP, bool UPPER>
 __global__ __launch_bounds__(kWave) void k_assemble_j2(
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
+    const int32_t* __restrict__ pptr, int64_t elem_offset,
     int64_t gp_offset, int64_t n_points, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ coords,
     const double* __restrict__ E, const double* __restrict__ nu,
@@ -511,7 +1021,9 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
     double* __restrict__ vals, double* __restrict__ f_int, int* __restrict__ flag) {
   __shared__ J2Lds<NN, NGP, UPPER> L;
   const int lane = threadIdx.x;
-  const int64_t le = elist[blockIdx.x];
+  const int32_t it_end = pptr[blockIdx.x + 1];
+  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
+  const int64_t le = elist[it];
   const int64_t ge = elem_offset + le;
   const int32_t* ce = conn + le * NN;
   stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
@@ -561,6 +1073,8 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
     }
     scatter_pair<NN>(L.D, se, a, b, K, UPPER && a != b, vals);
   }
+  __syncthreads();  // LDS is reused by the next element of the patch
+  }
 }
 
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
@@ -603,7 +1117,8 @@ struct VjpLds {
 
 template <int NN, int NGP>
 __global__ __launch_bounds__(kWave) void k_vjp(
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist, int64_t elem_offset,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
+    const int32_t* __restrict__ pptr, int64_t elem_offset,
     const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
     const double* __restrict__ nu, const double* __restrict__ u, const double* __restrict__ w,
@@ -611,7 +1126,9 @@ __global__ __launch_bounds__(kWave) void k_vjp(
     int* __restrict__ flag) {
   __shared__ VjpLds<NN, NGP> L;
   const int lane = threadIdx.x;
-  const int64_t le = elist[blockIdx.x];
+  const int32_t it_end = pptr[blockIdx.x + 1];
+  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
+  const int64_t le = elist[it];
   const int64_t ge = elem_offset + le;
   const int32_t* ce = conn + le * NN;
   stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
@@ -670,6 +1187,8 @@ __global__ __launch_bounds__(kWave) void k_vjp(
     xp[2] += a2;
   }
   node_forces<NN, NGP>(L.G, L.S, L.node, du, lane);
+  __syncthreads();
+  }
 }
 
 // ---- host-side dispatch -------------------------------------------------------
@@ -688,8 +1207,16 @@ const char* type_name(int t) {
   }
 }
 
-// Launch f once per (colour, block) in colour order; each launch is timed as
-// "<what>_<family>" when the context's timing is on.
+// One launch per (colour, block) in colour order, one workgroup per patch; each
+// launch is timed as "<what>_<family>" when the context's timing is on.
+struct LaunchSet {
+  size_t bi;
+  const cxg_block_dev& B;
+  const int32_t* elist;
+  const int32_t* pptr;
+  unsigned patches;
+};
+
 template <class F>
 void for_each_colour_block(cxg_context* ctx, const char* what, const cxg_mesh* m, F&& f) {
   for (int c = 0; c < m->n_colors; ++c)
@@ -697,17 +1224,19 @@ void for_each_colour_block(cxg_context* ctx, const char* what, const cxg_mesh* m
       const cxg_block_dev& B = m->blocks[bi];
       if (c < int(B.color_count.size()) && B.color_count[size_t(c)] > 0) {
         LaunchTimer timer(ctx, std::string(what) + "_" + type_name(B.type), B.color_count[size_t(c)]);
-        f(bi, B, B.color[size_t(c)].p, B.color_count[size_t(c)]);
+        f(LaunchSet{bi, B, B.color[size_t(c)].p, B.patch_ptr[size_t(c)].p,
+                    unsigned(B.patch_count[size_t(c)])});
+        CXG_HIP(hipGetLastError());
       }
     }
 }
 
-#define CXG_DISPATCH(TYPE, MACRO)                 \
-  switch (TYPE) {                                 \
-    case CXG_C3D8: MACRO(8, 8, false); break;     \
-    case CXG_C3D20: MACRO(20, 27, true); break;   \
-    case CXG_C3D20R: MACRO(20, 8, true); break;   \
-    case CXG_C3D6: MACRO(6, 2, false); break;     \
+#define CXG_DISPATCH(TYPE, MACRO)                                                   \
+  switch (TYPE) {                                                                   \
+    case CXG_C3D8: MACRO(8, 8, false, 8); break;                                    \
+    case CXG_C3D20: MACRO(20, 27, true, 3); break;                                  \
+    case CXG_C3D20R: MACRO(20, 8, true, 3); break;                                  \
+    case CXG_C3D6: MACRO(6, 2, false, 2); break;                                    \
     default: throw Error(CXG_ERR_UNSUPPORTED, "element type outside the GPU path"); \
   }
 
@@ -723,34 +1252,74 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
                             const double* E, const double* nu, double* vals, const double* u,
                             double* f_int) {
   const cxg_mesh* m = p->mesh;
+  if (ctx->asm_variant == 4) {
+    // Two-phase: element pair blocks to the arena, then one owner wave per node row.
+    cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+    if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
+    for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+      const cxg_block_dev& B = m->blocks[bi];
+      if (B.n_elem == 0) continue;
+      const ElemTable& T = ctx->tables[B.type];
+      const int32_t* conn = m->conn.p + B.conn_offset;
+      double* kp = pm->ke_arena.p + p->kp_base[bi];
+      LaunchTimer timer(ctx, std::string("elem_iso_") + type_name(B.type), B.n_elem);
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,    \
+                     ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E, nu, kp,            \
+                     ctx->flag_dev)
+      CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+      CXG_HIP(hipGetLastError());
+    }
+    if (m->n_nodes > 0) {
+      LaunchTimer timer(ctx, "row_gather", m->n_elem);
+      hipLaunchKernelGGL(k_row_gather, dim3(unsigned(m->n_nodes)), dim3(kWave), 0, ctx->stream,
+                         m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,
+                         p->dof_eq.p, p->row_ptr.p, pm->ke_arena.p, f_int ? u : nullptr, vals,
+                         f_int);
+      CXG_HIP(hipGetLastError());
+    }
+    return;
+  }
   fill_f64(ctx->stream, vals, p->nnz, 0.0);
   if (f_int) fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
-  for_each_colour_block(ctx, "assemble_iso", m, [&](size_t bi, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
-    const ElemTable& T = ctx->tables[B.type];
-    const int32_t* conn = m->conn.p + B.conn_offset;
-#define LAUNCH(NN, NGP, UP)                                                                     \
-  hipLaunchKernelGGL((k_assemble_iso<NN, NGP, UP>), dim3(unsigned(cnt)), dim3(kWave), 0,        \
-                     ctx->stream, conn, el, B.elem_offset, T.dN, T.w, coords, E, nu,           \
-                     p->dof_eq.p, p->row_ptr.p, p->slot[bi].p, vals, u, f_int, ctx->flag_dev)
-    CXG_DISPATCH(B.type, LAUNCH)
+  const bool v2 = ctx->asm_variant >= 2;
+  for_each_colour_block(ctx, v2 ? "assemble_iso2" : "assemble_iso", m, [&](const LaunchSet& S) {
+    const ElemTable& T = ctx->tables[S.B.type];
+    const int32_t* conn = m->conn.p + S.B.conn_offset;
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  if (ctx->asm_variant == 2)                                                                    \
+    hipLaunchKernelGGL((k_assemble_iso2<NN, NGP, QC, 4>), dim3(S.patches), dim3(kWave), 0,      \
+                       ctx->stream, conn, S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords,  \
+                       E, nu, p->dof_eq.p, p->row_ptr.p, (p->slot_all.p + p->slot_base[S.bi]), vals, u, f_int,       \
+                       ctx->flag_dev);                                                          \
+  else if (ctx->asm_variant == 3)                                                               \
+    hipLaunchKernelGGL((k_assemble_iso2<NN, NGP, QC, 3>), dim3(S.patches), dim3(kWave), 0,      \
+                       ctx->stream, conn, S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords,  \
+                       E, nu, p->dof_eq.p, p->row_ptr.p, (p->slot_all.p + p->slot_base[S.bi]), vals, u, f_int,       \
+                       ctx->flag_dev);                                                          \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_assemble_iso<NN, NGP, UP>), dim3(S.patches), dim3(kWave), 0,          \
+                       ctx->stream, conn, S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords,  \
+                       E, nu, p->dof_eq.p, p->row_ptr.p, (p->slot_all.p + p->slot_base[S.bi]), vals, u, f_int,       \
+                       ctx->flag_dev)
+    CXG_DISPATCH(S.B.type, LAUNCH)
 #undef LAUNCH
-    CXG_HIP(hipGetLastError());
   });
 }
 
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
                            const double* E, const double* nu, const double* u, double* f_int) {
   fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
-  for_each_colour_block(ctx, "internal_force", m, [&](size_t, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
-    const ElemTable& T = ctx->tables[B.type];
-    const int32_t* conn = m->conn.p + B.conn_offset;
-#define LAUNCH(NN, NGP, UP)                                                                   \
-  hipLaunchKernelGGL((k_internal_force<NN, NGP>), dim3(unsigned(cnt)), dim3(kWave), 0,        \
-                     ctx->stream, conn, el, B.elem_offset, T.dN, T.w, coords, E, nu, u,      \
-                     f_int, ctx->flag_dev)
-    CXG_DISPATCH(B.type, LAUNCH)
+  for_each_colour_block(ctx, "internal_force", m, [&](const LaunchSet& S) {
+    const ElemTable& T = ctx->tables[S.B.type];
+    const int32_t* conn = m->conn.p + S.B.conn_offset;
+#define LAUNCH(NN, NGP, UP, QC)                                                               \
+  hipLaunchKernelGGL((k_internal_force<NN, NGP>), dim3(S.patches), dim3(kWave), 0,            \
+                     ctx->stream, conn, S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords,  \
+                     E, nu, u, f_int, ctx->flag_dev)
+    CXG_DISPATCH(S.B.type, LAUNCH)
 #undef LAUNCH
-    CXG_HIP(hipGetLastError());
   });
 }
 
@@ -806,17 +1375,16 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
   const cxg_mesh* m = p->mesh;
   fill_f64(ctx->stream, vals, p->nnz, 0.0);
   fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
-  for_each_colour_block(ctx, "assemble_j2", m, [&](size_t bi, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
-    const ElemTable& T = ctx->tables[B.type];
-    const int32_t* conn = m->conn.p + B.conn_offset;
-#define LAUNCH(NN, NGP, UP)                                                                     \
-  hipLaunchKernelGGL((k_assemble_j2<NN, NGP, UP>), dim3(unsigned(cnt)), dim3(kWave), 0,         \
-                     ctx->stream, conn, el, B.elem_offset, B.gp_offset, m->n_points, T.dN,     \
-                     T.w, coords, E, nu, mat_id, dmats, u, committed, trial, p->dof_eq.p,      \
-                     p->row_ptr.p, p->slot[bi].p, vals, f_int, ctx->flag_dev)
-    CXG_DISPATCH(B.type, LAUNCH)
+  for_each_colour_block(ctx, "assemble_j2", m, [&](const LaunchSet& S) {
+    const ElemTable& T = ctx->tables[S.B.type];
+    const int32_t* conn = m->conn.p + S.B.conn_offset;
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_assemble_j2<NN, NGP, UP>), dim3(S.patches), dim3(kWave), 0,             \
+                     ctx->stream, conn, S.elist, S.pptr, S.B.elem_offset, S.B.gp_offset,        \
+                     m->n_points, T.dN, T.w, coords, E, nu, mat_id, dmats, u, committed, trial, \
+                     p->dof_eq.p, p->row_ptr.p, (p->slot_all.p + p->slot_base[S.bi]), vals, f_int, ctx->flag_dev)
+    CXG_DISPATCH(S.B.type, LAUNCH)
 #undef LAUNCH
-    CXG_HIP(hipGetLastError());
   });
 }
 
@@ -831,16 +1399,15 @@ void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const
                 double* du) {
   fill_f64(ctx->stream, dx, int64_t(m->n_nodes) * 3, 0.0);
   fill_f64(ctx->stream, du, int64_t(m->n_nodes) * 3, 0.0);
-  for_each_colour_block(ctx, "vjp", m, [&](size_t, const cxg_block_dev& B, const int32_t* el, int64_t cnt) {
-    const ElemTable& T = ctx->tables[B.type];
-    const int32_t* conn = m->conn.p + B.conn_offset;
-#define LAUNCH(NN, NGP, UP)                                                                   \
-  hipLaunchKernelGGL((k_vjp<NN, NGP>), dim3(unsigned(cnt)), dim3(kWave), 0, ctx->stream,      \
-                     conn, el, B.elem_offset, T.dN, T.w, coords, E, nu, u, w, dx, dE, du,     \
-                     ctx->flag_dev)
-    CXG_DISPATCH(B.type, LAUNCH)
+  for_each_colour_block(ctx, "vjp", m, [&](const LaunchSet& S) {
+    const ElemTable& T = ctx->tables[S.B.type];
+    const int32_t* conn = m->conn.p + S.B.conn_offset;
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_vjp<NN, NGP>), dim3(S.patches), dim3(kWave), 0, ctx->stream, conn,      \
+                     S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords, E, nu, u, w, dx, dE,  \
+                     du, ctx->flag_dev)
+    CXG_DISPATCH(S.B.type, LAUNCH)
 #undef LAUNCH
-    CXG_HIP(hipGetLastError());
   });
 }
 

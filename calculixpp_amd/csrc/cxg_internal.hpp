@@ -99,6 +99,10 @@ struct cxg_context {
   std::vector<hipEvent_t> event_pool;
   std::vector<Pending> pending;
   std::vector<Stat> stats;
+  // Tunables (cxg_set_option): elements per patch for meshes created afterwards,
+  // and the stiffness-kernel variant.
+  int patch_size = 8;
+  int asm_variant = 4;
 };
 
 namespace cxg {
@@ -114,13 +118,19 @@ struct LaunchTimer {
 void collect_timing(cxg_context* ctx);  // after a stream synchronization
 }  // namespace cxg
 
-// One element family run inside a mesh.
+// One element family run inside a mesh. Elements are grouped into patches of up to
+// patch_size consecutive elements; patches are coloured so that two patches of one
+// colour share no node. A workgroup walks one patch element by element, so repeated
+// updates of the same CSR rows by neighbouring elements happen back to back (cache
+// reuse), while patches of one colour run concurrently without conflicts.
 struct cxg_block_dev {
   int32_t type = 0, nn = 0, ngp = 0;
   int64_t n_elem = 0, elem_offset = 0, gp_offset = 0;
   int64_t conn_offset = 0;                     // into cxg_mesh::conn (entries)
-  std::vector<cxg::DevBuf<int32_t>> color;     // per colour: block-local element ids
-  std::vector<int64_t> color_count;
+  std::vector<cxg::DevBuf<int32_t>> color;     // per colour: block-local element ids, by patch
+  std::vector<cxg::DevBuf<int32_t>> patch_ptr; // per colour: patch p = color[ptr[p]..ptr[p+1])
+  std::vector<int64_t> color_count;            // elements per colour
+  std::vector<int64_t> patch_count;            // patches per colour
 };
 
 struct cxg_mesh {
@@ -136,6 +146,18 @@ struct cxg_mesh {
   int32_t max_candidates = 0;     // max over nodes of sum of incident element sizes
 };
 
+namespace cxg {
+// One (node, incident element) pair of the two-phase gather: where the node's three
+// rows of the element matrix start in the arena, where its slot row starts, the
+// element's connectivity offset and node count.
+struct IncRecord {
+  int64_t rows;   // arena offset of row 3*la of the element matrix
+  int64_t slot;   // offset of slot row la of the element
+  int64_t conn;   // offset of the element in the mesh connectivity
+  int64_t n;      // nodes of the element
+};
+}  // namespace cxg
+
 struct cxg_pattern {
   const cxg_mesh* mesh = nullptr;
   int32_t n_free = 0;
@@ -143,7 +165,21 @@ struct cxg_pattern {
   cxg::DevBuf<int32_t> dof_eq;                // 3*n_nodes
   cxg::DevBuf<int64_t> row_ptr;               // n_free+1
   cxg::DevBuf<int32_t> col_idx;               // nnz
-  std::vector<cxg::DevBuf<int32_t>> slot;     // per block: n_elem*nn*nn column offsets
+  // slot_all[sptr[e] + a*n + b]: offset of node b's column block inside the CSR rows of
+  // node a, for element e with n nodes (a, b local). slot_base[k] = sptr of block k's
+  // first element.
+  cxg::DevBuf<int32_t> slot_all;
+  cxg::DevBuf<int64_t> sptr;                  // n_elem+1
+  std::vector<int64_t> slot_base;
+  int64_t slot_total = 0;                     // sum over elements of n^2
+  // Two-phase assembly arena, allocated on first use: each element's full Ke
+  // (3n x 3n, row-major) at kpptr[e].
+  cxg::DevBuf<double> ke_arena;
+  cxg::DevBuf<int64_t> kpptr;                 // n_elem+1, in doubles
+  cxg::DevBuf<cxg::IncRecord> inc_rec;        // per mesh incidence entry (inc_elem order)
+  cxg::DevBuf<int8_t> nfm;                    // per node: bit c set if DOF c is free
+  std::vector<int64_t> kp_base;               // kpptr of each block's first element
+  int64_t kp_total = 0;
 };
 
 namespace cxg {

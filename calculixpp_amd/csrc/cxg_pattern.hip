@@ -171,6 +171,28 @@ __global__ void k_slot(int64_t n_elem, int nn, const int32_t* __restrict__ conn,
   }
 }
 
+// Incidence records of the two-phase gather and the per-node free masks.
+__global__ void k_inc_records(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
+                              const int32_t* __restrict__ inc_elem,
+                              const int64_t* __restrict__ eptr, const int32_t* __restrict__ conn,
+                              const int64_t* __restrict__ sptr, const int64_t* __restrict__ kpptr,
+                              const int32_t* __restrict__ dof_eq, IncRecord* __restrict__ rec,
+                              int8_t* __restrict__ nfm) {
+  const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_nodes) return;
+  const int64_t a3 = 3 * int64_t(a);
+  nfm[a] = int8_t((dof_eq[a3] >= 0 ? 1 : 0) | (dof_eq[a3 + 1] >= 0 ? 2 : 0) |
+                  (dof_eq[a3 + 2] >= 0 ? 4 : 0));
+  for (int64_t it = inc_ptr[a]; it < inc_ptr[a + 1]; ++it) {
+    const int32_t e = inc_elem[it];
+    const int64_t b = eptr[e];
+    const int n = int(eptr[e + 1] - b);
+    int la = 0;
+    while (la < n && conn[b + la] != a) ++la;
+    rec[it] = IncRecord{kpptr[e] + int64_t(3 * la) * (3 * n), sptr[e] + int64_t(la) * n, b, n};
+  }
+}
+
 }  // namespace
 
 void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_dev,
@@ -252,21 +274,51 @@ void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_de
                        cpre.p, P->dof_eq.p, P->row_ptr.p, P->col_idx.p);
   CXG_HIP(hipGetLastError());
 
-  // 5. element -> slot map per block
-  P->slot.clear();
+  // 5. element -> slot map, one flat array with per-element offsets
+  std::vector<int64_t> hsptr(size_t(m->n_elem) + 1, 0);
+  P->slot_base.clear();
   for (const cxg_block_dev& B : m->blocks) {
-    DevBuf<int32_t> sl(size_t(std::max<int64_t>(B.n_elem * B.nn * B.nn, 1)));
+    P->slot_base.push_back(hsptr[size_t(B.elem_offset)]);
+    for (int64_t e = 0; e < B.n_elem; ++e)
+      hsptr[size_t(B.elem_offset + e) + 1] = hsptr[size_t(B.elem_offset + e)] + int64_t(B.nn) * B.nn;
+  }
+  P->slot_total = hsptr.back();
+  std::vector<int64_t> hkp(size_t(m->n_elem) + 1, 0);
+  P->kp_base.clear();
+  for (const cxg_block_dev& B : m->blocks) {
+    P->kp_base.push_back(hkp[size_t(B.elem_offset)]);
+    for (int64_t e = 0; e < B.n_elem; ++e)
+      hkp[size_t(B.elem_offset + e) + 1] = hkp[size_t(B.elem_offset + e)] + 9 * int64_t(B.nn) * B.nn;
+  }
+  P->kp_total = hkp.back();
+  P->kpptr.alloc(hkp.size());
+  CXG_HIP(hipMemcpyAsync(P->kpptr.p, hkp.data(), hkp.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+  P->slot_all.alloc(size_t(std::max<int64_t>(hsptr.back(), 1)));
+  P->sptr.alloc(hsptr.size());
+  CXG_HIP(hipMemcpyAsync(P->sptr.p, hsptr.data(), hsptr.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+  for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+    const cxg_block_dev& B = m->blocks[bi];
     const int64_t total = B.n_elem * B.nn * B.nn;
     if (total > 0) {
       const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
       hipLaunchKernelGGL(k_slot, dim3(unsigned(blocks)), dim3(256), 0, s, B.n_elem, B.nn,
-                         m->conn.p + B.conn_offset, adj_ptr.p, adj.p, cpre.p, sl.p,
-                         ctx->flag_dev);
+                         m->conn.p + B.conn_offset, adj_ptr.p, adj.p, cpre.p,
+                         P->slot_all.p + P->slot_base[bi], ctx->flag_dev);
       CXG_HIP(hipGetLastError());
     }
-    P->slot.push_back(std::move(sl));
   }
-  check_flags(ctx);  // synchronizes: temporaries may be freed afterwards
+  const size_t n_inc = m->inc_elem.n;
+  P->inc_rec.alloc(std::max<size_t>(n_inc, 1));
+  P->nfm.alloc(size_t(std::max(nn_nodes, 1)));
+  if (nn_nodes > 0) {
+    hipLaunchKernelGGL(k_inc_records, dim3(unsigned((nn_nodes + 255) / 256)), dim3(256), 0, s,
+                       nn_nodes, m->inc_ptr.p, m->inc_elem.p, m->eptr.p, m->conn.p, P->sptr.p,
+                       P->kpptr.p, P->dof_eq.p, P->inc_rec.p, P->nfm.p);
+    CXG_HIP(hipGetLastError());
+  }
+  check_flags(ctx);  // synchronizes: temporaries (and hsptr) may be freed afterwards
 }
 
 }  // namespace cxg

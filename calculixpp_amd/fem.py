@@ -62,6 +62,10 @@ class Context:
     def check(self, status: int):
         _lib.check(self.h, status)
 
+    def set_option(self, key: str, value: int) -> None:
+        """cxg_set_option: "patch_size" (meshes created afterwards), "asm_variant"."""
+        self.check(self.lib.cxg_set_option(self.h, key.encode(), int(value)))
+
     def profile(self, enable: bool) -> None:
         """Bracket every library kernel launch with HIP events on its stream."""
         self.check(self.lib.cxg_profile_enable(self.h, int(enable)))

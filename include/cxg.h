@@ -89,6 +89,14 @@ cxg_status cxg_create(int device, void* stream, cxg_context** out);
 void cxg_destroy(cxg_context* ctx);
 cxg_status cxg_set_stream(cxg_context* ctx, void* stream);
 const char* cxg_last_error(const cxg_context* ctx);
+/* Tunables of the context (no counterpart in the reference):
+ *   "patch_size"  elements per patch of meshes created afterwards (>= 1, default 8)
+ *   "asm_variant" linear stiffness kernel: 1 = per-pair scatter, 2 = M-identity with
+ *                 row-coalesced scatter at 4 waves/SIMD, 3 = the same at 3, 4 =
+ *                 two-phase: element matrices to an arena, then each CSR row gathered
+ *                 and written once by its node (default)
+ * Returns CXG_ERR_ARG for an unknown key or value. */
+cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value);
 /* Library version string, e.g. "cxg 0.1 gfx950". */
 const char* cxg_version(void);
 int32_t cxg_nodes_per_element(int32_t type);   /* core/element.hpp:24-42 */

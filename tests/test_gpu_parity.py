@@ -176,3 +176,26 @@ def test_deterministic_bitwise(gpu):
     a.assemble(v1)
     a.assemble(v2)
     assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
+
+
+@pytest.mark.parametrize("variant,patch", [(1, 1), (1, 5), (2, 1), (2, 3), (2, 8), (3, 16), (4, 1), (4, 8)])
+@pytest.mark.parametrize("fam_i", range(4))
+def test_kernel_variants_and_patches(gpu, port, variant, patch, fam_i):
+    """Every stiffness kernel variant and patch length gives the same K and f_int."""
+    import torch
+    from calculixpp_amd import autotune, meshgen
+    m = meshgen.block(4, _families()[fam_i], seed_geo=31, seed_E=32, ny=3, nz=2)
+    exp = port.assemble_linear_static(m)
+    rp, ci, vals = exp.csr()
+    u = meshgen.smooth_field(m.coords, 33)
+    fexp = port.internal_force(m, u)
+    a = autotune.make_assembler(m, variant, patch)
+    v = a.new_values()
+    f = torch.empty(3 * m.n_nodes, dtype=torch.float64, device=a.ctx.device)
+    a.assemble(v, torch.from_numpy(u).to(a.ctx.device), f)
+    assert np.array_equal(a.pattern.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(a.pattern.col_idx.cpu().numpy(), ci)
+    assert rel_l2(v[: a.pattern.nnz].cpu().numpy(), vals) <= REL_K
+    assert rel_l2(f.cpu().numpy(), fexp) <= REL_K
+    a.ctx.set_option("asm_variant", 4)  # restore the defaults
+    a.ctx.set_option("patch_size", 8)
