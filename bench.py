@@ -44,6 +44,20 @@ def bytes_per_assembly(nnz: int, n_nodes: int, n_elem: int, nn: int, residual: b
     return b + (48 * n_nodes if residual else 0)
 
 
+def load_traffic(workload: str, variant: int):
+    """HBM bytes per assembly measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    copy-calibrated, as committed in profiles/traffic.json; None if not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("asm_variant") != variant:
+        return None
+    return rec.get("hbm_bytes_per_step")
+
+
 def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
     """Reference CPU assembly (+ internal force) on an n^3 sample of the same
     workload, one thread (the reference is single-threaded)."""
@@ -132,30 +146,43 @@ def main() -> int:
     value = model.n_elem / (ms_step * 1e-3)
     print(f"[bench] {ms_step:.3f} ms/step, {value:.4g} elements/s", file=sys.stderr, flush=True)
 
-    # Per-kernel durations from HIP events on the library's stream.
+    # Per-kernel durations from HIP events on the library's stream. One assembly is
+    # the element kernel followed by the row-gather kernel (two launches, one step);
+    # the roofline fraction is BASELINE's max(F/peak_fp64, B/peak_hbm) / t over both.
     ctx = asm.ctx
     ctx.profile_reset()
     ctx.profile(True)
-    for _ in range(3):
+    reps = 3
+    for _ in range(reps):
         asm.assemble(vals, u, f_int)
     ctx.profile(False)
     stats = ctx.profile_stats()
+    kernels = {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4),
+                   "ms_per_step": round(v[1] / reps, 4), "elements": v[2]}
+               for k, v in stats.items()}
+    step_ms = sum(v[1] for v in stats.values()) / reps
     name = max(stats, key=lambda k: stats[k][1])
-    launches, ms_total, elems = stats[name]
     nn, ngp = NODES[family], GAUSS[family]
     fpe = flops_per_element(nn, ngp, residual)
-    tflops = fpe * elems / (ms_total * 1e-3) / 1e12
+    flop = fpe * model.n_elem
     bpa = bytes_per_assembly(asm.pattern.nnz, model.n_nodes, model.n_elem, nn, residual)
-    asm_ms = ms_total / 3.0
-    gbs = bpa / (asm_ms * 1e-3) / 1e9
-    roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP64,
-                "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64, 4), "traffic": None,
-                "kernel": name, "launches_per_step": launches // 3,
-                "avg_launch_ms": round(ms_total / launches, 4),
-                "flop_per_element": fpe, "elements_per_launch": elems // launches,
-                "hbm_algorithmic_GBps": round(gbs, 1),
-                "hbm_frac": round(gbs / PEAK_HBM, 4)}
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 4), "elements": v[2]} for k, v in stats.items()}
+    t_flop, t_byte = flop / (PEAK_FP64 * 1e12), bpa / (PEAK_HBM * 1e9)
+    bound = "mfma" if t_flop >= t_byte else "hbm"
+    achieved_tf = flop / (step_ms * 1e-3) / 1e12
+    achieved_gb = bpa / (step_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"{args.family}-{args.n}", choice["asm_variant"])
+    roofline = {"bound": bound,
+                "achieved": round(achieved_tf if bound == "mfma" else achieved_gb, 3),
+                "peak": PEAK_FP64 if bound == "mfma" else PEAK_HBM,
+                "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
+                "frac": round(max(t_flop, t_byte) / (step_ms * 1e-3), 4),
+                "traffic": traffic,
+                "kernel": " + ".join(sorted(stats)), "dominant": name,
+                "step_kernel_ms": round(step_ms, 4),
+                "flop_per_element": fpe, "flop_per_step": flop,
+                "algorithmic_bytes_per_step": bpa,
+                "hbm_algorithmic_GBps": round(achieved_gb, 1),
+                "t_attainable_ms": round(max(t_flop, t_byte) * 1e3, 4)}
 
     cpu = None
     if not args.no_cpu:

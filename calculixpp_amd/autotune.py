@@ -25,7 +25,7 @@ from . import fem
 from .model import Model
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "calculixpp_amd", "autotune.json")
-CANDIDATES = [(4, 8), (2, 8), (2, 1), (3, 8), (1, 8)]
+CANDIDATES = [(4, 8), (2, 8), (1, 8)]
 
 
 def _key(model: Model) -> str:
