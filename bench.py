@@ -58,6 +58,42 @@ def load_traffic(workload: str, variant: int):
     return rec.get("hbm_bytes_per_step")
 
 
+def adjoint_ratio(reps: int = 20) -> dict:
+    """Adjoint/forward cost on the 96x48x48 C3D8 SIMP block (jitter 5%, seeds 5-8):
+    median over `reps` warm runs of the VJP kernel time over the forward r = K(x,rho)u
+    kernel time, both from HIP events on the library's stream."""
+    import statistics
+    import numpy as np
+    import torch
+    from calculixpp_amd import meshgen
+    from calculixpp_amd.autograd import ResidualMesh
+    m, rho = meshgen.adjoint_block()
+    mesh = ResidualMesh(m)
+    dev = mesh.ctx.device
+    x = torch.from_numpy(m.coords).to(dev)
+    E = torch.from_numpy(m.E).to(dev)
+    u = torch.from_numpy(meshgen.smooth_field(m.coords, 7)).to(dev)
+    w = torch.from_numpy(np.random.default_rng(8).normal(size=(m.n_nodes, 3))).to(dev)
+    r = torch.empty_like(u)
+    dx, du, dE = torch.empty_like(x), torch.empty_like(u), torch.empty_like(E)
+    mesh.forward_raw(x, E, u, r)
+    mesh.vjp_raw(x, E, u, w, dx, dE, du)
+    ctx = mesh.ctx
+    fwd, vjp = [], []
+    for _ in range(reps):
+        for fn, acc in ((lambda: mesh.forward_raw(x, E, u, r), fwd),
+                        (lambda: mesh.vjp_raw(x, E, u, w, dx, dE, du), vjp)):
+            ctx.profile_reset()
+            ctx.profile(True)
+            fn()
+            ctx.profile(False)
+            acc.append(sum(v[1] for k, v in ctx.profile_stats().items()))
+    f_ms, v_ms = statistics.median(fwd), statistics.median(vjp)
+    return {"workload": "96x48x48 C3D8 SIMP block, r = K(x,rho)u and its VJP (dx, drho, du)",
+            "elements": m.n_elem, "forward_ms": f_ms, "vjp_ms": v_ms, "ratio": v_ms / f_ms,
+            "reps": reps}
+
+
 def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
     """Reference CPU assembly (+ internal force) on an n^3 sample of the same
     workload, one thread (the reference is single-threaded)."""
@@ -83,6 +119,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=16, help="edge of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-reps", type=int, default=20)
     ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
     ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -184,6 +222,11 @@ def main() -> int:
                 "hbm_algorithmic_GBps": round(achieved_gb, 1),
                 "t_attainable_ms": round(max(t_flop, t_byte) * 1e3, 4)}
 
+    adjoint = None if args.no_adjoint else adjoint_ratio(args.adj_reps)
+    if adjoint:
+        print(f"[bench] adjoint/forward {adjoint['ratio']:.3f} ({adjoint['vjp_ms']:.4f} / "
+              f"{adjoint['forward_ms']:.4f} ms)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(family, args.cpu_n, residual)
@@ -205,6 +248,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "adjoint": adjoint,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(out), flush=True)
