@@ -94,6 +94,78 @@ def adjoint_ratio(reps: int = 20) -> dict:
             "reps": reps}
 
 
+def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
+              cpu_shape=(32, 32, 16), cpu: bool = True) -> dict:
+    """One J2 Newton iteration (radial return at every Gauss point, consistent tangent
+    K_t into the CSR values, internal force) on the 128x128x64 C3D8 block with seeded
+    committed history (seed 3). The smooth displacement (seed 7) is scaled by bisection
+    until the fraction of yielding points is within 0.005 of `target`."""
+    import statistics
+    import torch
+    from calculixpp_amd import fem, meshgen
+    dev = torch.device("cuda", 0)
+    m = meshgen.j2_block(*shape)
+    hist = meshgen.seeded_history(m.n_points, 3)
+    a = fem.TangentAssembler(m)
+    mp = fem.MaterialPoints(m.n_points, dev, committed=hist)
+    base = torch.from_numpy(meshgen.smooth_field(m.coords, 7, 1.0)).to(dev)
+    u = torch.empty_like(base)
+    vals = a.new_values()
+    f_int = torch.empty(3 * m.n_nodes, dtype=torch.float64, device=dev)
+
+    def yield_frac(amp: float) -> float:
+        torch.mul(base, amp, out=u)
+        a.assemble_tangent(u, mp, vals, f_int)
+        return float((mp.trial[6] > mp.committed[6]).double().mean())
+
+    lo, hi = 1e-6, 1e-1
+    amp, frac = hi, yield_frac(hi)
+    for _ in range(60):
+        amp = (lo * hi) ** 0.5
+        frac = yield_frac(amp)
+        if abs(frac - target) <= 0.005:
+            break
+        lo, hi = (amp, hi) if frac < target else (lo, amp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        a.assemble_tangent(u, mp, vals, f_int)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / reps
+    ctx = a.ctx
+    per = []
+    kern = {}
+    for _ in range(reps):
+        ctx.profile_reset()
+        ctx.profile(True)
+        a.assemble_tangent(u, mp, vals, f_int)
+        ctx.profile(False)
+        st = ctx.profile_stats()
+        per.append(sum(v[1] for v in st.values()))
+        for k, v in st.items():
+            kern.setdefault(k, []).append(v[1])
+    out = {"workload": f"{shape[0]}x{shape[1]}x{shape[2]} C3D8 J2 block, one Newton iteration "
+                       "(return map + K_t + f_int)",
+           "elements": m.n_elem, "gauss_points": m.n_points, "nnz": a.pattern.nnz,
+           "amplitude": amp, "yield_fraction": frac, "ms_per_iteration": wall_ms,
+           "kernel_ms": statistics.median(per),
+           "kernels": {k: round(statistics.median(v), 4) for k, v in kern.items()},
+           "elements_per_s": m.n_elem / (wall_ms * 1e-3)}
+    if cpu:
+        from oracle.oracle import Oracle, aos_state, ref_available
+        kind = "reference" if ref_available() else "port"
+        cm = meshgen.j2_block(*cpu_shape)
+        ch = meshgen.seeded_history(cm.n_points, 3)
+        cu = meshgen.smooth_field(cm.coords, 7, amp)
+        sec = Oracle(kind).time_material_tangent(cm, cu, aos_state(ch))
+        out["cpu_baseline"] = {"value": cm.n_elem / sec, "unit": "elements/s", "cores": 1,
+                               "kind": kind,
+                               "sample": f"{cpu_shape[0]}x{cpu_shape[1]}x{cpu_shape[2]} C3D8 J2 "
+                                         f"block, same seeds and amplitude, {sec:.2f} s"}
+        out["gpu_over_cpu"] = out["elements_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
     """Reference CPU assembly (+ internal force) on an n^3 sample of the same
     workload, one thread (the reference is single-threaded)."""
@@ -121,6 +193,7 @@ def main() -> int:
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-reps", type=int, default=20)
+    ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
     ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
     ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -227,6 +300,12 @@ def main() -> int:
         print(f"[bench] adjoint/forward {adjoint['ratio']:.3f} ({adjoint['vjp_ms']:.4f} / "
               f"{adjoint['forward_ms']:.4f} ms)", file=sys.stderr, flush=True)
 
+    j2 = None if args.no_j2 else j2_newton(cpu=not args.no_cpu)
+    if j2:
+        print(f"[bench] J2 Newton iteration {j2['ms_per_iteration']:.3f} ms "
+              f"({j2['elements_per_s']:.4g} elements/s, yield {j2['yield_fraction']:.3f})",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(family, args.cpu_n, residual)
@@ -249,6 +328,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "kernels": kernels,
         "adjoint": adjoint,
+        "j2_newton": j2,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(out), flush=True)

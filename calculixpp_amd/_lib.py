@@ -69,11 +69,12 @@ _SIGS = {
     "cxg_pattern_row_ptr": (_P, [_P]),
     "cxg_pattern_col_idx": (_P, [_P]),
     "cxg_pattern_copy": (C.c_int, [_P, _P, _P, _P]),
-    "cxg_assemble_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "cxg_pattern_compact": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64)]),
+    "cxg_assemble_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_internal_force": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "cxg_reduce_free": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "cxg_assemble_j2": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
-                                  _P, _P, _P, _P, _P]),
+                                  _P, _P, _P, _P, _P, _P]),
     "cxg_commit": (C.c_int, [_P, C.c_int64, _P, _P]),
     "cxg_vjp": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_profile_enable": (C.c_int, [_P, C.c_int]),

@@ -440,15 +440,28 @@ cxg_status cxg_pattern_copy(cxg_context* ctx, const cxg_pattern* p, int64_t* row
   });
 }
 
+cxg_status cxg_pattern_compact(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
+                               const double* vals, int64_t* row_ptr_out, int32_t* col_idx_out,
+                               double* vals_out, int64_t* nnz_out) {
+  if (!p || !live || !vals || !row_ptr_out || !col_idx_out || !vals_out || !nnz_out)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad pattern_compact arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    *nnz_out = cxg::compact_pattern(ctx, p, live, vals, row_ptr_out, col_idx_out, vals_out);
+  });
+}
+
 cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                const double* E, const double* nu, double* vals, const double* u,
-                               double* f_int) {
+                               double* f_int, uint8_t* live) {
   if (!p || !coords || !E || !nu || !vals) return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble arguments");
   if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  if (live && ctx->asm_variant != 4)
+    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4)");
   const bool with_f = u && f_int;
   return cxg::guarded(ctx, [&] {
     cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
-                                with_f ? f_int : nullptr);
+                                with_f ? f_int : nullptr, live);
     cxg::check_flags(ctx);
   });
 }
@@ -472,13 +485,16 @@ cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double*
                            const double* E, const double* nu, const int32_t* mat_id,
                            const cxg_j2_material* mats, int32_t n_mats, const double* u,
                            const double* committed, double* trial, double* vals,
-                           double* f_int) {
+                           double* f_int, uint8_t* live) {
   if (!p || !coords || !E || !nu || !u || !committed || !trial || !vals || !f_int ||
       (n_mats > 0 && (!mats || !mat_id)) || n_mats < 0)
     return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_j2 arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  if (live && ctx->asm_variant != 4)
+    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4)");
   return cxg::guarded(ctx, [&] {
     cxg::launch_assemble_j2(ctx, p, coords, E, nu, n_mats > 0 ? mat_id : nullptr, mats, n_mats, u,
-                            committed, trial, vals, f_int);
+                            committed, trial, vals, f_int, live);
     cxg::check_flags(ctx);
   });
 }

@@ -720,7 +720,8 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
 // capacity are accumulated in place in the CSR array (one owner, still race-free).
 // With u given, the same pass forms f_int[a] = sum_e (Ke ue)_a over the full DOF
 // set, as fem::internal_force does (stress.cpp:125-145), with a fixed-order wave
-// reduction.
+// reduction. With fe given instead (element force vectors of the J2 path), f_int[a]
+// is the sum of the incident elements' entries for a, in ascending element order.
 constexpr int kRowCap = 256;
 constexpr int kBatch = 4;  // incident elements whose loads are issued together
 
@@ -735,8 +736,10 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
     const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
-    const double* __restrict__ u, double* __restrict__ vals, double* __restrict__ f_int) {
+    const double* __restrict__ u, const double* __restrict__ fe, double* __restrict__ vals,
+    double* __restrict__ f_int, uint8_t* __restrict__ live) {
   __shared__ double buf[3 * kRowCap];
+  __shared__ uint8_t lbuf[3 * kRowCap];
   const int32_t a = blockIdx.x;
   const int lane = threadIdx.x;
   const int bl = lane / 3, jl = lane - 3 * (lane / 3);
@@ -752,12 +755,22 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   for (int i = 0; i < 3; ++i) {
     if (rs[i] < 0) continue;
     for (int64_t c = lane; c < Lrow; c += kWave) {
-      if (in_lds) buf[i * kRowCap + c] = 0.0;
-      else vals[rs[i] + c] = 0.0;
+      if (in_lds) {
+        buf[i * kRowCap + c] = 0.0;
+        lbuf[i * kRowCap + c] = 0;
+      } else {
+        vals[rs[i] + c] = 0.0;
+        if (live) live[rs[i] + c] = 0;
+      }
     }
   }
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
+  if (fe && lane < 3)  // element force vectors, summed in ascending element order
+    for (int64_t it = i0; it < i1; ++it) {
+      const IncRecord r = rec[it];
+      f0 += fe[3 * (r.conn + r.la) + lane];
+    }
   for (int64_t ib = i0; ib < i1; ib += kBatch) {
     // issue every load of the batch first ...
     double v[kBatch][3], uv[kBatch];
@@ -801,10 +814,20 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
           if (rs[0] >= 0) buf[pos] += v[k][0];
           if (rs[1] >= 0) buf[kRowCap + pos] += v[k][1];
           if (rs[2] >= 0) buf[2 * kRowCap + pos] += v[k][2];
+          if (live) {
+            if (v[k][0] != 0.0) lbuf[pos] = 1;
+            if (v[k][1] != 0.0) lbuf[kRowCap + pos] = 1;
+            if (v[k][2] != 0.0) lbuf[2 * kRowCap + pos] = 1;
+          }
         } else {
           if (rs[0] >= 0) vals[rs[0] + pos] += v[k][0];
           if (rs[1] >= 0) vals[rs[1] + pos] += v[k][1];
           if (rs[2] >= 0) vals[rs[2] + pos] += v[k][2];
+          if (live) {
+            if (rs[0] >= 0 && v[k][0] != 0.0) live[rs[0] + pos] = 1;
+            if (rs[1] >= 0 && v[k][1] != 0.0) live[rs[1] + pos] = 1;
+            if (rs[2] >= 0 && v[k][2] != 0.0) live[rs[2] + pos] = 1;
+          }
         }
       }
     }
@@ -815,6 +838,8 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     for (int i = 0; i < 3; ++i) {
       if (rs[i] < 0) continue;
       for (int64_t c = lane; c < Lrow; c += kWave) vals[rs[i] + c] = buf[i * kRowCap + c];
+      if (live)
+        for (int64_t c = lane; c < Lrow; c += kWave) live[rs[i] + c] = lbuf[i * kRowCap + c];
     }
   if (u) {
     f0 = wave_sum(f0);
@@ -826,6 +851,8 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
       fp[1] = f1;
       fp[2] = f2;
     }
+  } else if (fe && lane < 3) {
+    f_int[3 * int64_t(a) + lane] = f0;
   }
 }
 
@@ -1077,6 +1104,152 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
   }
 }
 
+// ---- K2, two-phase variant: return map, K_t and f_e to the element arenas ------------
+// One element per wave. Per Gauss point (one lane each): strain, radial return
+// (j2_point), algorithmic tangent D_t, trial history to the SoA state. Then the
+// element force f_e = sum_q s B^T sigma (one lane per node) and the pair blocks
+// K_ab = sum_q s B_a^T D_q B_b, with D_q B_b formed for QC Gauss points at a time in
+// LDS and shared by every pair of the wave. K_t is written as full rows through the
+// same staged, coalesced stores as k_elem_iso; f_e (3n per element) to its own arena.
+// The row gather then sums K_t rows and f_e entries per node in element order.
+template <int NN, int NGP, int QC>
+struct ElemJ2Lds {
+  Geo<NN, NGP> G;
+  double u[NN * 3];
+  double D[NGP * 36];
+  double S[NGP * 6];
+  union {
+    double DB[QC * NN * 18];
+    double stage[kStageNodes * 3 * 3 * NN];
+  } w;
+};
+
+template <int NN, int NGP, int QC>
+__global__ __launch_bounds__(kWave) void k_elem_j2(
+    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset, int64_t n_points,
+    const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
+    const J2Dev* __restrict__ mats, const double* __restrict__ u,
+    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
+    double* __restrict__ fe, int* __restrict__ flag) {
+  constexpr int NP = NN * (NN + 1) / 2;
+  constexpr int PPL = (NP + kWave - 1) / kWave;
+  constexpr int ND = 3 * NN;
+  __shared__ ElemJ2Lds<NN, NGP, QC> L;
+  __shared__ int32_t node[NN];
+  const int lane = threadIdx.x;
+  const int64_t le = blockIdx.x;
+  const int64_t ge = elem_offset + le;
+  const int32_t* ce = conn + le * NN;
+  stage_coords<NN, NGP>(L.G, node, ce, coords, lane);
+  stage_vec<NN>(L.u, ce, u, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  gradients<NN, NGP>(L.G, dN, lane);
+  __syncthreads();
+  const int32_t mid = mat_id ? mat_id[ge] : -1;
+  for (int q = lane; q < NGP; q += kWave) {
+    double H[3][3], e[6], sg[6];
+    grad_field<NN, NGP>(L.G, L.u, q, H);
+    voigt_strain(H, e);
+    const int64_t pt = gp_offset + le * NGP + q;
+    double* Dq = L.D + 36 * q;
+    if (mid >= 0) {
+      double hc[13], ht[13];
+#pragma unroll
+      for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
+      j2_point(mats[mid], e, hc, ht, sg, Dq);
+#pragma unroll
+      for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = ht[c];
+    } else {
+      double lam, mu;
+      lame(E[ge], nu[ge], lam, mu);
+      iso_stress(lam, mu, e, sg);
+      for (int i = 0; i < 36; ++i) Dq[i] = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Dq[6 * i + j] = (i == j) ? lam + 2.0 * mu : lam;
+      for (int i = 3; i < 6; ++i) Dq[7 * i] = mu;
+#pragma unroll
+      for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = committed[c * n_points + pt];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = L.G.sc[q] * sg[i];
+  }
+  __syncthreads();
+  if (lane < NN) {
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      const double* g = L.G.g + (q * NN + lane) * 3;
+      const double* sq = L.S + 6 * q;
+      f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
+      f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
+      f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
+    }
+    double* fp = fe + le * ND + 3 * lane;
+    fp[0] = f0;
+    fp[1] = f1;
+    fp[2] = f2;
+  }
+  int pa[PPL], pb[PPL];
+  double K[PPL][9];
+#pragma unroll
+  for (int t = 0; t < PPL; ++t) {
+    const int p = lane * PPL + t;
+    if (p < NP) {
+      decode_pair<NN, true>(p, pa[t], pb[t]);
+    } else {
+      pa[t] = -1;
+      pb[t] = -1;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
+  }
+  for (int q0 = 0; q0 < NGP; q0 += QC) {
+    for (int idx = lane; idx < QC * NN; idx += kWave) {
+      const int q = q0 + idx / NN;
+      if (q < NGP) d_times_b(L.D + 36 * q, L.G.g + 3 * (q * NN + idx % NN), L.w.DB + 18 * idx);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      if (pa[t] < 0) continue;
+      for (int qq = 0; qq < QC && q0 + qq < NGP; ++qq) {
+        const int q = q0 + qq;
+        bt_db(L.G.g + 3 * (q * NN + pa[t]), L.w.DB + 18 * (qq * NN + pb[t]), L.G.sc[q], K[t]);
+      }
+    }
+    __syncthreads();
+  }
+  double* kel = kp + le * (ND * ND);
+  for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      if (pa[t] < 0) continue;
+      if (pa[t] >= g0 && pa[t] < g0 + kStageNodes) {
+        double* ra = L.w.stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[t][3 * i + j];
+      }
+      if (pb[t] != pa[t] && pb[t] >= g0 && pb[t] < g0 + kStageNodes) {
+        double* rb = L.w.stage + (3 * (pb[t] - g0)) * ND + 3 * pa[t];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) rb[j * ND + i] = K[t][3 * i + j];
+      }
+    }
+    __syncthreads();
+    const int nrow = 3 * (NN - g0 < kStageNodes ? NN - g0 : kStageNodes);
+    double* dst = kel + int64_t(3 * g0) * ND;
+    for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = L.w.stage[idx];
+    __syncthreads();
+  }
+}
+
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -1250,7 +1423,7 @@ void fill_f64(hipStream_t s, double* p, int64_t n, double v) {
 
 void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             const double* E, const double* nu, double* vals, const double* u,
-                            double* f_int) {
+                            double* f_int, uint8_t* live) {
   const cxg_mesh* m = p->mesh;
   if (ctx->asm_variant == 4) {
     // Two-phase: element pair blocks to the arena, then one owner wave per node row.
@@ -1275,8 +1448,8 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
       LaunchTimer timer(ctx, "row_gather", m->n_elem);
       hipLaunchKernelGGL(k_row_gather, dim3(unsigned(m->n_nodes)), dim3(kWave), 0, ctx->stream,
                          m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,
-                         p->dof_eq.p, p->row_ptr.p, pm->ke_arena.p, f_int ? u : nullptr, vals,
-                         f_int);
+                         p->dof_eq.p, p->row_ptr.p, pm->ke_arena.p, f_int ? u : nullptr,
+                         nullptr, vals, f_int, live);
       CXG_HIP(hipGetLastError());
     }
     return;
@@ -1356,7 +1529,8 @@ static J2Dev prepare_j2(const cxg_j2_material& p) {
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
                         const cxg_j2_material* mats, int32_t n_mats, const double* u,
-                        const double* committed, double* trial, double* vals, double* f_int) {
+                        const double* committed, double* trial, double* vals, double* f_int,
+                        uint8_t* live) {
   // The device copy of the material table is re-uploaded only when it changes, so
   // the steady state of a Newton loop enqueues kernels only.
   std::vector<J2Dev> h(size_t(n_mats > 0 ? n_mats : 0));
@@ -1373,6 +1547,38 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
   }
   const J2Dev* dmats = bytes ? reinterpret_cast<const J2Dev*>(ctx->j2_dev.p) : nullptr;
   const cxg_mesh* m = p->mesh;
+  if (ctx->asm_variant == 4) {
+    // Two-phase: K_t and f_e of every element to the arenas, then the row gather.
+    cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+    if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
+    const size_t fe_n = size_t(3 * m->conn.n);
+    if (pm->fe_arena.n < fe_n) pm->fe_arena.alloc(fe_n);
+    for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+      const cxg_block_dev& B = m->blocks[bi];
+      if (B.n_elem == 0) continue;
+      const ElemTable& T = ctx->tables[B.type];
+      const int32_t* conn = m->conn.p + B.conn_offset;
+      double* kp = pm->ke_arena.p + p->kp_base[bi];
+      double* fe = pm->fe_arena.p + 3 * B.conn_offset;
+      LaunchTimer timer(ctx, std::string("elem_j2_") + type_name(B.type), B.n_elem);
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,        \
+                     ctx->stream, conn, B.elem_offset, B.gp_offset, m->n_points, T.dN, T.w,     \
+                     coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe, ctx->flag_dev)
+      CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+      CXG_HIP(hipGetLastError());
+    }
+    if (m->n_nodes > 0) {
+      LaunchTimer timer(ctx, "row_gather_j2", m->n_elem);
+      hipLaunchKernelGGL(k_row_gather, dim3(unsigned(m->n_nodes)), dim3(kWave), 0, ctx->stream,
+                         m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,
+                         p->dof_eq.p, p->row_ptr.p, pm->ke_arena.p, nullptr, pm->fe_arena.p,
+                         vals, f_int, live);
+      CXG_HIP(hipGetLastError());
+    }
+    return;
+  }
   fill_f64(ctx->stream, vals, p->nnz, 0.0);
   fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
   for_each_colour_block(ctx, "assemble_j2", m, [&](const LaunchSet& S) {

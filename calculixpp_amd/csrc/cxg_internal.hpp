@@ -154,7 +154,8 @@ struct IncRecord {
   int64_t rows;   // arena offset of row 3*la of the element matrix
   int64_t slot;   // offset of slot row la of the element
   int64_t conn;   // offset of the element in the mesh connectivity
-  int64_t n;      // nodes of the element
+  int32_t n;      // nodes of the element
+  int32_t la;     // local index of the node inside the element
 };
 }  // namespace cxg
 
@@ -175,6 +176,7 @@ struct cxg_pattern {
   // Two-phase assembly arena, allocated on first use: each element's full Ke
   // (3n x 3n, row-major) at kpptr[e].
   cxg::DevBuf<double> ke_arena;
+  cxg::DevBuf<double> fe_arena;               // element force vectors (3n at 3*eptr[e])
   cxg::DevBuf<int64_t> kpptr;                 // n_elem+1, in doubles
   cxg::DevBuf<cxg::IncRecord> inc_rec;        // per mesh incidence entry (inc_elem order)
   cxg::DevBuf<int8_t> nfm;                    // per node: bit c set if DOF c is free
@@ -187,7 +189,7 @@ namespace cxg {
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             const double* E, const double* nu, double* vals, const double* u,
-                            double* f_int);
+                            double* f_int, uint8_t* live);
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
                            const double* E, const double* nu, const double* u, double* f_int);
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
@@ -195,13 +197,18 @@ void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
                         const cxg_j2_material* mats, int32_t n_mats, const double* u,
-                        const double* committed, double* trial, double* vals, double* f_int);
+                        const double* committed, double* trial, double* vals, double* f_int,
+                        uint8_t* live);
 void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial);
 void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
                 const double* nu, const double* u, const double* w, double* dx, double* dE,
                 double* du);
 void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq, int32_t n_free,
                    cxg_pattern* out);
+// Keeps the entries with live != 0; returns the kept count (synchronizes the stream).
+int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
+                        const double* vals, int64_t* row_ptr_out, int32_t* col_idx_out,
+                        double* vals_out);
 // Read back and clear the device status word; throws on a raised flag.
 void check_flags(cxg_context* ctx);
 }  // namespace cxg

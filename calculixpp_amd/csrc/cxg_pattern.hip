@@ -189,11 +189,68 @@ __global__ void k_inc_records(int32_t n_nodes, const int64_t* __restrict__ inc_p
     const int n = int(eptr[e + 1] - b);
     int la = 0;
     while (la < n && conn[b + la] != a) ++la;
-    rec[it] = IncRecord{kpptr[e] + int64_t(3 * la) * (3 * n), sptr[e] + int64_t(la) * n, b, n};
+    rec[it] = IncRecord{kpptr[e] + int64_t(3 * la) * (3 * n), sptr[e] + int64_t(la) * n, b, n, la};
   }
 }
 
+// Entries kept per row (live != 0); one thread per row.
+__global__ void k_live_count(int32_t n_free, const int64_t* __restrict__ row_ptr,
+                             const uint8_t* __restrict__ live, int64_t* __restrict__ count) {
+  const int32_t r = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n_free) return;
+  int64_t c = 0;
+  for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) c += live[k] ? 1 : 0;
+  count[r] = c;
+}
+
+// Copies the kept entries of each row, in column order; one thread per row.
+__global__ void k_compact(int32_t n_free, const int64_t* __restrict__ row_ptr,
+                          const int32_t* __restrict__ col_idx, const double* __restrict__ vals,
+                          const uint8_t* __restrict__ live, const int64_t* __restrict__ row_out,
+                          int32_t* __restrict__ col_out, double* __restrict__ vals_out) {
+  const int32_t r = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n_free) return;
+  int64_t o = row_out[r];
+  for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+    if (live[k]) {
+      col_out[o] = col_idx[k];
+      vals_out[o] = vals[k];
+      ++o;
+    }
+}
+
 }  // namespace
+
+// The reference's key set: scatter_block (src/fem/assembly.cpp:90) skips element
+// contributions equal to 0.0, so an entry all of whose contributions are 0.0 never
+// enters its COO map; the live flags of the gather mark the others.
+int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
+                        const double* vals, int64_t* row_ptr_out, int32_t* col_idx_out,
+                        double* vals_out) {
+  hipStream_t s = ctx->stream;
+  const int32_t n = p->n_free;
+  std::vector<int64_t> hrow(size_t(n) + 1, 0);
+  if (n > 0) {
+    DevBuf<int64_t> count(static_cast<size_t>(n));
+    const unsigned blocks = unsigned((n + 255) / 256);
+    hipLaunchKernelGGL(k_live_count, dim3(blocks), dim3(256), 0, s, n, p->row_ptr.p, live,
+                       count.p);
+    CXG_HIP(hipGetLastError());
+    CXG_HIP(hipMemcpyAsync(hrow.data() + 1, count.p, size_t(n) * sizeof(int64_t),
+                           hipMemcpyDeviceToHost, s));
+    CXG_HIP(hipStreamSynchronize(s));
+    for (size_t r = 0; r < size_t(n); ++r) hrow[r + 1] += hrow[r];
+    CXG_HIP(hipMemcpyAsync(row_ptr_out, hrow.data(), hrow.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(256), 0, s, n, p->row_ptr.p, p->col_idx.p,
+                       vals, live, row_ptr_out, col_idx_out, vals_out);
+    CXG_HIP(hipGetLastError());
+  } else {
+    CXG_HIP(hipMemcpyAsync(row_ptr_out, hrow.data(), sizeof(int64_t), hipMemcpyHostToDevice, s));
+  }
+  CXG_HIP(hipStreamSynchronize(s));  // hrow and count may be released afterwards
+  return hrow.back();
+}
 
 void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_dev,
                    int32_t n_free, cxg_pattern* P) {

@@ -54,6 +54,7 @@ class Context:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(None, self.lib.cxg_create(self.device.index or 0, C.c_void_p(stream), C.byref(h)))
         self.h = h
+        self.options = {"asm_variant": 4, "patch_size": 8}
 
     def sync_stream(self):
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -65,6 +66,12 @@ class Context:
     def set_option(self, key: str, value: int) -> None:
         """cxg_set_option: "patch_size" (meshes created afterwards), "asm_variant"."""
         self.check(self.lib.cxg_set_option(self.h, key.encode(), int(value)))
+        self.options[key] = int(value)
+
+    @property
+    def tracks_live(self) -> bool:
+        """Whether assemblies can report entry liveness (the two-phase kernels)."""
+        return self.options.get("asm_variant", 4) == 4
 
     def profile(self, enable: bool) -> None:
         """Bracket every library kernel launch with HIP events on its stream."""
@@ -217,13 +224,18 @@ class Assembler:
     def new_values(self) -> torch.Tensor:
         return torch.empty(max(self.pattern.nnz, 1), dtype=torch.float64, device=self.ctx.device)
 
+    def new_live(self) -> torch.Tensor:
+        """Per-entry flags: 1 where some element contributed a value other than 0.0."""
+        return torch.empty(max(self.pattern.nnz, 1), dtype=torch.uint8, device=self.ctx.device)
+
     def assemble(self, vals: torch.Tensor, u: Optional[torch.Tensor] = None,
                  f_int: Optional[torch.Tensor] = None, coords: Optional[torch.Tensor] = None,
-                 E: Optional[torch.Tensor] = None) -> None:
+                 E: Optional[torch.Tensor] = None, live: Optional[torch.Tensor] = None) -> None:
         m = self.mesh
         self.ctx.check(self.ctx.lib.cxg_assemble_linear(
             self.ctx.h, self.pattern.h, _ptr(coords if coords is not None else m.coords),
-            _ptr(E if E is not None else m.E), _ptr(m.nu), _ptr(vals), _ptr(u), _ptr(f_int)))
+            _ptr(E if E is not None else m.E), _ptr(m.nu), _ptr(vals), _ptr(u), _ptr(f_int),
+            _ptr(live)))
 
     def rhs(self) -> torch.Tensor:
         """Reduced load: CLOADs through the transform minus the prescribed columns."""
@@ -242,9 +254,28 @@ class Assembler:
                                                     _ptr(f_c), _ptr(rhs)))
         return rhs[: self.n_free]
 
-    def system(self, vals: torch.Tensor) -> LinearSystem:
-        return LinearSystem(self.n_free, self.pattern.row_ptr, self.pattern.col_idx,
-                            vals[: self.pattern.nnz], self.rhs(), self.dof_eq, self.prescribed)
+    def compact(self, vals: torch.Tensor, live: torch.Tensor):
+        """(row_ptr, col_idx, vals) holding only the live entries: the key set of the
+        reference's COO map (scatter_block skips 0.0 contributions, assembly.cpp:90)."""
+        dev = self.ctx.device
+        nnz = max(self.pattern.nnz, 1)
+        rp = torch.empty(self.n_free + 1, dtype=torch.int64, device=dev)
+        ci = torch.empty(nnz, dtype=torch.int32, device=dev)
+        va = torch.empty(nnz, dtype=torch.float64, device=dev)
+        kept = C.c_int64()
+        self.ctx.check(self.ctx.lib.cxg_pattern_compact(self.ctx.h, self.pattern.h, _ptr(live),
+                                                        _ptr(vals), _ptr(rp), _ptr(ci), _ptr(va),
+                                                        C.byref(kept)))
+        return rp, ci[: kept.value], va[: kept.value]
+
+    def system(self, vals: torch.Tensor, live: Optional[torch.Tensor] = None) -> LinearSystem:
+        """The assembled system: on the structural pattern, or with live flags on the
+        reference's key set."""
+        if live is None:
+            rp, ci, va = self.pattern.row_ptr, self.pattern.col_idx, vals[: self.pattern.nnz]
+        else:
+            rp, ci, va = self.compact(vals, live)
+        return LinearSystem(self.n_free, rp, ci, va, self.rhs(), self.dof_eq, self.prescribed)
 
 
 def _only_active(model: Model) -> Model:
@@ -265,8 +296,9 @@ def assemble_linear_static(model: Model, device=None) -> LinearSystem:
     """fem::assemble_linear_static (fem/assembly.cpp:225-257) on the GPU."""
     a = Assembler(model, context(device))
     vals = a.new_values()
-    a.assemble(vals)
-    return a.system(vals)
+    live = a.new_live() if a.ctx.tracks_live else None
+    a.assemble(vals, live=live)
+    return a.system(vals, live)
 
 
 def internal_force(model: Model, u, device=None) -> torch.Tensor:
@@ -334,12 +366,12 @@ class TangentAssembler(Assembler):
         self.mat_id = torch.from_numpy(jid).to(dev)
 
     def assemble_tangent(self, u: torch.Tensor, mp: MaterialPoints, vals: torch.Tensor,
-                         f_int: torch.Tensor) -> None:
+                         f_int: torch.Tensor, live: Optional[torch.Tensor] = None) -> None:
         m = self.mesh
         self.ctx.check(self.ctx.lib.cxg_assemble_j2(
             self.ctx.h, self.pattern.h, _ptr(m.coords), _ptr(m.E), _ptr(m.nu), _ptr(self.mat_id),
             self.mats, self.n_mats, _ptr(u), _ptr(mp.committed), _ptr(mp.trial), _ptr(vals),
-            _ptr(f_int)))
+            _ptr(f_int), _ptr(live)))
 
 
 def assemble_material_tangent(model: Model, u, mp: MaterialPoints, device=None):
@@ -349,6 +381,7 @@ def assemble_material_tangent(model: Model, u, mp: MaterialPoints, device=None):
     u = torch.as_tensor(np.asarray(u, np.float64).reshape(-1, 3) if not torch.is_tensor(u) else u)
     u = u.to(a.ctx.device, torch.float64).contiguous()
     vals = a.new_values()
+    live = a.new_live() if a.ctx.tracks_live else None
     f_int = torch.empty(3 * model.n_nodes, dtype=torch.float64, device=a.ctx.device)
-    a.assemble_tangent(u, mp, vals, f_int)
-    return a.system(vals), f_int
+    a.assemble_tangent(u, mp, vals, f_int, live)
+    return a.system(vals, live), f_int

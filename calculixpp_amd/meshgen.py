@@ -121,13 +121,21 @@ def block(n: int = 64, family: int = _lib.C3D20, seed_geo: int = 0, seed_E: int 
     return Model(coords, [(family, conn)], E, 0.3)
 
 
-def j2_block(nx: int = 128, ny: int = 128, nz: int = 64, lengths=(2.0, 2.0, 1.0)):
+def j2_block(nx: int = 128, ny: int = 128, nz: int = 64, lengths=(2.0, 2.0, 1.0),
+             family: int = _lib.C3D8, elastic_every: int = 0, jitter_frac: float = 0.0,
+             seed_geo: int = 9):
     """Config 3 mesh + material: C3D8, E=210000, nu=0.3, sigma_y0 = 250, H = 2000
-    (table {(0, 250), (1, 2250)})."""
-    coords, conn = hex_grid(nx, ny, nz, lengths, _lib.C3D8)
+    (table {(0, 250), (1, 2250)}). Other families on the same grid for parity runs;
+    elastic_every = k > 0 leaves every k-th element isotropic elastic (no J2);
+    jitter_frac > 0 moves interior nodes by up to that fraction of the x spacing."""
+    coords, conn = hex_grid(nx, ny, nz, lengths, family)
+    if jitter_frac > 0.0:
+        coords = jitter(coords, lengths[0] / nx, jitter_frac, seed_geo, lengths)
     mat = J2Params(210000.0, 0.3, [0.0, 1.0], [250.0, 2250.0])
-    m = Model(coords, [(_lib.C3D8, conn)], 210000.0, 0.3,
-              j2_id=np.zeros(conn.shape[0], np.int32), j2=[mat])
+    jid = np.zeros(conn.shape[0], np.int32)
+    if elastic_every > 0:
+        jid[::elastic_every] = -1
+    m = Model(coords, [(family, conn)], 210000.0, 0.3, j2_id=jid, j2=[mat])
     return m
 
 

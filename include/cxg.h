@@ -132,16 +132,30 @@ const int32_t* cxg_pattern_col_idx(const cxg_pattern* p);
 cxg_status cxg_pattern_copy(cxg_context* ctx, const cxg_pattern* p, int64_t* row_ptr,
                             int32_t* col_idx);
 
+/* The reference's key set from a structural system: keeps the entries with live[k] != 0
+ * (see cxg_assemble_linear), in CSR order. Replaces the COO -> CSR step of the
+ * reference (scipp::sparse::CsrMatrix::from_coo over the map scatter_block fills,
+ * fem/assembly.cpp:76-100). Device outputs row_ptr_out[n_free+1], col_idx_out and
+ * vals_out with room for the structural nnz; *nnz_out (host) receives the kept count.
+ * Synchronizes the context's stream. */
+cxg_status cxg_pattern_compact(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
+                               const double* vals, int64_t* row_ptr_out,
+                               int32_t* col_idx_out, double* vals_out, int64_t* nnz_out);
+
 /* ---- K1 + K4 (+K5): linear-elastic stiffness into CSR values -----------------
  * Replaces the element loop of assemble_linear_static (fem/assembly.cpp:237-244):
  * element_stiffness (fem/element.cpp:614) per element and scatter_tangent /
  * scatter_block into the free/free matrix. vals[nnz] is overwritten with K_ff.
  * Optional: when u and f_int are both non-NULL, f_int[3*n_nodes] is overwritten
  * with the internal force sum_e Ke ue over the full DOF set (fem/stress.cpp:
- * 125-145) in the same pass. coords: n_nodes*3; E, nu: per model element. */
+ * 125-145) in the same pass. coords: n_nodes*3; E, nu: per model element.
+ * Optional live[nnz]: set to 1 where at least one element contributed a value other
+ * than 0.0, else 0. The reference leaves entries whose every contribution is 0.0 out
+ * of its COO map (scatter_block, fem/assembly.cpp:90); cxg_pattern_compact applies
+ * that rule. Needs the two-phase kernels (option "asm_variant" 4, the default). */
 cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                const double* E, const double* nu, double* vals,
-                               const double* u, double* f_int);
+                               const double* u, double* f_int, uint8_t* live);
 
 /* ---- K5: matrix-free internal force f_int = sum_e Ke(x,E) ue ----------------
  * fem::internal_force (fem/stress.cpp:125-145) without forming Ke: per Gauss point
@@ -167,12 +181,12 @@ cxg_status cxg_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof
  * mat_id[n_elem]: index into mats (host array of n_mats), -1 = linear elastic
  * with E/nu. committed, trial: SoA state, 13 planes of n_points (see above);
  * committed is read, trial is overwritten. vals[nnz] <- K_t, f_int[3*n_nodes]
- * <- internal force (full DOF set). */
+ * <- internal force (full DOF set). live: optional, as for cxg_assemble_linear. */
 cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                            const double* E, const double* nu, const int32_t* mat_id,
                            const cxg_j2_material* mats, int32_t n_mats, const double* u,
                            const double* committed, double* trial, double* vals,
-                           double* f_int);
+                           double* f_int, uint8_t* live);
 
 /* MaterialPoints::commit (fem/assembly.hpp:84-87): committed <- trial. */
 cxg_status cxg_commit(cxg_context* ctx, int64_t n_points, double* committed,

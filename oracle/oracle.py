@@ -273,6 +273,28 @@ class Oracle:
             return t1 - t0
         return self._with_model(model, run)
 
+    def time_material_tangent(self, model, u, state):
+        """Seconds spent in assemble_material_tangent alone (state: (n_points, 26) AoS,
+        copied first; model construction and result copies outside the clock)."""
+        import time
+        st = np.ascontiguousarray(state, np.float64).copy()
+        uu = np.ascontiguousarray(u, np.float64).reshape(-1)
+        fi = np.zeros(3 * model.n_nodes)
+
+        def run(m, arrays):
+            h = C.c_void_p()
+            f = self._f("assemble_material_tangent")
+            f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
+                          C.c_char_p, C.c_int]
+            t0 = time.perf_counter()
+            status = f(m, _p(uu), _p(st), _p(fi), C.byref(h), self._err, 512)
+            t1 = time.perf_counter()
+            if h.value:
+                self._f("system_free")(h)
+            self._check(status)
+            return t1 - t0
+        return self._with_model(model, run)
+
     def vjp(self, model, u, w):
         """(dx (n,3), dE (n_elem,), du (3n,), r (3n,)) of r = K(x,E)u, port only."""
         if self.kind != "port":
