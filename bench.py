@@ -133,6 +133,16 @@ def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3 / reps
     ctx = a.ctx
+    g = a.capture(u, mp, vals, f_int)  # the same iteration as one HIP graph
+    g.launch()
+    ctx.check_kernels()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    ctx.check_kernels()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+    graph_nodes = g.nodes
+    del g
     per = []
     kern = {}
     for _ in range(reps):
@@ -148,6 +158,7 @@ def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
                        "(return map + K_t + f_int)",
            "elements": m.n_elem, "gauss_points": m.n_points, "nnz": a.pattern.nnz,
            "amplitude": amp, "yield_fraction": frac, "ms_per_iteration": wall_ms,
+           "graph_ms_per_iteration": graph_ms, "graph_nodes": graph_nodes,
            "kernel_ms": statistics.median(per),
            "kernels": {k: round(statistics.median(v), 4) for k, v in kern.items()},
            "elements_per_s": m.n_elem / (wall_ms * 1e-3)}

@@ -77,6 +77,13 @@ _SIGS = {
                                   _P, _P, _P, _P, _P, _P]),
     "cxg_commit": (C.c_int, [_P, C.c_int64, _P, _P]),
     "cxg_vjp": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "cxg_graph_capture_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "cxg_graph_capture_j2": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
+                                       _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "cxg_graph_launch": (C.c_int, [_P, _P]),
+    "cxg_graph_info": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "cxg_graph_destroy": (None, [_P]),
+    "cxg_check": (C.c_int, [_P]),
     "cxg_profile_enable": (C.c_int, [_P, C.c_int]),
     "cxg_profile_reset": (C.c_int, [_P]),
     "cxg_profile_count": (C.c_int32, [_P]),

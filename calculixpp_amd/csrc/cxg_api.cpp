@@ -236,27 +236,6 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
   CXG_HIP(hipStreamSynchronize(s));
 }
 
-cxg_status fail(cxg_context* ctx, cxg_status st, const char* msg) {
-  if (ctx) ctx->err = msg;
-  return st;
-}
-
-template <class F>
-cxg_status guarded(cxg_context* ctx, F&& f) {
-  if (!ctx) return CXG_ERR_ARG;
-  try {
-    CXG_HIP(hipSetDevice(ctx->device));
-    f();
-    return CXG_OK;
-  } catch (const Error& e) {
-    return fail(ctx, e.status, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(ctx, CXG_ERR_HIP, "host allocation failed");
-  } catch (const std::exception& e) {
-    return fail(ctx, CXG_ERR_HIP, e.what());
-  }
-}
-
 }  // namespace
 }  // namespace cxg
 
@@ -493,7 +472,8 @@ cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double*
   if (live && ctx->asm_variant != 4)
     return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4)");
   return cxg::guarded(ctx, [&] {
-    cxg::launch_assemble_j2(ctx, p, coords, E, nu, n_mats > 0 ? mat_id : nullptr, mats, n_mats, u,
+    const void* table = cxg::upload_j2_table(ctx, mats, n_mats);
+    cxg::launch_assemble_j2(ctx, p, coords, E, nu, n_mats > 0 ? mat_id : nullptr, table, u,
                             committed, trial, vals, f_int, live);
     cxg::check_flags(ctx);
   });

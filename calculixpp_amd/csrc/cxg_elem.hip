@@ -1428,7 +1428,7 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
   if (ctx->asm_variant == 4) {
     // Two-phase: element pair blocks to the arena, then one owner wave per node row.
     cxg_pattern* pm = const_cast<cxg_pattern*>(p);
-    if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
+    prepare_arenas(p, false);
     for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
@@ -1526,33 +1526,45 @@ static J2Dev prepare_j2(const cxg_j2_material& p) {
   return d;
 }
 
-void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
-                        const double* E, const double* nu, const int32_t* mat_id,
-                        const cxg_j2_material* mats, int32_t n_mats, const double* u,
-                        const double* committed, double* trial, double* vals, double* f_int,
-                        uint8_t* live) {
-  // The device copy of the material table is re-uploaded only when it changes, so
-  // the steady state of a Newton loop enqueues kernels only.
+std::vector<unsigned char> j2_table_bytes(const cxg_j2_material* mats, int32_t n_mats) {
   std::vector<J2Dev> h(size_t(n_mats > 0 ? n_mats : 0));
   for (int32_t i = 0; i < n_mats; ++i) h[size_t(i)] = prepare_j2(mats[i]);
-  const size_t bytes = h.size() * sizeof(J2Dev);
-  const bool same = bytes == ctx->j2_host.size() && ctx->j2_dev.p &&
-                    std::memcmp(h.data(), ctx->j2_host.data(), bytes) == 0;
-  if (!same && bytes) {
-    ctx->j2_dev.alloc(bytes);
-    CXG_HIP(hipStreamSynchronize(ctx->stream));
-    CXG_HIP(hipMemcpy(ctx->j2_dev.p, h.data(), bytes, hipMemcpyHostToDevice));
-    ctx->j2_host.assign(reinterpret_cast<const unsigned char*>(h.data()),
-                        reinterpret_cast<const unsigned char*>(h.data()) + bytes);
+  const auto* b = reinterpret_cast<const unsigned char*>(h.data());
+  return std::vector<unsigned char>(b, b + h.size() * sizeof(J2Dev));
+}
+
+const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32_t n_mats) {
+  // The device copy of the material table is re-uploaded only when it changes, so
+  // the steady state of a Newton loop enqueues kernels only.
+  const std::vector<unsigned char> h = j2_table_bytes(mats, n_mats);
+  const bool same = h.size() == ctx->j2_host.size() && ctx->j2_dev.p &&
+                    std::memcmp(h.data(), ctx->j2_host.data(), h.size()) == 0;
+  if (!same && !h.empty()) {
+    CXG_HIP(hipStreamSynchronize(ctx->stream));  // earlier kernels may still read the old table
+    ctx->j2_dev.alloc(h.size());
+    CXG_HIP(hipMemcpy(ctx->j2_dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
+    ctx->j2_host = h;
   }
-  const J2Dev* dmats = bytes ? reinterpret_cast<const J2Dev*>(ctx->j2_dev.p) : nullptr;
+  return h.empty() ? nullptr : ctx->j2_dev.p;
+}
+
+void prepare_arenas(const cxg_pattern* p, bool element_forces) {
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
+  const size_t fe_n = size_t(3 * p->mesh->conn.n);
+  if (element_forces && pm->fe_arena.n < fe_n) pm->fe_arena.alloc(fe_n);
+}
+
+void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                        const double* E, const double* nu, const int32_t* mat_id,
+                        const void* j2_table, const double* u, const double* committed,
+                        double* trial, double* vals, double* f_int, uint8_t* live) {
+  const J2Dev* dmats = static_cast<const J2Dev*>(j2_table);
   const cxg_mesh* m = p->mesh;
   if (ctx->asm_variant == 4) {
     // Two-phase: K_t and f_e of every element to the arenas, then the row gather.
     cxg_pattern* pm = const_cast<cxg_pattern*>(p);
-    if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
-    const size_t fe_n = size_t(3 * m->conn.n);
-    if (pm->fe_arena.n < fe_n) pm->fe_arena.alloc(fe_n);
+    prepare_arenas(p, true);
     for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;

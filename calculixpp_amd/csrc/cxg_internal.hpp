@@ -194,11 +194,15 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
                            const double* E, const double* nu, const double* u, double* f_int);
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
                         const double* f_ext, const double* f_c, double* rhs);
+// J2 material tables: host bytes of the device form, and the context's cached upload.
+std::vector<unsigned char> j2_table_bytes(const cxg_j2_material* mats, int32_t n_mats);
+const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32_t n_mats);
+// Allocates the two-phase arenas of a pattern ahead of use (no-op when present).
+void prepare_arenas(const cxg_pattern* p, bool element_forces);
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
-                        const cxg_j2_material* mats, int32_t n_mats, const double* u,
-                        const double* committed, double* trial, double* vals, double* f_int,
-                        uint8_t* live);
+                        const void* j2_table, const double* u, const double* committed,
+                        double* trial, double* vals, double* f_int, uint8_t* live);
 void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial);
 void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
                 const double* nu, const double* u, const double* w, double* dx, double* dE,
@@ -211,4 +215,27 @@ int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* l
                         double* vals_out);
 // Read back and clear the device status word; throws on a raised flag.
 void check_flags(cxg_context* ctx);
+
+// Status + last-error text for the C ABI.
+inline cxg_status fail(cxg_context* ctx, cxg_status st, const char* msg) {
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+// Runs f on the context's device; exceptions become status codes at the ABI.
+template <class F>
+cxg_status guarded(cxg_context* ctx, F&& f) {
+  if (!ctx) return CXG_ERR_ARG;
+  try {
+    CXG_HIP(hipSetDevice(ctx->device));
+    f();
+    return CXG_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(ctx, CXG_ERR_HIP, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ctx, CXG_ERR_HIP, e.what());
+  }
+}
 }  // namespace cxg

@@ -63,6 +63,10 @@ class Context:
     def check(self, status: int):
         _lib.check(self.h, status)
 
+    def check_kernels(self) -> None:
+        """cxg_check: synchronize and raise kernel status flags (after graph replays)."""
+        self.check(self.lib.cxg_check(self.h))
+
     def set_option(self, key: str, value: int) -> None:
         """cxg_set_option: "patch_size" (meshes created afterwards), "asm_variant"."""
         self.check(self.lib.cxg_set_option(self.h, key.encode(), int(value)))
@@ -237,6 +241,16 @@ class Assembler:
             _ptr(E if E is not None else m.E), _ptr(m.nu), _ptr(vals), _ptr(u), _ptr(f_int),
             _ptr(live)))
 
+    def capture(self, vals: torch.Tensor, u: Optional[torch.Tensor] = None,
+                f_int: Optional[torch.Tensor] = None) -> Graph:
+        """One assembly (as assemble with these buffers) captured as a HIP graph."""
+        m = self.mesh
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.cxg_graph_capture_linear(
+            self.ctx.h, self.pattern.h, _ptr(m.coords), _ptr(m.E), _ptr(m.nu), _ptr(vals),
+            _ptr(u), _ptr(f_int), C.byref(h)))
+        return Graph(self.ctx, h, (self, vals, u, f_int))
+
     def rhs(self) -> torch.Tensor:
         """Reduced load: CLOADs through the transform minus the prescribed columns."""
         dev = self.ctx.device
@@ -276,6 +290,26 @@ class Assembler:
         else:
             rp, ci, va = self.compact(vals, live)
         return LinearSystem(self.n_free, rp, ci, va, self.rhs(), self.dof_eq, self.prescribed)
+
+
+class Graph:
+    """A captured assembly / Newton iteration (cxg_graph_*). Holds the tensors whose
+    addresses it captured; update their contents in place between launches."""
+
+    def __init__(self, ctx: Context, h, keep):
+        self.ctx, self.h, self._keep = ctx, h, keep
+        n = C.c_int64()
+        ctx.lib.cxg_graph_info(h, C.byref(n))
+        self.nodes = n.value
+
+    def launch(self) -> None:
+        self.ctx.check(self.ctx.lib.cxg_graph_launch(self.ctx.h, self.h))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self.ctx.lib.cxg_graph_destroy(h)
+            self.h = None
 
 
 def _only_active(model: Model) -> Model:
@@ -372,6 +406,19 @@ class TangentAssembler(Assembler):
             self.ctx.h, self.pattern.h, _ptr(m.coords), _ptr(m.E), _ptr(m.nu), _ptr(self.mat_id),
             self.mats, self.n_mats, _ptr(u), _ptr(mp.committed), _ptr(mp.trial), _ptr(vals),
             _ptr(f_int), _ptr(live)))
+
+    def capture(self, u: torch.Tensor, mp: MaterialPoints, vals: torch.Tensor,
+                f_int: torch.Tensor, f_ext: Optional[torch.Tensor] = None,
+                rhs: Optional[torch.Tensor] = None) -> Graph:
+        """One Newton iteration (return map + K_t + f_int, and rhs = f_ext - f_int on
+        the free DOFs when rhs is given) captured as a HIP graph."""
+        m = self.mesh
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.cxg_graph_capture_j2(
+            self.ctx.h, self.pattern.h, _ptr(m.coords), _ptr(m.E), _ptr(m.nu), _ptr(self.mat_id),
+            self.mats, self.n_mats, _ptr(u), _ptr(mp.committed), _ptr(mp.trial), _ptr(vals),
+            _ptr(f_int), _ptr(f_ext), _ptr(rhs), C.byref(h)))
+        return Graph(self.ctx, h, (self, u, mp.committed, mp.trial, vals, f_int, f_ext, rhs))
 
 
 def assemble_material_tangent(model: Model, u, mp: MaterialPoints, device=None):

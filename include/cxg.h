@@ -59,6 +59,7 @@ enum { CXG_HARD_ISOTROPIC = 0, CXG_HARD_KINEMATIC = 1, CXG_HARD_COMBINED = 2 };
 typedef struct cxg_context cxg_context;
 typedef struct cxg_mesh cxg_mesh;
 typedef struct cxg_pattern cxg_pattern;
+typedef struct cxg_graph cxg_graph;
 
 /* A run of elements of one type. Blocks of a mesh cover consecutive element
  * numbers: block k holds elements elem_offset .. elem_offset+n_elem-1 of the
@@ -202,11 +203,38 @@ cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
                    const double* E, const double* nu, const double* u, const double* w,
                    double* dx, double* dE, double* du);
 
+/* ---- HIP graphs: one assembly / one Newton iteration, replayed ---------------
+ * The reference re-assembles K_t, f_int and the residual at every Newton iteration
+ * of solve_nonlinear through assemble_material_tangent (fem/assembly.cpp:385-423)
+ * with the same mesh and buffers. cxg_graph_capture_j2 captures that iteration (the
+ * arguments of cxg_assemble_j2; with rhs non-NULL also rhs[n_free] = f_ext - f_int
+ * on the free DOFs, f_ext may be NULL) into a HIP graph; cxg_graph_launch replays it
+ * on the context's stream, reading the inputs from the captured addresses at replay
+ * time (update u in place between iterations). cxg_graph_capture_linear does the
+ * same for cxg_assemble_linear (no live flags in graphs). The graph keeps its own
+ * copy of the material table; it must be destroyed before its pattern and mesh.
+ * Kernel status (non-positive Jacobian) is reported by cxg_check after replays. */
+cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                    const double* E, const double* nu, double* vals,
+                                    const double* u, double* f_int, cxg_graph** out);
+cxg_status cxg_graph_capture_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                const double* E, const double* nu, const int32_t* mat_id,
+                                const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                                const double* committed, double* trial, double* vals,
+                                double* f_int, const double* f_ext, double* rhs,
+                                cxg_graph** out);
+cxg_status cxg_graph_launch(cxg_context* ctx, cxg_graph* g);
+cxg_status cxg_graph_info(const cxg_graph* g, int64_t* nodes);
+void cxg_graph_destroy(cxg_graph* g);
+/* Synchronizes the context's stream and reports (then clears) kernel status flags
+ * raised since the last check, as the eager entry points do before returning. */
+cxg_status cxg_check(cxg_context* ctx);
+
 /* ---- kernel timing ----------------------------------------------------------
  * When enabled, every kernel launch of the library is bracketed by HIP events on
  * the context's stream; per kernel name the library accumulates launches, total
  * milliseconds and the number of elements processed. No counterpart in the
- * reference (it has no timers, SURVEY: aux subsystems). Timing adds two event
+ * reference (it has no kernel timers). Timing adds two event
  * records per launch and a synchronization per call; leave it off when timing
  * whole calls. */
 cxg_status cxg_profile_enable(cxg_context* ctx, int enable);

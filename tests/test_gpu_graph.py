@@ -1,0 +1,139 @@
+"""HIP-graph replays of one assembly / one J2 Newton iteration (cxg_graph_*).
+
+Every replay is compared bitwise with the eager entry point on the same inputs, with
+eager library work, torch work and synchronizations between the replays, and with u
+changed in place between replays (the graph reads its inputs at replay time). The
+eager path itself is pinned to the oracle in test_gpu_parity.py; here the graph's
+first replay is also checked against the oracle directly.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from calculixpp_amd import fem
+    return fem
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _same(a, b):
+    return np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+@pytest.mark.parametrize("variant", [4, 1])
+def test_linear_graph_replays_match_eager(gpu, port, variant):
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    m = meshgen.block(4, _lib.C3D20, seed_geo=41, seed_E=42, ny=3, nz=3)
+    ctx = gpu.context()
+    ctx.set_option("asm_variant", variant)
+    try:
+        a = gpu.Assembler(m, ctx)
+        dev = ctx.device
+        u0 = torch.from_numpy(meshgen.smooth_field(m.coords, 43)).to(dev)
+        u1 = torch.from_numpy(meshgen.smooth_field(m.coords, 44)).to(dev)
+        u = u0.clone()
+        vals, f = a.new_values(), torch.empty(3 * m.n_nodes, dtype=torch.float64, device=dev)
+        g = a.capture(vals, u, f)
+        assert g.nodes >= 2
+        ve, fe = a.new_values(), torch.empty_like(f)
+        exp = port.assemble_linear_static(m)
+        for rep in range(4):
+            uu = u1 if rep == 2 else u0
+            u.copy_(uu)
+            vals.fill_(-1.0)
+            f.fill_(-1.0)
+            g.launch()
+            ctx.check_kernels()
+            a.assemble(ve, uu, fe)  # eager, on separate buffers
+            torch.cuda.synchronize()
+            assert _same(vals[: a.pattern.nnz], ve[: a.pattern.nnz]), rep
+            assert _same(f, fe), rep
+            if rep == 0:
+                assert rel_l2(vals[: a.pattern.nnz].cpu().numpy(), exp.csr()[2]) <= 1e-12
+                assert rel_l2(f.cpu().numpy(), port.internal_force(m, uu.cpu().numpy())) <= 1e-12
+            _ = (ve * 2.0).sum().item()  # unrelated torch work between replays
+    finally:
+        ctx.set_option("asm_variant", 4)
+
+
+@pytest.mark.parametrize("fam_i", [0, 1])
+def test_j2_newton_graph_replays_match_eager(gpu, port, fam_i):
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    from oracle.oracle import aos_state
+    fam = [_lib.C3D8, _lib.C3D20][fam_i]
+    m = meshgen.j2_block(4, 3, 3, (1.0, 0.8, 0.6), family=fam, elastic_every=4, jitter_frac=0.1)
+    hist = meshgen.seeded_history(m.n_points, 3)
+    ctx = gpu.context()
+    a = gpu.TangentAssembler(m, ctx)
+    dev = ctx.device
+    u0 = torch.from_numpy(meshgen.smooth_field(m.coords, 7, 1.5e-4)).to(dev)
+    u1 = torch.from_numpy(meshgen.smooth_field(m.coords, 8, 2.0e-4)).to(dev)
+    u = u0.clone()
+    mp = gpu.MaterialPoints(m.n_points, dev, committed=hist)
+    mpe = gpu.MaterialPoints(m.n_points, dev, committed=hist)
+    f_ext = torch.from_numpy(np.random.default_rng(5).normal(size=3 * m.n_nodes)).to(dev)
+    vals, f = a.new_values(), torch.empty(3 * m.n_nodes, dtype=torch.float64, device=dev)
+    rhs = torch.empty(max(a.n_free, 1), dtype=torch.float64, device=dev)
+    g = a.capture(u, mp, vals, f, f_ext, rhs)
+    ve, fe = a.new_values(), torch.empty_like(f)
+    free = torch.from_numpy(np.nonzero(a.dof_eq >= 0)[0]).to(dev)
+    sys_e, f_o, st_o = port.assemble_material_tangent(m, u0.cpu().numpy(), aos_state(hist))
+    for rep in range(4):
+        uu = u1 if rep == 2 else u0
+        u.copy_(uu)
+        vals.fill_(-1.0)
+        rhs.fill_(-1.0)
+        mp.trial.fill_(-1.0)
+        g.launch()
+        ctx.check_kernels()
+        a.assemble_tangent(uu, mpe, ve, fe)
+        torch.cuda.synchronize()
+        assert _same(vals[: a.pattern.nnz], ve[: a.pattern.nnz]), rep
+        assert _same(f, fe), rep
+        assert _same(mp.trial, mpe.trial), rep
+        assert _same(rhs[: a.n_free], (f_ext - fe)[free]), rep
+        if rep == 0:
+            assert rel_l2(vals[: a.pattern.nnz].cpu().numpy(), sys_e.csr()[2]) <= 1e-12
+            assert rel_l2(f.cpu().numpy(), f_o) <= 1e-12
+            for c in range(13):
+                assert rel_l2(mp.trial[c].cpu().numpy(), st_o[:, 13 + c]) <= 1e-10
+        _ = (fe * 3.0).sum().item()
+
+
+def test_graph_reports_bad_jacobian_on_check(gpu):
+    """A graph replay over an inverted element raises the reference's message at
+    cxg_check (the flag is read outside the graph)."""
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, conn = meshgen.hex_grid(2, 1, 1, (1.0, 0.5, 0.5), _lib.C3D8)
+    ctx = gpu.context()
+    m = Model(coords, [(_lib.C3D8, conn)], 210000.0, 0.3)
+    a = gpu.Assembler(m, ctx)
+    vals = a.new_values()
+    g = a.capture(vals)
+    g.launch()
+    ctx.check_kernels()  # valid geometry: no flag
+    # invert element 1 in place on the device: swap its bottom and top faces' z
+    bad = a.mesh.coords.clone()
+    a.mesh.coords[:, 2] = -bad[:, 2]
+    g.launch()
+    with pytest.raises(RuntimeError, match=r"non-positive element Jacobian"):
+        ctx.check_kernels()
+    a.mesh.coords.copy_(bad)
+    g.launch()
+    ctx.check_kernels()
+    torch.cuda.synchronize()
