@@ -270,7 +270,7 @@ def main() -> int:
 
     # Per-kernel durations from HIP events on the library's stream. One assembly is
     # the element kernel followed by the row-gather kernel (two launches, one step);
-    # the roofline fraction is BASELINE's max(F/peak_fp64, B/peak_hbm) / t over both.
+    # the roofline fraction is max(F/peak_fp64, B/peak_hbm) / t over both launches.
     ctx = asm.ctx
     ctx.profile_reset()
     ctx.profile(True)

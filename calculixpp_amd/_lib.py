@@ -71,6 +71,7 @@ _SIGS = {
     "cxg_pattern_copy": (C.c_int, [_P, _P, _P, _P]),
     "cxg_pattern_compact": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64)]),
     "cxg_assemble_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "cxg_assemble_linear_cases": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int32, _P, _P, _P]),
     "cxg_internal_force": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "cxg_reduce_free": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "cxg_assemble_j2": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
@@ -80,6 +81,8 @@ _SIGS = {
     "cxg_graph_capture_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "cxg_graph_capture_j2": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
                                        _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "cxg_graph_capture_linear_cases": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int32, _P, _P,
+                                                 C.POINTER(_P)]),
     "cxg_graph_launch": (C.c_int, [_P, _P]),
     "cxg_graph_info": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "cxg_graph_destroy": (None, [_P]),

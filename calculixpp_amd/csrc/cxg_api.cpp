@@ -258,6 +258,9 @@ cxg_status cxg_create(int device, void* stream, cxg_context** out) {
     CXG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->flag_host), sizeof(int), hipHostMallocDefault));
     *ctx->flag_host = 0;
     CXG_HIP(hipMemcpy(ctx->flag_dev, ctx->flag_host, sizeof(int), hipMemcpyHostToDevice));
+    int cus = 0;
+    CXG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (cus > 0) ctx->n_cu = cus;
     for (int t : {CXG_C3D8, CXG_C3D20, CXG_C3D20R, CXG_C3D6}) {
       std::vector<double> dN, w;
       cxg::host_tables(t, dN, w);
@@ -305,8 +308,16 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->patch_size = int(value);
     return CXG_OK;
   }
-  if (k == "asm_variant" && value >= 1 && value <= 4) {
+  if (k == "asm_variant" && value >= 1 && value <= 5) {
     ctx->asm_variant = int(value);
+    return CXG_OK;
+  }
+  if (k == "chunk_elems" && value >= 1 && value <= 4096) {
+    ctx->chunk_elems = int(value);
+    return CXG_OK;
+  }
+  if (k == "persistent_waves" && value >= 1 && value <= 32) {
+    ctx->persistent_waves = int(value);
     return CXG_OK;
   }
   return cxg::fail(ctx, CXG_ERR_ARG, "unknown option or value");
@@ -435,12 +446,27 @@ cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const dou
                                double* f_int, uint8_t* live) {
   if (!p || !coords || !E || !nu || !vals) return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble arguments");
   if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
-  if (live && ctx->asm_variant != 4)
-    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4)");
+  if (live && ctx->asm_variant < 4)
+    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4 or 5)");
   const bool with_f = u && f_int;
   return cxg::guarded(ctx, [&] {
     cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
                                 with_f ? f_int : nullptr, live);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_assemble_linear_cases(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                     const double* E, const double* nu, double* vals,
+                                     int32_t n_cases, const double* u, double* f_int,
+                                     uint8_t* live) {
+  if (!p || !coords || !E || !nu || !vals || !u || !f_int)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  if (live && ctx->asm_variant < 4)
+    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4 or 5)");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, u, f_int, live, int(n_cases));
     cxg::check_flags(ctx);
   });
 }
@@ -469,8 +495,8 @@ cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double*
       (n_mats > 0 && (!mats || !mat_id)) || n_mats < 0)
     return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_j2 arguments");
   if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
-  if (live && ctx->asm_variant != 4)
-    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4)");
+  if (live && ctx->asm_variant < 4)
+    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED, "entry liveness needs the two-phase assembly (asm_variant 4 or 5)");
   return cxg::guarded(ctx, [&] {
     const void* table = cxg::upload_j2_table(ctx, mats, n_mats);
     cxg::launch_assemble_j2(ctx, p, coords, E, nu, n_mats > 0 ? mat_id : nullptr, table, u,

@@ -20,6 +20,8 @@
 //   internal force         src/fem/stress.cpp:125-145 (as sum_q s B^T D B u)
 //   J2 return map/tangent  src/fem/plasticity.cpp:90-236
 //   scatter                src/fem/assembly.cpp:76-110 (SPC-only transform)
+#include <algorithm>
+#include <climits>
 #include <cstring>
 
 #include "cxg_internal.hpp"
@@ -556,6 +558,39 @@ __global__ __launch_bounds__(kWave, WAVES) void k_assemble_iso2(
 // no read-modify-write.
 constexpr int kStageNodes = 4;  // element-matrix node rows staged in LDS per store turn
 
+// J^-1 (row-major, cofactors) and sqrt(det J * w) of one Gauss point from the node
+// coordinates x[3n] and the natural derivatives d[n][3]; raises kFlagJacobian when
+// det J <= 0 (element.cpp:86-87).
+template <int NN>
+__device__ __forceinline__ void jacobian_inverse(const double* x, const double* __restrict__ d,
+                                                 const double w, double* A, double& rs,
+                                                 int* __restrict__ flag) {
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int k = 0; k < NN; ++k) {
+    const double x0 = x[3 * k], x1 = x[3 * k + 1], x2 = x[3 * k + 2];
+    const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+    J[0][0] += x0 * d0; J[0][1] += x0 * d1; J[0][2] += x0 * d2;
+    J[1][0] += x1 * d0; J[1][1] += x1 * d1; J[1][2] += x1 * d2;
+    J[2][0] += x2 * d0; J[2][1] += x2 * d1; J[2][2] += x2 * d2;
+  }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
+  const double r = 1.0 / det;
+  A[0] = c00 * r;
+  A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+  A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+  A[3] = c01 * r;
+  A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+  A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+  A[6] = c02 * r;
+  A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+  A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+  rs = sqrt(det * w);
+}
+
 template <int NN, int NGP, int QC>
 struct ElemLds {
   double x[NN * 3];
@@ -565,17 +600,17 @@ struct ElemLds {
   double stage[kStageNodes * 3 * 3 * NN];
 };
 
-template <int NN, int NGP, int QC, int WAVES>
-__global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
-    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
-    const double* __restrict__ wq, const double* __restrict__ coords,
-    const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
-    int* __restrict__ flag) {
+// One element: ce its connectivity row, kel its 3n x 3n block of the arena. Every
+// lane of the wave calls it; it ends with a barrier, so L can be reused at once.
+template <int NN, int NGP, int QC>
+__device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int lane,
+                                              const int32_t* __restrict__ ce, const double Ee,
+                                              const double nue, const double* __restrict__ dN,
+                                              const double* __restrict__ wq,
+                                              const double* __restrict__ coords,
+                                              double* __restrict__ kel, int* __restrict__ flag) {
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
-  __shared__ ElemLds<NN, NGP, QC> L;
-  const int lane = threadIdx.x;
-  const int64_t le = blockIdx.x;
   int pa[PPL], pb[PPL];
 #pragma unroll
   for (int t = 0; t < PPL; ++t) {
@@ -587,7 +622,6 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
       pb[t] = -1;
     }
   }
-  const int32_t* ce = conn + le * NN;
   if (lane < NN) {
     const double* xp = coords + 3 * int64_t(ce[lane]);
     L.x[3 * lane + 0] = xp[0];
@@ -595,34 +629,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
     L.x[3 * lane + 2] = xp[2];
   }
   __syncthreads();
-  for (int q = lane; q < NGP; q += kWave) {
-    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    const double* d = dN + q * NN * 3;
-    for (int k = 0; k < NN; ++k) {
-      const double x0 = L.x[3 * k], x1 = L.x[3 * k + 1], x2 = L.x[3 * k + 2];
-      const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
-      J[0][0] += x0 * d0; J[0][1] += x0 * d1; J[0][2] += x0 * d2;
-      J[1][0] += x1 * d0; J[1][1] += x1 * d1; J[1][2] += x1 * d2;
-      J[2][0] += x2 * d0; J[2][1] += x2 * d1; J[2][2] += x2 * d2;
-    }
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
-    const double r = 1.0 / det;
-    double* A = L.Ai + 9 * q;
-    A[0] = c00 * r;
-    A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
-    A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
-    A[3] = c01 * r;
-    A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
-    A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
-    A[6] = c02 * r;
-    A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
-    A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
-    L.rs[q] = sqrt(det * wq[q]);
-  }
+  for (int q = lane; q < NGP; q += kWave) jacobian_inverse<NN>(L.x, dN + q * NN * 3, wq[q], L.Ai + 9 * q, L.rs[q], flag);
   double M[PPL][9];
 #pragma unroll
   for (int t = 0; t < PPL; ++t)
@@ -663,7 +670,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
     __syncthreads();
   }
   double lam, mu;
-  lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
+  lame(Ee, nue, lam, mu);
   constexpr int ND = 3 * NN;
   // M -> K in place: K_ab = lam M + mu M^T + mu tr(M) I
 #pragma unroll
@@ -682,7 +689,6 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
   // Staged store: rows of kStageNodes nodes at a time. Block (a, b) goes to the rows of
   // a, its transpose to the rows of b; the staged rows are then one contiguous run of
   // the element's row-major matrix.
-  double* kel = kp + le * (ND * ND);
   for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
 #pragma unroll
     for (int t = 0; t < PPL; ++t) {
@@ -711,6 +717,18 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
   }
 }
 
+template <int NN, int NGP, int QC, int WAVES>
+__global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
+    int* __restrict__ flag) {
+  __shared__ ElemLds<NN, NGP, QC> L;
+  const int64_t le = blockIdx.x;
+  elem_iso_body<NN, NGP, QC>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
+                             nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
+}
+
 // ---- K4 (+K5), two-phase variant, phase B: CSR rows gathered by their owner node ----
 // One 64-lane wave per node a. Its (up to three) CSR rows share one column layout of
 // length L; they are accumulated in LDS from rows 3la..3la+2 of the element matrices
@@ -731,15 +749,19 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// NC displacement fields (load cases) at u + c*n_dof give NC internal forces at
+// f_int + c*n_dof from the same pass over the element rows. LIVE: also flag the
+// entries that received a contribution other than 0.0.
+template <int NC, bool LIVE>
 __global__ __launch_bounds__(kWave) void k_row_gather(
     const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
     const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
-    const double* __restrict__ u, const double* __restrict__ fe, double* __restrict__ vals,
-    double* __restrict__ f_int, uint8_t* __restrict__ live) {
+    const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
+    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live) {
   __shared__ double buf[3 * kRowCap];
-  __shared__ uint8_t lbuf[3 * kRowCap];
+  __shared__ uint8_t lbuf[LIVE ? 3 * kRowCap : 1];
   const int32_t a = blockIdx.x;
   const int lane = threadIdx.x;
   const int bl = lane / 3, jl = lane - 3 * (lane / 3);
@@ -757,23 +779,25 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     for (int64_t c = lane; c < Lrow; c += kWave) {
       if (in_lds) {
         buf[i * kRowCap + c] = 0.0;
-        lbuf[i * kRowCap + c] = 0;
+        if constexpr (LIVE) lbuf[i * kRowCap + c] = 0;
       } else {
         vals[rs[i] + c] = 0.0;
-        if (live) live[rs[i] + c] = 0;
+        if constexpr (LIVE) live[rs[i] + c] = 0;
       }
     }
   }
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  double f[NC][3];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) f[c][0] = f[c][1] = f[c][2] = 0.0;
   const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
   if (fe && lane < 3)  // element force vectors, summed in ascending element order
     for (int64_t it = i0; it < i1; ++it) {
       const IncRecord r = rec[it];
-      f0 += fe[3 * (r.conn + r.la) + lane];
+      f[0][0] += fe[3 * (r.conn + r.la) + lane];
     }
   for (int64_t ib = i0; ib < i1; ib += kBatch) {
     // issue every load of the batch first ...
-    double v[kBatch][3], uv[kBatch];
+    double v[kBatch][3], uv[kBatch][NC];
     int32_t colp[kBatch];
     int fm[kBatch];
     bool ok[kBatch];
@@ -781,7 +805,8 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     for (int k = 0; k < kBatch; ++k) {
       ok[k] = false;
       v[k][0] = v[k][1] = v[k][2] = 0.0;
-      uv[k] = 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) uv[k][c] = 0.0;
       colp[k] = 0;
       fm[k] = 0;
       if (ib + k < i1) {
@@ -796,7 +821,9 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
           v[k][1] = ke[ndof];
           v[k][2] = ke[2 * ndof];
           fm[k] = nfm[nd];
-          if (u) uv[k] = u[3 * int64_t(nd) + jl];
+          if (u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) uv[k][c] = u[c * n_dof + 3 * int64_t(nd) + jl];
         }
       }
     }
@@ -805,16 +832,19 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
 #pragma unroll
     for (int k = 0; k < kBatch; ++k) {
       if (!ok[k]) continue;
-      f0 += v[k][0] * uv[k];
-      f1 += v[k][1] * uv[k];
-      f2 += v[k][2] * uv[k];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        f[c][0] += v[k][0] * uv[k][c];
+        f[c][1] += v[k][1] * uv[k][c];
+        f[c][2] += v[k][2] * uv[k][c];
+      }
       if ((fm[k] >> jl) & 1) {
         const int64_t pos = int64_t(colp[k]) + __popc(fm[k] & ((1 << jl) - 1));
         if (in_lds) {
           if (rs[0] >= 0) buf[pos] += v[k][0];
           if (rs[1] >= 0) buf[kRowCap + pos] += v[k][1];
           if (rs[2] >= 0) buf[2 * kRowCap + pos] += v[k][2];
-          if (live) {
+          if constexpr (LIVE) {
             if (v[k][0] != 0.0) lbuf[pos] = 1;
             if (v[k][1] != 0.0) lbuf[kRowCap + pos] = 1;
             if (v[k][2] != 0.0) lbuf[2 * kRowCap + pos] = 1;
@@ -823,7 +853,7 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
           if (rs[0] >= 0) vals[rs[0] + pos] += v[k][0];
           if (rs[1] >= 0) vals[rs[1] + pos] += v[k][1];
           if (rs[2] >= 0) vals[rs[2] + pos] += v[k][2];
-          if (live) {
+          if constexpr (LIVE) {
             if (rs[0] >= 0 && v[k][0] != 0.0) live[rs[0] + pos] = 1;
             if (rs[1] >= 0 && v[k][1] != 0.0) live[rs[1] + pos] = 1;
             if (rs[2] >= 0 && v[k][2] != 0.0) live[rs[2] + pos] = 1;
@@ -838,21 +868,24 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     for (int i = 0; i < 3; ++i) {
       if (rs[i] < 0) continue;
       for (int64_t c = lane; c < Lrow; c += kWave) vals[rs[i] + c] = buf[i * kRowCap + c];
-      if (live)
+      if constexpr (LIVE)
         for (int64_t c = lane; c < Lrow; c += kWave) live[rs[i] + c] = lbuf[i * kRowCap + c];
     }
   if (u) {
-    f0 = wave_sum(f0);
-    f1 = wave_sum(f1);
-    f2 = wave_sum(f2);
-    if (lane == 0) {
-      double* fp = f_int + 3 * int64_t(a);
-      fp[0] = f0;
-      fp[1] = f1;
-      fp[2] = f2;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double s0 = wave_sum(f[c][0]);
+      const double s1 = wave_sum(f[c][1]);
+      const double s2 = wave_sum(f[c][2]);
+      if (lane == 0) {
+        double* fp = f_int + c * n_dof + 3 * int64_t(a);
+        fp[0] = s0;
+        fp[1] = s1;
+        fp[2] = s2;
+      }
     }
   } else if (fe && lane < 3) {
-    f_int[3 * int64_t(a) + lane] = f0;
+    f_int[3 * int64_t(a) + lane] = f[0][0];
   }
 }
 
@@ -936,10 +969,20 @@ __device__ __forceinline__ void yield_at(const J2Dev& m, double ep, double& sy, 
   h = 0.0;
 }
 
+// Algorithmic tangent of a J2 point in compact form: D_t = iso(lam, mu) - c n n^T,
+// iso(l, m) the isotropic Voigt matrix (l + 2m on the normal diagonal, l off it, m on
+// the engineering-shear diagonal). The reference's De - (1 - theta) Idev - 2 mu
+// theta_bar n n^T (plasticity.cpp:209-234) is exactly this with
+// lam = lam_e + (2/3) mu_e (1 - theta), mu = mu_e theta, c = 2 mu_e theta_bar.
+struct J2Tangent {
+  double lam, mu, c;
+  double n[6];
+};
+
 // Radial return of one Gauss point: reads the committed history hc[13], writes the
-// trial history ht[13], the stress and the algorithmic tangent D (6x6 row-major).
-__device__ void j2_point(const J2Dev& m, const double eps[6], const double hc[13],
-                         double ht[13], double sig[6], double D[36]) {
+// trial history ht[13], the stress and the compact algorithmic tangent.
+__device__ void j2_return(const J2Dev& m, const double eps[6], const double hc[13],
+                          double ht[13], double sig[6], J2Tangent& T) {
   double lam, mu;
   lame(m.E, m.nu, lam, mu);
   double ee[6];
@@ -955,11 +998,10 @@ __device__ void j2_point(const J2Dev& m, const double eps[6], const double hc[13
   double sy0, h0;
   yield_at(m, hc[6], sy0, h0);
   const double qtr = kSqrt32 * nrm;
-  // Elastic tangent De first; the plastic branch subtracts from it.
-  for (int i = 0; i < 36; ++i) D[i] = 0.0;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) D[6 * i + j] = (i == j) ? lam + 2.0 * mu : lam;
-  for (int i = 3; i < 6; ++i) D[7 * i] = mu;
+  T.lam = lam;
+  T.mu = mu;
+  T.c = 0.0;
+  for (int i = 0; i < 6; ++i) T.n[i] = 0.0;
   if (qtr - sy0 <= 0.0 || nrm == 0.0) {
     for (int i = 0; i < 6; ++i) sig[i] = st[i];
     for (int i = 0; i < 13; ++i) ht[i] = hc[i];
@@ -992,13 +1034,24 @@ __device__ void j2_point(const J2Dev& m, const double eps[6], const double hc[13
   yield_at(m, ep0 + dg, sy, h);
   const double theta = 1.0 - cdev / nrm;
   const double theta_bar = 1.0 / (1.0 + (h + m.hkin) / (3.0 * mu)) - (1.0 - theta);
-  const double soft = 1.0 - theta, c_nn = 2.0 * mu * theta_bar;
+  const double soft = 1.0 - theta;
+  T.lam = lam + (2.0 / 3.0) * mu * soft;
+  T.mu = mu * theta;
+  T.c = 2.0 * mu * theta_bar;
+  for (int i = 0; i < 6; ++i) T.n[i] = n[i];
+}
+
+// As j2_return, with the tangent expanded to a 6x6 row-major D.
+__device__ void j2_point(const J2Dev& m, const double eps[6], const double hc[13],
+                         double ht[13], double sig[6], double D[36]) {
+  J2Tangent T;
+  j2_return(m, eps, hc, ht, sig, T);
   for (int i = 0; i < 6; ++i)
     for (int j = 0; j < 6; ++j) {
-      double Pij;
-      if (i < 3 && j < 3) Pij = 2.0 * mu * ((i == j ? 1.0 : 0.0) - 1.0 / 3.0);
-      else Pij = (i == j) ? mu : 0.0;
-      D[6 * i + j] -= soft * Pij + c_nn * n[i] * n[j];
+      double d = 0.0;
+      if (i < 3 && j < 3) d = T.lam + (i == j ? 2.0 * T.mu : 0.0);
+      else if (i == j) d = T.mu;
+      D[6 * i + j] = d - T.c * T.n[i] * T.n[j];
     }
 }
 
@@ -1105,92 +1158,104 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
 }
 
 // ---- K2, two-phase variant: return map, K_t and f_e to the element arenas ------------
-// One element per wave. Per Gauss point (one lane each): strain, radial return
-// (j2_point), algorithmic tangent D_t, trial history to the SoA state. Then the
-// element force f_e = sum_q s B^T sigma (one lane per node) and the pair blocks
-// K_ab = sum_q s B_a^T D_q B_b, with D_q B_b formed for QC Gauss points at a time in
-// LDS and shared by every pair of the wave. K_t is written as full rows through the
-// same staged, coalesced stores as k_elem_iso; f_e (3n per element) to its own arena.
-// The row gather then sums K_t rows and f_e entries per node in element order.
+// One element per wave. Each lane owns one Gauss point: Jacobian, strain (gradients
+// formed on the fly), radial return (j2_return), trial history to the SoA state, and
+// the compact tangent (lam, mu, c, n) and rs*sigma to LDS. Then, QC Gauss points at a
+// time, h = rs g and t = B(h)^T n per node go to LDS, and each lane accumulates its
+// upper node-pair blocks
+//   K_ab += lam h_a h_b^T + mu h_b h_a^T + mu (h_a . h_b) I - c t_a t_b^T
+// (= s B_a^T D_t B_b with s = det J w, rs = sqrt(s)), while lanes < n accumulate the
+// element force f_a += B(h_a)^T (rs sigma). K_t is written as full rows through the
+// staged, coalesced stores of k_elem_iso; f_e (3n) to its own arena. The row gather
+// then sums K_t rows and f_e entries per node in element order.
 template <int NN, int NGP, int QC>
 struct ElemJ2Lds {
-  Geo<NN, NGP> G;
+  double x[NN * 3];
   double u[NN * 3];
-  double D[NGP * 36];
-  double S[NGP * 6];
+  double Ai[NGP * 9];
+  double rs[NGP];
+  double T[NGP * 9];  // lam, mu, c, n[6] per Gauss point
+  double S[NGP * 6];  // rs * sigma
   union {
-    double DB[QC * NN * 18];
+    struct {
+      double h[QC * NN * 3];
+      double t[QC * NN * 3];
+    } c;
     double stage[kStageNodes * 3 * 3 * NN];
   } w;
 };
 
+// One element (see above). mat == nullptr: isotropic elastic with Ee, nue. pt0: the
+// element's first Gauss point in the SoA state. Ends with a barrier.
 template <int NN, int NGP, int QC>
-__global__ __launch_bounds__(kWave) void k_elem_j2(
-    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset, int64_t n_points,
+__device__ __forceinline__ void elem_j2_body(
+    ElemJ2Lds<NN, NGP, QC>& L, const int lane, const int32_t* __restrict__ ce, const double Ee,
+    const double nue, const J2Dev* mat, const int64_t pt0, const int64_t n_points,
     const double* __restrict__ dN, const double* __restrict__ wq,
-    const double* __restrict__ coords, const double* __restrict__ E,
-    const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
-    const J2Dev* __restrict__ mats, const double* __restrict__ u,
-    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
-    double* __restrict__ fe, int* __restrict__ flag) {
+    const double* __restrict__ coords, const double* __restrict__ u,
+    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kel,
+    double* __restrict__ fel, int* __restrict__ flag) {
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
   constexpr int ND = 3 * NN;
-  __shared__ ElemJ2Lds<NN, NGP, QC> L;
-  __shared__ int32_t node[NN];
-  const int lane = threadIdx.x;
-  const int64_t le = blockIdx.x;
-  const int64_t ge = elem_offset + le;
-  const int32_t* ce = conn + le * NN;
-  stage_coords<NN, NGP>(L.G, node, ce, coords, lane);
-  stage_vec<NN>(L.u, ce, u, lane);
+  if (lane < NN) {
+    const int64_t n3 = 3 * int64_t(ce[lane]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      L.x[3 * lane + c] = coords[n3 + c];
+      L.u[3 * lane + c] = u[n3 + c];
+    }
+  }
   __syncthreads();
-  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
-  __syncthreads();
-  gradients<NN, NGP>(L.G, dN, lane);
-  __syncthreads();
-  const int32_t mid = mat_id ? mat_id[ge] : -1;
   for (int q = lane; q < NGP; q += kWave) {
-    double H[3][3], e[6], sg[6];
-    grad_field<NN, NGP>(L.G, L.u, q, H);
+    const double* d = dN + q * NN * 3;
+    double* A = L.Ai + 9 * q;
+    double rs;
+    jacobian_inverse<NN>(L.x, d, wq[q], A, rs, flag);
+    L.rs[q] = rs;
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < NN; ++k) {
+      const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+      const double g0 = A[0] * d0 + A[3] * d1 + A[6] * d2;
+      const double g1 = A[1] * d0 + A[4] * d1 + A[7] * d2;
+      const double g2 = A[2] * d0 + A[5] * d1 + A[8] * d2;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double v = L.u[3 * k + i];
+        H[i][0] += v * g0;
+        H[i][1] += v * g1;
+        H[i][2] += v * g2;
+      }
+    }
+    double e[6], sg[6];
     voigt_strain(H, e);
-    const int64_t pt = gp_offset + le * NGP + q;
-    double* Dq = L.D + 36 * q;
-    if (mid >= 0) {
+    const int64_t pt = pt0 + q;
+    J2Tangent T;
+    if (mat) {
       double hc[13], ht[13];
 #pragma unroll
       for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
-      j2_point(mats[mid], e, hc, ht, sg, Dq);
+      j2_return(*mat, e, hc, ht, sg, T);
 #pragma unroll
       for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = ht[c];
     } else {
-      double lam, mu;
-      lame(E[ge], nu[ge], lam, mu);
-      iso_stress(lam, mu, e, sg);
-      for (int i = 0; i < 36; ++i) Dq[i] = 0.0;
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Dq[6 * i + j] = (i == j) ? lam + 2.0 * mu : lam;
-      for (int i = 3; i < 6; ++i) Dq[7 * i] = mu;
+      lame(Ee, nue, T.lam, T.mu);
+      iso_stress(T.lam, T.mu, e, sg);
+      T.c = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) T.n[i] = 0.0;
 #pragma unroll
       for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = committed[c * n_points + pt];
     }
+    double* Tq = L.T + 9 * q;
+    Tq[0] = T.lam;
+    Tq[1] = T.mu;
+    Tq[2] = T.c;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = L.G.sc[q] * sg[i];
-  }
-  __syncthreads();
-  if (lane < NN) {
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int q = 0; q < NGP; ++q) {
-      const double* g = L.G.g + (q * NN + lane) * 3;
-      const double* sq = L.S + 6 * q;
-      f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
-      f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
-      f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
+    for (int i = 0; i < 6; ++i) {
+      Tq[3 + i] = T.n[i];
+      L.S[6 * q + i] = rs * sg[i];
     }
-    double* fp = fe + le * ND + 3 * lane;
-    fp[0] = f0;
-    fp[1] = f1;
-    fp[2] = f2;
   }
   int pa[PPL], pb[PPL];
   double K[PPL][9];
@@ -1206,23 +1271,77 @@ __global__ __launch_bounds__(kWave) void k_elem_j2(
 #pragma unroll
     for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
   }
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  __syncthreads();
   for (int q0 = 0; q0 < NGP; q0 += QC) {
     for (int idx = lane; idx < QC * NN; idx += kWave) {
       const int q = q0 + idx / NN;
-      if (q < NGP) d_times_b(L.D + 36 * q, L.G.g + 3 * (q * NN + idx % NN), L.w.DB + 18 * idx);
+      double h0 = 0.0, h1 = 0.0, h2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+      if (q < NGP) {
+        const double* A = L.Ai + 9 * q;
+        const double* d = dN + (q * NN + idx % NN) * 3;
+        const double s = L.rs[q];
+        h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
+        h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
+        h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
+        const double* n = L.T + 9 * q + 3;
+        t0 = h0 * n[0] + h1 * n[3] + h2 * n[4];
+        t1 = h1 * n[1] + h0 * n[3] + h2 * n[5];
+        t2 = h2 * n[2] + h0 * n[4] + h1 * n[5];
+      }
+      L.w.c.h[3 * idx + 0] = h0;
+      L.w.c.h[3 * idx + 1] = h1;
+      L.w.c.h[3 * idx + 2] = h2;
+      L.w.c.t[3 * idx + 0] = t0;
+      L.w.c.t[3 * idx + 1] = t1;
+      L.w.c.t[3 * idx + 2] = t2;
     }
     __syncthreads();
+    const int qn = NGP - q0 < QC ? NGP - q0 : QC;
+    if (lane < NN)
+      for (int qq = 0; qq < qn; ++qq) {
+        const double* g = L.w.c.h + (qq * NN + lane) * 3;
+        const double* sq = L.S + 6 * (q0 + qq);
+        f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
+        f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
+        f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
+      }
 #pragma unroll
     for (int t = 0; t < PPL; ++t) {
       if (pa[t] < 0) continue;
-      for (int qq = 0; qq < QC && q0 + qq < NGP; ++qq) {
-        const int q = q0 + qq;
-        bt_db(L.G.g + 3 * (q * NN + pa[t]), L.w.DB + 18 * (qq * NN + pb[t]), L.G.sc[q], K[t]);
+#pragma unroll 1
+      for (int qq = 0; qq < qn; ++qq) {
+        const double* Tq = L.T + 9 * (q0 + qq);
+        const double lam = Tq[0], mu = Tq[1], c = Tq[2];
+        const double* ha = L.w.c.h + (qq * NN + pa[t]) * 3;
+        const double* hb = L.w.c.h + (qq * NN + pb[t]) * 3;
+        const double* ta = L.w.c.t + (qq * NN + pa[t]) * 3;
+        const double* tb = L.w.c.t + (qq * NN + pb[t]) * 3;
+        const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+        const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
+        const double la[3] = {lam * a0, lam * a1, lam * a2};
+        const double mb[3] = {mu * b0, mu * b1, mu * b2};
+        const double ct[3] = {c * ta[0], c * ta[1], c * ta[2]};
+        const double av[3] = {a0, a1, a2}, bv[3] = {b0, b1, b2};
+        const double dg = mu * (a0 * b0 + a1 * b1 + a2 * b2);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            K[t][3 * i + j] += la[i] * bv[j] + mb[i] * av[j] - ct[i] * tb[j];
+        K[t][0] += dg;
+        K[t][4] += dg;
+        K[t][8] += dg;
       }
     }
     __syncthreads();
   }
-  double* kel = kp + le * (ND * ND);
+  if (lane < NN) {
+    double* fp = fel + 3 * lane;
+    fp[0] = f0;
+    fp[1] = f1;
+    fp[2] = f2;
+  }
   for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
 #pragma unroll
     for (int t = 0; t < PPL; ++t) {
@@ -1247,6 +1366,136 @@ __global__ __launch_bounds__(kWave) void k_elem_j2(
     double* dst = kel + int64_t(3 * g0) * ND;
     for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = L.w.stage[idx];
     __syncthreads();
+  }
+}
+
+template <int NN, int NGP, int QC>
+__global__ __launch_bounds__(kWave) void k_elem_j2(
+    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset, int64_t n_points,
+    const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
+    const J2Dev* __restrict__ mats, const double* __restrict__ u,
+    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
+    double* __restrict__ fe, int* __restrict__ flag) {
+  __shared__ ElemJ2Lds<NN, NGP, QC> L;
+  const int64_t le = blockIdx.x;
+  const int64_t ge = elem_offset + le;
+  const int32_t mid = mat_id ? mat_id[ge] : -1;
+  elem_j2_body<NN, NGP, QC>(L, int(threadIdx.x), conn + le * NN, E[ge], nu[ge],
+                            mid >= 0 ? mats + mid : nullptr, gp_offset + le * NGP, n_points, dN,
+                            wq, coords, u, committed, trial, kp + le * (9 * NN * NN),
+                            fe + le * (3 * NN), flag);
+}
+
+// ---- Persistent element kernels (asm_variant 5) ------------------------------------
+// One launch covers every block of the mesh, whatever its family: a grid of resident
+// workgroups (one wave each) takes work items (runs of consecutive elements of one
+// block) from a shared counter until the counter passes the last item, so meshes of
+// very uneven size and mixed families balance across the compute units without one
+// launch per family. Items are ordered costliest family first. The element arithmetic
+// is the same device code as the per-family kernels (elem_iso_body / elem_j2_body);
+// the LDS is the union of the families' layouts.
+union IsoLdsAll {
+  ElemLds<8, 8, 8> c3d8;
+  ElemLds<20, 27, 3> c3d20;
+  ElemLds<20, 8, 3> c3d20r;
+  ElemLds<6, 2, 2> c3d6;
+};
+union J2LdsAll {
+  ElemJ2Lds<8, 8, 8> c3d8;
+  ElemJ2Lds<20, 27, 3> c3d20;
+  ElemJ2Lds<20, 8, 3> c3d20r;
+  ElemJ2Lds<6, 2, 2> c3d6;
+};
+
+// Next work item for the whole wave (lane 0 takes it from the counter).
+__device__ __forceinline__ int32_t next_item(int32_t* queue, int32_t* slot, int lane) {
+  if (lane == 0) *slot = atomicAdd(queue, 1);
+  __syncthreads();
+  const int32_t w = *slot;
+  __syncthreads();  // every lane has read the slot before lane 0 writes it again
+  return w;
+}
+
+__global__ void k_reset_queue(int32_t* queue) {
+  if (threadIdx.x == 0) *queue = 0;
+}
+
+__global__ __launch_bounds__(kWave, 4) void k_elem_iso_persistent(
+    const WorkChunk* __restrict__ work, int32_t n_work, int32_t* __restrict__ queue,
+    const BlockDesc* __restrict__ blocks, const int32_t* __restrict__ conn,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, double* __restrict__ kp, int* __restrict__ flag) {
+  __shared__ IsoLdsAll L;
+  __shared__ int32_t slot;
+  const int lane = threadIdx.x;
+  for (;;) {
+    const int32_t w = next_item(queue, &slot, lane);
+    if (w >= n_work) break;
+    const WorkChunk c = work[w];
+    const BlockDesc B = blocks[c.block];
+    // The lane index is passed through an empty register-only asm per element, so the
+    // compiler cannot hoist the per-family lane -> node-pair decoding out of the loops
+    // (hoisted for all four families at once it would exceed the register budget).
+#define CXG_ISO_RUN(NN, NGP, QC, LDS)                                                      \
+  for (int32_t i = 0; i < c.count; ++i) {                                                \
+    const int64_t le = int64_t(c.first) + i;                                             \
+    const int64_t ge = B.elem_offset + le;                                               \
+    int ln = lane;                                                                       \
+    asm volatile("" : "+v"(ln));                                                         \
+    elem_iso_body<NN, NGP, QC>(LDS, ln, conn + B.conn_offset + le * NN, E[ge], nu[ge],   \
+                               B.dN, B.w, coords, kp + B.kp_base + le * (9 * NN * NN),   \
+                               flag);                                                    \
+  }
+    switch (B.type) {
+      case CXG_C3D8: CXG_ISO_RUN(8, 8, 8, L.c3d8) break;
+      case CXG_C3D20: CXG_ISO_RUN(20, 27, 3, L.c3d20) break;
+      case CXG_C3D20R: CXG_ISO_RUN(20, 8, 3, L.c3d20r) break;
+      case CXG_C3D6: CXG_ISO_RUN(6, 2, 2, L.c3d6) break;
+      default: break;
+    }
+#undef CXG_ISO_RUN
+  }
+}
+
+__global__ __launch_bounds__(kWave, 2) void k_elem_j2_persistent(
+    const WorkChunk* __restrict__ work, int32_t n_work, int32_t* __restrict__ queue,
+    const BlockDesc* __restrict__ blocks, const int32_t* __restrict__ conn,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
+    const J2Dev* __restrict__ mats, int64_t n_points, const double* __restrict__ u,
+    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
+    double* __restrict__ fe, int* __restrict__ flag) {
+  __shared__ J2LdsAll L;
+  __shared__ int32_t slot;
+  const int lane = threadIdx.x;
+  for (;;) {
+    const int32_t w = next_item(queue, &slot, lane);
+    if (w >= n_work) break;
+    const WorkChunk c = work[w];
+    const BlockDesc B = blocks[c.block];
+#define CXG_J2_RUN(NN, NGP, QC, LDS)                                                        \
+  for (int32_t i = 0; i < c.count; ++i) {                                                 \
+    const int64_t le = int64_t(c.first) + i;                                              \
+    const int64_t ge = B.elem_offset + le;                                                \
+    const int64_t co = B.conn_offset + le * NN;                                           \
+    const int32_t mid = mat_id ? mat_id[ge] : -1;                                         \
+    int ln = lane;                                                                        \
+    asm volatile("" : "+v"(ln));                                                          \
+    elem_j2_body<NN, NGP, QC>(LDS, ln, conn + co, E[ge], nu[ge],                          \
+                              mid >= 0 ? mats + mid : nullptr, B.gp_offset + le * NGP,    \
+                              n_points, B.dN, B.w, coords, u, committed, trial,           \
+                              kp + B.kp_base + le * (9 * NN * NN), fe + 3 * co, flag);    \
+  }
+    switch (B.type) {
+      case CXG_C3D8: CXG_J2_RUN(8, 8, 8, L.c3d8) break;
+      case CXG_C3D20: CXG_J2_RUN(20, 27, 3, L.c3d20) break;
+      case CXG_C3D20R: CXG_J2_RUN(20, 8, 3, L.c3d20r) break;
+      case CXG_C3D6: CXG_J2_RUN(6, 2, 2, L.c3d6) break;
+      default: break;
+    }
+#undef CXG_J2_RUN
   }
 }
 
@@ -1365,6 +1614,33 @@ __global__ __launch_bounds__(kWave) void k_vjp(
 }
 
 // ---- host-side dispatch -------------------------------------------------------
+// Phase B over every node of the pattern's mesh (see k_row_gather).
+void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
+                   double* vals, double* f_int, uint8_t* live, int n_cases) {
+  const cxg_mesh* m = p->mesh;
+  const int64_t n_dof = 3 * int64_t(m->n_nodes);
+#define GATHER(NC, LV)                                                                          \
+  hipLaunchKernelGGL((k_row_gather<NC, LV>), dim3(unsigned(m->n_nodes)), dim3(kWave), 0,        \
+                     ctx->stream, m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p,         \
+                     p->nfm.p, p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals,    \
+                     f_int, live)
+  if (n_cases == 4) {
+    if (live) GATHER(4, true); else GATHER(4, false);
+  } else if (n_cases == 2) {
+    if (live) GATHER(2, true); else GATHER(2, false);
+  } else {
+    if (live) GATHER(1, true); else GATHER(1, false);
+  }
+#undef GATHER
+}
+
+// Resident workgroups of a persistent launch: waves per CU times CUs, at most one per
+// work item.
+unsigned persistent_grid(const cxg_context* ctx, const cxg_pattern* p) {
+  const int64_t g = std::min<int64_t>(int64_t(ctx->n_cu) * ctx->persistent_waves, p->n_work);
+  return unsigned(g > 0 ? g : 1);
+}
+
 inline int grid_for(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return int(b < 4096 ? (b > 0 ? b : 1) : 4096);
@@ -1423,13 +1699,24 @@ void fill_f64(hipStream_t s, double* p, int64_t n, double v) {
 
 void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             const double* E, const double* nu, double* vals, const double* u,
-                            double* f_int, uint8_t* live) {
+                            double* f_int, uint8_t* live, int n_cases) {
   const cxg_mesh* m = p->mesh;
-  if (ctx->asm_variant == 4) {
+  if (n_cases != 1 && (ctx->asm_variant < 4 || (n_cases != 2 && n_cases != 4)))
+    throw Error(CXG_ERR_UNSUPPORTED, "load cases: 1, or 2 or 4 with the two-phase assembly");
+  if (ctx->asm_variant >= 4) {
     // Two-phase: element pair blocks to the arena, then one owner wave per node row.
     cxg_pattern* pm = const_cast<cxg_pattern*>(p);
     prepare_arenas(p, false);
-    for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+    if (ctx->asm_variant == 5) {
+      prepare_work(ctx, p);
+      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, pm->queue.p);
+      LaunchTimer timer(ctx, "elem_iso_persistent", m->n_elem);
+      hipLaunchKernelGGL(k_elem_iso_persistent, dim3(persistent_grid(ctx, p)), dim3(kWave), 0,
+                         ctx->stream, pm->work.p, pm->n_work, pm->queue.p, pm->bdesc.p, m->conn.p,
+                         coords, E, nu, pm->ke_arena.p, ctx->flag_dev);
+      CXG_HIP(hipGetLastError());
+    }
+    for (size_t bi = 0; ctx->asm_variant == 4 && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
       const ElemTable& T = ctx->tables[B.type];
@@ -1446,10 +1733,7 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
     }
     if (m->n_nodes > 0) {
       LaunchTimer timer(ctx, "row_gather", m->n_elem);
-      hipLaunchKernelGGL(k_row_gather, dim3(unsigned(m->n_nodes)), dim3(kWave), 0, ctx->stream,
-                         m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,
-                         p->dof_eq.p, p->row_ptr.p, pm->ke_arena.p, f_int ? u : nullptr,
-                         nullptr, vals, f_int, live);
+      launch_gather(ctx, p, f_int ? u : nullptr, nullptr, vals, f_int, live, n_cases);
       CXG_HIP(hipGetLastError());
     }
     return;
@@ -1555,17 +1839,65 @@ void prepare_arenas(const cxg_pattern* p, bool element_forces) {
   if (element_forces && pm->fe_arena.n < fe_n) pm->fe_arena.alloc(fe_n);
 }
 
+void prepare_work(cxg_context* ctx, const cxg_pattern* p) {
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  if (pm->work_chunk == ctx->chunk_elems && pm->queue.p) return;
+  const cxg_mesh* m = p->mesh;
+  std::vector<BlockDesc> desc(m->blocks.size());
+  std::vector<size_t> order(m->blocks.size());
+  for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+    const cxg_block_dev& B = m->blocks[bi];
+    const ElemTable& T = ctx->tables[B.type];
+    desc[bi] = BlockDesc{B.type, B.nn, B.ngp, 0, B.elem_offset, B.gp_offset, B.conn_offset,
+                         p->kp_base[bi], T.dN, T.w};
+    order[bi] = bi;
+  }
+  // costliest family first (pair blocks times Gauss points per element)
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    const auto cost = [&](size_t i) { return int64_t(desc[i].ngp) * desc[i].nn * desc[i].nn; };
+    return cost(a) > cost(b);
+  });
+  const int32_t chunk = ctx->chunk_elems;
+  std::vector<WorkChunk> items;
+  for (size_t bi : order) {
+    const int64_t n = m->blocks[bi].n_elem;
+    for (int64_t f = 0; f < n; f += chunk)
+      items.push_back(WorkChunk{int32_t(bi), int32_t(f), int32_t(std::min<int64_t>(chunk, n - f)), 0});
+  }
+  if (items.size() > size_t(INT32_MAX)) throw Error(CXG_ERR_UNSUPPORTED, "too many work items");
+  pm->work.alloc(std::max<size_t>(items.size(), 1));
+  pm->bdesc.alloc(std::max<size_t>(desc.size(), 1));
+  if (!pm->queue.p) pm->queue.alloc(1);
+  CXG_HIP(hipStreamSynchronize(ctx->stream));
+  if (!items.empty())
+    CXG_HIP(hipMemcpy(pm->work.p, items.data(), items.size() * sizeof(WorkChunk), hipMemcpyHostToDevice));
+  if (!desc.empty())
+    CXG_HIP(hipMemcpy(pm->bdesc.p, desc.data(), desc.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
+  pm->n_work = int32_t(items.size());
+  pm->work_chunk = chunk;
+}
+
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
                         const void* j2_table, const double* u, const double* committed,
                         double* trial, double* vals, double* f_int, uint8_t* live) {
   const J2Dev* dmats = static_cast<const J2Dev*>(j2_table);
   const cxg_mesh* m = p->mesh;
-  if (ctx->asm_variant == 4) {
+  if (ctx->asm_variant >= 4) {
     // Two-phase: K_t and f_e of every element to the arenas, then the row gather.
     cxg_pattern* pm = const_cast<cxg_pattern*>(p);
     prepare_arenas(p, true);
-    for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+    if (ctx->asm_variant == 5) {
+      prepare_work(ctx, p);
+      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, pm->queue.p);
+      LaunchTimer timer(ctx, "elem_j2_persistent", m->n_elem);
+      hipLaunchKernelGGL(k_elem_j2_persistent, dim3(persistent_grid(ctx, p)), dim3(kWave), 0,
+                         ctx->stream, pm->work.p, pm->n_work, pm->queue.p, pm->bdesc.p, m->conn.p,
+                         coords, E, nu, mat_id, dmats, m->n_points, u, committed, trial,
+                         pm->ke_arena.p, pm->fe_arena.p, ctx->flag_dev);
+      CXG_HIP(hipGetLastError());
+    }
+    for (size_t bi = 0; ctx->asm_variant == 4 && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
       const ElemTable& T = ctx->tables[B.type];
@@ -1583,10 +1915,7 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
     }
     if (m->n_nodes > 0) {
       LaunchTimer timer(ctx, "row_gather_j2", m->n_elem);
-      hipLaunchKernelGGL(k_row_gather, dim3(unsigned(m->n_nodes)), dim3(kWave), 0, ctx->stream,
-                         m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,
-                         p->dof_eq.p, p->row_ptr.p, pm->ke_arena.p, nullptr, pm->fe_arena.p,
-                         vals, f_int, live);
+      launch_gather(ctx, p, nullptr, pm->fe_arena.p, vals, f_int, live, 1);
       CXG_HIP(hipGetLastError());
     }
     return;

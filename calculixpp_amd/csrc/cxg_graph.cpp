@@ -95,10 +95,33 @@ cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, cons
     auto g = std::make_unique<cxg_graph>();
     g->ctx = ctx;
     g->elements = p->mesh->n_elem;
-    if (ctx->asm_variant == 4) cxg::prepare_arenas(p, false);
+    if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
+    if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
                                   with_f ? f_int : nullptr, nullptr);
+    });
+    *out = g.release();
+  });
+}
+
+cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p,
+                                          const double* coords, const double* E,
+                                          const double* nu, double* vals, int32_t n_cases,
+                                          const double* u, double* f_int, cxg_graph** out) {
+  if (!out) return cxg::fail(ctx, CXG_ERR_ARG, "null graph output");
+  *out = nullptr;
+  if (!p || !coords || !E || !nu || !vals || !u || !f_int)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad capture arguments");
+  if (ctx && p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    auto g = std::make_unique<cxg_graph>();
+    g->ctx = ctx;
+    g->elements = p->mesh->n_elem;
+    if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
+    if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
+    cxg::capture(ctx, g.get(), [&] {
+      cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, u, f_int, nullptr, int(n_cases));
     });
     *out = g.release();
   });
@@ -125,7 +148,8 @@ cxg_status cxg_graph_capture_j2(cxg_context* ctx, const cxg_pattern* p, const do
       g->j2_table.alloc(table.size());
       CXG_HIP(hipMemcpy(g->j2_table.p, table.data(), table.size(), hipMemcpyHostToDevice));
     }
-    if (ctx->asm_variant == 4) cxg::prepare_arenas(p, true);
+    if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, true);
+    if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
     const int32_t* mid = n_mats > 0 ? mat_id : nullptr;
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_j2(ctx, p, coords, E, nu, mid, g->j2_table.p, u, committed, trial,

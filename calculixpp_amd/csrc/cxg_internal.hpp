@@ -103,6 +103,11 @@ struct cxg_context {
   // and the stiffness-kernel variant.
   int patch_size = 8;
   int asm_variant = 4;
+  // Persistent element kernels (asm_variant 5): elements per work item, and resident
+  // waves per compute unit of the persistent grid.
+  int chunk_elems = 4;
+  int persistent_waves = 8;
+  int n_cu = 256;  // compute units of the device (hipDeviceAttributeMultiprocessorCount)
 };
 
 namespace cxg {
@@ -147,6 +152,21 @@ struct cxg_mesh {
 };
 
 namespace cxg {
+// A work item of the persistent element kernels: `count` consecutive elements of
+// block `block`, starting at block-local element `first`.
+struct WorkChunk {
+  int32_t block;
+  int32_t first;
+  int32_t count;
+  int32_t pad;
+};
+// What the persistent kernels need of one block.
+struct BlockDesc {
+  int32_t type, nn, ngp, pad;
+  int64_t elem_offset, gp_offset, conn_offset, kp_base;
+  const double* dN;
+  const double* w;
+};
 // One (node, incident element) pair of the two-phase gather: where the node's three
 // rows of the element matrix start in the arena, where its slot row starts, the
 // element's connectivity offset and node count.
@@ -182,6 +202,13 @@ struct cxg_pattern {
   cxg::DevBuf<int8_t> nfm;                    // per node: bit c set if DOF c is free
   std::vector<int64_t> kp_base;               // kpptr of each block's first element
   int64_t kp_total = 0;
+  // Work queue of the persistent element kernels, built for work_chunk elements per
+  // item (0: not built), costliest family first; queue is the shared item counter.
+  cxg::DevBuf<cxg::WorkChunk> work;
+  cxg::DevBuf<cxg::BlockDesc> bdesc;
+  cxg::DevBuf<int32_t> queue;
+  int32_t n_work = 0;
+  int32_t work_chunk = 0;
 };
 
 namespace cxg {
@@ -189,7 +216,7 @@ namespace cxg {
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             const double* E, const double* nu, double* vals, const double* u,
-                            double* f_int, uint8_t* live);
+                            double* f_int, uint8_t* live, int n_cases = 1);
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
                            const double* E, const double* nu, const double* u, double* f_int);
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
@@ -197,8 +224,10 @@ void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq
 // J2 material tables: host bytes of the device form, and the context's cached upload.
 std::vector<unsigned char> j2_table_bytes(const cxg_j2_material* mats, int32_t n_mats);
 const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32_t n_mats);
-// Allocates the two-phase arenas of a pattern ahead of use (no-op when present).
+// Allocates the two-phase arenas of a pattern ahead of use (no-op when present), and
+// for asm_variant 5 the work queue of the persistent kernels.
 void prepare_arenas(const cxg_pattern* p, bool element_forces);
+void prepare_work(cxg_context* ctx, const cxg_pattern* p);
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
                         const void* j2_table, const double* u, const double* committed,

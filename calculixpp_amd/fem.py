@@ -75,7 +75,7 @@ class Context:
     @property
     def tracks_live(self) -> bool:
         """Whether assemblies can report entry liveness (the two-phase kernels)."""
-        return self.options.get("asm_variant", 4) == 4
+        return self.options.get("asm_variant", 4) >= 4
 
     def profile(self, enable: bool) -> None:
         """Bracket every library kernel launch with HIP events on its stream."""

@@ -162,3 +162,71 @@ def adjoint_block(nx: int = 96, ny: int = 48, nz: int = 48, lengths=(2.0, 1.0, 1
     coords = jitter(coords, lengths[0] / nx, 0.05, 5, lengths)
     rho = np.random.default_rng(6).uniform(0.05, 1.0, size=conn.shape[0])
     return Model(coords, [(_lib.C3D8, conn)], E0 * rho ** penal, 0.3), rho
+
+
+def mixed_box(n_elem: int, seed: int, jitter_frac: float = 0.1):
+    """A structured box of about n_elem elements whose cells are drawn uniformly as one
+    C3D8, one C3D20 or two C3D6 (a hex split along its 0-2 / 4-6 diagonal plane).
+    Nodes live on the quadratic lattice; unused lattice nodes are dropped. Interior
+    nodes are jittered by up to jitter_frac of the spacing. Returns (coords, blocks)
+    with blocks in the order C3D8, C3D20, C3D6 (empty runs left out)."""
+    rng = np.random.default_rng(seed)
+    cells = max(1, int(round(n_elem * 0.75)))  # 4/3 elements per cell on average
+    nx = max(1, int(round(cells ** (1.0 / 3.0))))
+    ny = max(1, int(round((cells / nx) ** 0.5)))
+    nz = max(1, int(round(cells / (nx * ny))))
+    h = 1.0 / max(nx, ny, nz)
+    L = (nx * h, ny * h, nz * h)
+    coords, q = hex_grid(nx, ny, nz, L, _lib.C3D20)
+    kind = rng.integers(0, 3, size=q.shape[0])
+    c8 = q[kind == 0, :8]
+    c20 = q[kind == 1]
+    hx = q[kind == 2, :8]
+    c6 = np.empty((2 * len(hx), 6), np.int32)
+    c6[0::2] = hx[:, [0, 1, 2, 4, 5, 6]]
+    c6[1::2] = hx[:, [0, 2, 3, 4, 6, 7]]
+    used = np.zeros(len(coords), bool)
+    for c in (c8, c20, c6):
+        used[c.ravel()] = True
+    renum = np.full(len(coords), -1, np.int64)
+    renum[used] = np.arange(int(used.sum()))
+    coords = jitter(coords[used], h, jitter_frac, seed + 1, L)
+    blocks = [(t, renum[c].astype(np.int32)) for t, c in
+              ((_lib.C3D8, c8), (_lib.C3D20, c20), (_lib.C3D6, c6)) if len(c)]
+    return coords, blocks
+
+
+J2_TABLE = ([0.0, 1.0], [250.0, 2250.0])
+
+
+def batch_meshes(n_meshes: int = 512, seed: int = 4, e_min: int = 100, e_max: int = 50000,
+                 n_cases: int = 4):
+    """Config 4: n_meshes independent mixed-family boxes with element counts log-uniform
+    in [e_min, e_max] (seed 4), per-mesh E ~ U[70000, 210000] and nu ~ U[0.25, 0.35];
+    half of the meshes (a seeded draw) take a J2 step with the table {(0,250),(1,2250)}
+    and seeded history. Each mesh carries n_cases smooth displacement fields (max |u|
+    1e-3 for linear meshes, 2e-4 for J2 meshes). Returns a list of dicts with keys
+    model, j2 (bool), u (n_cases, n_nodes, 3) and, for J2 meshes, history (13, n_points)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(e_min), np.log(e_max), size=n_meshes)).astype(np.int64)
+    E = rng.uniform(70000.0, 210000.0, size=n_meshes)
+    nu = rng.uniform(0.25, 0.35, size=n_meshes)
+    j2 = np.zeros(n_meshes, bool)
+    j2[rng.choice(n_meshes, n_meshes // 2, replace=False)] = True
+    out = []
+    for i in range(n_meshes):
+        ms = 1000 * (seed + 1) + 10 * i
+        coords, blocks = mixed_box(int(sizes[i]), ms)
+        n_el = sum(c.shape[0] for _, c in blocks)
+        kw = {}
+        if j2[i]:
+            kw = dict(j2_id=np.zeros(n_el, np.int32),
+                      j2=[J2Params(float(E[i]), float(nu[i]), *J2_TABLE)])
+        m = Model(coords, blocks, float(E[i]), float(nu[i]), **kw)
+        amp = 2e-4 if j2[i] else 1e-3
+        u = np.stack([smooth_field(coords, ms + 2 + c, amp) for c in range(n_cases)])
+        rec = {"model": m, "j2": bool(j2[i]), "u": u}
+        if j2[i]:
+            rec["history"] = seeded_history(m.n_points, ms + 7)
+        out.append(rec)
+    return out

@@ -85,7 +85,7 @@ typedef struct {
 /* ---- context ------------------------------------------------------------- */
 /* Create a context on HIP device `device`, enqueueing on `stream` (a
  * hipStream_t; NULL = the null stream). No counterpart in the reference, which is
- * single-threaded host code (SURVEY: src/compute/cpu_backend.cpp:108-111). */
+ * single-threaded host code (src/compute/cpu_backend.cpp:108-111). */
 cxg_status cxg_create(int device, void* stream, cxg_context** out);
 void cxg_destroy(cxg_context* ctx);
 cxg_status cxg_set_stream(cxg_context* ctx, void* stream);
@@ -95,7 +95,10 @@ const char* cxg_last_error(const cxg_context* ctx);
  *   "asm_variant" linear stiffness kernel: 1 = per-pair scatter, 2 = M-identity with
  *                 row-coalesced scatter at 4 waves/SIMD, 3 = the same at 3, 4 =
  *                 two-phase: element matrices to an arena, then each CSR row gathered
- *                 and written once by its node (default)
+ *                 and written once by its node (default), 5 = two-phase with one
+ *                 persistent element launch over all families, fed by a work queue
+ *   "chunk_elems"      elements per work item of the persistent launch (1..4096, 4)
+ *   "persistent_waves" resident waves per compute unit of that launch (1..32, 8)
  * Returns CXG_ERR_ARG for an unknown key or value. */
 cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value);
 /* Library version string, e.g. "cxg 0.1 gfx950". */
@@ -153,10 +156,19 @@ cxg_status cxg_pattern_compact(cxg_context* ctx, const cxg_pattern* p, const uin
  * Optional live[nnz]: set to 1 where at least one element contributed a value other
  * than 0.0, else 0. The reference leaves entries whose every contribution is 0.0 out
  * of its COO map (scatter_block, fem/assembly.cpp:90); cxg_pattern_compact applies
- * that rule. Needs the two-phase kernels (option "asm_variant" 4, the default). */
+ * that rule. Needs the two-phase kernels (option "asm_variant" 4, the default, or 5). */
 cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                const double* E, const double* nu, double* vals,
                                const double* u, double* f_int, uint8_t* live);
+
+/* cxg_assemble_linear for n_cases displacement fields at once (load cases of one
+ * mesh or batch): u and f_int hold n_cases consecutive vectors of 3*n_nodes; K is
+ * formed once and every f_int[c] = K_full u[c] comes from the same pass over the
+ * element rows. n_cases 1, 2 or 4 (2 and 4 need asm_variant 4 or 5). */
+cxg_status cxg_assemble_linear_cases(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                     const double* E, const double* nu, double* vals,
+                                     int32_t n_cases, const double* u, double* f_int,
+                                     uint8_t* live);
 
 /* ---- K5: matrix-free internal force f_int = sum_e Ke(x,E) ue ----------------
  * fem::internal_force (fem/stress.cpp:125-145) without forming Ke: per Gauss point
@@ -217,6 +229,11 @@ cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
 cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                     const double* E, const double* nu, double* vals,
                                     const double* u, double* f_int, cxg_graph** out);
+/* As cxg_graph_capture_linear for cxg_assemble_linear_cases. */
+cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p,
+                                          const double* coords, const double* E,
+                                          const double* nu, double* vals, int32_t n_cases,
+                                          const double* u, double* f_int, cxg_graph** out);
 cxg_status cxg_graph_capture_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                 const double* E, const double* nu, const int32_t* mat_id,
                                 const cxg_j2_material* mats, int32_t n_mats, const double* u,

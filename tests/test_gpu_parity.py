@@ -111,11 +111,11 @@ def test_internal_force_fused_and_matrix_free(gpu, port, fam_i):
     assert rel_l2(f.cpu().numpy(), exp) <= REL_K
 
 
-@pytest.mark.parametrize("variant", [4, 1])
+@pytest.mark.parametrize("variant", [4, 1, 5])
 @pytest.mark.parametrize("fam_i", range(4))
 def test_j2_tangent_force_state(gpu, port, fam_i, variant):
-    """K_t, f_int and trial history of the two-phase (4) and colour-scatter (1) paths;
-    every 5th element stays isotropic elastic."""
+    """K_t, f_int and trial history of the two-phase (4), colour-scatter (1) and
+    persistent (5) paths; every 5th element stays isotropic elastic."""
     from calculixpp_amd import meshgen
     from oracle.oracle import aos_state
     fam = _families()[fam_i]
@@ -188,7 +188,8 @@ def test_deterministic_bitwise(gpu):
     assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
 
 
-@pytest.mark.parametrize("variant,patch", [(1, 1), (1, 5), (2, 1), (2, 3), (2, 8), (3, 16), (4, 1), (4, 8)])
+@pytest.mark.parametrize("variant,patch", [(1, 1), (1, 5), (2, 1), (2, 3), (2, 8), (3, 16), (4, 1), (4, 8),
+                                           (5, 8)])
 @pytest.mark.parametrize("fam_i", range(4))
 def test_kernel_variants_and_patches(gpu, port, variant, patch, fam_i):
     """Every stiffness kernel variant and patch length gives the same K and f_int."""
