@@ -177,6 +177,89 @@ def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
     return out
 
 
+def batch_step(n_meshes: int = 512, reps: int = 3, cpu: bool = True, cpu_seconds: float = 15.0,
+               log=None) -> dict:
+    """The batched workload: n_meshes mixed C3D8/C3D20/C3D6 boxes with log-uniform
+    element counts in [100, 50000] (seed 4), half of them J2, 4 load cases each. One
+    step = the linear meshes' K plus f_int for the 4 cases, and for the J2 meshes per
+    case the return map, K_t and f_int; elements of every mesh and family go through one
+    persistent launch per assembly. Throughput counts element evaluations (a linear
+    element once, a J2 element once per case)."""
+    import torch
+    from calculixpp_amd import autotune, meshgen
+    from calculixpp_amd.batch import BatchStep
+    from calculixpp_amd import fem
+    t0 = time.perf_counter()
+    recs = meshgen.batch_meshes(n_meshes)
+    t1 = time.perf_counter()
+    ctx = fem.context()
+    step = BatchStep(recs, ctx, variant=5)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    if log:
+        log(f"[bench] batch: {n_meshes} meshes generated in {t1 - t0:.1f}s, uploaded + "
+            f"patterns in {t2 - t1:.1f}s")
+    tune = autotune.tune_persistent(step.run, ctx, log=log)
+    n_lin = step.lin.model.n_elem if step.lin else 0
+    n_j2 = step.j2.model.n_elem if step.j2 else 0
+    evals = n_lin + step.n_cases * n_j2
+    step.run()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(reps):
+        step.run()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - ts) * 1e3 / reps
+    step.capture()
+    step.launch()
+    ctx.check_kernels()
+    ts = time.perf_counter()
+    for _ in range(reps):
+        step.launch()
+    ctx.check_kernels()
+    graph_ms = (time.perf_counter() - ts) * 1e3 / reps
+    ctx.profile_reset()
+    ctx.profile(True)
+    step.launch()
+    step.run()
+    ctx.profile(False)
+    kern = {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in ctx.profile_stats().items()}
+    nnz = sum(a.pattern.nnz for a in [getattr(step, "lin_asm", None), getattr(step, "j2_asm", None)]
+              if a is not None)
+    out = {"workload": f"{n_meshes} mixed C3D8/C3D20/C3D6 meshes, log-uniform 100..50000 "
+                       f"elements, half J2, {step.n_cases} load cases",
+           "meshes": n_meshes, "elements": n_lin + n_j2, "linear_elements": n_lin,
+           "j2_elements": n_j2, "element_evaluations": evals, "nnz": nnz,
+           "ms_per_step": eager_ms, "graph_ms_per_step": graph_ms,
+           "evaluations_per_s": evals / (min(eager_ms, graph_ms) * 1e-3),
+           "autotune": tune, "kernels_one_graph_plus_one_eager_step": kern}
+    ctx.set_option("asm_variant", 4)
+    del step
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle.oracle import Oracle, aos_state, ref_available
+        kind = "reference" if ref_available() else "port"
+        o = Oracle(kind)
+        sec, ev, used = 0.0, 0, 0
+        for r in recs:  # the batch's meshes in order, until about cpu_seconds of work
+            if sec >= cpu_seconds:
+                break
+            used += 1
+            m = r["model"]
+            if r["j2"]:
+                for u in r["u"]:
+                    sec += o.time_material_tangent(m, u, aos_state(r["history"]))
+                ev += len(r["u"]) * m.n_elem
+            else:
+                sec += o.time_linear_static(m, r["u"][0])
+                ev += m.n_elem
+        out["cpu_baseline"] = {"value": ev / sec, "unit": "element evaluations/s", "cores": 1,
+                               "kind": kind, "sample": f"the batch's first {used} meshes "
+                               f"({ev} element evaluations, {sec:.1f} s on one host thread)"}
+        out["gpu_over_cpu"] = out["evaluations_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
     """Reference CPU assembly (+ internal force) on an n^3 sample of the same
     workload, one thread (the reference is single-threaded)."""
@@ -205,6 +288,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-reps", type=int, default=20)
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
+    ap.add_argument("--batch-meshes", type=int, default=512)
     ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
     ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -317,6 +402,13 @@ def main() -> int:
               f"({j2['elements_per_s']:.4g} elements/s, yield {j2['yield_fraction']:.3f})",
               file=sys.stderr, flush=True)
 
+    batch = None if args.no_batch else batch_step(
+        args.batch_meshes, cpu=not args.no_cpu, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if batch:
+        print(f"[bench] batch step {batch['ms_per_step']:.2f} ms eager, "
+              f"{batch['graph_ms_per_step']:.2f} ms graph ({batch['evaluations_per_s']:.4g} "
+              "element evaluations/s)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(family, args.cpu_n, residual)
@@ -340,6 +432,7 @@ def main() -> int:
         "kernels": kernels,
         "adjoint": adjoint,
         "j2_newton": j2,
+        "batch": batch,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(out), flush=True)

@@ -11,6 +11,12 @@ Two knobs decide how the stiffness assembly maps onto the GPU:
 The tuner times each candidate on the actual mesh (HIP events on the library's
 stream) and caches the winner in a JSON file keyed by element family, element
 count and nonzero count, so later runs on the same workload reuse it.
+
+The persistent launch (asm_variant 5, batches of uneven meshes) has two more knobs,
+tuned by tune_persistent on the actual batch step:
+  chunk_elems       elements per work item (the unit a workgroup takes from the
+                    queue: small items balance better, large ones cost fewer atomics)
+  persistent_waves  resident waves per compute unit of the persistent grid.
 """
 from __future__ import annotations
 
@@ -97,4 +103,34 @@ def tune_linear(model: Model, u=None, candidates: Iterable = CANDIDATES, steps: 
     if cache_path:
         cache[key] = best
         save_cache(cache, cache_path)
+    return best
+
+
+PERSISTENT_CANDIDATES = [(1, 8), (2, 8), (4, 8), (8, 8), (16, 8), (4, 4), (4, 16), (8, 16)]
+
+
+def tune_persistent(run, ctx: fem.Context, candidates: Iterable = PERSISTENT_CANDIDATES,
+                    steps: int = 2, log=None) -> dict:
+    """Pick (chunk_elems, persistent_waves) for a callable `run` that enqueues one step
+    on ctx's stream with asm_variant 5 (e.g. batch.BatchStep.run); the winner is left
+    set on ctx. Returns {"chunk_elems", "persistent_waves", "ms", "trials"}."""
+    trials, best = [], None
+    for chunk, waves in candidates:
+        ctx.set_option("chunk_elems", chunk)
+        ctx.set_option("persistent_waves", waves)
+        run()  # rebuilds the work queue for this item size
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        trials.append({"chunk_elems": chunk, "persistent_waves": waves, "ms": ms})
+        if log:
+            log(f"[autotune] chunk {chunk} waves {waves}: {ms:.3f} ms")
+        if best is None or ms < best["ms"]:
+            best = {"chunk_elems": chunk, "persistent_waves": waves, "ms": ms}
+    ctx.set_option("chunk_elems", best["chunk_elems"])
+    ctx.set_option("persistent_waves", best["persistent_waves"])
+    best["trials"] = trials
     return best
