@@ -87,6 +87,10 @@ _SIGS = {
     "cxg_graph_info": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "cxg_graph_destroy": (None, [_P]),
     "cxg_check": (C.c_int, [_P]),
+    "cxg_malloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "cxg_free": (C.c_int, [_P, _P]),
+    "cxg_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "cxg_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "cxg_profile_enable": (C.c_int, [_P, C.c_int]),
     "cxg_profile_reset": (C.c_int, [_P]),
     "cxg_profile_count": (C.c_int32, [_P]),

@@ -288,6 +288,8 @@ void cxg_destroy(cxg_context* ctx) {
     (void)hipEventDestroy(p.b);
   }
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (!ctx->allocs.empty()) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->flag_dev) (void)hipFree(ctx->flag_dev);
   if (ctx->flag_host) (void)hipHostFree(ctx->flag_host);
   delete ctx;
@@ -300,6 +302,43 @@ cxg_status cxg_set_stream(cxg_context* ctx, void* stream) {
 }
 
 const char* cxg_last_error(const cxg_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+cxg_status cxg_malloc(cxg_context* ctx, size_t bytes, void** out) {
+  if (!out) return cxg::fail(ctx, CXG_ERR_ARG, "null output");
+  *out = nullptr;
+  return cxg::guarded(ctx, [&] {
+    void* p = nullptr;
+    CXG_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    ctx->allocs.insert(p);
+    *out = p;
+  });
+}
+
+cxg_status cxg_free(cxg_context* ctx, void* p) {
+  if (!p) return CXG_OK;
+  return cxg::guarded(ctx, [&] {
+    if (ctx->allocs.erase(p) == 0) throw cxg::Error(CXG_ERR_ARG, "pointer not allocated by cxg_malloc");
+    CXG_HIP(hipFree(p));
+  });
+}
+
+cxg_status cxg_copy_to_device(cxg_context* ctx, void* dst, const void* src, size_t bytes) {
+  if ((!dst || !src) && bytes) return cxg::fail(ctx, CXG_ERR_ARG, "null copy argument");
+  return cxg::guarded(ctx, [&] {
+    if (!bytes) return;
+    CXG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    CXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+cxg_status cxg_copy_to_host(cxg_context* ctx, void* dst, const void* src, size_t bytes) {
+  if ((!dst || !src) && bytes) return cxg::fail(ctx, CXG_ERR_ARG, "null copy argument");
+  return cxg::guarded(ctx, [&] {
+    if (!bytes) return;
+    CXG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    CXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
 
 cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
   if (!ctx || !key) return CXG_ERR_ARG;

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "cxg.h"
@@ -108,6 +109,7 @@ struct cxg_context {
   int chunk_elems = 4;
   int persistent_waves = 8;
   int n_cu = 256;  // compute units of the device (hipDeviceAttributeMultiprocessorCount)
+  std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
 };
 
 namespace cxg {

@@ -1,0 +1,427 @@
+// cxpp::gpu — the reference's assembly entry points on libcxg.so (see cxpp_gpu.hpp).
+//
+// Each call turns the Model into the C ABI's arrays (node coordinates in index order,
+// connectivity as node indices in runs of one element family, per-element E, nu and
+// J2 material), assembles on the device over the pattern in which every DOF is free,
+// and reduces on the host into the reference's LinearSystem: a free/free entry lands
+// at (dof_eq[row], dof_eq[col]) when some element contributed a value other than 0.0
+// to it (the key rule of scatter_block, src/fem/assembly.cpp:76-100); an entry in a
+// prescribed column moves to the load, rhs[dof_eq[row]] -= value * prescribed[col]
+// (assembly.cpp:95-96); rows of constrained DOFs are dropped (assembly.cpp:86). The
+// DOF map and the constraint transform are the reference's own (build_dof_map,
+// assembly.cpp:19-42, over fem::build_constraint_transform) and must reduce to single
+// point constraints; external loads come from fem::external_load_vector, as in
+// assemble_linear_static (assembly.cpp:250).
+#include "cxpp_gpu.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "calculixpp/fem/constraint_transform.hpp"
+#include "calculixpp/fem/loads.hpp"
+#include "cxg.h"
+
+namespace cxpp::gpu {
+namespace {
+
+constexpr int kDof = 3;
+
+thread_local int t_device = 0;
+thread_local cxg_context* t_ctx = nullptr;  // kept until the process ends
+thread_local int t_ctx_device = -1;
+
+cxg_context* context() {
+  if (!t_ctx || t_ctx_device != t_device) {
+    if (t_ctx) cxg_destroy(t_ctx);
+    t_ctx = nullptr;
+    cxg_context* h = nullptr;
+    if (cxg_create(t_device, nullptr, &h) != CXG_OK)
+      throw std::runtime_error("cxpp::gpu: no libcxg context on device " + std::to_string(t_device));
+    t_ctx = h;
+    t_ctx_device = t_device;
+  }
+  return t_ctx;
+}
+
+void check(cxg_context* c, cxg_status st) {
+  if (st != CXG_OK) throw std::runtime_error(cxg_last_error(c));
+}
+
+// Device buffer owned by the context (cxg_malloc / cxg_free).
+template <class T>
+class Buf {
+ public:
+  Buf(cxg_context* c, size_t n) : c_(c), n_(n) {
+    void* p = nullptr;
+    check(c, cxg_malloc(c, n * sizeof(T), &p));
+    p_ = static_cast<T*>(p);
+  }
+  Buf(cxg_context* c, const std::vector<T>& host) : Buf(c, host.size()) { put(host.data()); }
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() {
+    if (p_) cxg_free(c_, p_);
+  }
+  T* get() const { return p_; }
+  void put(const T* src) { check(c_, cxg_copy_to_device(c_, p_, src, n_ * sizeof(T))); }
+  std::vector<T> fetch(size_t n) const {
+    std::vector<T> out(n);
+    check(c_, cxg_copy_to_host(c_, out.data(), p_, n * sizeof(T)));
+    return out;
+  }
+
+ private:
+  cxg_context* c_;
+  size_t n_;
+  T* p_ = nullptr;
+};
+
+bool on_gpu_path(ElementType t) {
+  return t == ElementType::C3D8 || t == ElementType::C3D20 || t == ElementType::C3D20R ||
+         t == ElementType::C3D6;
+}
+
+int gauss_points(ElementType t) {
+  const int n = cxg_gauss_points(static_cast<int32_t>(t));
+  if (n <= 0) throw std::runtime_error("cxpp::gpu: element type outside the GPU path");
+  return n;
+}
+
+// A model's active elements on the device, as runs of one family in model order.
+struct DeviceMesh {
+  cxg_context* c;
+  std::vector<size_t> elem;       // reference element index of each device element
+  std::vector<int64_t> gp_first;  // first device Gauss point of each device element
+  int32_t n_nodes = 0;
+  int64_t n_points = 0;
+  std::unique_ptr<Buf<double>> coords, E, nu;
+  std::unique_ptr<Buf<int32_t>> conn;
+  cxg_mesh* mesh = nullptr;
+
+  DeviceMesh(cxg_context* ctx, const Model& model) : c(ctx) {
+    const Mesh& m = model.mesh;
+    n_nodes = static_cast<int32_t>(m.num_nodes());
+    std::vector<double> x(m.num_nodes() * kDof);
+    for (size_t i = 0; i < m.num_nodes(); ++i)
+      for (int k = 0; k < kDof; ++k) x[kDof * i + size_t(k)] = m.nodes()[i].x[size_t(k)];
+    const std::vector<ElasticIso> el = model.element_elastic();
+    const std::vector<bool> active = model.element_active_mask();
+    std::vector<int32_t> cn;
+    std::vector<double> ev, nv;
+    std::vector<cxg_block> blocks;
+    std::vector<size_t> block_conn;
+    for (size_t e = 0; e < m.num_elements(); ++e) {
+      if (!active[e]) continue;  // *MODEL CHANGE, REMOVE: no stiffness (assembly.cpp:238)
+      const Element& elem_e = m.elements()[e];
+      if (!on_gpu_path(elem_e.type))
+        throw std::runtime_error("cxpp::gpu: element type outside the GPU path (C3D8, C3D20, "
+                                 "C3D20R, C3D6)");
+      const int32_t t = static_cast<int32_t>(elem_e.type);
+      if (blocks.empty() || blocks.back().type != t) {
+        blocks.push_back(cxg_block{t, 0, int64_t(elem.size()), n_points, nullptr});
+        block_conn.push_back(cn.size());
+      }
+      ++blocks.back().n_elem;
+      for (Index id : elem_e.nodes) {
+        const Index ni = m.node_index(id);
+        if (ni < 0) throw std::runtime_error("element references unknown node");
+        cn.push_back(ni);
+      }
+      elem.push_back(e);
+      gp_first.push_back(n_points);
+      n_points += gauss_points(elem_e.type);
+      ev.push_back(el[e].E);
+      nv.push_back(el[e].nu);
+    }
+    coords = std::make_unique<Buf<double>>(c, x);
+    conn = std::make_unique<Buf<int32_t>>(c, cn);
+    E = std::make_unique<Buf<double>>(c, ev);
+    nu = std::make_unique<Buf<double>>(c, nv);
+    for (size_t b = 0; b < blocks.size(); ++b) blocks[b].conn = conn->get() + block_conn[b];
+    check(c, cxg_mesh_create(c, n_nodes, int32_t(blocks.size()), blocks.data(), &mesh));
+  }
+  DeviceMesh(const DeviceMesh&) = delete;
+  DeviceMesh& operator=(const DeviceMesh&) = delete;
+  ~DeviceMesh() {
+    if (mesh) cxg_mesh_destroy(mesh);
+  }
+};
+
+// The pattern in which every DOF of the mesh is a free equation.
+struct AllFreePattern {
+  cxg_context* c;
+  cxg_pattern* p = nullptr;
+  int64_t nnz = 0;
+  int32_t n = 0;
+  AllFreePattern(cxg_context* ctx, const cxg_mesh* mesh, int32_t n_nodes) : c(ctx) {
+    const int64_t nd = int64_t(n_nodes) * kDof;
+    if (nd > INT32_MAX) throw std::runtime_error("cxpp::gpu: model too large for int32 equations");
+    n = int32_t(nd);
+    std::vector<int32_t> eq(static_cast<size_t>(n));
+    for (int32_t g = 0; g < n; ++g) eq[size_t(g)] = g;
+    Buf<int32_t> d(c, eq);
+    check(c, cxg_pattern_build(c, mesh, d.get(), n, &p));
+    int32_t nf = 0;
+    check(c, cxg_pattern_info(p, &nf, &nnz));
+  }
+  AllFreePattern(const AllFreePattern&) = delete;
+  AllFreePattern& operator=(const AllFreePattern&) = delete;
+  ~AllFreePattern() {
+    if (p) cxg_pattern_destroy(p);
+  }
+};
+
+// build_dof_map (assembly.cpp:19-42) through the public constraint transform; the GPU
+// path takes single point constraints only.
+void dof_map(const Model& model, fem::LinearSystem& sys) {
+  const Mesh& mesh = model.mesh;
+  const size_t n_dof = mesh.num_nodes() * kDof;
+  sys.prescribed.assign(n_dof, 0.0);
+  sys.prescribed_amp.assign(n_dof, std::string{});
+  std::vector<bool> constrained(n_dof, false);
+  for (const Spc& spc : model.spcs) {
+    const Index ni = mesh.node_index(spc.node_id);
+    if (ni < 0) throw std::runtime_error("*BOUNDARY references unknown node");
+    if (spc.comp < 1 || spc.comp > kDof) throw std::runtime_error("*BOUNDARY dof out of range");
+    const size_t g = size_t(ni) * kDof + size_t(spc.comp - 1);
+    constrained[g] = true;
+    sys.prescribed[g] = spc.value;
+    sys.prescribed_amp[g] = spc.amplitude;
+  }
+  sys.transform = fem::build_constraint_transform(model, constrained, sys.prescribed);
+  sys.dof_eq = sys.transform.dof_eq;
+  sys.n_free = sys.transform.n_free;
+  sys.rhs.assign(size_t(sys.n_free), 0.0);
+  for (size_t g = 0; g < n_dof; ++g) {
+    const fem::DofExpansion& x = sys.transform.expansion[g];
+    const bool single = sys.dof_eq[g] >= 0
+                            ? x.terms.size() == 1 && x.terms[0].eq == sys.dof_eq[g] &&
+                                  x.terms[0].coeff == 1.0 && x.constant == 0.0
+                            : x.terms.empty() && !sys.transform.is_slave[g];
+    if (!single)
+      throw std::runtime_error("cxpp::gpu: *EQUATION / MPC / rigid / tie constraints are "
+                               "outside the GPU path");
+  }
+}
+
+void require_mechanical(const Model& model) {
+  if (!model.springs.empty()) throw std::runtime_error("cxpp::gpu: *SPRING is outside the GPU path");
+  for (Real v : fem::thermal_strain_load_vector(model))
+    if (v != 0.0) throw std::runtime_error("cxpp::gpu: thermal loads are outside the GPU path");
+}
+
+// All-free CSR + live flags -> the reference's reduced COO and load correction.
+void reduce(const AllFreePattern& pat, const std::vector<double>& vals,
+            const std::vector<uint8_t>& live, fem::LinearSystem& sys) {
+  Buf<int64_t> rp_dev(pat.c, size_t(pat.n) + 1);
+  Buf<int32_t> ci_dev(pat.c, size_t(std::max<int64_t>(pat.nnz, 1)));
+  check(pat.c, cxg_pattern_copy(pat.c, pat.p, rp_dev.get(), ci_dev.get()));
+  const std::vector<int64_t> rp = rp_dev.fetch(size_t(pat.n) + 1);
+  const std::vector<int32_t> ci = ci_dev.fetch(size_t(pat.nnz));
+  sys.rows.clear();
+  sys.cols.clear();
+  sys.vals.clear();
+  for (int32_t g = 0; g < pat.n; ++g) {
+    const Index r = sys.dof_eq[size_t(g)];
+    if (r < 0) continue;  // constrained row
+    for (int64_t k = rp[size_t(g)]; k < rp[size_t(g) + 1]; ++k) {
+      if (!live[size_t(k)]) continue;  // every contribution was 0.0
+      const int32_t h = ci[size_t(k)];
+      const Index col = sys.dof_eq[size_t(h)];
+      if (col >= 0) {
+        sys.rows.push_back(r);
+        sys.cols.push_back(col);
+        sys.vals.push_back(vals[size_t(k)]);
+      } else if (sys.prescribed[size_t(h)] != 0.0) {
+        sys.rhs[size_t(r)] -= vals[size_t(k)] * sys.prescribed[size_t(h)];
+      }
+    }
+  }
+}
+
+void add_external_load(const Model& model, fem::LinearSystem& sys) {
+  const std::vector<Real> f = fem::external_load_vector(model);
+  for (size_t g = 0; g < f.size(); ++g)
+    if (f[g] != 0.0 && sys.dof_eq[g] >= 0) sys.rhs[size_t(sys.dof_eq[g])] += f[g];
+}
+
+cxg_j2_material to_cxg(const ElasticIso& el, const Plastic& pl) {
+  cxg_j2_material m{};
+  m.E = el.E;
+  m.nu = el.nu;
+  m.hardening = pl.hardening == Plastic::Hardening::Kinematic  ? CXG_HARD_KINEMATIC
+                : pl.hardening == Plastic::Hardening::Combined ? CXG_HARD_COMBINED
+                                                               : CXG_HARD_ISOTROPIC;
+  if (pl.yield.empty() || pl.yield.size() > CXG_MAX_TABLE || pl.yield.size() != pl.eqplastic.size())
+    throw std::runtime_error("cxpp::gpu: *PLASTIC table outside the GPU path (1.." +
+                             std::to_string(CXG_MAX_TABLE) + " points)");
+  m.n_table = int32_t(pl.yield.size());
+  for (size_t i = 0; i < pl.yield.size(); ++i) {
+    m.eqplastic[i] = pl.eqplastic[i];
+    m.yield[i] = pl.yield[i];
+  }
+  m.kinematic_modulus = pl.kinematic_modulus;
+  return m;
+}
+
+}  // namespace
+
+void set_device(int device) { t_device = device; }
+
+fem::LinearSystem assemble_linear_static(const Model& model) {
+  require_mechanical(model);
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
+  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
+  Buf<double> vals(c, nnz);
+  Buf<uint8_t> live(c, nnz);
+  check(c, cxg_assemble_linear(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(), vals.get(),
+                               nullptr, nullptr, live.get()));
+  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  add_external_load(model, sys);
+  return sys;
+}
+
+fem::LinearSystem assemble_material_tangent(const Model& model, const std::vector<Vec3>& u,
+                                            fem::MaterialPoints& mp, std::vector<Real>& f_int) {
+  require_mechanical(model);
+  for (const auto& h : model.element_hyperelastic())
+    if (h && !h->empty()) throw std::runtime_error("cxpp::gpu: *HYPERELASTIC is outside the GPU path");
+  for (const auto& um : model.element_user_material())
+    if (um && !um->empty()) throw std::runtime_error("cxpp::gpu: *USER MATERIAL is outside the GPU path");
+  const Mesh& mesh = model.mesh;
+  if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
+  if (mp.state.size() != mesh.num_elements())
+    throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  const size_t ne = dm.elem.size();
+  // J2 materials: one table entry per distinct (elastic, plastic) pair
+  const std::vector<ElasticIso> el = model.element_elastic();
+  const std::vector<std::optional<Plastic>> pl = model.element_plastic();
+  std::vector<cxg_j2_material> mats;
+  std::map<std::vector<double>, int32_t> seen;
+  std::vector<int32_t> mat_id(ne, -1);
+  for (size_t k = 0; k < ne; ++k) {
+    const size_t e = dm.elem[k];
+    if (!pl[e] || pl[e]->empty()) continue;
+    const cxg_j2_material m = to_cxg(el[e], *pl[e]);
+    std::vector<double> key{m.E, m.nu, double(m.hardening), double(m.n_table), m.kinematic_modulus};
+    key.insert(key.end(), m.eqplastic, m.eqplastic + m.n_table);
+    key.insert(key.end(), m.yield, m.yield + m.n_table);
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      it = seen.emplace(key, int32_t(mats.size())).first;
+      mats.push_back(m);
+    }
+    mat_id[k] = it->second;
+  }
+  // committed history, SoA: plastic strain (6), eqplastic, back stress (6)
+  const size_t P = size_t(dm.n_points);
+  std::vector<double> committed(size_t(CXG_STATE_COMPONENTS) * P);
+  for (size_t k = 0; k < ne; ++k) {
+    const std::vector<fem::MaterialState>& st = mp.state[dm.elem[k]];
+    const size_t ng = size_t(gauss_points(mesh.elements()[dm.elem[k]].type));
+    if (st.size() != ng) throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
+    for (size_t q = 0; q < ng; ++q) {
+      const size_t pt = size_t(dm.gp_first[k]) + q;
+      for (size_t i = 0; i < 6; ++i) {
+        committed[i * P + pt] = st[q].committed_plastic_strain[i];
+        committed[(7 + i) * P + pt] = st[q].committed_back_stress[i];
+      }
+      committed[6 * P + pt] = st[q].committed_eqplastic;
+    }
+  }
+  std::vector<double> uh(mesh.num_nodes() * kDof);
+  for (size_t i = 0; i < mesh.num_nodes(); ++i)
+    for (int k = 0; k < kDof; ++k) uh[kDof * i + size_t(k)] = u[i][size_t(k)];
+  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
+  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
+  Buf<double> vals(c, nnz), fd(c, uh.size()), ud(c, uh), cd(c, committed), td(c, committed.size());
+  Buf<uint8_t> live(c, nnz);
+  Buf<int32_t> mid(c, std::max<size_t>(ne, 1));
+  if (ne) mid.put(mat_id.data());
+  check(c, cxg_assemble_j2(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(),
+                           mats.empty() ? nullptr : mid.get(), mats.empty() ? nullptr : mats.data(),
+                           int32_t(mats.size()), ud.get(), cd.get(), td.get(), vals.get(),
+                           fd.get(), live.get()));
+  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  add_external_load(model, sys);
+  f_int = fd.fetch(uh.size());
+  const std::vector<double> trial = td.fetch(committed.size());
+  for (size_t k = 0; k < ne; ++k) {
+    std::vector<fem::MaterialState>& st = mp.state[dm.elem[k]];
+    for (size_t q = 0; q < st.size(); ++q) {
+      const size_t pt = size_t(dm.gp_first[k]) + q;
+      for (size_t i = 0; i < 6; ++i) {
+        st[q].plastic_strain[i] = trial[i * P + pt];
+        st[q].back_stress[i] = trial[(7 + i) * P + pt];
+      }
+      st[q].eqplastic = trial[6 * P + pt];
+    }
+  }
+  return sys;
+}
+
+std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u) {
+  require_mechanical(model);
+  const Mesh& mesh = model.mesh;
+  if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  std::vector<double> uh(mesh.num_nodes() * kDof);
+  for (size_t i = 0; i < mesh.num_nodes(); ++i)
+    for (int k = 0; k < kDof; ++k) uh[kDof * i + size_t(k)] = u[i][size_t(k)];
+  Buf<double> ud(c, uh), fd(c, uh.size());
+  check(c, cxg_internal_force(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(), ud.get(),
+                              fd.get()));
+  return fd.fetch(uh.size());
+}
+
+std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
+                                    const ElasticIso& elastic) {
+  if (!on_gpu_path(type))
+    throw std::runtime_error("cxpp::gpu: element type outside the GPU path (C3D8, C3D20, C3D20R, C3D6)");
+  const int n = cxg_nodes_per_element(static_cast<int32_t>(type));
+  if (coords.size() != size_t(n)) throw std::runtime_error("cxpp::gpu: coords do not match the element type");
+  cxg_context* c = context();
+  std::vector<double> x(size_t(n) * kDof);
+  std::vector<int32_t> cn(static_cast<size_t>(n));
+  for (int a = 0; a < n; ++a) {
+    cn[size_t(a)] = a;
+    for (int k = 0; k < kDof; ++k) x[size_t(kDof * a + k)] = coords[size_t(a)][size_t(k)];
+  }
+  Buf<double> xd(c, x), Ed(c, std::vector<double>{elastic.E}), nd(c, std::vector<double>{elastic.nu});
+  Buf<int32_t> cd(c, cn);
+  cxg_block b{static_cast<int32_t>(type), 1, 0, 0, cd.get()};
+  cxg_mesh* mesh = nullptr;
+  check(c, cxg_mesh_create(c, n, 1, &b, &mesh));
+  std::unique_ptr<cxg_mesh, void (*)(cxg_mesh*)> hold(mesh, cxg_mesh_destroy);
+  const AllFreePattern pat(c, mesh, n);
+  Buf<double> vals(c, size_t(pat.nnz));
+  check(c, cxg_assemble_linear(c, pat.p, xd.get(), Ed.get(), nd.get(), vals.get(), nullptr,
+                               nullptr, nullptr));
+  Buf<int64_t> rp_dev(c, size_t(pat.n) + 1);
+  Buf<int32_t> ci_dev(c, size_t(pat.nnz));
+  check(c, cxg_pattern_copy(c, pat.p, rp_dev.get(), ci_dev.get()));
+  const std::vector<int64_t> rp = rp_dev.fetch(size_t(pat.n) + 1);
+  const std::vector<int32_t> ci = ci_dev.fetch(size_t(pat.nnz));
+  const std::vector<double> v = vals.fetch(size_t(pat.nnz));
+  const size_t nd3 = size_t(kDof * n);
+  std::vector<Real> K(nd3 * nd3, 0.0);
+  for (size_t r = 0; r < nd3; ++r)
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) K[r * nd3 + size_t(ci[size_t(k)])] = v[size_t(k)];
+  return K;
+}
+
+}  // namespace cxpp::gpu

@@ -1,0 +1,52 @@
+// Drop-in GPU versions of the reference's assembly entry points, for C++20 callers.
+//
+// Same signatures, argument meaning, ownership (value semantics) and error behaviour
+// (std::runtime_error with the reference's messages) as the cxpp::fem functions they
+// replace; they run on libcxg.so through its C ABI (include/cxg.h):
+//
+//   cxpp::gpu::assemble_linear_static      fem::assemble_linear_static
+//                                          (include/calculixpp/fem/assembly.hpp:42,
+//                                           src/fem/assembly.cpp:225-257)
+//   cxpp::gpu::assemble_material_tangent   fem::assemble_material_tangent
+//                                          (assembly.hpp:105-108, assembly.cpp:385-423)
+//   cxpp::gpu::internal_force              fem::internal_force
+//                                          (fem/stress.hpp:16, stress.cpp:125-145)
+//   cxpp::gpu::element_stiffness           fem::element_stiffness
+//                                          (fem/element.hpp:62, element.cpp:614-634)
+//
+// This file and cxpp_gpu.cpp compile inside the reference's tree (they include its
+// headers) and link against libcxg.so. A model the GPU path does not cover (element
+// types other than C3D8/C3D20/C3D20R/C3D6, MPC/rigid/tie equations, springs, thermal
+// loads, hyperelastic or user materials) raises std::runtime_error naming the
+// feature; nothing falls back to the CPU.
+//
+// Results: K as COO triplets in CSR order (rows ascending, columns ascending inside a
+// row) with the reference's key set (entries whose every element contribution is 0.0
+// are left out, assembly.cpp:90); values within rel-L2 1e-12 of the reference's.
+// One libcxg context per host thread, on the device chosen by set_device (default
+// 0), created on first use and kept until the process ends.
+#pragma once
+
+#include <span>
+#include <vector>
+
+#include "calculixpp/core/model.hpp"
+#include "calculixpp/fem/assembly.hpp"
+#include "calculixpp/fem/element.hpp"
+
+namespace cxpp::gpu {
+
+// HIP device of the calling thread's context (takes effect on the next call).
+void set_device(int device);
+
+fem::LinearSystem assemble_linear_static(const Model& model);
+
+fem::LinearSystem assemble_material_tangent(const Model& model, const std::vector<Vec3>& u,
+                                            fem::MaterialPoints& mp, std::vector<Real>& f_int);
+
+std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u);
+
+std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
+                                    const ElasticIso& elastic);
+
+}  // namespace cxpp::gpu

@@ -27,6 +27,7 @@
  */
 #ifndef CXG_H
 #define CXG_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -101,6 +102,15 @@ const char* cxg_last_error(const cxg_context* ctx);
  *   "persistent_waves" resident waves per compute unit of that launch (1..32, 8)
  * Returns CXG_ERR_ARG for an unknown key or value. */
 cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value);
+/* Device memory owned by the context, for callers without a device allocator of
+ * their own (host-language bindings). Blocks still held at cxg_destroy are released
+ * there. cxg_copy_to_device / cxg_copy_to_host enqueue on the context's stream and
+ * return when the copy is complete. No counterpart in the reference (host memory
+ * only); the reference's buffers map onto these one to one. */
+cxg_status cxg_malloc(cxg_context* ctx, size_t bytes, void** out);
+cxg_status cxg_free(cxg_context* ctx, void* p);
+cxg_status cxg_copy_to_device(cxg_context* ctx, void* dst, const void* src, size_t bytes);
+cxg_status cxg_copy_to_host(cxg_context* ctx, void* dst, const void* src, size_t bytes);
 /* Library version string, e.g. "cxg 0.1 gfx950". */
 const char* cxg_version(void);
 int32_t cxg_nodes_per_element(int32_t type);   /* core/element.hpp:24-42 */

@@ -3,6 +3,11 @@
   Oracle(kind="port")       liborc.so, our CPU restatement of the reference path
   Oracle(kind="reference")  _ref/libcxpp_ref.so, the reference's calculixpp_core
                             compiled from its own sources (oracle/Makefile `ref`)
+  Oracle(kind="dropin")     _ref/libcxpp_dropin.so, the same build with its assembly
+                            calls (assemble_linear_static, assemble_material_tangent,
+                            internal_force, element_stiffness) routed to the GPU drop-in
+                            cxpp::gpu; the model is still the reference's cxpp::Model.
+                            Not a checker: what the drop-in tests check.
 
 Both expose the same methods on the same array-level model (orc_model.h). Only
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
@@ -18,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 PORT_LIB = os.path.join(HERE, "liborc.so")
 REF_LIB = os.path.join(HERE, "_ref", "libcxpp_ref.so")
+DROPIN_LIB = os.path.join(HERE, "_ref", "libcxpp_dropin.so")
 ORC_STATE = 26
 MAX_TABLE = 16
 
@@ -43,13 +49,18 @@ class OrcModel(C.Structure):
 
 
 def build(ref: bool = False) -> None:
-    """Compile liborc.so (and _ref when asked and the reference tree is present)."""
-    targets = ["all"] + (["ref"] if ref else [])
+    """Compile liborc.so (and, when asked and the reference tree is present, _ref with
+    the reference core and the drop-in build; the latter links libcxg.so)."""
+    targets = ["all"] + (["ref", "dropin"] if ref else [])
     subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
 
 
 def ref_available() -> bool:
     return os.path.exists(REF_LIB)
+
+
+def dropin_available() -> bool:
+    return os.path.exists(DROPIN_LIB)
 
 
 def _p(a):
@@ -125,7 +136,7 @@ class System:
 class Oracle:
     def __init__(self, kind: str = "port"):
         self.kind = kind
-        path = PORT_LIB if kind == "port" else REF_LIB
+        path = {"port": PORT_LIB, "reference": REF_LIB, "dropin": DROPIN_LIB}[kind]
         if not os.path.exists(path):
             raise OracleError(f"{path} not built")
         self.L = C.CDLL(path)

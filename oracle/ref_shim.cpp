@@ -30,6 +30,16 @@
 
 using namespace cxpp;
 
+// Built a second time with CXPP_DROPIN (oracle/Makefile, target dropin): the same
+// entry points then run the GPU drop-in cxpp::gpu (calculixpp_amd/host/cxpp_gpu.cpp)
+// on the very same cxpp::Model, so a test can hold the two side by side.
+#ifdef CXPP_DROPIN
+#include "cxpp_gpu.hpp"
+namespace impl = cxpp::gpu;
+#else
+namespace impl = cxpp::fem;
+#endif
+
 namespace {
 
 void set_err(char* err, int len, const char* msg) {
@@ -138,8 +148,8 @@ int ref_element_stiffness(int type, const double* coords, double E, double nu, d
   try {
     const int n = nodes_per_element(static_cast<ElementType>(type));
     const std::vector<Vec3> x = as_vec3(coords, size_t(n));
-    const std::vector<Real> K = fem::element_stiffness(static_cast<ElementType>(type), x,
-                                                      ElasticIso{E, nu});
+    const std::vector<Real> K = impl::element_stiffness(static_cast<ElementType>(type), x,
+                                                       ElasticIso{E, nu});
     std::memcpy(Ke, K.data(), K.size() * sizeof(double));
     return 0;
   } catch (const std::exception& e) {
@@ -214,7 +224,7 @@ void ref_model_free(ref_model* h) { delete h; }
 int ref_assemble_linear_static(const ref_model* h, ref_system** out, char* err, int errlen) {
   try {
     auto* s = new ref_system();
-    s->sys = fem::assemble_linear_static(*h->model);
+    s->sys = impl::assemble_linear_static(*h->model);
     *out = s;
     return 0;
   } catch (const std::exception& e) {
@@ -239,7 +249,7 @@ int ref_assemble_material_tangent(const ref_model* h, const double* u, double* s
     const std::vector<Vec3> uu = as_vec3(u, M.mesh.num_nodes());
     std::vector<Real> fi;
     auto* s = new ref_system();
-    s->sys = fem::assemble_material_tangent(M, uu, mp, fi);
+    s->sys = impl::assemble_material_tangent(M, uu, mp, fi);
     off = 0;
     for (auto& es : mp.state)
       for (auto& st : es) {
@@ -259,7 +269,7 @@ int ref_internal_force(const ref_model* h, const double* u, double* f_int, char*
                        int errlen) {
   try {
     const std::vector<Vec3> uu = as_vec3(u, h->model->mesh.num_nodes());
-    const std::vector<Real> fi = fem::internal_force(*h->model, uu);
+    const std::vector<Real> fi = impl::internal_force(*h->model, uu);
     std::memcpy(f_int, fi.data(), fi.size() * sizeof(double));
     return 0;
   } catch (const std::exception& e) {

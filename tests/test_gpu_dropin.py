@@ -1,0 +1,130 @@
+"""The C++ drop-in cxpp::gpu (calculixpp_amd/host/cxpp_gpu.cpp) against the reference.
+
+Both sides take the reference's own cxpp::Model, built once per case from the same
+arrays: oracle/_ref/libcxpp_ref.so runs cxpp::fem::{assemble_linear_static,
+assemble_material_tangent, internal_force, element_stiffness} of the reference core,
+oracle/_ref/libcxpp_dropin.so the cxpp::gpu functions of the same names on libcxg.so.
+  COO key set            identical (as sets of (row, col))
+  K, rhs, f_int          rel-L2 <= 1e-12
+  trial history          rel-L2 <= 1e-10
+  dof_eq                 identical
+Both libraries are built in the source container (oracle/Makefile `ref`, `dropin`).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def pair():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from oracle.oracle import Oracle, dropin_available, ref_available
+    if not (ref_available() and dropin_available()):
+        pytest.skip("oracle/_ref reference and drop-in builds not present")
+    return Oracle("reference"), Oracle("dropin")
+
+
+def rel(a, b):
+    a, b = np.ravel(a), np.ravel(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def same_system(got, exp):
+    assert got.n_free == exp.n_free
+    assert np.array_equal(got.dof_eq, exp.dof_eq)
+    rg, cg, vg = got.csr()
+    re, ce, ve = exp.csr()
+    assert np.array_equal(rg, re) and np.array_equal(cg, ce)
+    assert rel(vg, ve) <= REL, rel(vg, ve)
+    if np.linalg.norm(exp.rhs) > 0:
+        assert rel(got.rhs, exp.rhs) <= REL
+    else:
+        assert np.linalg.norm(got.rhs) == 0
+
+
+def test_cantilever_config1(pair):
+    from calculixpp_amd import meshgen
+    ref, gpu = pair
+    m = meshgen.cantilever()
+    same_system(gpu.assemble_linear_static(m), ref.assemble_linear_static(m))
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_jittered_block(pair, fam_i):
+    from calculixpp_amd import _lib, meshgen
+    ref, gpu = pair
+    fam = [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6][fam_i]
+    m = meshgen.block(3, fam, seed_geo=21, seed_E=22)
+    same_system(gpu.assemble_linear_static(m), ref.assemble_linear_static(m))
+    u = meshgen.smooth_field(m.coords, 23)
+    assert rel(gpu.internal_force(m, u), ref.internal_force(m, u)) <= REL
+
+
+def test_prescribed_loads_and_mixed_families(pair):
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    ref, gpu = pair
+    coords, hexes = meshgen.hex_grid(4, 3, 2, (1.0, 0.75, 0.5), _lib.C3D8)
+    coords = meshgen.jitter(coords, 0.25, 0.1, 13, (1.0, 0.75, 0.5))
+    wed = np.concatenate([hexes[12:][:, [0, 1, 2, 4, 5, 6]], hexes[12:][:, [0, 2, 3, 4, 6, 7]]])
+    rng = np.random.default_rng(14)
+    x0 = np.nonzero(coords[:, 0] < 1e-12)[0]
+    cl = rng.choice(len(coords), 5, replace=False)
+    m = Model(coords, [(_lib.C3D8, hexes[:12]), (_lib.C3D6, wed)],
+              np.linspace(1e5, 2e5, 12 + len(wed)), 0.3,
+              spc_node=np.concatenate([x0, x0[:3]]),
+              spc_comp=np.concatenate([np.full(len(x0), 1), np.full(3, 2)]),
+              spc_value=rng.uniform(-1e-3, 1e-3, len(x0) + 3),
+              cload_node=cl, cload_comp=rng.integers(1, 4, 5), cload_value=rng.normal(size=5))
+    same_system(gpu.assemble_linear_static(m), ref.assemble_linear_static(m))
+
+
+@pytest.mark.parametrize("fam_i", [0, 1, 3])
+def test_material_tangent_with_history(pair, fam_i):
+    from calculixpp_amd import _lib, meshgen
+    from oracle.oracle import aos_state
+    ref, gpu = pair
+    fam = [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6][fam_i]
+    m = meshgen.j2_block(3, 3, 2, (1.0, 0.9, 0.6), family=fam, elastic_every=4, jitter_frac=0.1)
+    st = aos_state(meshgen.seeded_history(m.n_points, 3))
+    u = meshgen.smooth_field(m.coords, 7, 1.5e-4)
+    s_e, f_e, st_e = ref.assemble_material_tangent(m, u, st)
+    s_g, f_g, st_g = gpu.assemble_material_tangent(m, u, st)
+    same_system(s_g, s_e)
+    assert rel(f_g, f_e) <= REL
+    assert np.array_equal(st_g[:, :13], st_e[:, :13])  # committed slots untouched
+    for c in range(13, 26):
+        assert rel(st_g[:, c], st_e[:, c]) <= 1e-10, c
+    assert (st_e[:, 19] > st_e[:, 6]).mean() > 0.05
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_element_stiffness(pair, fam_i):
+    from calculixpp_amd import _lib, meshgen
+    ref, gpu = pair
+    fam = [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6][fam_i]
+    m = meshgen.block(1, fam, seed_geo=31, jitter_frac=0.0)
+    conn = m.blocks[0][1][0]
+    x = m.coords[conn] + np.random.default_rng(32).uniform(-0.05, 0.05, (len(conn), 3))
+    ke_g = gpu.element_stiffness(fam, x, 2.1e5, 0.3)
+    ke_e = ref.element_stiffness(fam, x, 2.1e5, 0.3)
+    assert rel(ke_g, ke_e) <= REL
+
+
+def test_inverted_element_message(pair):
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    from oracle.oracle import OracleError
+    ref, gpu = pair
+    coords, conn = meshgen.hex_grid(2, 1, 1, (1.0, 0.5, 0.5), _lib.C3D8)
+    bad = conn.copy()
+    bad[1] = bad[1][[4, 5, 6, 7, 0, 1, 2, 3]]
+    m = Model(coords, [(_lib.C3D8, bad)], 210000.0, 0.3)
+    for o in (ref, gpu):
+        with pytest.raises(OracleError, match=r"non-positive element Jacobian \(bad connectivity\?\)"):
+            o.assemble_linear_static(m)
