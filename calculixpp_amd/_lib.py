@@ -83,6 +83,8 @@ _SIGS = {
                                        _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "cxg_graph_capture_linear_cases": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int32, _P, _P,
                                                  C.POINTER(_P)]),
+    "cxg_pcg": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, C.c_double, C.c_int32,
+                          C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "cxg_graph_launch": (C.c_int, [_P, _P]),
     "cxg_graph_info": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "cxg_graph_destroy": (None, [_P]),

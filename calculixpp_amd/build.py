@@ -21,7 +21,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libcxg.so")
 OBJ = os.path.join(ROOT, "_build", "cxg")
 
-SOURCES = ["cxg_tables.cpp", "cxg_api.cpp", "cxg_graph.cpp", "cxg_pattern.hip", "cxg_elem.hip"]
+SOURCES = ["cxg_tables.cpp", "cxg_api.cpp", "cxg_graph.cpp", "cxg_pattern.hip", "cxg_elem.hip",
+           "cxg_solve.hip"]
 ARCH = os.environ.get("CXG_ARCH", "gfx950")
 
 

@@ -240,6 +240,10 @@ void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const
                 double* du);
 void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq, int32_t n_free,
                    cxg_pattern* out);
+// Jacobi-preconditioned CG on a CSR (cxg_solve.hip).
+void pcg(cxg_context* ctx, int32_t n, const int64_t* rp, const int32_t* ci, const double* va,
+         const double* b, double* x, double tol, int32_t max_iter, int32_t& iters,
+         double& rel_res);
 // Keeps the entries with live != 0; returns the kept count (synchronizes the stream).
 int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
                         const double* vals, int64_t* row_ptr_out, int32_t* col_idx_out,

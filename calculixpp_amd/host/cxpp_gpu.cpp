@@ -424,4 +424,57 @@ std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coor
   return K;
 }
 
+std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector<Index>& cols,
+                               const std::vector<Real>& vals, Index n,
+                               const std::vector<Real>& rhs, compute::SolverKind) {
+  if (rows.size() != cols.size() || rows.size() != vals.size() || rhs.size() != size_t(n) || n < 0)
+    throw std::runtime_error("cxpp::gpu::solve_sparse: inconsistent system sizes");
+  // COO -> CSR with duplicates summed in triplet order
+  std::vector<int64_t> rp(size_t(n) + 1, 0);
+  for (Index r : rows) {
+    if (r < 0 || r >= n) throw std::runtime_error("cxpp::gpu::solve_sparse: row out of range");
+    ++rp[size_t(r) + 1];
+  }
+  for (size_t i = 0; i < size_t(n); ++i) rp[i + 1] += rp[i];
+  std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+  std::vector<int32_t> ci(rows.size());
+  std::vector<double> va(rows.size());
+  for (size_t k = 0; k < rows.size(); ++k) {
+    if (cols[k] < 0 || cols[k] >= n) throw std::runtime_error("cxpp::gpu::solve_sparse: column out of range");
+    const int64_t at = fill[size_t(rows[k])]++;
+    ci[size_t(at)] = cols[k];
+    va[size_t(at)] = vals[k];
+  }
+  std::vector<int64_t> out_rp(size_t(n) + 1, 0);
+  std::vector<int32_t> out_ci;
+  std::vector<double> out_va;
+  out_ci.reserve(ci.size());
+  out_va.reserve(va.size());
+  std::vector<std::pair<int32_t, double>> row;
+  for (size_t r = 0; r < size_t(n); ++r) {
+    row.clear();
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) row.emplace_back(ci[size_t(k)], va[size_t(k)]);
+    std::stable_sort(row.begin(), row.end(),
+                     [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (size_t k = 0; k < row.size(); ++k) {
+      if (k > 0 && row[k].first == out_ci.back()) {
+        out_va.back() += row[k].second;
+      } else {
+        out_ci.push_back(row[k].first);
+        out_va.push_back(row[k].second);
+      }
+    }
+    out_rp[r + 1] = int64_t(out_ci.size());
+  }
+  cxg_context* c = context();
+  Buf<int64_t> rpd(c, out_rp);
+  Buf<int32_t> cid(c, out_ci);
+  Buf<double> vad(c, out_va), bd(c, rhs), xd(c, std::vector<double>(size_t(n), 0.0));
+  int32_t iters = 0;
+  double rel = 0.0;
+  check(c, cxg_pcg(c, n, rpd.get(), cid.get(), vad.get(), bd.get(), xd.get(), 1e-13,
+                   std::max<int32_t>(100, 20 * n), &iters, &rel));
+  return xd.fetch(size_t(n));
+}
+
 }  // namespace cxpp::gpu

@@ -13,6 +13,9 @@
 //                                          (fem/stress.hpp:16, stress.cpp:125-145)
 //   cxpp::gpu::element_stiffness           fem::element_stiffness
 //                                          (fem/element.hpp:62, element.cpp:614-634)
+//   cxpp::gpu::solve_sparse                compute::ComputeBackend::solve_sparse
+//                                          (compute/backend.hpp:51-56), as a function:
+//                                          a HIP backend class wraps it one to one
 //
 // This file and cxpp_gpu.cpp compile inside the reference's tree (they include its
 // headers) and link against libcxg.so. A model the GPU path does not cover (element
@@ -30,6 +33,7 @@
 #include <span>
 #include <vector>
 
+#include "calculixpp/compute/backend.hpp"
 #include "calculixpp/core/model.hpp"
 #include "calculixpp/fem/assembly.hpp"
 #include "calculixpp/fem/element.hpp"
@@ -48,5 +52,12 @@ std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u)
 
 std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
                                     const ElasticIso& elastic);
+
+// A x = rhs for the SPD system given as COO triplets (duplicates summed), on the
+// device by Jacobi-preconditioned conjugate gradients to ||b - A x|| <= 1e-13 ||b||
+// (cxg_pcg) whichever SolverKind is asked for; throws if that is not reached.
+std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector<Index>& cols,
+                               const std::vector<Real>& vals, Index n,
+                               const std::vector<Real>& rhs, compute::SolverKind solver);
 
 }  // namespace cxpp::gpu

@@ -225,6 +225,20 @@ cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
                    const double* E, const double* nu, const double* u, const double* w,
                    double* dx, double* dE, double* du);
 
+/* ---- f1: sparse SPD solve on the device --------------------------------------
+ * Replaces ComputeBackend::solve_sparse (compute/backend.hpp:51-56) for the SPD
+ * systems of linear and Newton steps (the CPU backend hands them to SciPP,
+ * src/compute/cpu_backend.cpp:33-46): Jacobi-preconditioned conjugate gradients on
+ * the CSR (row_ptr[n+1], col_idx, vals: device, e.g. a pattern and its values). x is
+ * the start on entry and the solution on return; iterations stop once
+ * ||b - A x|| <= tol ||b|| (checked every 16 iterations) or after max_iter
+ * (CXG_ERR_STATE, x holds the last iterate). Deterministic: every reduction has a
+ * fixed order. iters / rel_residual (host, may be NULL) receive the count and the
+ * final relative residual. */
+cxg_status cxg_pcg(cxg_context* ctx, int32_t n, const int64_t* row_ptr, const int32_t* col_idx,
+                   const double* vals, const double* b, double* x, double tol, int32_t max_iter,
+                   int32_t* iters, double* rel_residual);
+
 /* ---- HIP graphs: one assembly / one Newton iteration, replayed ---------------
  * The reference re-assembles K_t, f_int and the residual at every Newton iteration
  * of solve_nonlinear through assemble_material_tangent (fem/assembly.cpp:385-423)

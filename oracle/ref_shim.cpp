@@ -297,4 +297,21 @@ void ref_system_dof_eq(const ref_system* s, int32_t* dof_eq) {
 
 void ref_system_free(ref_system* s) { delete s; }
 
+#ifdef CXPP_DROPIN
+// COO system -> cxpp::gpu::solve_sparse (drop-in build only).
+int ref_solve_sparse(int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                     const double* vals, const double* rhs, double* x, char* err, int errlen) {
+  try {
+    const std::vector<Index> r(rows, rows + nnz), c(cols, cols + nnz);
+    const std::vector<Real> v(vals, vals + nnz), b(rhs, rhs + n);
+    const std::vector<Real> u = cxpp::gpu::solve_sparse(r, c, v, n, b, compute::SolverKind::CG);
+    std::memcpy(x, u.data(), u.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+#endif
+
 }  // extern "C"

@@ -116,6 +116,25 @@ def test_element_stiffness(pair, fam_i):
     assert rel(ke_g, ke_e) <= REL
 
 
+def test_solve_sparse_config1(pair):
+    """cxpp::gpu::solve_sparse on the reference's own COO system of config 1 against a
+    dense direct solve (SciPP's spsolve is absent here)."""
+    import ctypes as C
+    from calculixpp_amd import meshgen
+    ref, gpu = pair
+    s = ref.assemble_linear_static(meshgen.cantilever())
+    rows, cols = s.rows.astype(np.int32), s.cols.astype(np.int32)
+    vals, rhs = np.ascontiguousarray(s.vals), np.ascontiguousarray(s.rhs)
+    x = np.zeros(s.n_free)
+    err = C.create_string_buffer(512)
+    f = gpu.L.ref_solve_sparse
+    f.argtypes = [C.c_int32, C.c_int64] + [C.c_void_p] * 5 + [C.c_char_p, C.c_int]
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert f(s.n_free, len(vals), p(rows), p(cols), p(vals), p(rhs), p(x), err, 512) == 0, err.value
+    exact = np.linalg.solve(s.dense(), rhs)
+    assert rel(x, exact) <= 1e-10
+
+
 def test_inverted_element_message(pair):
     from calculixpp_amd import _lib, meshgen
     from calculixpp_amd.model import Model
