@@ -347,7 +347,7 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->patch_size = int(value);
     return CXG_OK;
   }
-  if (k == "asm_variant" && value >= 1 && value <= 5) {
+  if (k == "asm_variant" && value >= 1 && value <= 6) {
     ctx->asm_variant = int(value);
     return CXG_OK;
   }

@@ -729,6 +729,124 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
 }
 
+// ---- K1 on the matrix cores (asm_variant 6): M = H^T H by FP64 MFMA -------------------
+// Same M-identity as k_elem_iso, as one small GEMM per element: with
+// H[q][3a+i] = sqrt(det J w)_q g_{q,a,i} (NGP x 3n), M = H^T H holds every
+// M_ab = sum_q h_a h_b^T at once, and K_ab = lam M_ab + mu M_ab^T + mu tr(M_ab) I.
+// One element per wave. Each lane computes exactly the H entries it feeds to
+// v_mfma_f64_16x16x4f64 (row k = lane/16 of a 4-deep K step, column lane%16 of a
+// 16-wide tile), so H never touches LDS; the upper tiles of M (TI <= TJ) accumulate
+// in registers over ceil(NGP/4) K steps. M goes to LDS, and each lane then writes
+// consecutive entries of the row-major 3n x 3n Ke,
+//   K[3a+i][3b+j] = lam M(3a+i, 3b+j) + mu M(3a+j, 3b+i) + [i == j] mu tr M_ab,
+// which, M being symmetric, holds for every (a, b): Ke comes out bitwise symmetric,
+// stored with fully coalesced writes.
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+
+template <int NN, int NGP>
+struct MfmaLds {
+  static constexpr int ND = 3 * NN;
+  static constexpr int LD = ND + 1;  // row stride of M (odd: spreads LDS banks)
+  double x[NN * 3];
+  double Ai[NGP * 9];
+  double rs[NGP];
+  double M[ND * LD];
+};
+
+template <int NN, int NGP>
+__global__ __launch_bounds__(kWave) void k_elem_iso_mfma(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
+    int* __restrict__ flag) {
+  constexpr int ND = 3 * NN;
+  constexpr int NT = (ND + 15) / 16;   // 16-wide column tiles of H
+  constexpr int KS = (NGP + 3) / 4;    // 4-deep steps over the Gauss points
+  constexpr int LD = MfmaLds<NN, NGP>::LD;
+  __shared__ MfmaLds<NN, NGP> L;
+  const int lane = threadIdx.x;
+  const int64_t le = blockIdx.x;
+  const int32_t* ce = conn + le * NN;
+  if (lane < NN) {
+    const double* xp = coords + 3 * int64_t(ce[lane]);
+    L.x[3 * lane + 0] = xp[0];
+    L.x[3 * lane + 1] = xp[1];
+    L.x[3 * lane + 2] = xp[2];
+  }
+  __syncthreads();
+  for (int q = lane; q < NGP; q += kWave)
+    jacobian_inverse<NN>(L.x, dN + q * NN * 3, wq[q], L.Ai + 9 * q, L.rs[q], flag);
+  __syncthreads();
+  // this lane's H entries: h[s][t] = H[4s + lane/16][16t + lane%16]
+  const int qs = lane / 16, cs = lane % 16;
+  double h[KS][NT];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int q = 4 * s + qs, col = 16 * t + cs;
+      double v = 0.0;
+      if (q < NGP && col < ND) {
+        const int a = col / 3, i = col - 3 * (col / 3);
+        const double* A = L.Ai + 9 * q;
+        const double* d = dN + (q * NN + a) * 3;
+        v = L.rs[q] * (A[i] * d[0] + A[3 + i] * d[1] + A[6 + i] * d[2]);
+      }
+      h[s][t] = v;
+    }
+  // upper tiles of M = H^T H
+  mf64x4 acc[NT * (NT + 1) / 2];
+#pragma unroll
+  for (int k = 0; k < NT * (NT + 1) / 2; ++k) acc[k] = mf64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    int k = 0;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+      for (int tj = ti; tj < NT; ++tj, ++k)
+        acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(h[s][ti], h[s][tj], acc[k], 0, 0, 0);
+  }
+  // D layout (gfx950, checked by an in-bounds probe): element m of a lane's result is
+  // row lane/16 + 4m, column lane%16 of its tile
+  {
+    int k = 0;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+      for (int tj = ti; tj < NT; ++tj, ++k) {
+        const int c = 16 * tj + cs;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int r = 16 * ti + qs + 4 * m;
+          if (r < ND && c < ND) {
+            L.M[r * LD + c] = acc[k][m];
+            L.M[c * LD + r] = acc[k][m];  // M is symmetric: store both triangles
+          }
+        }
+      }
+  }
+  __syncthreads();
+  double lam, mu;
+  lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
+  double* kel = kp + le * (ND * ND);
+  // lane = column 3b + j of Ke; rows walk node by node
+  for (int c = lane; c < ND; c += kWave) {
+    const int b3 = c - c % 3, j = c % 3;
+    for (int a3 = 0; a3 < ND; a3 += 3) {
+      const double* Ma = L.M + a3 * LD;
+      const double tr = mu * (Ma[b3] + Ma[LD + b3 + 1] + Ma[2 * LD + b3 + 2]);
+      const double* Mj = Ma + j * LD + b3;  // M(a3 + j, b3 + i) = Mj[i]
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double v = lam * Ma[i * LD + c] + mu * Mj[i];
+        if (i == j) v += tr;
+        kel[(a3 + i) * ND + c] = v;
+      }
+    }
+  }
+}
+
 // ---- K4 (+K5), two-phase variant, phase B: CSR rows gathered by their owner node ----
 // One 64-lane wave per node a. Its (up to three) CSR rows share one column layout of
 // length L; they are accumulated in LDS from rows 3la..3la+2 of the element matrices
@@ -1716,6 +1834,21 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
                          coords, E, nu, pm->ke_arena.p, ctx->flag_dev);
       CXG_HIP(hipGetLastError());
     }
+    for (size_t bi = 0; ctx->asm_variant == 6 && bi < m->blocks.size(); ++bi) {
+      const cxg_block_dev& B = m->blocks[bi];
+      if (B.n_elem == 0) continue;
+      const ElemTable& T = ctx->tables[B.type];
+      const int32_t* conn = m->conn.p + B.conn_offset;
+      double* kp = pm->ke_arena.p + p->kp_base[bi];
+      LaunchTimer timer(ctx, std::string("elem_iso_mfma_") + type_name(B.type), B.n_elem);
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_elem_iso_mfma<NN, NGP>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,      \
+                     ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E, nu, kp,            \
+                     ctx->flag_dev)
+      CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+      CXG_HIP(hipGetLastError());
+    }
     for (size_t bi = 0; ctx->asm_variant == 4 && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
@@ -1897,7 +2030,7 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
                          pm->ke_arena.p, pm->fe_arena.p, ctx->flag_dev);
       CXG_HIP(hipGetLastError());
     }
-    for (size_t bi = 0; ctx->asm_variant == 4 && bi < m->blocks.size(); ++bi) {
+    for (size_t bi = 0; ctx->asm_variant != 5 && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
       const ElemTable& T = ctx->tables[B.type];

@@ -97,7 +97,9 @@ const char* cxg_last_error(const cxg_context* ctx);
  *                 row-coalesced scatter at 4 waves/SIMD, 3 = the same at 3, 4 =
  *                 two-phase: element matrices to an arena, then each CSR row gathered
  *                 and written once by its node (default), 5 = two-phase with one
- *                 persistent element launch over all families, fed by a work queue
+ *                 persistent element launch over all families, fed by a work queue,
+ *                 6 = two-phase with the element matrices on the FP64 matrix cores
+ *                 (M = H^T H by MFMA)
  *   "chunk_elems"      elements per work item of the persistent launch (1..4096, 4)
  *   "persistent_waves" resident waves per compute unit of that launch (1..32, 8)
  * Returns CXG_ERR_ARG for an unknown key or value. */

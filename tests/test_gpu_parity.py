@@ -189,7 +189,7 @@ def test_deterministic_bitwise(gpu):
 
 
 @pytest.mark.parametrize("variant,patch", [(1, 1), (1, 5), (2, 1), (2, 3), (2, 8), (3, 16), (4, 1), (4, 8),
-                                           (5, 8)])
+                                           (5, 8), (6, 8)])
 @pytest.mark.parametrize("fam_i", range(4))
 def test_kernel_variants_and_patches(gpu, port, variant, patch, fam_i):
     """Every stiffness kernel variant and patch length gives the same K and f_int."""
