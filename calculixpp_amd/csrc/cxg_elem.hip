@@ -1276,24 +1276,32 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
 }
 
 // ---- K2, two-phase variant: return map, K_t and f_e to the element arenas ------------
-// One element per wave. Each lane owns one Gauss point: Jacobian, strain (gradients
-// formed on the fly), radial return (j2_return), trial history to the SoA state, and
-// the compact tangent (lam, mu, c, n) and rs*sigma to LDS. Then, QC Gauss points at a
-// time, h = rs g and t = B(h)^T n per node go to LDS, and each lane accumulates its
-// upper node-pair blocks
+// Phase 1, one lane per Gauss point: Jacobian, strain (gradients formed on the fly),
+// radial return (j2_return), trial history to the SoA state, and the compact tangent
+// (lam, mu, c, n) and rs*sigma to LDS. The per-family kernel runs EPW elements per wave
+// here, so that up to all 64 lanes carry a Gauss point (8 C3D8 elements of 8 points,
+// 2 C3D20 of 27, 32 C3D6 of 2); the persistent kernel runs one element per wave.
+// Phase 2, one element at a time on the whole wave: QC Gauss points at a time,
+// h = rs g and t = B(h)^T n per node go to LDS, and each lane accumulates its upper
+// node-pair blocks
 //   K_ab += lam h_a h_b^T + mu h_b h_a^T + mu (h_a . h_b) I - c t_a t_b^T
 // (= s B_a^T D_t B_b with s = det J w, rs = sqrt(s)), while lanes < n accumulate the
 // element force f_a += B(h_a)^T (rs sigma). K_t is written as full rows through the
 // staged, coalesced stores of k_elem_iso; f_e (3n) to its own arena. The row gather
 // then sums K_t rows and f_e entries per node in element order.
-template <int NN, int NGP, int QC>
-struct ElemJ2Lds {
+template <int NN, int NGP>
+struct J2Elem {
   double x[NN * 3];
   double u[NN * 3];
   double Ai[NGP * 9];
   double rs[NGP];
   double T[NGP * 9];  // lam, mu, c, n[6] per Gauss point
   double S[NGP * 6];  // rs * sigma
+};
+
+template <int NN, int NGP, int QC, int EPW>
+struct ElemJ2Lds {
+  J2Elem<NN, NGP> e[EPW];
   union {
     struct {
       double h[QC * NN * 3];
@@ -1303,78 +1311,88 @@ struct ElemJ2Lds {
   } w;
 };
 
-// One element (see above). mat == nullptr: isotropic elastic with Ee, nue. pt0: the
-// element's first Gauss point in the SoA state. Ends with a barrier.
-template <int NN, int NGP, int QC>
-__device__ __forceinline__ void elem_j2_body(
-    ElemJ2Lds<NN, NGP, QC>& L, const int lane, const int32_t* __restrict__ ce, const double Ee,
-    const double nue, const J2Dev* mat, const int64_t pt0, const int64_t n_points,
-    const double* __restrict__ dN, const double* __restrict__ wq,
-    const double* __restrict__ coords, const double* __restrict__ u,
-    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kel,
-    double* __restrict__ fel, int* __restrict__ flag) {
+// Elements per wave of the per-family J2 kernel.
+constexpr int j2_epw(int ngp) { return ngp >= 27 ? 2 : (64 / ngp > 32 ? 32 : 64 / ngp); }
+
+// Node k of an element into its LDS record.
+template <int NN, int NGP>
+__device__ __forceinline__ void j2_stage(J2Elem<NN, NGP>& P, const int32_t* __restrict__ ce,
+                                         const double* __restrict__ coords,
+                                         const double* __restrict__ u, int k) {
+  const int64_t n3 = 3 * int64_t(ce[k]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    P.x[3 * k + c] = coords[n3 + c];
+    P.u[3 * k + c] = u[n3 + c];
+  }
+}
+
+// Phase 1 for Gauss point q (x, u staged). mat == nullptr: isotropic elastic with
+// Ee, nue. pt: the point's index in the SoA state.
+template <int NN, int NGP>
+__device__ __forceinline__ void j2_gauss_point(
+    J2Elem<NN, NGP>& P, const int q, const double Ee, const double nue, const J2Dev* mat,
+    const int64_t pt, const int64_t n_points, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ committed,
+    double* __restrict__ trial, int* __restrict__ flag) {
+  const double* d = dN + q * NN * 3;
+  double* A = P.Ai + 9 * q;
+  double rs;
+  jacobian_inverse<NN>(P.x, d, wq[q], A, rs, flag);
+  P.rs[q] = rs;
+  double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int k = 0; k < NN; ++k) {
+    const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+    const double g0 = A[0] * d0 + A[3] * d1 + A[6] * d2;
+    const double g1 = A[1] * d0 + A[4] * d1 + A[7] * d2;
+    const double g2 = A[2] * d0 + A[5] * d1 + A[8] * d2;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double v = P.u[3 * k + i];
+      H[i][0] += v * g0;
+      H[i][1] += v * g1;
+      H[i][2] += v * g2;
+    }
+  }
+  double e[6], sg[6];
+  voigt_strain(H, e);
+  J2Tangent T;
+  if (mat) {
+    double hc[13], ht[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
+    j2_return(*mat, e, hc, ht, sg, T);
+#pragma unroll
+    for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = ht[c];
+  } else {
+    lame(Ee, nue, T.lam, T.mu);
+    iso_stress(T.lam, T.mu, e, sg);
+    T.c = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) T.n[i] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = committed[c * n_points + pt];
+  }
+  double* Tq = P.T + 9 * q;
+  Tq[0] = T.lam;
+  Tq[1] = T.mu;
+  Tq[2] = T.c;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    Tq[3 + i] = T.n[i];
+    P.S[6 * q + i] = rs * sg[i];
+  }
+}
+
+// Phase 2 for one element: every lane of the wave calls it; ends with a barrier.
+template <int NN, int NGP, int QC, class W>
+__device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w, const int lane,
+                                                  const double* __restrict__ dN,
+                                                  double* __restrict__ kel,
+                                                  double* __restrict__ fel) {
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
   constexpr int ND = 3 * NN;
-  if (lane < NN) {
-    const int64_t n3 = 3 * int64_t(ce[lane]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      L.x[3 * lane + c] = coords[n3 + c];
-      L.u[3 * lane + c] = u[n3 + c];
-    }
-  }
-  __syncthreads();
-  for (int q = lane; q < NGP; q += kWave) {
-    const double* d = dN + q * NN * 3;
-    double* A = L.Ai + 9 * q;
-    double rs;
-    jacobian_inverse<NN>(L.x, d, wq[q], A, rs, flag);
-    L.rs[q] = rs;
-    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int k = 0; k < NN; ++k) {
-      const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
-      const double g0 = A[0] * d0 + A[3] * d1 + A[6] * d2;
-      const double g1 = A[1] * d0 + A[4] * d1 + A[7] * d2;
-      const double g2 = A[2] * d0 + A[5] * d1 + A[8] * d2;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const double v = L.u[3 * k + i];
-        H[i][0] += v * g0;
-        H[i][1] += v * g1;
-        H[i][2] += v * g2;
-      }
-    }
-    double e[6], sg[6];
-    voigt_strain(H, e);
-    const int64_t pt = pt0 + q;
-    J2Tangent T;
-    if (mat) {
-      double hc[13], ht[13];
-#pragma unroll
-      for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
-      j2_return(*mat, e, hc, ht, sg, T);
-#pragma unroll
-      for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = ht[c];
-    } else {
-      lame(Ee, nue, T.lam, T.mu);
-      iso_stress(T.lam, T.mu, e, sg);
-      T.c = 0.0;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) T.n[i] = 0.0;
-#pragma unroll
-      for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = committed[c * n_points + pt];
-    }
-    double* Tq = L.T + 9 * q;
-    Tq[0] = T.lam;
-    Tq[1] = T.mu;
-    Tq[2] = T.c;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      Tq[3 + i] = T.n[i];
-      L.S[6 * q + i] = rs * sg[i];
-    }
-  }
   int pa[PPL], pb[PPL];
   double K[PPL][9];
 #pragma unroll
@@ -1390,36 +1408,35 @@ __device__ __forceinline__ void elem_j2_body(
     for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
   }
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  __syncthreads();
   for (int q0 = 0; q0 < NGP; q0 += QC) {
     for (int idx = lane; idx < QC * NN; idx += kWave) {
       const int q = q0 + idx / NN;
       double h0 = 0.0, h1 = 0.0, h2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
       if (q < NGP) {
-        const double* A = L.Ai + 9 * q;
+        const double* A = P.Ai + 9 * q;
         const double* d = dN + (q * NN + idx % NN) * 3;
-        const double s = L.rs[q];
+        const double s = P.rs[q];
         h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
         h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
         h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
-        const double* n = L.T + 9 * q + 3;
+        const double* n = P.T + 9 * q + 3;
         t0 = h0 * n[0] + h1 * n[3] + h2 * n[4];
         t1 = h1 * n[1] + h0 * n[3] + h2 * n[5];
         t2 = h2 * n[2] + h0 * n[4] + h1 * n[5];
       }
-      L.w.c.h[3 * idx + 0] = h0;
-      L.w.c.h[3 * idx + 1] = h1;
-      L.w.c.h[3 * idx + 2] = h2;
-      L.w.c.t[3 * idx + 0] = t0;
-      L.w.c.t[3 * idx + 1] = t1;
-      L.w.c.t[3 * idx + 2] = t2;
+      w.c.h[3 * idx + 0] = h0;
+      w.c.h[3 * idx + 1] = h1;
+      w.c.h[3 * idx + 2] = h2;
+      w.c.t[3 * idx + 0] = t0;
+      w.c.t[3 * idx + 1] = t1;
+      w.c.t[3 * idx + 2] = t2;
     }
     __syncthreads();
     const int qn = NGP - q0 < QC ? NGP - q0 : QC;
     if (lane < NN)
       for (int qq = 0; qq < qn; ++qq) {
-        const double* g = L.w.c.h + (qq * NN + lane) * 3;
-        const double* sq = L.S + 6 * (q0 + qq);
+        const double* g = w.c.h + (qq * NN + lane) * 3;
+        const double* sq = P.S + 6 * (q0 + qq);
         f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
         f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
         f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
@@ -1429,12 +1446,12 @@ __device__ __forceinline__ void elem_j2_body(
       if (pa[t] < 0) continue;
 #pragma unroll 1
       for (int qq = 0; qq < qn; ++qq) {
-        const double* Tq = L.T + 9 * (q0 + qq);
+        const double* Tq = P.T + 9 * (q0 + qq);
         const double lam = Tq[0], mu = Tq[1], c = Tq[2];
-        const double* ha = L.w.c.h + (qq * NN + pa[t]) * 3;
-        const double* hb = L.w.c.h + (qq * NN + pb[t]) * 3;
-        const double* ta = L.w.c.t + (qq * NN + pa[t]) * 3;
-        const double* tb = L.w.c.t + (qq * NN + pb[t]) * 3;
+        const double* ha = w.c.h + (qq * NN + pa[t]) * 3;
+        const double* hb = w.c.h + (qq * NN + pb[t]) * 3;
+        const double* ta = w.c.t + (qq * NN + pa[t]) * 3;
+        const double* tb = w.c.t + (qq * NN + pb[t]) * 3;
         const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
         const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
         const double la[3] = {lam * a0, lam * a1, lam * a2};
@@ -1465,14 +1482,14 @@ __device__ __forceinline__ void elem_j2_body(
     for (int t = 0; t < PPL; ++t) {
       if (pa[t] < 0) continue;
       if (pa[t] >= g0 && pa[t] < g0 + kStageNodes) {
-        double* ra = L.w.stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
+        double* ra = w.stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[t][3 * i + j];
       }
       if (pb[t] != pa[t] && pb[t] >= g0 && pb[t] < g0 + kStageNodes) {
-        double* rb = L.w.stage + (3 * (pb[t] - g0)) * ND + 3 * pa[t];
+        double* rb = w.stage + (3 * (pb[t] - g0)) * ND + 3 * pa[t];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -1482,28 +1499,62 @@ __device__ __forceinline__ void elem_j2_body(
     __syncthreads();
     const int nrow = 3 * (NN - g0 < kStageNodes ? NN - g0 : kStageNodes);
     double* dst = kel + int64_t(3 * g0) * ND;
-    for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = L.w.stage[idx];
+    for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = w.stage[idx];
     __syncthreads();
   }
 }
 
+// One element per wave (the persistent kernel). Ends with a barrier.
+template <int NN, int NGP, int QC>
+__device__ __forceinline__ void elem_j2_body(
+    ElemJ2Lds<NN, NGP, QC, 1>& L, const int lane, const int32_t* __restrict__ ce,
+    const double Ee, const double nue, const J2Dev* mat, const int64_t pt0,
+    const int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ u,
+    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kel,
+    double* __restrict__ fel, int* __restrict__ flag) {
+  if (lane < NN) j2_stage<NN, NGP>(L.e[0], ce, coords, u, lane);
+  __syncthreads();
+  for (int q = lane; q < NGP; q += kWave)
+    j2_gauss_point<NN, NGP>(L.e[0], q, Ee, nue, mat, pt0 + q, n_points, dN, wq, committed,
+                            trial, flag);
+  __syncthreads();
+  j2_element_matrix<NN, NGP, QC>(L.e[0], L.w, lane, dN, kel, fel);
+}
+
+// EPW elements per wave: phase 1 for all their Gauss points at once, then phase 2
+// element by element.
 template <int NN, int NGP, int QC>
 __global__ __launch_bounds__(kWave) void k_elem_j2(
-    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset, int64_t n_points,
-    const double* __restrict__ dN, const double* __restrict__ wq,
+    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t n_elem, int64_t gp_offset,
+    int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
     const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
     const J2Dev* __restrict__ mats, const double* __restrict__ u,
     const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
     double* __restrict__ fe, int* __restrict__ flag) {
-  __shared__ ElemJ2Lds<NN, NGP, QC> L;
-  const int64_t le = blockIdx.x;
-  const int64_t ge = elem_offset + le;
-  const int32_t mid = mat_id ? mat_id[ge] : -1;
-  elem_j2_body<NN, NGP, QC>(L, int(threadIdx.x), conn + le * NN, E[ge], nu[ge],
-                            mid >= 0 ? mats + mid : nullptr, gp_offset + le * NGP, n_points, dN,
-                            wq, coords, u, committed, trial, kp + le * (9 * NN * NN),
-                            fe + le * (3 * NN), flag);
+  constexpr int EPW = j2_epw(NGP);
+  __shared__ ElemJ2Lds<NN, NGP, QC, EPW> L;
+  const int lane = threadIdx.x;
+  const int64_t le0 = int64_t(blockIdx.x) * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  for (int idx = lane; idx < ne * NN; idx += kWave) {
+    const int e = idx / NN;
+    j2_stage<NN, NGP>(L.e[e], conn + (le0 + e) * NN, coords, u, idx - e * NN);
+  }
+  __syncthreads();
+  for (int idx = lane; idx < ne * NGP; idx += kWave) {
+    const int e = idx / NGP, q = idx - e * NGP;
+    const int64_t ge = elem_offset + le0 + e;
+    const int32_t mid = mat_id ? mat_id[ge] : -1;
+    j2_gauss_point<NN, NGP>(L.e[e], q, E[ge], nu[ge], mid >= 0 ? mats + mid : nullptr,
+                            gp_offset + (le0 + e) * NGP + q, n_points, dN, wq, committed, trial,
+                            flag);
+  }
+  __syncthreads();
+  for (int e = 0; e < ne; ++e)
+    j2_element_matrix<NN, NGP, QC>(L.e[e], L.w, lane, dN, kp + (le0 + e) * (9 * NN * NN),
+                                   fe + (le0 + e) * (3 * NN));
 }
 
 // ---- Persistent element kernels (asm_variant 5) ------------------------------------
@@ -1521,10 +1572,10 @@ union IsoLdsAll {
   ElemLds<6, 2, 2> c3d6;
 };
 union J2LdsAll {
-  ElemJ2Lds<8, 8, 8> c3d8;
-  ElemJ2Lds<20, 27, 3> c3d20;
-  ElemJ2Lds<20, 8, 3> c3d20r;
-  ElemJ2Lds<6, 2, 2> c3d6;
+  ElemJ2Lds<8, 8, 8, 1> c3d8;
+  ElemJ2Lds<20, 27, 3, 1> c3d20;
+  ElemJ2Lds<20, 8, 3, 1> c3d20r;
+  ElemJ2Lds<6, 2, 2, 1> c3d6;
 };
 
 // Next work item for the whole wave (lane 0 takes it from the counter).
@@ -2039,9 +2090,11 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
       double* fe = pm->fe_arena.p + 3 * B.conn_offset;
       LaunchTimer timer(ctx, std::string("elem_j2_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,        \
-                     ctx->stream, conn, B.elem_offset, B.gp_offset, m->n_points, T.dN, T.w,     \
-                     coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe, ctx->flag_dev)
+  hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC>),                                                  \
+                     dim3(unsigned((B.n_elem + j2_epw(NGP) - 1) / j2_epw(NGP))), dim3(kWave), 0, \
+                     ctx->stream, conn, B.elem_offset, B.n_elem, B.gp_offset, m->n_points, T.dN, \
+                     T.w, coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe,            \
+                     ctx->flag_dev)
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
       CXG_HIP(hipGetLastError());
