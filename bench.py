@@ -290,6 +290,8 @@ def main() -> int:
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
     ap.add_argument("--batch-meshes", type=int, default=512)
+    ap.add_argument("--only", choices=["j2", "batch", "adjoint"], default=None,
+                    help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
     ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -304,6 +306,18 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": n * 8, "launches": 3}), flush=True)
+        return 0
+    if args.only:
+        import torch
+        torch.cuda.set_device(0)
+        if args.only == "j2":
+            res = j2_newton(reps=args.steps, cpu=not args.no_cpu)
+        elif args.only == "batch":
+            res = batch_step(args.batch_meshes, reps=args.steps, cpu=not args.no_cpu,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
+        else:
+            res = adjoint_ratio(args.adj_reps)
+        print(json.dumps(res), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
