@@ -290,6 +290,8 @@ def main() -> int:
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
     ap.add_argument("--batch-meshes", type=int, default=512)
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="extra cxg_set_option for the library context (repeatable)")
     ap.add_argument("--only", choices=["j2", "batch", "adjoint"], default=None,
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
@@ -345,6 +347,9 @@ def main() -> int:
         choice = autotune.tune_linear(model, u, cache_path=None, log=lambda m: print(
             m, file=sys.stderr, flush=True))
     t2 = time.perf_counter()
+    for opt in args.option:
+        key, val = opt.split("=", 1)
+        fem.context().set_option(key, int(val))
     asm = autotune.make_assembler(model, choice["asm_variant"], choice["patch_size"])
     torch.cuda.synchronize()
     t3 = time.perf_counter()

@@ -355,6 +355,10 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->chunk_elems = int(value);
     return CXG_OK;
   }
+  if (k == "xcd_remap" && (value == 0 || value == 1)) {
+    ctx->xcd_remap = int(value);
+    return CXG_OK;
+  }
   if (k == "persistent_waves" && value >= 1 && value <= 32) {
     ctx->persistent_waves = int(value);
     return CXG_OK;

@@ -36,6 +36,14 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// Block b of a launch of n blocks -> work index: with slab > 0 each of the 8 XCDs (which
+// receive consecutive workgroups in turn) walks one contiguous range of slab indices;
+// -1 past the end. slab 0: identity.
+__device__ __forceinline__ int64_t xcd_index(int64_t b, int64_t n, int64_t slab) {
+  const int64_t i = slab > 0 ? (b % 8) * slab + b / 8 : b;
+  return i < n ? i : -1;
+}
+
 // ---- LDS staging shared by all element kernels ------------------------------
 template <int NN, int NGP>
 struct Geo {
@@ -722,9 +730,10 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
     const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ coords,
     const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
-    int* __restrict__ flag) {
+    int* __restrict__ flag, int64_t n_elem, int64_t slab) {
   __shared__ ElemLds<NN, NGP, QC> L;
-  const int64_t le = blockIdx.x;
+  const int64_t le = xcd_index(blockIdx.x, n_elem, slab);
+  if (le < 0) return;
   elem_iso_body<NN, NGP, QC>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
 }
@@ -877,10 +886,15 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
-    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live) {
+    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
+    int32_t n_nodes, int32_t slab) {
   __shared__ double buf[3 * kRowCap];
   __shared__ uint8_t lbuf[LIVE ? 3 * kRowCap : 1];
-  const int32_t a = blockIdx.x;
+  // one contiguous node range per XCD (slab > 0): its L2 sees the neighbouring nodes'
+  // data again
+  const int64_t ai = xcd_index(blockIdx.x, n_nodes, slab);
+  if (ai < 0) return;
+  const int32_t a = int32_t(ai);
   const int lane = threadIdx.x;
   const int bl = lane / 3, jl = lane - 3 * (lane / 3);
   int64_t rs[3];
@@ -1534,11 +1548,13 @@ __global__ __launch_bounds__(kWave) void k_elem_j2(
     const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
     const J2Dev* __restrict__ mats, const double* __restrict__ u,
     const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
-    double* __restrict__ fe, int* __restrict__ flag) {
+    double* __restrict__ fe, int* __restrict__ flag, int64_t slab) {
   constexpr int EPW = j2_epw(NGP);
   __shared__ ElemJ2Lds<NN, NGP, QC, EPW> L;
   const int lane = threadIdx.x;
-  const int64_t le0 = int64_t(blockIdx.x) * EPW;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
   const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
   for (int idx = lane; idx < ne * NN; idx += kWave) {
     const int e = idx / NN;
@@ -1785,16 +1801,25 @@ __global__ __launch_bounds__(kWave) void k_vjp(
 }
 
 // ---- host-side dispatch -------------------------------------------------------
+// Work items per XCD slab (0: no remap) and the grid that covers them.
+int64_t xcd_slab(const cxg_context* ctx, int64_t n) { return ctx->xcd_remap ? (n + 7) / 8 : 0; }
+unsigned xcd_grid(const cxg_context* ctx, int64_t n) {
+  const int64_t s = xcd_slab(ctx, n);
+  return unsigned(s > 0 ? 8 * s : (n > 0 ? n : 1));
+}
+
 // Phase B over every node of the pattern's mesh (see k_row_gather).
 void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
                    double* vals, double* f_int, uint8_t* live, int n_cases) {
   const cxg_mesh* m = p->mesh;
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
+  const int32_t slab = int32_t(xcd_slab(ctx, m->n_nodes));
+  const unsigned grid = xcd_grid(ctx, m->n_nodes);
 #define GATHER(NC, LV)                                                                          \
-  hipLaunchKernelGGL((k_row_gather<NC, LV>), dim3(unsigned(m->n_nodes)), dim3(kWave), 0,        \
-                     ctx->stream, m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p,         \
-                     p->nfm.p, p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals,    \
-                     f_int, live)
+  hipLaunchKernelGGL((k_row_gather<NC, LV>), dim3(grid), dim3(kWave), 0, ctx->stream,           \
+                     m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
+                     p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
+                     live, m->n_nodes, slab)
   if (n_cases == 4) {
     if (live) GATHER(4, true); else GATHER(4, false);
   } else if (n_cases == 2) {
@@ -1910,9 +1935,9 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
       double* kp = pm->ke_arena.p + p->kp_base[bi];
       LaunchTimer timer(ctx, std::string("elem_iso_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,    \
-                     ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E, nu, kp,            \
-                     ctx->flag_dev)
+  hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4>), dim3(xcd_grid(ctx, B.n_elem)), dim3(kWave), \
+                     0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E, nu, kp,         \
+                     ctx->flag_dev, B.n_elem, xcd_slab(ctx, B.n_elem))
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
       CXG_HIP(hipGetLastError());
@@ -2093,10 +2118,10 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
       LaunchTimer timer(ctx, std::string("elem_j2_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
   hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC>),                                                  \
-                     dim3(unsigned((B.n_elem + j2_epw(NGP) - 1) / j2_epw(NGP))), dim3(kWave), 0, \
-                     ctx->stream, conn, B.elem_offset, B.n_elem, B.gp_offset, m->n_points, T.dN, \
-                     T.w, coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe,            \
-                     ctx->flag_dev)
+                     dim3(xcd_grid(ctx, (B.n_elem + j2_epw(NGP) - 1) / j2_epw(NGP))), dim3(kWave), \
+                     0, ctx->stream, conn, B.elem_offset, B.n_elem, B.gp_offset, m->n_points,   \
+                     T.dN, T.w, coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe,      \
+                     ctx->flag_dev, xcd_slab(ctx, (B.n_elem + j2_epw(NGP) - 1) / j2_epw(NGP)))
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
       CXG_HIP(hipGetLastError());

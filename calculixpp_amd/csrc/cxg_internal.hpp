@@ -109,6 +109,7 @@ struct cxg_context {
   int chunk_elems = 4;
   int persistent_waves = 8;
   int n_cu = 256;  // compute units of the device (hipDeviceAttributeMultiprocessorCount)
+  int xcd_remap = 1;  // two-phase kernels: one contiguous work range per XCD
   std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
 };
 

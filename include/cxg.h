@@ -102,6 +102,9 @@ const char* cxg_last_error(const cxg_context* ctx);
  *                 (M = H^T H by MFMA)
  *   "chunk_elems"      elements per work item of the persistent launch (1..4096, 4)
  *   "persistent_waves" resident waves per compute unit of that launch (1..32, 8)
+ *   "xcd_remap"   1: the two-phase kernels give each of the 8 XCDs one contiguous range
+ *                 of nodes / elements (its L2 then sees neighbouring data again); 0: the
+ *                 plain block order (default 1)
  * Returns CXG_ERR_ARG for an unknown key or value. */
 cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value);
 /* Device memory owned by the context, for callers without a device allocator of
