@@ -229,6 +229,24 @@ __device__ __forceinline__ void decode_pair(int p, int& a, int& b) {
   }
 }
 
+// Lane -> (node a, first partner b0) of its group of up to G consecutive upper pairs
+// (a, b0 .. b0+G-1), b >= a, groups ordered a-major; (-1, -1) past the last group.
+template <int NN, int G>
+__device__ __forceinline__ void decode_group(int lane, int& a, int& b0) {
+  int rem = lane;
+  a = -1;
+  b0 = -1;
+  for (int aa = 0; aa < NN; ++aa) {
+    const int items = (NN - aa + G - 1) / G;
+    if (rem < items) {
+      a = aa;
+      b0 = aa + rem * G;
+      return;
+    }
+    rem -= items;
+  }
+}
+
 // Add the 3x3 block K (rows: DOFs of node a, cols: DOFs of node b) into the CSR
 // values; with `mirror` also its transpose into the rows of node b.
 template <int NN>
@@ -619,17 +637,18 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
                                               double* __restrict__ kel, int* __restrict__ flag) {
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
+  // A lane owns node a and up to PPL consecutive partners b (pairs (a, b0..b0+PPL-1)),
+  // so that h_a is loaded once per Gauss point for all of them.
   int pa[PPL], pb[PPL];
+  int ga, gb0;
+  decode_group<NN, PPL>(lane, ga, gb0);
 #pragma unroll
   for (int t = 0; t < PPL; ++t) {
-    const int p = lane * PPL + t;
-    if (p < NP) {
-      decode_pair<NN, true>(p, pa[t], pb[t]);
-    } else {
-      pa[t] = -1;
-      pb[t] = -1;
-    }
+    const bool ok = ga >= 0 && gb0 + t < NN;
+    pa[t] = ok ? ga : -1;
+    pb[t] = ok ? gb0 + t : -1;
   }
+  const int ga_c = ga >= 0 ? ga : 0;  // clamped: unowned slots compute unused values
   if (lane < NN) {
     const double* xp = coords + 3 * int64_t(ce[lane]);
     L.x[3 * lane + 0] = xp[0];
@@ -661,14 +680,14 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
       L.h[3 * idx + 2] = h2;
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < PPL; ++t) {
-      if (pa[t] < 0) continue;
 #pragma unroll 1
-      for (int qq = 0; qq < QC; ++qq) {
-        const double* ha = L.h + (qq * NN + pa[t]) * 3;
-        const double* hb = L.h + (qq * NN + pb[t]) * 3;
-        const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+    for (int qq = 0; qq < QC; ++qq) {
+      const double* ha = L.h + (qq * NN + ga_c) * 3;
+      const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+#pragma unroll
+      for (int t = 0; t < PPL; ++t) {
+        const int b = gb0 + t < NN ? gb0 + t : NN - 1;
+        const double* hb = L.h + (qq * NN + (b >= 0 ? b : 0)) * 3;
         const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
         M[t][0] += a0 * b0; M[t][1] += a0 * b1; M[t][2] += a0 * b2;
         M[t][3] += a1 * b0; M[t][4] += a1 * b1; M[t][5] += a1 * b2;
