@@ -1428,17 +1428,17 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
   constexpr int ND = 3 * NN;
+  // a lane owns node a and up to PPL consecutive partners b (see decode_group)
   int pa[PPL], pb[PPL];
+  int ga, gb0;
+  decode_group<NN, PPL>(lane, ga, gb0);
+  const int ga_c = ga >= 0 ? ga : 0;
   double K[PPL][9];
 #pragma unroll
   for (int t = 0; t < PPL; ++t) {
-    const int p = lane * PPL + t;
-    if (p < NP) {
-      decode_pair<NN, true>(p, pa[t], pb[t]);
-    } else {
-      pa[t] = -1;
-      pb[t] = -1;
-    }
+    const bool ok = ga >= 0 && gb0 + t < NN;
+    pa[t] = ok ? ga : -1;
+    pb[t] = ok ? gb0 + t : -1;
 #pragma unroll
     for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
   }
@@ -1476,29 +1476,31 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
         f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
         f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
       }
-#pragma unroll
-    for (int t = 0; t < PPL; ++t) {
-      if (pa[t] < 0) continue;
 #pragma unroll 1
-      for (int qq = 0; qq < qn; ++qq) {
-        const double* Tq = P.T + 9 * (q0 + qq);
-        const double lam = Tq[0], mu = Tq[1], c = Tq[2];
-        const double* ha = w.c.h + (qq * NN + pa[t]) * 3;
-        const double* hb = w.c.h + (qq * NN + pb[t]) * 3;
-        const double* ta = w.c.t + (qq * NN + pa[t]) * 3;
-        const double* tb = w.c.t + (qq * NN + pb[t]) * 3;
-        const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+    for (int qq = 0; qq < qn; ++qq) {
+      const double* Tq = P.T + 9 * (q0 + qq);
+      const double lam = Tq[0], mu = Tq[1], c = Tq[2];
+      const double* ha = w.c.h + (qq * NN + ga_c) * 3;
+      const double* ta = w.c.t + (qq * NN + ga_c) * 3;
+      const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
+      const double av[3] = {a0, a1, a2};
+      const double la[3] = {lam * a0, lam * a1, lam * a2};
+      const double ct[3] = {c * ta[0], c * ta[1], c * ta[2]};
+#pragma unroll
+      for (int t = 0; t < PPL; ++t) {
+        const int b = gb0 + t < NN ? (gb0 + t >= 0 ? gb0 + t : 0) : NN - 1;
+        const double* hb = w.c.h + (qq * NN + b) * 3;
+        const double* tb = w.c.t + (qq * NN + b) * 3;
         const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
-        const double la[3] = {lam * a0, lam * a1, lam * a2};
+        const double bv[3] = {b0, b1, b2};
         const double mb[3] = {mu * b0, mu * b1, mu * b2};
-        const double ct[3] = {c * ta[0], c * ta[1], c * ta[2]};
-        const double av[3] = {a0, a1, a2}, bv[3] = {b0, b1, b2};
+        const double tv[3] = {tb[0], tb[1], tb[2]};
         const double dg = mu * (a0 * b0 + a1 * b1 + a2 * b2);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j)
-            K[t][3 * i + j] += la[i] * bv[j] + mb[i] * av[j] - ct[i] * tb[j];
+            K[t][3 * i + j] += la[i] * bv[j] + mb[i] * av[j] - ct[i] * tv[j];
         K[t][0] += dg;
         K[t][4] += dg;
         K[t][8] += dg;
@@ -1539,42 +1541,19 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
   }
 }
 
-// One element per wave (the persistent kernel). Ends with a barrier.
-template <int NN, int NGP, int QC>
-__device__ __forceinline__ void elem_j2_body(
-    ElemJ2Lds<NN, NGP, QC, 1>& L, const int lane, const int32_t* __restrict__ ce,
-    const double Ee, const double nue, const J2Dev* mat, const int64_t pt0,
+// ne <= EPW consecutive elements of one block (first block-local le0) on one wave:
+// phase 1 for all their Gauss points at once, then phase 2 element by element. kp and
+// fe point at the block's arenas. Ends with a barrier.
+template <int NN, int NGP, int QC, int EPW>
+__device__ __forceinline__ void j2_group(
+    ElemJ2Lds<NN, NGP, QC, EPW>& L, const int lane, const int ne, const int64_t le0,
+    const int32_t* __restrict__ conn, const int64_t elem_offset, const int64_t gp_offset,
     const int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
-    const double* __restrict__ coords, const double* __restrict__ u,
-    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kel,
-    double* __restrict__ fel, int* __restrict__ flag) {
-  if (lane < NN) j2_stage<NN, NGP>(L.e[0], ce, coords, u, lane);
-  __syncthreads();
-  for (int q = lane; q < NGP; q += kWave)
-    j2_gauss_point<NN, NGP>(L.e[0], q, Ee, nue, mat, pt0 + q, n_points, dN, wq, committed,
-                            trial, flag);
-  __syncthreads();
-  j2_element_matrix<NN, NGP, QC>(L.e[0], L.w, lane, dN, kel, fel);
-}
-
-// EPW elements per wave: phase 1 for all their Gauss points at once, then phase 2
-// element by element.
-template <int NN, int NGP, int QC>
-__global__ __launch_bounds__(kWave) void k_elem_j2(
-    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t n_elem, int64_t gp_offset,
-    int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
     const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
     const J2Dev* __restrict__ mats, const double* __restrict__ u,
     const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
-    double* __restrict__ fe, int* __restrict__ flag, int64_t slab) {
-  constexpr int EPW = j2_epw(NGP);
-  __shared__ ElemJ2Lds<NN, NGP, QC, EPW> L;
-  const int lane = threadIdx.x;
-  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
-  if (grp < 0) return;
-  const int64_t le0 = grp * EPW;
-  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+    double* __restrict__ fe, int* __restrict__ flag) {
   for (int idx = lane; idx < ne * NN; idx += kWave) {
     const int e = idx / NN;
     j2_stage<NN, NGP>(L.e[e], conn + (le0 + e) * NN, coords, u, idx - e * NN);
@@ -1594,13 +1573,34 @@ __global__ __launch_bounds__(kWave) void k_elem_j2(
                                    fe + (le0 + e) * (3 * NN));
 }
 
+// EPW elements per wave (see j2_group).
+template <int NN, int NGP, int QC>
+__global__ __launch_bounds__(kWave) void k_elem_j2(
+    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t n_elem, int64_t gp_offset,
+    int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const int32_t* __restrict__ mat_id,
+    const J2Dev* __restrict__ mats, const double* __restrict__ u,
+    const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
+    double* __restrict__ fe, int* __restrict__ flag, int64_t slab) {
+  constexpr int EPW = j2_epw(NGP);
+  __shared__ ElemJ2Lds<NN, NGP, QC, EPW> L;
+  const int lane = threadIdx.x;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  j2_group<NN, NGP, QC, EPW>(L, lane, ne, le0, conn, elem_offset, gp_offset, n_points, dN, wq,
+                             coords, E, nu, mat_id, mats, u, committed, trial, kp, fe, flag);
+}
+
 // ---- Persistent element kernels (asm_variant 5) ------------------------------------
 // One launch covers every block of the mesh, whatever its family: a grid of resident
 // workgroups (one wave each) takes work items (runs of consecutive elements of one
 // block) from a shared counter until the counter passes the last item, so meshes of
 // very uneven size and mixed families balance across the compute units without one
 // launch per family. Items are ordered costliest family first. The element arithmetic
-// is the same device code as the per-family kernels (elem_iso_body / elem_j2_body);
+// is the same device code as the per-family kernels (elem_iso_body / j2_group);
 // the LDS is the union of the families' layouts.
 union IsoLdsAll {
   ElemLds<8, 8, 8> c3d8;
@@ -1609,10 +1609,10 @@ union IsoLdsAll {
   ElemLds<6, 2, 2> c3d6;
 };
 union J2LdsAll {
-  ElemJ2Lds<8, 8, 8, 1> c3d8;
-  ElemJ2Lds<20, 27, 3, 1> c3d20;
-  ElemJ2Lds<20, 8, 3, 1> c3d20r;
-  ElemJ2Lds<6, 2, 2, 1> c3d6;
+  ElemJ2Lds<8, 8, 8, j2_epw(8)> c3d8;
+  ElemJ2Lds<20, 27, 3, j2_epw(27)> c3d20;
+  ElemJ2Lds<20, 8, 3, j2_epw(8)> c3d20r;
+  ElemJ2Lds<6, 2, 2, j2_epw(2)> c3d6;
 };
 
 // Next work item for the whole wave (lane 0 takes it from the counter).
@@ -1682,17 +1682,15 @@ __global__ __launch_bounds__(kWave, 2) void k_elem_j2_persistent(
     const WorkChunk c = work[w];
     const BlockDesc B = blocks[c.block];
 #define CXG_J2_RUN(NN, NGP, QC, LDS)                                                        \
-  for (int32_t i = 0; i < c.count; ++i) {                                                 \
-    const int64_t le = int64_t(c.first) + i;                                              \
-    const int64_t ge = B.elem_offset + le;                                                \
-    const int64_t co = B.conn_offset + le * NN;                                           \
-    const int32_t mid = mat_id ? mat_id[ge] : -1;                                         \
+  for (int32_t i = 0; i < c.count; i += j2_epw(NGP)) {                                    \
+    const int ne = c.count - i < j2_epw(NGP) ? c.count - i : j2_epw(NGP);                 \
     int ln = lane;                                                                        \
     asm volatile("" : "+v"(ln));                                                          \
-    elem_j2_body<NN, NGP, QC>(LDS, ln, conn + co, E[ge], nu[ge],                          \
-                              mid >= 0 ? mats + mid : nullptr, B.gp_offset + le * NGP,    \
-                              n_points, B.dN, B.w, coords, u, committed, trial,           \
-                              kp + B.kp_base + le * (9 * NN * NN), fe + 3 * co, flag);    \
+    j2_group<NN, NGP, QC, j2_epw(NGP)>(LDS, ln, ne, int64_t(c.first) + i,                 \
+                                       conn + B.conn_offset, B.elem_offset, B.gp_offset,  \
+                                       n_points, B.dN, B.w, coords, E, nu, mat_id, mats,  \
+                                       u, committed, trial, kp + B.kp_base,               \
+                                       fe + 3 * B.conn_offset, flag);                     \
   }
     switch (B.type) {
       case CXG_C3D8: CXG_J2_RUN(8, 8, 8, L.c3d8) break;
@@ -2087,13 +2085,15 @@ void prepare_work(cxg_context* ctx, const cxg_pattern* p) {
     const auto cost = [&](size_t i) { return int64_t(desc[i].ngp) * desc[i].nn * desc[i].nn; };
     return cost(a) > cost(b);
   });
-  const int32_t chunk = ctx->chunk_elems;
+  // an item is chunk_elems groups of the J2 kernel's elements per wave
   std::vector<WorkChunk> items;
   for (size_t bi : order) {
     const int64_t n = m->blocks[bi].n_elem;
+    const int64_t chunk = int64_t(ctx->chunk_elems) * j2_epw(m->blocks[bi].ngp);
     for (int64_t f = 0; f < n; f += chunk)
       items.push_back(WorkChunk{int32_t(bi), int32_t(f), int32_t(std::min<int64_t>(chunk, n - f)), 0});
   }
+  const int32_t chunk = ctx->chunk_elems;
   if (items.size() > size_t(INT32_MAX)) throw Error(CXG_ERR_UNSUPPORTED, "too many work items");
   pm->work.alloc(std::max<size_t>(items.size(), 1));
   pm->bdesc.alloc(std::max<size_t>(desc.size(), 1));
