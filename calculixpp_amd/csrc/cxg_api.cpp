@@ -363,6 +363,10 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->persistent_waves = int(value);
     return CXG_OK;
   }
+  if (k == "gather_lds_cols" && value >= 8 && value <= 256) {
+    ctx->gather_lds_cols = int(value);
+    return CXG_OK;
+  }
   return cxg::fail(ctx, CXG_ERR_ARG, "unknown option or value");
 }
 

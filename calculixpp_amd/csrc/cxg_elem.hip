@@ -895,27 +895,22 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// NC displacement fields (load cases) at u + c*n_dof give NC internal forces at
-// f_int + c*n_dof from the same pass over the element rows. LIVE: also flag the
-// entries that received a contribution other than 0.0.
-template <int NC, bool LIVE>
-__global__ __launch_bounds__(kWave) void k_row_gather(
+// One node's three rows. HALVES = 2: two elements per step, one per half-wave, for
+// nodes whose incident elements all have <= 10 nodes (3n <= 30 columns fit in 32 lanes).
+template <int NC, bool LIVE, int HALVES>
+__device__ __forceinline__ void row_gather_node(
+    int32_t a, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
     const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
     const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
-    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    int32_t n_nodes, int32_t slab) {
-  __shared__ double buf[3 * kRowCap];
-  __shared__ uint8_t lbuf[LIVE ? 3 * kRowCap : 1];
-  // one contiguous node range per XCD (slab > 0): its L2 sees the neighbouring nodes'
-  // data again
-  const int64_t ai = xcd_index(blockIdx.x, n_nodes, slab);
-  if (ai < 0) return;
-  const int32_t a = int32_t(ai);
+    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live) {
   const int lane = threadIdx.x;
-  const int bl = lane / 3, jl = lane - 3 * (lane / 3);
+  constexpr int halves = HALVES;
+  const int h = HALVES == 2 ? lane / 32 : 0;
+  const int sub = HALVES == 2 ? lane % 32 : lane;
+  const int bl = sub / 3, jl = sub - 3 * (sub / 3);
   int64_t rs[3];
   int64_t Lrow = 0;
 #pragma unroll
@@ -924,31 +919,31 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     rs[i] = eq >= 0 ? row_ptr[eq] : -1;
     if (eq >= 0) Lrow = row_ptr[eq + 1] - row_ptr[eq];
   }
-  const bool in_lds = Lrow <= kRowCap;
+  const bool in_lds = Lrow <= cap;
   for (int i = 0; i < 3; ++i) {
     if (rs[i] < 0) continue;
     for (int64_t c = lane; c < Lrow; c += kWave) {
       if (in_lds) {
-        buf[i * kRowCap + c] = 0.0;
-        if constexpr (LIVE) lbuf[i * kRowCap + c] = 0;
+        buf[i * cap + c] = 0.0;
+        if constexpr (LIVE) lbuf[i * cap + c] = 0;
       } else {
         vals[rs[i] + c] = 0.0;
         if constexpr (LIVE) live[rs[i] + c] = 0;
       }
     }
   }
-  double f[NC][3];
+  if (!in_lds) __syncthreads();  // the zeros in vals are seen by the lanes that add
+  constexpr int NU = NC > 0 ? NC : 1;
+  double f[NU][3];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) f[c][0] = f[c][1] = f[c][2] = 0.0;
+  for (int c = 0; c < NU; ++c) f[c][0] = f[c][1] = f[c][2] = 0.0;
+  // element force vectors (fe mode): lanes sub < 3 of each half sum their elements'
+  // component sub in ascending element order; the halves are added at the end
+  double fsum = 0.0;
   const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
-  if (fe && lane < 3)  // element force vectors, summed in ascending element order
-    for (int64_t it = i0; it < i1; ++it) {
-      const IncRecord r = rec[it];
-      f[0][0] += fe[3 * (r.conn + r.la) + lane];
-    }
-  for (int64_t ib = i0; ib < i1; ib += kBatch) {
+  for (int64_t ib = i0; ib < i1; ib += kBatch * halves) {
     // issue every load of the batch first ...
-    double v[kBatch][3], uv[kBatch][NC];
+    double v[kBatch][3], uv[kBatch][NU], fv[kBatch];
     int32_t colp[kBatch];
     int fm[kBatch];
     bool ok[kBatch];
@@ -957,11 +952,15 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
       ok[k] = false;
       v[k][0] = v[k][1] = v[k][2] = 0.0;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) uv[k][c] = 0.0;
+      for (int c = 0; c < NU; ++c) uv[k][c] = 0.0;
       colp[k] = 0;
       fm[k] = 0;
-      if (ib + k < i1) {
-        const IncRecord r = rec[ib + k];
+      fv[k] = 0.0;
+      const int64_t it = ib + int64_t(k) * halves + h;
+      if (it < i1) {
+        const IncRecord r = rec[it];
+        if constexpr (NC == 0)
+          if (sub < 3) fv[k] = fe[3 * (r.conn + r.la) + sub];
         if (bl < r.n) {
           ok[k] = true;
           const int32_t nd = conn[r.conn + bl];
@@ -972,17 +971,22 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
           v[k][1] = ke[ndof];
           v[k][2] = ke[2 * ndof];
           fm[k] = nfm[nd];
-          if (u)
+          if constexpr (NC > 0)
+            if (u)
 #pragma unroll
-            for (int c = 0; c < NC; ++c) uv[k][c] = u[c * n_dof + 3 * int64_t(nd) + jl];
+              for (int c = 0; c < NC; ++c) uv[k][c] = u[c * n_dof + 3 * int64_t(nd) + jl];
         }
       }
     }
-    // ... then accumulate in ascending element order (one wave: LDS updates of
-    // successive elements are ordered by program order)
+    // ... then accumulate in ascending element order: element (k, half 0) before
+    // (k, half 1) before (k + 1, half 0). LDS updates of one wave are ordered by program
+    // order; the global read-modify-writes of long rows are separated by barriers.
 #pragma unroll
-    for (int k = 0; k < kBatch; ++k) {
-      if (!ok[k]) continue;
+    for (int k = 0; k < kBatch; ++k)
+    for (int hh = 0; hh < halves; ++hh) {
+      if (!in_lds && (k > 0 || hh > 0)) __syncthreads();
+      if (!ok[k] || h != hh) continue;
+      if constexpr (NC == 0) fsum += fv[k];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         f[c][0] += v[k][0] * uv[k][c];
@@ -993,12 +997,12 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
         const int64_t pos = int64_t(colp[k]) + __popc(fm[k] & ((1 << jl) - 1));
         if (in_lds) {
           if (rs[0] >= 0) buf[pos] += v[k][0];
-          if (rs[1] >= 0) buf[kRowCap + pos] += v[k][1];
-          if (rs[2] >= 0) buf[2 * kRowCap + pos] += v[k][2];
+          if (rs[1] >= 0) buf[cap + pos] += v[k][1];
+          if (rs[2] >= 0) buf[2 * cap + pos] += v[k][2];
           if constexpr (LIVE) {
             if (v[k][0] != 0.0) lbuf[pos] = 1;
-            if (v[k][1] != 0.0) lbuf[kRowCap + pos] = 1;
-            if (v[k][2] != 0.0) lbuf[2 * kRowCap + pos] = 1;
+            if (v[k][1] != 0.0) lbuf[cap + pos] = 1;
+            if (v[k][2] != 0.0) lbuf[2 * cap + pos] = 1;
           }
         } else {
           if (rs[0] >= 0) vals[rs[0] + pos] += v[k][0];
@@ -1018,11 +1022,14 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   if (in_lds)
     for (int i = 0; i < 3; ++i) {
       if (rs[i] < 0) continue;
-      for (int64_t c = lane; c < Lrow; c += kWave) vals[rs[i] + c] = buf[i * kRowCap + c];
+      for (int64_t c = lane; c < Lrow; c += kWave) vals[rs[i] + c] = buf[i * cap + c];
       if constexpr (LIVE)
-        for (int64_t c = lane; c < Lrow; c += kWave) live[rs[i] + c] = lbuf[i * kRowCap + c];
+        for (int64_t c = lane; c < Lrow; c += kWave) live[rs[i] + c] = lbuf[i * cap + c];
     }
-  if (u) {
+  if constexpr (NC == 0) {
+    const double other = __shfl(fsum, lane + 32);  // half 1's sum (0.0 for one half)
+    if (lane < 3) f_int[3 * int64_t(a) + lane] = halves == 2 ? fsum + other : fsum;
+  } else if (u) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const double s0 = wave_sum(f[c][0]);
@@ -1035,9 +1042,39 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
         fp[2] = s2;
       }
     }
-  } else if (fe && lane < 3) {
-    f_int[3 * int64_t(a) + lane] = f[0][0];
   }
+}
+
+// NC displacement fields (load cases) at u + c*n_dof give NC internal forces at
+// f_int + c*n_dof from the same pass over the element rows (u may be null: K only).
+// NC = 0: f_int is the sum of the element force vectors fe instead. LIVE: also flag the
+// entries that received a contribution other than 0.0. One wave per node.
+template <int NC, bool LIVE>
+__global__ __launch_bounds__(kWave) void k_row_gather(
+    const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
+    const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
+    const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
+    const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
+    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
+    int32_t n_nodes, int32_t slab, int32_t cap) {
+  // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
+  // rows accumulate in vals) and, for LIVE, their flags
+  extern __shared__ double row_lds[];
+  double* buf = row_lds;
+  uint8_t* lbuf = reinterpret_cast<uint8_t*>(row_lds + 3 * cap);
+  // one contiguous node range per XCD (slab > 0): its L2 sees the neighbouring nodes'
+  // data again
+  const int64_t ai = xcd_index(blockIdx.x, n_nodes, slab);
+  if (ai < 0) return;
+  const int32_t a = int32_t(ai);
+  // nfm bit 3: every incident element has at most 10 nodes (wave-uniform)
+  if (__builtin_amdgcn_readfirstlane(int(nfm[a])) & 8)
+    row_gather_node<NC, LIVE, 2>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
+                                 row_ptr, kp, u, n_dof, fe, vals, f_int, live);
+  else
+    row_gather_node<NC, LIVE, 1>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
+                                 row_ptr, kp, u, n_dof, fe, vals, f_int, live);
 }
 
 // ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
@@ -1832,12 +1869,17 @@ void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, cons
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
   const int32_t slab = int32_t(xcd_slab(ctx, m->n_nodes));
   const unsigned grid = xcd_grid(ctx, m->n_nodes);
+  const int32_t cap = int32_t(std::min<int64_t>((std::max<int64_t>(p->max_row, 1) + 7) / 8 * 8,
+                                                std::min(kRowCap, ctx->gather_lds_cols)));
+  const size_t lds = size_t(3 * cap) * (sizeof(double) + (live ? 1 : 0));
 #define GATHER(NC, LV)                                                                          \
-  hipLaunchKernelGGL((k_row_gather<NC, LV>), dim3(grid), dim3(kWave), 0, ctx->stream,           \
+  hipLaunchKernelGGL((k_row_gather<NC, LV>), dim3(grid), dim3(kWave), lds, ctx->stream,         \
                      m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
                      p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
-                     live, m->n_nodes, slab)
-  if (n_cases == 4) {
+                     live, m->n_nodes, slab, cap)
+  if (fe) {
+    if (live) GATHER(0, true); else GATHER(0, false);
+  } else if (n_cases == 4) {
     if (live) GATHER(4, true); else GATHER(4, false);
   } else if (n_cases == 2) {
     if (live) GATHER(2, true); else GATHER(2, false);

@@ -110,6 +110,7 @@ struct cxg_context {
   int persistent_waves = 8;
   int n_cu = 256;  // compute units of the device (hipDeviceAttributeMultiprocessorCount)
   int xcd_remap = 1;  // two-phase kernels: one contiguous work range per XCD
+  int gather_lds_cols = 256;  // longest row the row gather accumulates in LDS
   std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
 };
 
@@ -186,6 +187,7 @@ struct cxg_pattern {
   const cxg_mesh* mesh = nullptr;
   int32_t n_free = 0;
   int64_t nnz = 0;
+  int64_t max_row = 0;  // longest row (columns of one node's rows)
   cxg::DevBuf<int32_t> dof_eq;                // 3*n_nodes
   cxg::DevBuf<int64_t> row_ptr;               // n_free+1
   cxg::DevBuf<int32_t> col_idx;               // nnz

@@ -181,16 +181,18 @@ __global__ void k_inc_records(int32_t n_nodes, const int64_t* __restrict__ inc_p
   const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n_nodes) return;
   const int64_t a3 = 3 * int64_t(a);
-  nfm[a] = int8_t((dof_eq[a3] >= 0 ? 1 : 0) | (dof_eq[a3 + 1] >= 0 ? 2 : 0) |
-                  (dof_eq[a3 + 2] >= 0 ? 4 : 0));
+  int small = 8;  // bit 3: every incident element has at most 10 nodes
   for (int64_t it = inc_ptr[a]; it < inc_ptr[a + 1]; ++it) {
     const int32_t e = inc_elem[it];
     const int64_t b = eptr[e];
     const int n = int(eptr[e + 1] - b);
+    if (n > 10) small = 0;
     int la = 0;
     while (la < n && conn[b + la] != a) ++la;
     rec[it] = IncRecord{kpptr[e] + int64_t(3 * la) * (3 * n), sptr[e] + int64_t(la) * n, b, n, la};
   }
+  nfm[a] = int8_t((dof_eq[a3] >= 0 ? 1 : 0) | (dof_eq[a3 + 1] >= 0 ? 2 : 0) |
+                  (dof_eq[a3 + 2] >= 0 ? 4 : 0) | small);
 }
 
 // Entries kept per row (live != 0); one thread per row.
@@ -320,6 +322,8 @@ void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_de
     if (eq >= 0) hrow[size_t(eq) + 1] = hrow[size_t(eq)] + hrowlen[size_t(g / 3)];
   }
   P->nnz = hrow.back();
+  P->max_row = 0;
+  for (int64_t nd = 0; nd < nn_nodes; ++nd) P->max_row = std::max(P->max_row, hrowlen[size_t(nd)]);
   P->row_ptr.alloc(hrow.size());
   CXG_HIP(hipMemcpyAsync(P->row_ptr.p, hrow.data(), hrow.size() * sizeof(int64_t),
                          hipMemcpyHostToDevice, s));

@@ -106,6 +106,8 @@ const char* cxg_last_error(const cxg_context* ctx);
  *   "xcd_remap"   1: the two-phase kernels give each of the 8 XCDs one contiguous range
  *                 of nodes / elements (its L2 then sees neighbouring data again); 0: the
  *                 plain block order (default 1)
+ *   "gather_lds_cols" longest CSR row the two-phase row gather accumulates in LDS; longer
+ *                 rows accumulate in the value array itself (8..256, default 256)
  * Returns CXG_ERR_ARG for an unknown key or value. */
 cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value);
 /* Device memory owned by the context, for callers without a device allocator of

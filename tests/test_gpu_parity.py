@@ -268,3 +268,42 @@ def test_reference_key_set_rounding_noise(gpu, port, fam_i):
     ve = np.array([de.get(k, 0.0) for k in keys])
     vg = np.array([dg.get(k, 0.0) for k in keys])
     assert rel_l2(vg, ve) <= REL_K
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_gather_long_rows_match_lds_rows(gpu, port, fam_i):
+    """Rows longer than the gather's LDS capacity accumulate in the value array itself
+    (gather_lds_cols 8 sends every row there): same additions in the same order, so
+    K, f_int and the live flags equal the LDS path's bit for bit, for the linear
+    (displacement) and the J2 (element force) gathers."""
+    import torch
+    from calculixpp_amd import meshgen
+    fam = _families()[fam_i]
+    m = meshgen.j2_block(4, 3, 3, (1.0, 0.8, 0.6), family=fam, elastic_every=5, jitter_frac=0.1)
+    hist = meshgen.seeded_history(m.n_points, 3)
+    u_np = meshgen.smooth_field(m.coords, 7, 1.5e-4)
+    ctx = gpu.context()
+    a = gpu.TangentAssembler(m, ctx)
+    u = torch.from_numpy(u_np).to(ctx.device)
+    out = {}
+    try:
+        for cols in (256, 8):
+            ctx.set_option("gather_lds_cols", cols)
+            v, lv = a.new_values(), a.new_live()
+            f = torch.empty(3 * m.n_nodes, dtype=torch.float64, device=ctx.device)
+            a.assemble(v, u, f, live=lv)
+            mp = gpu.MaterialPoints(m.n_points, committed=hist)
+            vt, lt = a.new_values(), a.new_live()
+            ft = torch.empty_like(f)
+            a.assemble_tangent(u, mp, vt, ft, lt)
+            nnz = a.pattern.nnz
+            out[cols] = [v[:nnz].cpu().numpy(), lv[:nnz].cpu().numpy(), f.cpu().numpy(),
+                         vt[:nnz].cpu().numpy(), lt[:nnz].cpu().numpy(), ft.cpu().numpy()]
+    finally:
+        ctx.set_option("gather_lds_cols", 256)
+    for x, y in zip(out[256], out[8]):
+        assert np.array_equal(x, y)
+    from oracle.oracle import aos_state
+    assert rel_l2(out[8][2], port.internal_force(m, u_np)) <= REL_K
+    _, f_e, _ = port.assemble_material_tangent(m, u_np, aos_state(hist))
+    assert rel_l2(out[8][5], f_e) <= REL_K
