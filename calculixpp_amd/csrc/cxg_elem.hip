@@ -253,7 +253,7 @@ template <int NN>
 __device__ __forceinline__ void scatter_pair(const DofLds<NN>& D, const int32_t* __restrict__ se,
                                              int a, int b, const double K[9], bool mirror,
                                              double* __restrict__ vals) {
-  const int32_t cab = se[a * NN + b];
+  const int32_t cab = se[a * NN + b] & kSlotMask;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const int64_t r = D.row[3 * a + i];
@@ -266,7 +266,7 @@ __device__ __forceinline__ void scatter_pair(const DofLds<NN>& D, const int32_t*
     }
   }
   if (mirror) {
-    const int32_t cba = se[b * NN + a];
+    const int32_t cba = se[b * NN + a] & kSlotMask;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int64_t r = D.row[3 * b + j];
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_assemble_iso2(
       L.col[lane] = eq >= 0 ? before : -1;
     }
     const int32_t* se = slot + le * NN * NN;
-    for (int i = lane; i < NN * NN; i += kWave) L.slot[i] = se[i];
+    for (int i = lane; i < NN * NN; i += kWave) L.slot[i] = se[i] & kSlotMask;
     __syncthreads();
     for (int q = lane; q < NGP; q += kWave) {
       double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -963,18 +963,20 @@ __device__ __forceinline__ void row_gather_node(
           if (sub < 3) fv[k] = fe[3 * (r.conn + r.la) + sub];
         if (bl < r.n) {
           ok[k] = true;
-          const int32_t nd = conn[r.conn + bl];
-          colp[k] = slot_all[r.slot + bl];
+          const int32_t sl = slot_all[r.slot + bl];
+          colp[k] = sl & kSlotMask;
+          fm[k] = sl >> kSlotFreeShift;
           const double* ke = kp + r.rows + 3 * bl + jl;
           const int ndof = 3 * r.n;
           v[k][0] = ke[0];
           v[k][1] = ke[ndof];
           v[k][2] = ke[2 * ndof];
-          fm[k] = nfm[nd];
           if constexpr (NC > 0)
-            if (u)
+            if (u) {
+              const int32_t nd = conn[r.conn + bl];
 #pragma unroll
               for (int c = 0; c < NC; ++c) uv[k][c] = u[c * n_dof + 3 * int64_t(nd) + jl];
+            }
         }
       }
     }

@@ -32,6 +32,10 @@ inline const char* kBadJacobian = "non-positive element Jacobian (bad connectivi
 
 // Bits of the device status word every kernel may raise.
 enum : int { kFlagJacobian = 1, kFlagValence = 2, kFlagPattern = 4 };
+// slot_all entries: bits 0..27 the column offset, bits 28..30 the free-DOF mask of the
+// column node (bit c: DOF c is free)
+constexpr int32_t kSlotMask = (1 << 28) - 1;
+constexpr int kSlotFreeShift = 28;
 
 // Shape-derivative tables of one element family, resident in device memory:
 // dN[q][k][j] = dN_k/dxi_j at Gauss point q, and the Gauss weights w[q].
@@ -192,7 +196,8 @@ struct cxg_pattern {
   cxg::DevBuf<int64_t> row_ptr;               // n_free+1
   cxg::DevBuf<int32_t> col_idx;               // nnz
   // slot_all[sptr[e] + a*n + b]: offset of node b's column block inside the CSR rows of
-  // node a, for element e with n nodes (a, b local). slot_base[k] = sptr of block k's
+  // node a (bits 0..27) and node b's free-DOF mask (bits 28..30), for element e with n
+  // nodes (a, b local). slot_base[k] = sptr of block k's
   // first element.
   cxg::DevBuf<int32_t> slot_all;
   cxg::DevBuf<int64_t> sptr;                  // n_elem+1

@@ -18,7 +18,8 @@
 //   3. row_ptr by a host prefix sum over free equations;
 //   4. col_idx, one wave per node;
 //   5. slot[e][a][b] = column offset of node b's block inside the rows of node a,
-//      by binary search of the sorted adjacency (one thread per element node pair).
+//      by binary search of the sorted adjacency (one thread per element node pair),
+//      with node b's free-DOF mask in bits 28..30.
 #include <algorithm>
 #include <climits>
 #include <vector>
@@ -140,8 +141,8 @@ __global__ __launch_bounds__(kWave) void k_col_idx(const int64_t* __restrict__ a
 
 __global__ void k_slot(int64_t n_elem, int nn, const int32_t* __restrict__ conn,
                        const int64_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj,
-                       const int32_t* __restrict__ cpre, int32_t* __restrict__ slot,
-                       int* __restrict__ flag) {
+                       const int32_t* __restrict__ cpre, const int32_t* __restrict__ dof_eq,
+                       int32_t* __restrict__ slot, int* __restrict__ flag) {
   const int64_t total = n_elem * nn * nn;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += stride) {
@@ -161,11 +162,14 @@ __global__ void k_slot(int64_t n_elem, int nn, const int32_t* __restrict__ conn,
       if (v < nb) lo = mid + 1;
       else hi = mid - 1;
     }
-    if (found < 0) {
+    const int64_t b3 = 3 * int64_t(nb);
+    const int32_t free_b = (dof_eq[b3] >= 0 ? 1 : 0) | (dof_eq[b3 + 1] >= 0 ? 2 : 0) |
+                           (dof_eq[b3 + 2] >= 0 ? 4 : 0);
+    if (found < 0 || cpre[found] > kSlotMask) {
       atomicOr(flag, kFlagPattern);
       slot[idx] = 0;
     } else {
-      slot[idx] = cpre[found];
+      slot[idx] = cpre[found] | (free_b << kSlotFreeShift);
       (void)p0;
     }
   }
@@ -365,7 +369,7 @@ void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_de
     if (total > 0) {
       const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
       hipLaunchKernelGGL(k_slot, dim3(unsigned(blocks)), dim3(256), 0, s, B.n_elem, B.nn,
-                         m->conn.p + B.conn_offset, adj_ptr.p, adj.p, cpre.p,
+                         m->conn.p + B.conn_offset, adj_ptr.p, adj.p, cpre.p, P->dof_eq.p,
                          P->slot_all.p + P->slot_base[bi], ctx->flag_dev);
       CXG_HIP(hipGetLastError());
     }
