@@ -552,6 +552,24 @@ cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double*
   });
 }
 
+cxg_status cxg_recover_fields(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                              const double* E, const double* nu, const int32_t* mat_id,
+                              const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                              const double* committed, const double* f_ext, double* strain,
+                              double* stress, double* reaction, double* f_int,
+                              double* gauss_stress) {
+  if (!mesh || !coords || !E || !nu || !u || n_mats < 0 ||
+      (n_mats > 0 && (!mats || !mat_id || !committed)))
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad recover_fields arguments");
+  if (mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "mesh belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    const void* table = n_mats > 0 ? cxg::upload_j2_table(ctx, mats, n_mats) : nullptr;
+    cxg::launch_recover(ctx, mesh, coords, E, nu, u, n_mats > 0 ? mat_id : nullptr, table,
+                        committed, f_ext, strain, stress, reaction, f_int, gauss_stress);
+    cxg::check_flags(ctx);
+  });
+}
+
 cxg_status cxg_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial) {
   if (!committed || !trial || n_points < 0) return cxg::fail(ctx, CXG_ERR_ARG, "bad commit arguments");
   return cxg::guarded(ctx, [&] { cxg::launch_commit(ctx, n_points, committed, trial); });

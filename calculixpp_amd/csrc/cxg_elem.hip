@@ -1856,6 +1856,129 @@ __global__ __launch_bounds__(kWave) void k_vjp(
   }
 }
 
+// ---- K7: field recovery (stress.cpp:147-353) ----------------------------------------
+// Phase 1, one wave per element: Gauss-point strain (B u, engineering shear) and stress
+// (D eps, or the radial return from the committed history when a J2 table is given),
+// then per element the Gauss-point means (the hex/wedge extrapolation of
+// extrapolation.hpp:26-55 is the mean at every node), the element force
+// f_e = sum_q det*w B^T sigma into the element-force arena, and optionally every point's
+// stress. Phase 2, one thread per node: mean over incident elements in ascending element
+// order, f_int from the arena, RF = f_int - f_ext.
+template <int NN, int NGP>
+struct RecoverLds {
+  Geo<NN, NGP> G;
+  double u[NN * 3];
+  double eps[NGP * 6];
+  double sig[NGP * 6];
+  double S[NGP * 6];  // det J * w * sigma
+  int32_t node[NN];
+};
+
+template <int NN, int NGP>
+__global__ __launch_bounds__(kWave) void k_gauss_fields(
+    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
+    int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
+    const double* __restrict__ coords, const double* __restrict__ E,
+    const double* __restrict__ nu, const double* __restrict__ u,
+    const int32_t* __restrict__ mat_id, const J2Dev* __restrict__ mats,
+    const double* __restrict__ committed, double* __restrict__ emean,
+    double* __restrict__ fe, double* __restrict__ gauss_stress, int* __restrict__ flag) {
+  __shared__ RecoverLds<NN, NGP> L;
+  const int lane = threadIdx.x;
+  const int64_t le = blockIdx.x;
+  const int64_t eg = elem_offset + le;
+  const int32_t* ce = conn + le * NN;
+  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
+  stage_vec<NN>(L.u, ce, u, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  gradients<NN, NGP>(L.G, dN, lane);
+  __syncthreads();
+  double lam, mu;
+  lame(E[eg], nu[eg], lam, mu);
+  const int32_t mid = (mats && mat_id) ? mat_id[eg] : -1;
+  for (int q = lane; q < NGP; q += kWave) {
+    double H[3][3], e[6], sg[6];
+    grad_field<NN, NGP>(L.G, L.u, q, H);
+    voigt_strain(H, e);
+    const int64_t pt = gp_offset + le * NGP + q;
+    if (mid >= 0) {
+      // the committed history is read, never advanced (the reference evaluates a copy)
+      double hc[13], ht[13];
+      for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
+      J2Tangent T;
+      j2_return(mats[mid], e, hc, ht, sg, T);
+    } else {
+      iso_stress(lam, mu, e, sg);
+    }
+    for (int i = 0; i < 6; ++i) {
+      L.eps[6 * q + i] = e[i];
+      L.sig[6 * q + i] = sg[i];
+      L.S[6 * q + i] = L.G.sc[q] * sg[i];
+    }
+    if (gauss_stress)
+      for (int i = 0; i < 6; ++i) gauss_stress[6 * pt + i] = sg[i];
+  }
+  __syncthreads();
+  if (lane < 12) {  // Gauss-point means: sum in point order, times 1/NGP
+    const double* v = lane < 6 ? L.eps + lane : L.sig + (lane - 6);
+    double m = 0.0;
+    for (int q = 0; q < NGP; ++q) m += v[6 * q];
+    emean[12 * eg + lane] = m * (1.0 / NGP);
+  }
+  for (int k = lane; k < NN; k += kWave) {
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      const double* g = L.G.g + (q * NN + k) * 3;
+      const double* sq = L.S + 6 * q;
+      f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
+      f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
+      f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
+    }
+    double* fp = fe + 3 * (le * NN + k);
+    fp[0] = f0;
+    fp[1] = f1;
+    fp[2] = f2;
+  }
+}
+
+// Phase 2: one thread per node. fe holds the element forces at 3*(conn offset + local
+// node) + component, as the J2 element-force arena.
+__global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
+                              const int32_t* __restrict__ inc_elem,
+                              const int64_t* __restrict__ eptr, const int32_t* __restrict__ conn,
+                              const double* __restrict__ emean, const double* __restrict__ fe,
+                              const double* __restrict__ f_ext, double* __restrict__ strain,
+                              double* __restrict__ stress, double* __restrict__ reaction,
+                              double* __restrict__ f_int) {
+  const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_nodes) return;
+  double m[12], f[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 12; ++c) m[c] = 0.0;
+  const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
+  for (int64_t it = i0; it < i1; ++it) {
+    const int32_t e = inc_elem[it];
+    for (int c = 0; c < 12; ++c) m[c] += emean[12 * int64_t(e) + c];
+    const int64_t b = eptr[e], n = eptr[e + 1] - b;
+    int64_t la = 0;
+    while (la < n && conn[b + la] != a) ++la;
+    if (la < n)  // always: the incidence lists hold only elements containing the node
+      for (int c = 0; c < 3; ++c) f[c] += fe[3 * (b + la) + c];
+  }
+  const int64_t cnt = i1 - i0;
+  if (cnt > 0)
+    for (int c = 0; c < 12; ++c) m[c] /= double(cnt);
+  for (int c = 0; c < 6; ++c) {
+    if (strain) strain[6 * int64_t(a) + c] = m[c];
+    if (stress) stress[6 * int64_t(a) + c] = m[6 + c];
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (f_int) f_int[3 * int64_t(a) + c] = f[c];
+    if (reaction) reaction[3 * int64_t(a) + c] = f[c] - (f_ext ? f_ext[3 * int64_t(a) + c] : 0.0);
+  }
+}
+
 // ---- host-side dispatch -------------------------------------------------------
 // Work items per XCD slab (0: no remap) and the grid that covers them.
 int64_t xcd_slab(const cxg_context* ctx, int64_t n) { return ctx->xcd_remap ? (n + 7) / 8 : 0; }
@@ -2215,6 +2338,39 @@ void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const 
   const int64_t n = n_points * CXG_STATE_COMPONENTS;
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, committed, trial, n);
   CXG_HIP(hipGetLastError());
+}
+
+void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
+                    const double* nu, const double* u, const int32_t* mat_id,
+                    const void* j2_table, const double* committed, const double* f_ext,
+                    double* strain, double* stress, double* reaction, double* f_int,
+                    double* gauss_stress) {
+  m->rec_mean.ensure(size_t(std::max<int64_t>(12 * m->n_elem, 1)));
+  m->rec_fe.ensure(std::max<size_t>(3 * m->conn.n, 1));
+  const J2Dev* dmats = static_cast<const J2Dev*>(j2_table);
+  for (const cxg_block_dev& B : m->blocks) {
+    if (B.n_elem == 0) continue;
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+    double* feb = m->rec_fe.p + 3 * B.conn_offset;
+    LaunchTimer timer(ctx, "gauss_fields", B.n_elem);
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_gauss_fields<NN, NGP>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,       \
+                     ctx->stream, conn, B.elem_offset, B.gp_offset, m->n_points, T.dN, T.w,     \
+                     coords, E, nu, u, mat_id, dmats, committed, m->rec_mean.p, feb,            \
+                     gauss_stress, ctx->flag_dev)
+    CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+    CXG_HIP(hipGetLastError());
+  }
+  if (m->n_nodes > 0 && (strain || stress || reaction || f_int)) {
+    LaunchTimer timer(ctx, "node_fields", m->n_nodes);
+    hipLaunchKernelGGL(k_node_fields, dim3(unsigned((m->n_nodes + 255) / 256)), dim3(256), 0,
+                       ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_elem.p, m->eptr.p,
+                       m->conn.p, m->rec_mean.p, m->rec_fe.p, f_ext, strain, stress, reaction,
+                       f_int);
+    CXG_HIP(hipGetLastError());
+  }
 }
 
 void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,

@@ -75,6 +75,10 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
+  // Grow to at least `count` elements (contents are not kept).
+  void ensure(size_t count) {
+    if (n < count) alloc(count);
+  }
 };
 
 }  // namespace cxg
@@ -157,6 +161,10 @@ struct cxg_mesh {
   cxg::DevBuf<int64_t> inc_ptr;   // node -> incident elements (n_nodes+1)
   cxg::DevBuf<int32_t> inc_elem;  // incident model element numbers, ascending
   int32_t max_candidates = 0;     // max over nodes of sum of incident element sizes
+  // Field recovery arenas, allocated on first use: per element the Gauss-point means of
+  // strain and stress (12), per element node its force vector (3, at 3 * conn offset).
+  mutable cxg::DevBuf<double> rec_mean;
+  mutable cxg::DevBuf<double> rec_fe;
 };
 
 namespace cxg {
@@ -243,6 +251,11 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
                         const void* j2_table, const double* u, const double* committed,
                         double* trial, double* vals, double* f_int, uint8_t* live);
 void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial);
+// Field recovery (K7): Gauss-point strain/stress, node means, f_int and reactions.
+void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
+                    const double* nu, const double* u, const int32_t* mat_id, const void* j2_table,
+                    const double* committed, const double* f_ext, double* strain,
+                    double* stress, double* reaction, double* f_int, double* gauss_stress);
 void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
                 const double* nu, const double* u, const double* w, double* dx, double* dE,
                 double* du);

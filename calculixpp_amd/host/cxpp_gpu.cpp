@@ -269,6 +269,109 @@ cxg_j2_material to_cxg(const ElasticIso& el, const Plastic& pl) {
   return m;
 }
 
+// The material-point path covers J2 (*PLASTIC) and linear elastic elements only.
+void require_material_points(const Model& model, const fem::MaterialPoints& mp) {
+  for (const auto& h : model.element_hyperelastic())
+    if (h && !h->empty()) throw std::runtime_error("cxpp::gpu: *HYPERELASTIC is outside the GPU path");
+  for (const auto& um : model.element_user_material())
+    if (um && !um->empty()) throw std::runtime_error("cxpp::gpu: *USER MATERIAL is outside the GPU path");
+  if (mp.state.size() != model.mesh.num_elements())
+    throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
+}
+
+// J2 inputs of the device elements: the material table (one entry per distinct
+// elastic + plastic pair), each element's entry (-1: elastic) and the committed history
+// as SoA planes: plastic strain (6), eqplastic, back stress (6).
+struct J2Inputs {
+  std::vector<cxg_j2_material> mats;
+  std::vector<int32_t> mat_id;
+  std::vector<double> committed;
+};
+
+J2Inputs j2_inputs(const Model& model, const DeviceMesh& dm, const fem::MaterialPoints& mp) {
+  J2Inputs in;
+  const size_t ne = dm.elem.size();
+  const std::vector<ElasticIso> el = model.element_elastic();
+  const std::vector<std::optional<Plastic>> pl = model.element_plastic();
+  std::map<std::vector<double>, int32_t> seen;
+  in.mat_id.assign(ne, -1);
+  for (size_t k = 0; k < ne; ++k) {
+    const size_t e = dm.elem[k];
+    if (!pl[e] || pl[e]->empty()) continue;
+    const cxg_j2_material m = to_cxg(el[e], *pl[e]);
+    std::vector<double> key{m.E, m.nu, double(m.hardening), double(m.n_table), m.kinematic_modulus};
+    key.insert(key.end(), m.eqplastic, m.eqplastic + m.n_table);
+    key.insert(key.end(), m.yield, m.yield + m.n_table);
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      it = seen.emplace(key, int32_t(in.mats.size())).first;
+      in.mats.push_back(m);
+    }
+    in.mat_id[k] = it->second;
+  }
+  const size_t P = size_t(dm.n_points);
+  in.committed.assign(size_t(CXG_STATE_COMPONENTS) * P, 0.0);
+  for (size_t k = 0; k < ne; ++k) {
+    const std::vector<fem::MaterialState>& st = mp.state[dm.elem[k]];
+    const size_t ng = size_t(gauss_points(model.mesh.elements()[dm.elem[k]].type));
+    if (st.size() != ng) throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
+    for (size_t q = 0; q < ng; ++q) {
+      const size_t pt = size_t(dm.gp_first[k]) + q;
+      for (size_t i = 0; i < 6; ++i) {
+        in.committed[i * P + pt] = st[q].committed_plastic_strain[i];
+        in.committed[(7 + i) * P + pt] = st[q].committed_back_stress[i];
+      }
+      in.committed[6 * P + pt] = st[q].committed_eqplastic;
+    }
+  }
+  return in;
+}
+
+std::vector<double> flat(const std::vector<Vec3>& v) {
+  std::vector<double> h(v.size() * kDof);
+  for (size_t i = 0; i < v.size(); ++i)
+    for (int k = 0; k < kDof; ++k) h[kDof * i + size_t(k)] = v[i][size_t(k)];
+  return h;
+}
+
+// recover_fields in both forms (mp == nullptr: every element elastic).
+void recover(const Model& model, StaticFields& fields, const fem::MaterialPoints* mp) {
+  require_mechanical(model);
+  if (mp) require_material_points(model, *mp);
+  const Mesh& mesh = model.mesh;
+  const size_t nn = mesh.num_nodes();
+  if (fields.displacement.size() != nn)
+    throw std::runtime_error("cxpp::gpu: the displacement does not match the mesh");
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  J2Inputs j2;
+  if (mp) j2 = j2_inputs(model, dm, *mp);
+  const bool plastic = !j2.mats.empty();
+  Buf<double> ud(c, flat(fields.displacement)), fx(c, fem::external_load_vector(model));
+  Buf<double> sd(c, 6 * nn), ed(c, 6 * nn), rd(c, kDof * nn);
+  Buf<double> cd(c, std::max<size_t>(j2.committed.size(), 1));
+  Buf<int32_t> mid(c, std::max<size_t>(j2.mat_id.size(), 1));
+  if (plastic) {
+    cd.put(j2.committed.data());
+    mid.put(j2.mat_id.data());
+  }
+  check(c, cxg_recover_fields(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(),
+                              plastic ? mid.get() : nullptr, plastic ? j2.mats.data() : nullptr,
+                              int32_t(j2.mats.size()), ud.get(), plastic ? cd.get() : nullptr,
+                              fx.get(), ed.get(), sd.get(), rd.get(), nullptr, nullptr));
+  const std::vector<double> s = sd.fetch(6 * nn), e = ed.fetch(6 * nn), r = rd.fetch(kDof * nn);
+  fields.stress.assign(nn, Voigt6{});
+  fields.strain.assign(nn, Voigt6{});
+  fields.reaction.assign(nn, Vec3{0, 0, 0});
+  for (size_t i = 0; i < nn; ++i) {
+    for (size_t k = 0; k < 6; ++k) {
+      fields.stress[i][k] = s[6 * i + k];
+      fields.strain[i][k] = e[6 * i + k];
+    }
+    for (size_t k = 0; k < size_t(kDof); ++k) fields.reaction[i][k] = r[kDof * i + k];
+  }
+}
+
 }  // namespace
 
 void set_device(int device) { t_device = device; }
@@ -293,58 +396,20 @@ fem::LinearSystem assemble_linear_static(const Model& model) {
 fem::LinearSystem assemble_material_tangent(const Model& model, const std::vector<Vec3>& u,
                                             fem::MaterialPoints& mp, std::vector<Real>& f_int) {
   require_mechanical(model);
-  for (const auto& h : model.element_hyperelastic())
-    if (h && !h->empty()) throw std::runtime_error("cxpp::gpu: *HYPERELASTIC is outside the GPU path");
-  for (const auto& um : model.element_user_material())
-    if (um && !um->empty()) throw std::runtime_error("cxpp::gpu: *USER MATERIAL is outside the GPU path");
+  require_material_points(model, mp);
   const Mesh& mesh = model.mesh;
   if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
-  if (mp.state.size() != mesh.num_elements())
-    throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
   fem::LinearSystem sys;
   dof_map(model, sys);
   cxg_context* c = context();
   const DeviceMesh dm(c, model);
   const size_t ne = dm.elem.size();
-  // J2 materials: one table entry per distinct (elastic, plastic) pair
-  const std::vector<ElasticIso> el = model.element_elastic();
-  const std::vector<std::optional<Plastic>> pl = model.element_plastic();
-  std::vector<cxg_j2_material> mats;
-  std::map<std::vector<double>, int32_t> seen;
-  std::vector<int32_t> mat_id(ne, -1);
-  for (size_t k = 0; k < ne; ++k) {
-    const size_t e = dm.elem[k];
-    if (!pl[e] || pl[e]->empty()) continue;
-    const cxg_j2_material m = to_cxg(el[e], *pl[e]);
-    std::vector<double> key{m.E, m.nu, double(m.hardening), double(m.n_table), m.kinematic_modulus};
-    key.insert(key.end(), m.eqplastic, m.eqplastic + m.n_table);
-    key.insert(key.end(), m.yield, m.yield + m.n_table);
-    auto it = seen.find(key);
-    if (it == seen.end()) {
-      it = seen.emplace(key, int32_t(mats.size())).first;
-      mats.push_back(m);
-    }
-    mat_id[k] = it->second;
-  }
-  // committed history, SoA: plastic strain (6), eqplastic, back stress (6)
+  const J2Inputs j2 = j2_inputs(model, dm, mp);
+  const std::vector<cxg_j2_material>& mats = j2.mats;
+  const std::vector<int32_t>& mat_id = j2.mat_id;
+  const std::vector<double>& committed = j2.committed;
   const size_t P = size_t(dm.n_points);
-  std::vector<double> committed(size_t(CXG_STATE_COMPONENTS) * P);
-  for (size_t k = 0; k < ne; ++k) {
-    const std::vector<fem::MaterialState>& st = mp.state[dm.elem[k]];
-    const size_t ng = size_t(gauss_points(mesh.elements()[dm.elem[k]].type));
-    if (st.size() != ng) throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
-    for (size_t q = 0; q < ng; ++q) {
-      const size_t pt = size_t(dm.gp_first[k]) + q;
-      for (size_t i = 0; i < 6; ++i) {
-        committed[i * P + pt] = st[q].committed_plastic_strain[i];
-        committed[(7 + i) * P + pt] = st[q].committed_back_stress[i];
-      }
-      committed[6 * P + pt] = st[q].committed_eqplastic;
-    }
-  }
-  std::vector<double> uh(mesh.num_nodes() * kDof);
-  for (size_t i = 0; i < mesh.num_nodes(); ++i)
-    for (int k = 0; k < kDof; ++k) uh[kDof * i + size_t(k)] = u[i][size_t(k)];
+  const std::vector<double> uh = flat(u);
   const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
   const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
   Buf<double> vals(c, nnz), fd(c, uh.size()), ud(c, uh), cd(c, committed), td(c, committed.size());
@@ -475,6 +540,36 @@ std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector
   check(c, cxg_pcg(c, n, rpd.get(), cid.get(), vad.get(), bd.get(), xd.get(), 1e-13,
                    std::max<int32_t>(100, 20 * n), &iters, &rel));
   return xd.fetch(size_t(n));
+}
+
+void recover_fields(const Model& model, StaticFields& fields) { recover(model, fields, nullptr); }
+
+void recover_fields(const Model& model, StaticFields& fields, fem::MaterialPoints& mp) {
+  recover(model, fields, &mp);
+}
+
+std::vector<std::vector<Voigt6>> recover_gauss_stress(const Model& model,
+                                                      const std::vector<Vec3>& u) {
+  require_mechanical(model);
+  const Mesh& mesh = model.mesh;
+  if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  const size_t P = size_t(dm.n_points);
+  Buf<double> ud(c, flat(u)), gd(c, std::max<size_t>(6 * P, 1));
+  check(c, cxg_recover_fields(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(), nullptr,
+                              nullptr, 0, ud.get(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, gd.get()));
+  const std::vector<double> g = gd.fetch(6 * P);
+  std::vector<std::vector<Voigt6>> out(mesh.num_elements());  // removed elements: empty
+  for (size_t k = 0; k < dm.elem.size(); ++k) {
+    const size_t ng = size_t(gauss_points(mesh.elements()[dm.elem[k]].type));
+    std::vector<Voigt6>& v = out[dm.elem[k]];
+    v.resize(ng);
+    for (size_t q = 0; q < ng; ++q)
+      for (size_t i = 0; i < 6; ++i) v[q][i] = g[6 * (size_t(dm.gp_first[k]) + q) + i];
+  }
+  return out;
 }
 
 }  // namespace cxpp::gpu

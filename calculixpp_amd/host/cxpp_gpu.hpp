@@ -13,6 +13,10 @@
 //                                          (fem/stress.hpp:16, stress.cpp:125-145)
 //   cxpp::gpu::element_stiffness           fem::element_stiffness
 //                                          (fem/element.hpp:62, element.cpp:614-634)
+//   cxpp::gpu::recover_fields (both forms) fem::recover_fields
+//                                          (fem/stress.hpp:24,43, stress.cpp:189-353)
+//   cxpp::gpu::recover_gauss_stress        fem::recover_gauss_stress
+//                                          (fem/stress.hpp:35, stress.cpp:147-187)
 //   cxpp::gpu::solve_sparse                compute::ComputeBackend::solve_sparse
 //                                          (compute/backend.hpp:51-56), as a function:
 //                                          a HIP backend class wraps it one to one
@@ -35,6 +39,7 @@
 
 #include "calculixpp/compute/backend.hpp"
 #include "calculixpp/core/model.hpp"
+#include "calculixpp/core/results.hpp"
 #include "calculixpp/fem/assembly.hpp"
 #include "calculixpp/fem/element.hpp"
 
@@ -52,6 +57,16 @@ std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u)
 
 std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
                                     const ElasticIso& elastic);
+
+// Averaged nodal stress and strain and the reactions RF = f_int - f_ext of
+// fields.displacement; with mp, J2 elements take the radial-return stress from the
+// committed history (read only).
+void recover_fields(const Model& model, StaticFields& fields);
+void recover_fields(const Model& model, StaticFields& fields, fem::MaterialPoints& mp);
+
+// Per element, the stress at each Gauss point of u (removed elements: empty).
+std::vector<std::vector<Voigt6>> recover_gauss_stress(const Model& model,
+                                                      const std::vector<Vec3>& u);
 
 // A x = rhs for the SPD system given as COO triplets (duplicates summed), on the
 // device by Jacobi-preconditioned conjugate gradients to ||b - A x|| <= 1e-13 ||b||

@@ -1,21 +1,24 @@
 """Displacement solves on the device: linear static and the Newton-Raphson driver.
 
   solve_linear_static(model)       numerics::solve_linear_static (src/numerics/
-                                   linear_static.cpp:29): assemble K_ff, rhs; solve;
-                                   expand to the nodal displacement field
+                                   linear_static.cpp:29-50): assemble K_ff, rhs; solve;
+                                   expand to the nodal displacement field; recover
+                                   stress, strain and reactions
   solve_nonlinear_static(model)    numerics::solve_nonlinear_static (src/numerics/
                                    nonlinear_static.cpp:266-295): load factor 0 -> 1
                                    with automatic incrementation and cutback
                                    (run_increments, :197-233), a Newton loop per
                                    increment (newton_iterate, :159-186) whose
                                    assemble + return map + residual is one HIP graph,
-                                   and a commit of the material points per increment
+                                   a commit of the material points per increment,
+                                   and the field recovery of :276-288
 
 The sparse solves run on the device (cxg_pcg, Jacobi-preconditioned CG) in place of
-SciPP (ComputeBackend::solve_sparse). Displacements only: stress recovery
-(recover_fields) is outside this path. Loads and prescribed values follow the
-default linear ramp (no *AMPLITUDE), the models carry single point constraints only,
-and the Newton loop takes no line search (NonlinearOptions defaults).
+SciPP (ComputeBackend::solve_sparse). Both drivers return the reference's
+StaticFields (calculixpp_amd.recover, cxg_recover_fields). Loads and prescribed values
+follow the default linear ramp (no *AMPLITUDE), the models carry single point
+constraints only, and the Newton loop takes no line search (NonlinearOptions
+defaults).
 """
 from __future__ import annotations
 
@@ -28,6 +31,7 @@ import torch
 
 from . import fem
 from .model import Model
+from .recover import FieldRecovery, StaticFields
 
 
 @dataclass
@@ -78,8 +82,8 @@ def _expand(n_nodes: int, free_idx: torch.Tensor, spc_idx: torch.Tensor, spc_val
     return out
 
 
-def solve_linear_static(model: Model, tol: float = 1e-13, device=None) -> np.ndarray:
-    """Nodal displacements (n_nodes, 3) of a linear static step."""
+def solve_linear_static(model: Model, tol: float = 1e-13, device=None) -> StaticFields:
+    """Displacements, nodal stress and strain, and reactions of a linear static step."""
     ctx = fem.context(device)
     a = fem.Assembler(model, ctx)
     vals = a.new_values()
@@ -92,7 +96,16 @@ def solve_linear_static(model: Model, tol: float = 1e-13, device=None) -> np.nda
     u = torch.empty(3 * model.n_nodes, dtype=torch.float64, device=dev)
     _expand(model.n_nodes, free, spc, torch.from_numpy(a.prescribed[a.dof_eq < 0]).to(dev),
             x, 1.0, u)
-    return u.cpu().numpy().reshape(-1, 3)
+    return _fields(model, ctx, u)
+
+
+def _fields(model: Model, ctx: fem.Context, u: torch.Tensor,
+            mp: Optional[fem.MaterialPoints] = None) -> StaticFields:
+    """recover_fields (linear_static.cpp:48; nonlinear_static.cpp:284-288: the
+    material-point form for models with a nonlinear material)."""
+    o = FieldRecovery(model, ctx).run(u.reshape(-1, 3), mp if model.j2 else None)
+    return StaticFields(u.reshape(-1, 3).cpu().numpy(), o["stress"].cpu().numpy(),
+                        o["strain"].cpu().numpy(), o["reaction"].cpu().numpy())
 
 
 @dataclass
@@ -164,8 +177,8 @@ class _Newton:
 def solve_nonlinear_static(model: Model, controls: Optional[NonlinearControls] = None,
                            increment: Optional[Incrementation] = None, cg_tol: float = 1e-13,
                            device=None):
-    """Returns (displacement (n_nodes, 3), NonlinearReport, MaterialPoints with the
-    committed history)."""
+    """Returns (StaticFields, NonlinearReport, MaterialPoints with the committed
+    history)."""
     c = controls or NonlinearControls()
     inc = increment or Incrementation()
     st = _Newton(model, fem.context(device), cg_tol)
@@ -196,4 +209,4 @@ def solve_nonlinear_static(model: Model, controls: Optional[NonlinearControls] =
     u = _expand(model.n_nodes, st.free, st.spc, st.spc_val, st.uf, lam,
                 torch.empty_like(st.u))
     del st.graph
-    return u.cpu().numpy().reshape(-1, 3), rep, st.mp
+    return _fields(model, st.ctx, u, st.mp), rep, st.mp

@@ -233,6 +233,29 @@ cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
                    const double* E, const double* nu, const double* u, const double* w,
                    double* dx, double* dE, double* du);
 
+/* ---- K7: field recovery -----------------------------------------------------
+ * Replaces recover_fields (fem/stress.cpp:189-273), its material-point form
+ * (fem/stress.cpp:275-353) and recover_gauss_stress (fem/stress.cpp:147-187) for the
+ * C3D8/C3D20/C3D20R/C3D6 families, whose integration-point -> node extrapolation is
+ * the Gauss-point mean at every node (fem/extrapolation.hpp:26-55).
+ * u[3*n_nodes]: the displacement of every node. Per Gauss point: strain = B u
+ * (engineering shear); stress = D strain, or, when mats is given, the radial return
+ * of J2 elements (mat_id >= 0) from the committed history (13 planes of n_points;
+ * read, never advanced), as recover_fields(model, fields, mp).
+ * Outputs (device, each may be NULL, overwritten):
+ *   strain, stress [n_nodes*6]  mean over the node's elements of each element's
+ *                               Gauss-point mean (Voigt xx,yy,zz,xy,xz,yz)
+ *   f_int [3*n_nodes]           sum_e of det*w B^T stress over the Gauss points
+ *   reaction [3*n_nodes]        f_int - f_ext (f_ext: full-DOF load, NULL = zero)
+ *   gauss_stress [n_points*6]   per point, at the point index of the state arrays
+ * Inactive elements (*MODEL CHANGE, REMOVE) must be left out of the mesh. */
+cxg_status cxg_recover_fields(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                              const double* E, const double* nu, const int32_t* mat_id,
+                              const cxg_j2_material* mats, int32_t n_mats, const double* u,
+                              const double* committed, const double* f_ext, double* strain,
+                              double* stress, double* reaction, double* f_int,
+                              double* gauss_stress);
+
 /* ---- f1: sparse SPD solve on the device --------------------------------------
  * Replaces ComputeBackend::solve_sparse (compute/backend.hpp:51-56) for the SPD
  * systems of linear and Newton steps (the CPU backend hands them to SciPP,

@@ -22,6 +22,9 @@
 //   internal force     f = K u identity, _ref
 //   vjp (adjoint)      central finite differences of w^T K(x,E) u; closed forms
 //                      dphi/dE_e = phi_e / E_e and du = K w.
+//   recover_fields     _ref bitwise (elastic, material-point and Gauss-point forms);
+//                      linear displacement fields give the exact uniform strain and
+//                      D strain at every node; reactions balance the loads.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -934,6 +937,122 @@ int orc_vjp(const orc_model* m, const double* u, const double* w, double* dx, do
         du[3 * size_t(G.idx[k / 3]) + k % 3] += ldu[size_t(k)];
       }
     }
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// Field recovery. state == NULL: recover_fields (stress.cpp:189-273), every element
+// elastic with f_int = Ke ue (accumulate_internal_force, stress.cpp:52-69) and
+// stress = D strain (strain_stress_at, stress.cpp:20-40, no thermal strain). state given
+// (26 doubles per point, element-major): the material-point form (stress.cpp:275-353),
+// f_int from element_tangent_force and the point stress from the material evaluated on
+// a copy of the history. Hex/wedge extrapolation is the Gauss-point mean at every node
+// (extrapolation.hpp:37-45); node values are summed over elements in element order and
+// divided by the count; RF = f_int - f_ext (CLOADs at full magnitude).
+// gauss_stress: the per-point stress (recover_gauss_stress, stress.cpp:147-187, for
+// state == NULL), element-major; inactive elements' points stay zero. Outputs may be NULL.
+int orc_recover_fields(const orc_model* m, const double* u, const double* state,
+                       double* strain, double* stress, double* reaction, double* f_int,
+                       double* gauss_stress, char* err, int errlen) {
+  try {
+    const size_t nn = size_t(m->n_nodes);
+    std::vector<double> se(nn * 6, 0.0), ss(nn * 6, 0.0), fi(nn * 3, 0.0);
+    std::vector<int> count(nn, 0);
+    std::vector<J2Mat> mats;
+    for (int i = 0; i < m->n_j2; ++i) mats.push_back(make_j2(m->j2[i]));
+    Gathered G;
+    std::vector<double> Ke, fe, st;
+    double g[kMaxN][3];
+    size_t poff = 0;  // first point of element e
+    for (int64_t e = 0; e < m->n_elem; ++e) {
+      const int type = m->etype[e];
+      const auto& R = rule(type);
+      const size_t ng = R.size();
+      if (!is_active(*m, e)) {
+        if (gauss_stress) std::fill(gauss_stress + 6 * poff, gauss_stress + 6 * (poff + ng), 0.0);
+        poff += ng;
+        continue;
+      }
+      gather(*m, e, u, G);
+      const int n = G.n, ndof = 3 * n;
+      const Mat6 De = iso_D(m->E[e], m->nu[e]);
+      const int32_t jid = (state && m->j2_id) ? m->j2_id[e] : -1;
+      Ke.resize(size_t(ndof) * ndof);
+      fe.resize(size_t(ndof));
+      if (!state) {
+        stiffness(type, G.x.data(), m->E[e], m->nu[e], Ke.data());
+        for (int a = 0; a < ndof; ++a) {
+          double v = 0.0;
+          for (int b = 0; b < ndof; ++b) v += Ke[size_t(a) * ndof + b] * G.u[size_t(b)];
+          fi[size_t(G.idx[a / 3]) * 3 + a % 3] += v;
+        }
+      } else {
+        st.assign(state + poff * ORC_STATE, state + (poff + ng) * ORC_STATE);
+        tangent_force(type, G.x.data(), G.u.data(), m->E[e], m->nu[e],
+                      jid >= 0 ? &mats[size_t(jid)] : nullptr, st.data(), Ke.data(), fe.data());
+        for (int a = 0; a < ndof; ++a) fi[size_t(G.idx[a / 3]) * 3 + a % 3] += fe[size_t(a)];
+      }
+      double me[6] = {0, 0, 0, 0, 0, 0}, ms[6] = {0, 0, 0, 0, 0, 0};
+      for (size_t q = 0; q < ng; ++q) {
+        const GP& p = R[q];
+        const ShapeVals sv = shape_at(type, p.xi, p.et, p.ze);
+        gradients(sv, G.x.data(), g);
+        double eps[6], sig[6];
+        strain_of(g, n, G.u.data(), eps);
+        if (jid >= 0) {
+          double scratch[ORC_STATE];
+          std::copy(state + (poff + q) * ORC_STATE, state + (poff + q + 1) * ORC_STATE, scratch);
+          Mat6 Dt;
+          j2_eval(mats[size_t(jid)], eps, scratch, sig, Dt);
+        } else {
+          for (int i = 0; i < 6; ++i) {
+            double v = 0.0;
+            for (int j = 0; j < 6; ++j) v += De.m[i][j] * (eps[j] - 0.0);
+            sig[i] = v;
+          }
+        }
+        for (int c = 0; c < 6; ++c) {
+          me[c] += eps[c];
+          ms[c] += sig[c];
+        }
+        if (gauss_stress)
+          for (int c = 0; c < 6; ++c) gauss_stress[6 * (poff + q) + c] = sig[c];
+      }
+      const double inv = 1.0 / double(ng);
+      for (int c = 0; c < 6; ++c) {
+        me[c] *= inv;
+        ms[c] *= inv;
+      }
+      for (int i = 0; i < n; ++i) {
+        const size_t gi = size_t(G.idx[size_t(i)]);
+        for (int c = 0; c < 6; ++c) {
+          se[6 * gi + c] += me[c];
+          ss[6 * gi + c] += ms[c];
+        }
+        ++count[gi];
+      }
+      poff += ng;
+    }
+    for (size_t i = 0; i < nn; ++i)
+      if (count[i] > 0)
+        for (int c = 0; c < 6; ++c) {
+          se[6 * i + c] /= count[i];
+          ss[6 * i + c] /= count[i];
+        }
+    std::vector<double> fx(nn * 3, 0.0);
+    for (int i = 0; i < m->n_cload; ++i) {
+      const int32_t nd = m->cload_node[i];
+      if (nd >= 0 && nd < m->n_nodes && m->cload_comp[i] >= 1 && m->cload_comp[i] <= 3)
+        fx[size_t(nd) * 3 + size_t(m->cload_comp[i] - 1)] += 1.0 * m->cload_value[i];
+    }
+    if (strain) std::copy(se.begin(), se.end(), strain);
+    if (stress) std::copy(ss.begin(), ss.end(), stress);
+    if (f_int) std::copy(fi.begin(), fi.end(), f_int);
+    if (reaction)
+      for (size_t k = 0; k < nn * 3; ++k) reaction[k] = fi[k] - fx[k];
     return 0;
   } catch (const std::exception& e) {
     set_err(err, errlen, e.what());

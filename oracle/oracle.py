@@ -306,6 +306,30 @@ class Oracle:
             return t1 - t0
         return self._with_model(model, run)
 
+    def recover_fields(self, model, u, state=None, gauss_stress=False):
+        """recover_fields (state None) or its material-point form (state: (n_points, 26)
+        AoS history, committed slots read): dict of strain, stress (n, 6), reaction,
+        f_int (n, 3) and, with gauss_stress, the per-point stress (n_points, 6) in
+        model element order (zeros for removed elements). For the reference, f_int is
+        reaction + f_ext and gauss_stress exists for state None only."""
+        uu = np.ascontiguousarray(u, np.float64).reshape(-1)
+        st = None if state is None else np.ascontiguousarray(state, np.float64).copy()
+        n = model.n_nodes
+        out = {"strain": np.zeros((n, 6)), "stress": np.zeros((n, 6)),
+               "reaction": np.zeros((n, 3)), "f_int": np.zeros((n, 3))}
+        gs = np.zeros((max(model.n_points, 1), 6)) if gauss_stress else None
+
+        def run(m, arrays):
+            f = self._f("recover_fields")
+            f.argtypes = [C.c_void_p] + [C.c_void_p] * 7 + [C.c_char_p, C.c_int]
+            self._check(f(m, _p(uu), None if st is None else _p(st), _p(out["strain"]),
+                          _p(out["stress"]), _p(out["reaction"]), _p(out["f_int"]),
+                          None if gs is None else _p(gs), self._err, 512))
+        self._with_model(model, run)
+        if gs is not None:
+            out["gauss_stress"] = gs[: model.n_points]
+        return out
+
     def vjp(self, model, u, w):
         """(dx (n,3), dE (n_elem,), du (3n,), r (3n,)) of r = K(x,E)u, port only."""
         if self.kind != "port":

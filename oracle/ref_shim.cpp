@@ -13,6 +13,8 @@
 //   fem::make_material_points        (src/fem/assembly.cpp:357)
 //   fem::assemble_material_tangent   (src/fem/assembly.cpp:385)
 //   fem::internal_force              (src/fem/stress.cpp:125)
+//   fem::recover_fields              (src/fem/stress.cpp:189, 275)
+//   fem::recover_gauss_stress        (src/fem/stress.cpp:147)
 #include <cstring>
 #include <map>
 #include <memory>
@@ -25,6 +27,7 @@
 #include "calculixpp/fem/assembly.hpp"
 #include "calculixpp/fem/element.hpp"
 #include "calculixpp/fem/material_model.hpp"
+#include "calculixpp/fem/loads.hpp"
 #include "calculixpp/fem/stress.hpp"
 #include "orc_model.h"
 
@@ -271,6 +274,59 @@ int ref_internal_force(const ref_model* h, const double* u, double* f_int, char*
     const std::vector<Vec3> uu = as_vec3(u, h->model->mesh.num_nodes());
     const std::vector<Real> fi = impl::internal_force(*h->model, uu);
     std::memcpy(f_int, fi.data(), fi.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// recover_fields (state NULL) or its material-point form (state: 26 doubles per point,
+// the committed slots loaded into make_material_points' history). f_int is derived as
+// reaction + f_ext (the reference keeps its f_int internal). gauss_stress (state NULL
+// only): recover_gauss_stress flattened in element order, zeros for removed elements.
+int ref_recover_fields(const ref_model* h, const double* u, const double* state,
+                       double* strain, double* stress, double* reaction, double* f_int,
+                       double* gauss_stress, char* err, int errlen) {
+  try {
+    const Model& M = *h->model;
+    const size_t nn = M.mesh.num_nodes();
+    StaticFields fields;
+    fields.displacement = as_vec3(u, nn);
+    if (state) {
+      fem::MaterialPoints mp = fem::make_material_points(M);
+      size_t off = 0;
+      for (auto& es : mp.state)
+        for (auto& s : es) {
+          load_state(state + off, s);
+          off += ORC_STATE;
+        }
+      impl::recover_fields(M, fields, mp);
+    } else {
+      impl::recover_fields(M, fields);
+    }
+    const std::vector<Real> fx = fem::external_load_vector(M);
+    for (size_t i = 0; i < nn; ++i) {
+      for (int c = 0; c < 6; ++c) {
+        if (strain) strain[6 * i + c] = fields.strain[i][size_t(c)];
+        if (stress) stress[6 * i + c] = fields.stress[i][size_t(c)];
+      }
+      for (int c = 0; c < 3; ++c) {
+        if (reaction) reaction[3 * i + c] = fields.reaction[i][size_t(c)];
+        if (f_int) f_int[3 * i + c] = fields.reaction[i][size_t(c)] + fx[3 * i + size_t(c)];
+      }
+    }
+    if (gauss_stress && !state) {
+      const auto gp = impl::recover_gauss_stress(M, fields.displacement);
+      size_t off = 0;
+      for (size_t e = 0; e < gp.size(); ++e) {
+        const size_t ng = fem::gauss_rule(M.mesh.elements()[e].type).size();
+        for (size_t q = 0; q < ng; ++q)
+          for (int c = 0; c < 6; ++c)
+            gauss_stress[6 * (off + q) + c] = gp[e].empty() ? 0.0 : gp[e][q][size_t(c)];
+        off += ng;
+      }
+    }
     return 0;
   } catch (const std::exception& e) {
     set_err(err, errlen, e.what());

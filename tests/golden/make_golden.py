@@ -6,7 +6,8 @@ Every array in the fixtures is an output of a reference function called through
 oracle/ref_shim.cpp; inputs are seeded. The fixtures are data (inputs + expected
 outputs), committed so that tests pin the oracle without the reference present.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py recover    # one fixture group
 """
 import os
 import sys
@@ -49,8 +50,39 @@ def system_arrays(sys_, prefix=""):
             prefix + "rhs": sys_.rhs, prefix + "dof_eq": sys_.dof_eq}
 
 
+def recover(R):
+    """6. Field recovery: recover_fields and recover_gauss_stress on the 2x2x2 blocks,
+    the loaded cantilever, and the material-point form on a J2 block."""
+    rec = {}
+    for name, t in FAMILIES.items():
+        m = meshgen.block(2, t, seed_geo=21, seed_E=22)
+        u = meshgen.smooth_field(m.coords, 41)
+        o = R.recover_fields(m, u, gauss_stress=True)
+        rec[name + "_u"] = u
+        for k in ("strain", "stress", "reaction", "gauss_stress"):
+            rec[name + "_" + k] = o[k]
+    m = meshgen.cantilever()
+    u = meshgen.smooth_field(m.coords, 42)
+    o = R.recover_fields(m, u)
+    rec["cantilever_u"] = u
+    for k in ("strain", "stress", "reaction"):
+        rec["cantilever_" + k] = o[k]
+    m = meshgen.j2_block(3, 2, 2, (1.0, 0.8, 0.6), family=_lib.C3D20, elastic_every=5,
+                         jitter_frac=0.1)
+    hist = meshgen.seeded_history(m.n_points, 43)
+    u = meshgen.smooth_field(m.coords, 44, 1.5e-4)
+    o = R.recover_fields(m, u, aos_state(hist))
+    rec["j2_u"], rec["j2_history"] = u, hist
+    for k in ("strain", "stress", "reaction"):
+        rec["j2_" + k] = o[k]
+    save("recover", **rec)
+
+
 def main():
     R = Oracle("reference")
+    if sys.argv[1:] == ["recover"]:
+        recover(R)
+        return
     # 1. config 1 cantilever: K_ff, rhs; the reference key set equals the structural set
     m = meshgen.cantilever()
     s = R.assemble_linear_static(m)
@@ -112,6 +144,8 @@ def main():
     s, fi, st = R.assemble_material_tangent(m, u, aos_state(hist))
     arr = system_arrays(s)
     save("j2_mesh_c3d8", u=u, history=hist, f_int=fi, state_out=st, **arr)
+
+    recover(R)
 
 
 if __name__ == "__main__":

@@ -2,10 +2,13 @@
 
 Both sides take the reference's own cxpp::Model, built once per case from the same
 arrays: oracle/_ref/libcxpp_ref.so runs cxpp::fem::{assemble_linear_static,
-assemble_material_tangent, internal_force, element_stiffness} of the reference core,
-oracle/_ref/libcxpp_dropin.so the cxpp::gpu functions of the same names on libcxg.so.
+assemble_material_tangent, internal_force, element_stiffness, recover_fields,
+recover_gauss_stress} of the reference core, oracle/_ref/libcxpp_dropin.so the
+cxpp::gpu functions of the same names on libcxg.so.
   COO key set            identical (as sets of (row, col))
   K, rhs, f_int          rel-L2 <= 1e-12
+  nodal stress, strain,
+  reactions, point stress rel-L2 <= 1e-12
   trial history          rel-L2 <= 1e-10
   dof_eq                 identical
 Both libraries are built in the source container (oracle/Makefile `ref`, `dropin`).
@@ -147,3 +150,24 @@ def test_inverted_element_message(pair):
     for o in (ref, gpu):
         with pytest.raises(OracleError, match=r"non-positive element Jacobian \(bad connectivity\?\)"):
             o.assemble_linear_static(m)
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_recover_fields(pair, fam_i):
+    """recover_fields and recover_gauss_stress, and the material-point form on a J2
+    block, with the reference's StaticFields and MaterialPoints on both sides."""
+    from calculixpp_amd import _lib, meshgen
+    from oracle.oracle import aos_state
+    ref, gpu = pair
+    fam = [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6][fam_i]
+    m = meshgen.block(3, fam, seed_geo=121, seed_E=122, ny=2, nz=2)
+    u = meshgen.smooth_field(m.coords, 123)
+    a, b = gpu.recover_fields(m, u, gauss_stress=True), ref.recover_fields(m, u, gauss_stress=True)
+    for k in ("strain", "stress", "reaction", "gauss_stress"):
+        assert rel(a[k], b[k]) <= REL, k
+    m = meshgen.j2_block(3, 2, 2, (1.0, 0.8, 0.6), family=fam, elastic_every=4, jitter_frac=0.1)
+    h = aos_state(meshgen.seeded_history(m.n_points, 124))
+    u = meshgen.smooth_field(m.coords, 125, 1.5e-4)
+    a, b = gpu.recover_fields(m, u, h), ref.recover_fields(m, u, h)
+    for k in ("strain", "stress", "reaction"):
+        assert rel(a[k], b[k]) <= REL, k

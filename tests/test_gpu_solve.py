@@ -9,7 +9,11 @@ Pinned by:
     Gauss point, lateral strain -nu sigma/E - eps_p/2 — rel <= 1e-10;
   - a J2 block (clamped end, prescribed tip displacement, loads) against the oracle's
     CPU restatement of the same driver (oracle/newton.py): equal increment and
-    iteration counts, displacements and committed history rel <= 1e-10.
+    iteration counts, displacements and committed history rel <= 1e-10;
+  - the recovered fields: the reference's test_hex (one C3D8 cube in uniaxial
+    tension: s_xx = E delta, no lateral stress, x-face reactions -E delta) and
+    test_plasticity's uniaxial end-to-end cube (mean s_zz = sy0 + H eps_p), and the
+    oracle's recovery of the same displacement — rel <= 1e-10.
 """
 import numpy as np
 import pytest
@@ -35,10 +39,17 @@ def test_config1_displacements(gpu, port):
     from calculixpp_amd import meshgen
     from oracle import newton
     m = meshgen.cantilever()
-    u = gpu.solve_linear_static(m)
+    f = gpu.solve_linear_static(m)
+    u = f.displacement
     u_e = newton.solve_linear_static(port, m)
     assert rel(u, u_e) <= 1e-10
     assert u[:, 2].min() < 0  # the tip load points down
+    # the recovered fields of the solved displacement match the oracle's recovery
+    o = port.recover_fields(m, u_e)
+    for k in ("stress", "strain", "reaction"):
+        assert rel(getattr(f, k), o[k]) <= 1e-10, k
+    # equilibrium: the reactions balance the tip load
+    assert abs(f.reaction[:, 2].sum() + m.external_load()[2::3].sum()) <= 1e-9
 
 
 def test_pcg_is_deterministic(gpu):
@@ -78,10 +89,10 @@ def _bar(n=2, delta=0.01, j2=True):
 def test_elastic_newton_equals_linear(gpu):
     from calculixpp_amd import meshgen
     m = meshgen.cantilever()
-    u_lin = gpu.solve_linear_static(m)
-    u_nl, rep, _ = gpu.solve_nonlinear_static(m)
+    u_lin = gpu.solve_linear_static(m).displacement
+    f_nl, rep, _ = gpu.solve_nonlinear_static(m)
     assert rep.converged and rep.increments == 1
-    assert rel(u_nl, u_lin) <= 1e-10
+    assert rel(f_nl.displacement, u_lin) <= 1e-10
 
 
 def test_uniaxial_j2_bar_closed_form(gpu, port):
@@ -89,7 +100,8 @@ def test_uniaxial_j2_bar_closed_form(gpu, port):
     E, nu, sy0, H, eps = 210000.0, 0.3, 250.0, 2000.0, 0.01
     m = _bar(2, eps)
     tight = gpu.NonlinearControls(force_tol=1e-12, disp_tol=1e-12)
-    u, rep, mp = gpu.solve_nonlinear_static(m, controls=tight)
+    f, rep, mp = gpu.solve_nonlinear_static(m, controls=tight)
+    u = f.displacement
     assert rep.converged
     ep = (E * eps - sy0) / (E + H)
     sig = sy0 + H * ep
@@ -120,7 +132,8 @@ def test_j2_block_newton_matches_oracle(gpu, port):
               cload_value=np.array([5.0, 5.0]),
               j2_id=np.zeros(len(conn), np.int32),
               j2=[J2Params(210000.0, 0.3, [0.0, 1.0], [250.0, 2250.0])])
-    u, rep, mp = gpu.solve_nonlinear_static(m)
+    f, rep, mp = gpu.solve_nonlinear_static(m)
+    u = f.displacement
     u_e, rep_e, st_e = newton.solve_nonlinear_static(port, m)
     assert rep.converged and rep_e["final_load_factor"] == 1.0
     assert (rep.increments, rep.iterations, rep.cutbacks) == \
@@ -130,3 +143,66 @@ def test_j2_block_newton_matches_oracle(gpu, port):
     assert (st[6] > 0).mean() > 0.1  # a good part of the points yielded
     for k in range(13):
         assert rel(st[k], st_e[:, k]) <= 1e-10, k
+
+
+def _cube(uniaxial_x: bool, delta: float, j2=None):
+    """The single C3D8 unit cube of the reference's test_hex (make_cube: x = 0 face
+    held in x, x = 1 face at u_x = delta, rigid modes removed at nodes 1, 4, 5) or of
+    test_plasticity (uniaxial_cube: symmetry on the three back faces, top at u_z)."""
+    from calculixpp_amd import _lib
+    from calculixpp_amd.model import Model
+    x = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                  [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    conn = np.arange(8, dtype=np.int32)[None, :]
+    if uniaxial_x:
+        spc = [(n, 1, 0.0) for n in (1, 4, 5, 8)] + [(1, 2, 0.0), (1, 3, 0.0), (4, 3, 0.0),
+                                                     (5, 2, 0.0)]
+        spc += [(n, 1, delta) for n in (2, 3, 6, 7)]
+    else:
+        spc = [(n, 1, 0.0) for n in (1, 4, 5, 8)] + [(n, 2, 0.0) for n in (1, 2, 5, 6)]
+        spc += [(n, 3, 0.0) for n in (1, 2, 3, 4)] + [(n, 3, delta) for n in (5, 6, 7, 8)]
+    kw = {} if j2 is None else dict(j2_id=np.zeros(1, np.int32), j2=[j2])
+    return Model(x, [(_lib.C3D8, conn)], 210000.0, 0.3,
+                 spc_node=np.array([s[0] - 1 for s in spc]),
+                 spc_comp=np.array([s[1] for s in spc]),
+                 spc_value=np.array([s[2] for s in spc]), **kw)
+
+
+def test_hex_uniaxial_fields(gpu, port):
+    """tests/test_hex.cpp:58-81 on the device: homogeneous uniaxial solution."""
+    from oracle import newton
+    E, nu, delta = 210000.0, 0.3, 0.01
+    m = _cube(True, delta)
+    f = gpu.solve_linear_static(m)
+    i7 = 6
+    assert abs(f.displacement[i7, 0] - delta) <= 1e-10
+    assert abs(f.displacement[i7, 1] + nu * delta) <= 1e-10
+    assert abs(f.stress[i7, 0] - E * delta) <= 1e-9 * E * delta
+    assert abs(f.stress[i7, 1]) <= 1e-9 * E * delta and abs(f.stress[i7, 2]) <= 1e-9 * E * delta
+    assert abs(f.strain[i7, 0] - delta) <= 1e-12 and abs(f.strain[i7, 1] + nu * delta) <= 1e-12
+    rf0 = f.reaction[[0, 3, 4, 7], 0].sum()
+    assert abs(rf0 + E * delta) <= 1e-9 * E * delta
+    o = port.recover_fields(m, newton.solve_linear_static(port, m))
+    for k in ("stress", "strain", "reaction"):
+        assert rel(getattr(f, k), o[k]) <= 1e-10, k
+
+
+def test_plastic_cube_fields(gpu, port):
+    """tests/test_plasticity.cpp:162-197 on the device: the recovered axial stress of
+    a J2 cube follows sy0 + H eps_p (material-point recovery)."""
+    from calculixpp_amd.model import J2Params
+    from oracle import newton
+    from oracle.oracle import aos_state
+    E, H, sy0, uz = 210000.0, 8000.0, 800.0, 0.01
+    m = _cube(False, uz, J2Params(E, 0.3, [0.0, 0.02], [800.0, 960.0]))
+    tight = gpu.NonlinearControls(force_tol=1e-12, disp_tol=1e-12)
+    f, rep, mp = gpu.solve_nonlinear_static(m, controls=tight)
+    assert rep.converged
+    eps_p = (E * uz - sy0) / (E + H)
+    sigma = sy0 + H * eps_p
+    assert abs(f.stress[:, 2].mean() - sigma) <= 1e-9 * sigma
+    assert np.max(np.abs(f.stress[:, :2])) <= 1e-8 * sigma  # uniaxial
+    u_e, _, st_e = newton.solve_nonlinear_static(port, m, controls=tight)
+    o = port.recover_fields(m, u_e, st_e)
+    for k in ("stress", "strain", "reaction"):
+        assert rel(getattr(f, k), o[k]) <= 1e-10, k

@@ -369,3 +369,89 @@ def test_vjp_central_differences(port):
     # du = K w with the assembled operator (no BCs)
     s = port.assemble_linear_static(m)
     assert rel(du, s.dense() @ w.ravel()) < 1e-12
+
+
+# ---- field recovery (recover_fields / recover_gauss_stress) ------------------------
+
+def _recover_fixture_models():
+    for name, t in FAMS.items():
+        yield name, meshgen.block(2, t, seed_geo=21, seed_E=22)
+
+
+def test_golden_recover(port):
+    """recover_fields / recover_gauss_stress against the reference core's outputs."""
+    from oracle.oracle import aos_state
+    g = gold("recover")
+    for name, m in _recover_fixture_models():
+        o = port.recover_fields(m, g[name + "_u"], gauss_stress=True)
+        for k in ("strain", "stress", "reaction", "gauss_stress"):
+            assert np.array_equal(o[k], g[name + "_" + k]), (name, k)
+    m = meshgen.cantilever()
+    o = port.recover_fields(m, g["cantilever_u"])
+    for k in ("strain", "stress", "reaction"):
+        assert np.array_equal(o[k], g["cantilever_" + k]), k
+    m = meshgen.j2_block(3, 2, 2, (1.0, 0.8, 0.6), family=_lib.C3D20, elastic_every=5,
+                         jitter_frac=0.1)
+    o = port.recover_fields(m, g["j2_u"], aos_state(g["j2_history"]))
+    for k in ("strain", "stress", "reaction"):
+        assert np.array_equal(o[k], g["j2_" + k]), k
+
+
+@pytest.mark.parametrize("name", list(FAMS))
+def test_reference_bitwise_recover(port, ref, name):
+    from oracle.oracle import aos_state
+    m = meshgen.block(3, FAMS[name], seed_geo=111, seed_E=112, ny=2, nz=2)
+    u = meshgen.smooth_field(m.coords, 113)
+    a, b = port.recover_fields(m, u, gauss_stress=True), ref.recover_fields(m, u, gauss_stress=True)
+    for k in ("strain", "stress", "reaction", "gauss_stress"):
+        assert np.array_equal(a[k], b[k]), k
+    m = meshgen.j2_block(3, 2, 2, (1.0, 0.8, 0.6), family=FAMS[name], elastic_every=4,
+                         jitter_frac=0.1)
+    h = aos_state(meshgen.seeded_history(m.n_points, 114))
+    u = meshgen.smooth_field(m.coords, 115, 1.5e-4)
+    a, b = port.recover_fields(m, u, h), ref.recover_fields(m, u, h)
+    for k in ("strain", "stress", "reaction"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("name", list(FAMS))
+def test_recover_linear_field_closed_form(port, name):
+    """u = A x + b: every Gauss point, element and node carries the strain of A
+    (engineering shear) and D of it; the internal forces sum to zero."""
+    jittered = meshgen.block(3, FAMS[name], seed_geo=12, ny=2, nz=2,
+                             jitter_frac=0.0 if name == "C3D20R" else 0.1)
+    E, nu = 2.0e5, 0.3
+    m = Model(jittered.coords, jittered.blocks, E, nu)
+    rng = np.random.default_rng(13)
+    A = rng.normal(size=(3, 3)) * 1e-3
+    u = m.coords @ A.T + rng.normal(size=3) * 1e-3
+    eps = np.array([A[0, 0], A[1, 1], A[2, 2], A[0, 1] + A[1, 0], A[0, 2] + A[2, 0],
+                    A[1, 2] + A[2, 1]])
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    sig = np.concatenate([lam * eps[:3].sum() + 2 * mu * eps[:3], mu * eps[3:]])
+    o = port.recover_fields(m, u, gauss_stress=True)
+    assert np.abs(o["strain"] - eps).max() <= 1e-12 * np.abs(eps).max()
+    assert np.abs(o["stress"] - sig).max() <= 1e-12 * np.abs(sig).max()
+    assert np.abs(o["gauss_stress"] - sig).max() <= 1e-12 * np.abs(sig).max()
+    f = o["f_int"]
+    assert np.abs(f.sum(axis=0)).max() <= 1e-12 * np.abs(f).max()
+    assert np.array_equal(o["reaction"], f)  # no loads
+
+
+def test_recover_node_average_weights(port):
+    """Two elements of different stiffness under the same uniform strain: a shared
+    node's stress is the plain mean of the two elements' stresses (counted once per
+    element), an unshared node's is its own element's."""
+    c, conn = meshgen.hex_grid(2, 1, 1, (2.0, 1.0, 1.0), _lib.C3D8)
+    E = np.array([1.0e5, 3.0e5])
+    m = Model(c, [(_lib.C3D8, conn)], E, 0.25)
+    u = np.zeros_like(c)
+    u[:, 0] = 1e-3 * c[:, 0]
+    o = port.recover_fields(m, u)
+    s_e = [port.recover_fields(Model(c, [(_lib.C3D8, conn[i:i + 1])], E[i], 0.25), u)
+           ["stress"][conn[i][0]] for i in range(2)]
+    shared = np.intersect1d(conn[0], conn[1])
+    only0 = np.setdiff1d(conn[0], conn[1])
+    assert np.allclose(o["stress"][shared], 0.5 * (s_e[0] + s_e[1]), rtol=1e-14, atol=0)
+    assert np.allclose(o["stress"][only0], s_e[0], rtol=1e-14, atol=0)
+    assert np.all(o["stress"][shared, 0] > s_e[0][0]) and np.all(o["stress"][shared, 0] < s_e[1][0])
