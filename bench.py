@@ -260,6 +260,62 @@ def batch_step(n_meshes: int = 512, reps: int = 3, cpu: bool = True, cpu_seconds
     return out
 
 
+def recover_leg(model, u, family: int, reps: int = 5, cpu_n: int = 16, cpu: bool = True) -> dict:
+    """Field recovery (recover_fields: nodal stress and strain, f_int, reactions) of the
+    displacement u on the assembly workload, from HIP events on the library's stream.
+    Algorithmic counts per element: n_gp (72 n + 80) FLOP (Jacobian, gradients,
+    displacement gradient and B^T sigma at 9n FMA each, inverse, strain and stress);
+    bytes 192 N_nodes (coords, u, f_ext in; strain, stress, reaction out) +
+    (4n + 16) N_e (connectivity, E, nu)."""
+    import statistics
+    import torch
+    from calculixpp_amd import recover
+    from calculixpp_amd.model import GAUSS, NODES
+    r = recover.FieldRecovery(model)
+    ctx = r.ctx
+    r.run(u)
+    times = []
+    for _ in range(reps):
+        ctx.profile_reset()
+        ctx.profile(True)
+        r.run(u)
+        ctx.profile(False)
+        times.append(sum(v[1] for v in ctx.profile_stats().values()))
+    stats = ctx.profile_stats()
+    ms = statistics.median(times)
+    nn, ngp = NODES[family], GAUSS[family]
+    fpe = ngp * (72 * nn + 80)
+    flop = fpe * model.n_elem
+    byt = 192 * model.n_nodes + (4 * nn + 16) * model.n_elem
+    t_att = max(flop / (PEAK_FP64 * 1e12), byt / (PEAK_HBM * 1e9))
+    out = {"workload": "recover_fields of a smooth displacement on the assembly block",
+           "elements": model.n_elem, "nodes": model.n_nodes, "ms": ms,
+           "elements_per_s": model.n_elem / (ms * 1e-3),
+           "kernels_ms": {k: round(v[1], 4) for k, v in stats.items()},
+           "flop_per_element": fpe, "algorithmic_bytes": byt,
+           "roofline": {"bound": "mfma" if flop / (PEAK_FP64 * 1e12) >= byt / (PEAK_HBM * 1e9)
+                        else "hbm", "t_attainable_ms": round(t_att * 1e3, 4),
+                        "frac": round(t_att / (ms * 1e-3), 4)}}
+    del r
+    torch.cuda.synchronize()
+    if cpu:
+        from calculixpp_amd import meshgen
+        from oracle.oracle import Oracle, ref_available
+        kind = "reference" if ref_available() else "port"
+        cm = meshgen.block(cpu_n, family)
+        cu = meshgen.smooth_field(cm.coords, 2)
+        o = Oracle(kind)
+        t0 = time.perf_counter()
+        o.recover_fields(cm, cu)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cm.n_elem / sec, "unit": "elements/s", "cores": 1,
+                               "kind": kind, "sample": f"{cpu_n}^3 block of the same family "
+                               f"and seeds ({cm.n_elem} elements, recover_fields, {sec:.1f} s "
+                               "on one host thread)"}
+        out["gpu_over_cpu"] = out["elements_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
     """Reference CPU assembly (+ internal force) on an n^3 sample of the same
     workload, one thread (the reference is single-threaded)."""
@@ -289,6 +345,7 @@ def main() -> int:
     ap.add_argument("--adj-reps", type=int, default=20)
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
+    ap.add_argument("--no-recover", action="store_true", help="skip the field-recovery leg")
     ap.add_argument("--batch-meshes", type=int, default=512)
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="extra cxg_set_option for the library context (repeatable)")
@@ -421,6 +478,12 @@ def main() -> int:
               f"({j2['elements_per_s']:.4g} elements/s, yield {j2['yield_fraction']:.3f})",
               file=sys.stderr, flush=True)
 
+    rec = None
+    if not args.no_recover and u is not None:
+        rec = recover_leg(model, u.reshape(-1, 3), family, cpu_n=args.cpu_n, cpu=not args.no_cpu)
+        print(f"[bench] recover_fields {rec['ms']:.3f} ms ({rec['elements_per_s']:.4g} "
+              f"elements/s)", file=sys.stderr, flush=True)
+
     batch = None if args.no_batch else batch_step(
         args.batch_meshes, cpu=not args.no_cpu, log=lambda m: print(m, file=sys.stderr, flush=True))
     if batch:
@@ -451,6 +514,7 @@ def main() -> int:
         "kernels": kernels,
         "adjoint": adjoint,
         "j2_newton": j2,
+        "recover": rec,
         "batch": batch,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
