@@ -78,6 +78,8 @@ _SIGS = {
                                   _P, _P, _P, _P, _P, _P]),
     "cxg_commit": (C.c_int, [_P, C.c_int64, _P, _P]),
     "cxg_vjp": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "cxg_assemble_mass": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P]),
+    "cxg_assemble_geometric": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "cxg_recover_fields": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
                                      _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_graph_capture_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),

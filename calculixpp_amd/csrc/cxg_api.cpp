@@ -262,13 +262,16 @@ cxg_status cxg_create(int device, void* stream, cxg_context** out) {
     CXG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (cus > 0) ctx->n_cu = cus;
     for (int t : {CXG_C3D8, CXG_C3D20, CXG_C3D20R, CXG_C3D6}) {
-      std::vector<double> dN, w;
-      cxg::host_tables(t, dN, w);
+      std::vector<double> dN, w, N;
+      cxg::host_tables(t, dN, w, N);
       ctx->tab_dN[t].alloc(dN.size());
       ctx->tab_w[t].alloc(w.size());
+      ctx->tab_N[t].alloc(N.size());
       CXG_HIP(hipMemcpy(ctx->tab_dN[t].p, dN.data(), dN.size() * sizeof(double), hipMemcpyHostToDevice));
       CXG_HIP(hipMemcpy(ctx->tab_w[t].p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
-      ctx->tables[t] = {cxg::nodes_per_element(t), cxg::gauss_points(t), ctx->tab_dN[t].p, ctx->tab_w[t].p};
+      CXG_HIP(hipMemcpy(ctx->tab_N[t].p, N.data(), N.size() * sizeof(double), hipMemcpyHostToDevice));
+      ctx->tables[t] = {cxg::nodes_per_element(t), cxg::gauss_points(t), ctx->tab_dN[t].p,
+                        ctx->tab_w[t].p, ctx->tab_N[t].p};
     }
   });
   if (st != CXG_OK) {
@@ -548,6 +551,27 @@ cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double*
     const void* table = cxg::upload_j2_table(ctx, mats, n_mats);
     cxg::launch_assemble_j2(ctx, p, coords, E, nu, n_mats > 0 ? mat_id : nullptr, table, u,
                             committed, trial, vals, f_int, live);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_assemble_mass(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                             const double* rho, int32_t lumped, double* vals, uint8_t* live) {
+  if (!p || !coords || !rho || !vals) return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_mass arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_scalar(ctx, p, coords, lumped ? 1 : 0, rho, nullptr, vals, live);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_assemble_geometric(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                  const double* gp_stress, double* vals, uint8_t* live) {
+  if (!p || !coords || !gp_stress || !vals)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_geometric arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_scalar(ctx, p, coords, 2, nullptr, gp_stress, vals, live);
     cxg::check_flags(ctx);
   });
 }

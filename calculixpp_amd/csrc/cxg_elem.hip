@@ -1979,6 +1979,86 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
   }
 }
 
+// ---- K8: mass and geometric stiffness into the element arena (element.cpp:786-854) ---
+// Both matrices are k_ab I3 blocks: the consistent mass k_ab = sum_q rho detJ w N_a N_b
+// (element_mass), the lumped mass its row sums on the diagonal (element_mass_lumped),
+// the geometric stiffness k_ab = sum_q detJ w g_a^T sigma_q g_b (element_geometric_
+// stiffness with accumulate_kgeo). One element per wave, lanes over the upper node
+// pairs. The full (3n)^2 matrix goes to the arena with exact zeros off the block
+// diagonals, so the row gather's liveness flags give the reference's key set.
+enum : int { kMassConsistent = 0, kMassLumped = 1, kGeometric = 2 };
+
+template <int NN, int NGP>
+struct ScalarLds {
+  Geo<NN, NGP> G;
+  int32_t node[NN];
+  double k[NN * NN];
+  double row[NN];
+};
+
+template <int NN, int NGP, int KIND>
+__global__ __launch_bounds__(kWave) void k_elem_scalar(
+    const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
+    const double* __restrict__ dN, const double* __restrict__ wq, const double* __restrict__ Nq,
+    const double* __restrict__ coords, const double* __restrict__ rho,
+    const double* __restrict__ sig, double* __restrict__ kp, int* __restrict__ flag) {
+  __shared__ ScalarLds<NN, NGP> L;
+  const int lane = threadIdx.x;
+  const int64_t le = blockIdx.x;
+  stage_coords<NN, NGP>(L.G, L.node, conn + le * NN, coords, lane);
+  __syncthreads();
+  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
+  __syncthreads();
+  if constexpr (KIND == kGeometric) {
+    gradients<NN, NGP>(L.G, dN, lane);
+    __syncthreads();
+  }
+  const double r = KIND == kGeometric ? 0.0 : rho[elem_offset + le];
+  constexpr int NP = NN * (NN + 1) / 2;
+  for (int p = lane; p < NP; p += kWave) {
+    int a, b;
+    decode_pair<NN, true>(p, a, b);
+    double k = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      if constexpr (KIND == kGeometric) {
+        const double* s = sig + 6 * (gp_offset + le * NGP + q);
+        const double* ga = L.G.g + (q * NN + a) * 3;
+        const double* gb = L.G.g + (q * NN + b) * 3;
+        const double se = s[0] * ga[0] * gb[0] + s[1] * ga[1] * gb[1] + s[2] * ga[2] * gb[2] +
+                          s[3] * (ga[0] * gb[1] + ga[1] * gb[0]) +
+                          s[4] * (ga[0] * gb[2] + ga[2] * gb[0]) +
+                          s[5] * (ga[1] * gb[2] + ga[2] * gb[1]);
+        k += L.G.sc[q] * se;
+      } else {
+        k += r * L.G.sc[q] * Nq[q * NN + a] * Nq[q * NN + b];
+      }
+    }
+    L.k[a * NN + b] = k;
+    L.k[b * NN + a] = k;
+  }
+  __syncthreads();
+  if constexpr (KIND == kMassLumped) {
+    for (int a = lane; a < NN; a += kWave) {
+      double sum = 0.0;
+      for (int b = 0; b < NN; ++b) sum += L.k[a * NN + b];
+      L.row[a] = sum;
+    }
+    __syncthreads();
+  }
+  constexpr int ND = 3 * NN;
+  double* ke = kp + le * ND * ND;
+  for (int idx = lane; idx < ND * ND; idx += kWave) {
+    const int i = idx / ND, j = idx - ND * (idx / ND);
+    const int a = i / 3, b = j / 3;
+    double v = 0.0;
+    if (i - 3 * a == j - 3 * b) {
+      if constexpr (KIND == kMassLumped) v = a == b ? L.row[a] : 0.0;
+      else v = L.k[a * NN + b];
+    }
+    ke[idx] = v;
+  }
+}
+
 // ---- host-side dispatch -------------------------------------------------------
 // Work items per XCD slab (0: no remap) and the grid that covers them.
 int64_t xcd_slab(const cxg_context* ctx, int64_t n) { return ctx->xcd_remap ? (n + 7) / 8 : 0; }
@@ -2338,6 +2418,41 @@ void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const 
   const int64_t n = n_points * CXG_STATE_COMPONENTS;
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, ctx->stream, committed, trial, n);
   CXG_HIP(hipGetLastError());
+}
+
+void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                            int kind, const double* rho, const double* sig, double* vals,
+                            uint8_t* live) {
+  const cxg_mesh* m = p->mesh;
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  prepare_arenas(p, false);
+  static const char* const names[3] = {"elem_mass_", "elem_mass_lumped_", "elem_geometric_"};
+  for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+    const cxg_block_dev& B = m->blocks[bi];
+    if (B.n_elem == 0) continue;
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+    double* kp = pm->ke_arena.p + p->kp_base[bi];
+    LaunchTimer timer(ctx, std::string(names[kind]) + type_name(B.type), B.n_elem);
+#define LAUNCH_K(NN, NGP, KIND)                                                                 \
+  hipLaunchKernelGGL((k_elem_scalar<NN, NGP, KIND>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,  \
+                     ctx->stream, conn, B.elem_offset, B.gp_offset, T.dN, T.w, T.N, coords,     \
+                     rho, sig, kp, ctx->flag_dev)
+#define LAUNCH(NN, NGP, UP, QC)                                          \
+  if (kind == kMassConsistent) LAUNCH_K(NN, NGP, kMassConsistent);       \
+  else if (kind == kMassLumped) LAUNCH_K(NN, NGP, kMassLumped);          \
+  else LAUNCH_K(NN, NGP, kGeometric);
+    CXG_DISPATCH(B.type, LAUNCH)
+#undef LAUNCH
+#undef LAUNCH_K
+    CXG_HIP(hipGetLastError());
+  }
+  if (m->n_nodes > 0) {
+    LaunchTimer timer(ctx, kind == kGeometric ? "row_gather_geometric" : "row_gather_mass",
+                      m->n_elem);
+    launch_gather(ctx, p, nullptr, nullptr, vals, nullptr, live, 1);
+    CXG_HIP(hipGetLastError());
+  }
 }
 
 void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,

@@ -37,18 +37,21 @@ enum : int { kFlagJacobian = 1, kFlagValence = 2, kFlagPattern = 4 };
 constexpr int32_t kSlotMask = (1 << 28) - 1;
 constexpr int kSlotFreeShift = 28;
 
-// Shape-derivative tables of one element family, resident in device memory:
-// dN[q][k][j] = dN_k/dxi_j at Gauss point q, and the Gauss weights w[q].
+// Shape tables of one element family, resident in device memory:
+// dN[q][k][j] = dN_k/dxi_j at Gauss point q, the Gauss weights w[q] and the
+// shape-function values N[q][k].
 struct ElemTable {
   int nn = 0, ngp = 0;
   const double* dN = nullptr;  // device, ngp*nn*3
   const double* w = nullptr;   // device, ngp
+  const double* N = nullptr;   // device, ngp*nn
 };
 
 int nodes_per_element(int type);  // 0 for types outside the GPU path
 int gauss_points(int type);       // 0 for types outside the GPU path
 // Host tables (natural coordinates of the Gauss points and derivatives).
-void host_tables(int type, std::vector<double>& dN, std::vector<double>& w);
+void host_tables(int type, std::vector<double>& dN, std::vector<double>& w,
+                 std::vector<double>& N);
 
 // Device buffer owning helper (hipMalloc/hipFree), move-only.
 template <class T>
@@ -89,7 +92,7 @@ struct cxg_context {
   std::string err;
   int* flag_dev = nullptr;   // device status word (cxg::kFlag*)
   int* flag_host = nullptr;  // pinned mirror
-  cxg::DevBuf<double> tab_dN[8], tab_w[8];
+  cxg::DevBuf<double> tab_dN[8], tab_w[8], tab_N[8];
   cxg::ElemTable tables[8];
   std::vector<unsigned char> j2_host;  // last uploaded J2 material table (bytes)
   cxg::DevBuf<unsigned char> j2_dev;
@@ -251,6 +254,11 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
                         const void* j2_table, const double* u, const double* committed,
                         double* trial, double* vals, double* f_int, uint8_t* live);
 void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial);
+// Mass (kind 0 consistent, 1 lumped; rho per element) or geometric stiffness (kind 2;
+// sig per Gauss point) into the CSR values (K8 + the row gather).
+void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                            int kind, const double* rho, const double* sig, double* vals,
+                            uint8_t* live);
 // Field recovery (K7): Gauss-point strain/stress, node means, f_int and reactions.
 void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
                     const double* nu, const double* u, const int32_t* mat_id, const void* j2_table,

@@ -101,6 +101,37 @@ void d_wedge6(const double p[3], double d[][3]) {
     for (int j = 0; j < 3; ++j) d[k][j] = t[k][j];
 }
 
+// Shape-function values N_k at a point, same families and node orders as above.
+void n_values(int type, const double p[3], double* N) {
+  if (type == CXG_C3D8) {
+    for (int k = 0; k < 8; ++k)
+      N[k] = (1.0 + kSign[k][0] * p[0]) * (1.0 + kSign[k][1] * p[1]) *
+             (1.0 + kSign[k][2] * p[2]) / 8.0;
+  } else if (type == CXG_C3D6) {
+    const double a = 1.0 - p[0] - p[1];
+    const double lo = 0.5 * (1.0 - p[2]), hi = 0.5 * (1.0 + p[2]);
+    const double v[6] = {a * lo, p[0] * lo, p[1] * lo, a * hi, p[0] * hi, p[1] * hi};
+    for (int k = 0; k < 6; ++k) N[k] = v[k];
+  } else {
+    for (int k = 0; k < 8; ++k) {
+      double f = 1.0, q = -2.0;
+      for (int a = 0; a < 3; ++a) {
+        f *= 1.0 + kSign[k][a] * p[a];
+        q += kSign[k][a] * p[a];
+      }
+      N[k] = f * q / 8.0;
+    }
+    static const int from[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+    static const int axis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+    for (int e = 0; e < 12; ++e) {
+      const int ax = axis[e], c = from[e];
+      const int m1 = (ax + 1) % 3, m2 = (ax + 2) % 3;
+      N[8 + e] = (1.0 - p[ax] * p[ax]) * (1.0 + kSign[c][m1] * p[m1]) *
+                 (1.0 + kSign[c][m2] * p[m2]) / 4.0;
+    }
+  }
+}
+
 }  // namespace
 
 int nodes_per_element(int type) {
@@ -123,11 +154,13 @@ int gauss_points(int type) {
   }
 }
 
-void host_tables(int type, std::vector<double>& dN, std::vector<double>& w) {
+void host_tables(int type, std::vector<double>& dN, std::vector<double>& w,
+                 std::vector<double>& N) {
   const int nn = nodes_per_element(type);
   const std::vector<Pt> g = gauss(type);
   dN.assign(g.size() * size_t(nn) * 3, 0.0);
   w.assign(g.size(), 0.0);
+  N.assign(g.size() * size_t(nn), 0.0);
   double d[20][3];
   for (size_t q = 0; q < g.size(); ++q) {
     const double p[3] = {g[q].xi, g[q].et, g[q].ze};
@@ -137,6 +170,7 @@ void host_tables(int type, std::vector<double>& dN, std::vector<double>& w) {
     for (int k = 0; k < nn; ++k)
       for (int j = 0; j < 3; ++j) dN[(q * nn + k) * 3 + j] = d[k][j];
     w[q] = g[q].w;
+    n_values(type, p, N.data() + q * size_t(nn));
   }
 }
 

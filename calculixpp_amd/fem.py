@@ -9,6 +9,10 @@ arrays (torch tensors on the current HIP device) in place of std::vector:
   assemble_material_tangent(model, u, mp)       fem/assembly.cpp:385  -> (LinearSystem, f_int)
   internal_force(model, u)                      fem/stress.cpp:125
   element_stiffness(type, coords, E, nu)        fem/element.cpp:614
+  assemble_mass(model, lumped)                  fem/assembly.cpp:259
+  assemble_geometric_stiffness(model, gp_stress) fem/assembly.cpp:299
+
+Field recovery (recover_fields, recover_gauss_stress) lives in calculixpp_amd.recover.
 
 The returned LinearSystem holds the free/free stiffness as canonical CSR (rows
 ascending, columns ascending), which is what SciPP's CsrMatrix::from_coo builds from
@@ -241,6 +245,29 @@ class Assembler:
             _ptr(E if E is not None else m.E), _ptr(m.nu), _ptr(vals), _ptr(u), _ptr(f_int),
             _ptr(live)))
 
+    def assemble_mass(self, vals: torch.Tensor, lumped: bool = False,
+                      rho: Optional[torch.Tensor] = None, live: Optional[torch.Tensor] = None) -> None:
+        """Consistent (or row-sum lumped) mass into vals on this pattern (cxg_assemble_mass);
+        rho per element defaults to the model's densities."""
+        m = self.mesh
+        if rho is None:
+            rho = torch.from_numpy(self.model.rho).to(self.ctx.device)
+        self.ctx.check(self.ctx.lib.cxg_assemble_mass(self.ctx.h, self.pattern.h, _ptr(m.coords),
+                                                      _ptr(rho), int(bool(lumped)), _ptr(vals),
+                                                      _ptr(live)))
+
+    def assemble_geometric(self, vals: torch.Tensor, gp_stress: torch.Tensor,
+                           live: Optional[torch.Tensor] = None) -> None:
+        """Geometric stiffness of the Gauss-point stress (n_points, 6) into vals
+        (cxg_assemble_geometric)."""
+        m = self.mesh
+        g = gp_stress.to(self.ctx.device, torch.float64).contiguous()
+        if g.shape != (m.n_points, 6):
+            raise ValueError(f"gp_stress must be ({m.n_points}, 6)")
+        self.ctx.check(self.ctx.lib.cxg_assemble_geometric(self.ctx.h, self.pattern.h,
+                                                           _ptr(m.coords), _ptr(g), _ptr(vals),
+                                                           _ptr(live)))
+
     def capture(self, vals: torch.Tensor, u: Optional[torch.Tensor] = None,
                 f_int: Optional[torch.Tensor] = None) -> Graph:
         """One assembly (as assemble with these buffers) captured as a HIP graph."""
@@ -323,7 +350,7 @@ def _only_active(model: Model) -> Model:
         off += c.shape[0]
     return Model(model.coords, blocks, model.E[keep], model.nu[keep], model.spc_node,
                  model.spc_comp, model.spc_value, model.cload_node, model.cload_comp,
-                 model.cload_value)
+                 model.cload_value, rho=model.rho[keep])
 
 
 def assemble_linear_static(model: Model, device=None) -> LinearSystem:
@@ -333,6 +360,45 @@ def assemble_linear_static(model: Model, device=None) -> LinearSystem:
     live = a.new_live() if a.ctx.tracks_live else None
     a.assemble(vals, live=live)
     return a.system(vals, live)
+
+
+def _load_free_system(a: Assembler, vals: torch.Tensor, live: torch.Tensor) -> LinearSystem:
+    """The compacted system of a matrix that drives no load: assemble_mass and
+    assemble_geometric_stiffness discard the eliminated-column side effect."""
+    rp, ci, va = a.compact(vals, live)
+    rhs = torch.zeros(a.n_free, dtype=torch.float64, device=a.ctx.device)
+    return LinearSystem(a.n_free, rp, ci, va, rhs, a.dof_eq, a.prescribed)
+
+
+def assemble_mass(model: Model, lumped: bool = False, device=None) -> LinearSystem:
+    """fem::assemble_mass (fem/assembly.cpp:259-297) with model.rho (*DENSITY):
+    consistent element_mass or row-sum element_mass_lumped (fem/element.cpp:786-831).
+    *MASS point masses are outside this path."""
+    a = Assembler(model, context(device))
+    vals, live = a.new_values(), a.new_live()
+    a.assemble_mass(vals, lumped, live=live)
+    return _load_free_system(a, vals, live)
+
+
+def assemble_geometric_stiffness(model: Model, gp_stress, device=None) -> LinearSystem:
+    """fem::assemble_geometric_stiffness (fem/assembly.cpp:299-330) with
+    element_geometric_stiffness (fem/element.cpp:833-854). gp_stress: per model element
+    its (n_gp, 6) Gauss-point stress (recover_gauss_stress's list; removed elements
+    empty), or one (n_points, 6) array in model element order."""
+    a = Assembler(model, context(device))
+    if isinstance(gp_stress, (list, tuple)):
+        rows = [np.asarray(g, np.float64).reshape(-1, 6) for g in gp_stress]
+        keep = [r for i, r in enumerate(rows)
+                if model.active is None or bool(model.active[i])]
+        g = np.concatenate(keep) if keep else np.zeros((0, 6))
+    else:
+        g = np.asarray(gp_stress, np.float64).reshape(-1, 6)
+        if model.active is not None and not np.all(model.active):
+            ng = np.concatenate([np.full(c.shape[0], GAUSS[t]) for t, c in model.blocks])
+            g = g[np.repeat(np.asarray(model.active, bool), ng)]
+    vals, live = a.new_values(), a.new_live()
+    a.assemble_geometric(vals, torch.from_numpy(np.ascontiguousarray(g)), live=live)
+    return _load_free_system(a, vals, live)
 
 
 def internal_force(model: Model, u, device=None) -> torch.Tensor:

@@ -572,4 +572,60 @@ std::vector<std::vector<Voigt6>> recover_gauss_stress(const Model& model,
   return out;
 }
 
+namespace {
+
+// assemble_mass / assemble_geometric_stiffness: the k_ab I3 matrices on the all-free
+// pattern, reduced like the stiffness with the eliminated-column load discarded.
+fem::LinearSystem assemble_scalar(const Model& model, int kind,
+                                  const std::vector<std::vector<Voigt6>>* gp_stress) {
+  const Mesh& mesh = model.mesh;
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
+  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
+  Buf<double> vals(c, nnz);
+  Buf<uint8_t> live(c, nnz);
+  if (kind < 2) {
+    const std::vector<Real> rho = model.element_density();
+    std::vector<double> r(std::max<size_t>(dm.elem.size(), 1), 0.0);
+    for (size_t k = 0; k < dm.elem.size(); ++k) r[k] = rho[dm.elem[k]];
+    Buf<double> rd(c, r);
+    check(c, cxg_assemble_mass(c, pat.p, dm.coords->get(), rd.get(), kind == 1, vals.get(),
+                               live.get()));
+  } else {
+    const size_t P = size_t(dm.n_points);
+    std::vector<double> g(std::max<size_t>(6 * P, 1), 0.0);  // missing stress: zero
+    for (size_t k = 0; k < dm.elem.size(); ++k) {
+      const size_t e = dm.elem[k];
+      if (e >= gp_stress->size()) continue;
+      const std::vector<Voigt6>& es = (*gp_stress)[e];
+      const size_t ng = size_t(gauss_points(mesh.elements()[e].type));
+      for (size_t q = 0; q < ng && q < es.size(); ++q)
+        for (size_t i = 0; i < 6; ++i) g[6 * (size_t(dm.gp_first[k]) + q) + i] = es[q][i];
+    }
+    Buf<double> gd(c, g);
+    check(c, cxg_assemble_geometric(c, pat.p, dm.coords->get(), gd.get(), vals.get(), live.get()));
+  }
+  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  std::fill(sys.rhs.begin(), sys.rhs.end(), 0.0);  // mass and K_geo never drive a load
+  return sys;
+}
+
+}  // namespace
+
+fem::LinearSystem assemble_mass(const Model& model, bool lumped) {
+  require_mechanical(model);
+  if (!model.point_masses.empty())
+    throw std::runtime_error("cxpp::gpu: *MASS point masses are outside the GPU path");
+  return assemble_scalar(model, lumped ? 1 : 0, nullptr);
+}
+
+fem::LinearSystem assemble_geometric_stiffness(const Model& model,
+                                               const std::vector<std::vector<Voigt6>>& gp_stress) {
+  require_mechanical(model);
+  return assemble_scalar(model, 2, &gp_stress);
+}
+
 }  // namespace cxpp::gpu

@@ -13,6 +13,10 @@
 //                                          (fem/stress.hpp:16, stress.cpp:125-145)
 //   cxpp::gpu::element_stiffness           fem::element_stiffness
 //                                          (fem/element.hpp:62, element.cpp:614-634)
+//   cxpp::gpu::assemble_mass               fem::assemble_mass
+//                                          (assembly.hpp:55, assembly.cpp:259-297)
+//   cxpp::gpu::assemble_geometric_stiffness fem::assemble_geometric_stiffness
+//                                          (assembly.hpp:68, assembly.cpp:299-330)
 //   cxpp::gpu::recover_fields (both forms) fem::recover_fields
 //                                          (fem/stress.hpp:24,43, stress.cpp:189-353)
 //   cxpp::gpu::recover_gauss_stress        fem::recover_gauss_stress
@@ -57,6 +61,14 @@ std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u)
 
 std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
                                     const ElasticIso& elastic);
+
+// Consistent (or row-sum lumped) mass from the elements' *DENSITY; *MASS point masses
+// raise std::runtime_error.
+fem::LinearSystem assemble_mass(const Model& model, bool lumped = false);
+
+// Geometric stiffness of the per-element Gauss-point stresses (missing: zero).
+fem::LinearSystem assemble_geometric_stiffness(const Model& model,
+                                               const std::vector<std::vector<Voigt6>>& gp_stress);
 
 // Averaged nodal stress and strain and the reactions RF = f_int - f_ext of
 // fields.displacement; with mp, J2 elements take the radial-return stress from the

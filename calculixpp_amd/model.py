@@ -15,6 +15,7 @@ the model here is the same information as arrays:
   cload_*  node index, component 1..3, value     (*CLOAD)
   j2_id    (n_elem,) int32, -1 = elastic; j2 = list of J2Params  (*PLASTIC)
   active   (n_elem,) bool or None                (*MODEL CHANGE, REMOVE)
+  rho      (n_elem,) float64, default 0          (*DENSITY, Model::element_density)
 """
 from __future__ import annotations
 
@@ -67,6 +68,7 @@ class Model:
     j2_id: Optional[np.ndarray] = None
     j2: list = field(default_factory=list)
     active: Optional[np.ndarray] = None
+    rho: Optional[np.ndarray] = None
 
     def __post_init__(self):
         self.coords = np.ascontiguousarray(self.coords, np.float64)
@@ -79,6 +81,8 @@ class Model:
         n = self.n_elem
         self.E = np.array(np.broadcast_to(np.asarray(self.E, np.float64), (n,)))
         self.nu = np.array(np.broadcast_to(np.asarray(self.nu, np.float64), (n,)))
+        self.rho = np.zeros(n) if self.rho is None else \
+            np.array(np.broadcast_to(np.asarray(self.rho, np.float64), (n,)))
         for name in ("spc_node", "spc_comp", "cload_node", "cload_comp"):
             setattr(self, name, np.ascontiguousarray(getattr(self, name), np.int32))
         for name in ("spc_value", "cload_value"):

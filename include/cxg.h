@@ -233,6 +233,21 @@ cxg_status cxg_vjp(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
                    const double* E, const double* nu, const double* u, const double* w,
                    double* dx, double* dE, double* du);
 
+/* ---- K8: mass and geometric stiffness ----------------------------------------
+ * Replace assemble_mass (fem/assembly.cpp:259-297) with element_mass /
+ * element_mass_lumped (fem/element.cpp:786-831), and assemble_geometric_stiffness
+ * (fem/assembly.cpp:299-330) with element_geometric_stiffness (fem/element.cpp:
+ * 833-854), on the pattern p. rho[n_elem]: density per element (0: no mass);
+ * lumped != 0: row-sum lumping. gp_stress[n_points*6]: Cauchy stress per Gauss point
+ * (Voigt xx,yy,zz,xy,xz,yz; point index as the state arrays, e.g. the gauss_stress of
+ * cxg_recover_fields). vals[nnz] is overwritten; live (optional) flags the entries that
+ * received a contribution other than 0.0, for cxg_pattern_compact (the reference's
+ * key set). *MASS point masses are not part of this path. */
+cxg_status cxg_assemble_mass(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                             const double* rho, int32_t lumped, double* vals, uint8_t* live);
+cxg_status cxg_assemble_geometric(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                  const double* gp_stress, double* vals, uint8_t* live);
+
 /* ---- K7: field recovery -----------------------------------------------------
  * Replaces recover_fields (fem/stress.cpp:189-273), its material-point form
  * (fem/stress.cpp:275-353) and recover_gauss_stress (fem/stress.cpp:147-187) for the

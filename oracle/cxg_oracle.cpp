@@ -22,6 +22,9 @@
 //   internal force     f = K u identity, _ref
 //   vjp (adjoint)      central finite differences of w^T K(x,E) u; closed forms
 //                      dphi/dE_e = phi_e / E_e and du = K w.
+//   mass, geometric    _ref bitwise; total mass rho V per direction, lumped = row sums,
+//                      geometric stiffness annihilates rigid translations and is linear
+//                      in the stress; hydrostatic stress gives p times the Laplacian
 //   recover_fields     _ref bitwise (elastic, material-point and Gauss-point forms);
 //                      linear displacement fields give the exact uniform strain and
 //                      D strain at every node; reactions balance the loads.
@@ -493,6 +496,72 @@ void tangent_force(int type, const double* x, const double* ue, double E, double
   mirror(ndof, Ke);
 }
 
+// Consistent mass (element.cpp:786-812): m_ab = rho detJ w N_a N_b on each direction.
+void mass(int type, const double* x, double rho, double* Me) {
+  const int n = nodes_of(type), ndof = 3 * n;
+  std::fill(Me, Me + size_t(ndof) * ndof, 0.0);
+  if (rho == 0.0) return;
+  double g[kMaxN][3];
+  for (const GP& p : rule(type)) {
+    const ShapeVals s = shape_at(type, p.xi, p.et, p.ze);
+    const double det = gradients(s, x, g);
+    const double scale = rho * det * p.w;
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        const double m = scale * s.N[a] * s.N[b];
+        for (int d = 0; d < 3; ++d) Me[size_t(3 * a + d) * ndof + size_t(3 * b + d)] += m;
+      }
+  }
+}
+
+// Row-sum lumped mass (element.cpp:814-831).
+void mass_lumped(int type, const double* x, double rho, double* Ml) {
+  const int ndof = 3 * nodes_of(type);
+  std::vector<double> Mc(size_t(ndof) * ndof);
+  mass(type, x, rho, Mc.data());
+  std::fill(Ml, Ml + size_t(ndof) * ndof, 0.0);
+  for (int i = 0; i < ndof; ++i) {
+    double row = 0.0;
+    for (int j = 0; j < ndof; ++j) row += Mc[size_t(i) * ndof + j];
+    Ml[size_t(i) * ndof + i] = row;
+  }
+}
+
+// Geometric stiffness (element.cpp:833-854 with accumulate_kgeo, :482-508): upper
+// node pairs, sigma-energy of the two gradients on each direction, then mirrored.
+// sig: ngp Voigt stresses, or NULL (zero field).
+void geometric(int type, const double* x, const double* sig, double* Ke) {
+  const int n = nodes_of(type), ndof = 3 * n;
+  std::fill(Ke, Ke + size_t(ndof) * ndof, 0.0);
+  double g[kMaxN][3];
+  const auto& R = rule(type);
+  for (size_t q = 0; q < R.size(); ++q) {
+    const GP& p = R[q];
+    const ShapeVals sv = shape_at(type, p.xi, p.et, p.ze);
+    const double det = gradients(sv, x, g);
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    if (sig)
+      for (int c = 0; c < 6; ++c) s[c] = sig[6 * q + c];
+    const double scale = det * p.w;
+    for (int a = 0; a < n; ++a) {
+      const double* ga = g[a];
+      for (int b = a; b < n; ++b) {
+        const double* gb = g[b];
+        const double se = s[0] * ga[0] * gb[0] + s[1] * ga[1] * gb[1] + s[2] * ga[2] * gb[2] +
+                          s[3] * (ga[0] * gb[1] + ga[1] * gb[0]) +
+                          s[4] * (ga[0] * gb[2] + ga[2] * gb[0]) +
+                          s[5] * (ga[1] * gb[2] + ga[2] * gb[1]);
+        const double v = scale * se;
+        for (int d = 0; d < 3; ++d) {
+          const int ia = 3 * a + d, ib = 3 * b + d;
+          if (ib >= ia) Ke[size_t(ia) * ndof + ib] += v;
+        }
+      }
+    }
+  }
+  mirror(ndof, Ke);
+}
+
 // ---- Model-level assembly (reference src/fem/assembly.cpp) -------------------
 struct DofMap {
   int32_t n_free = 0;
@@ -937,6 +1006,50 @@ int orc_vjp(const orc_model* m, const double* u, const double* w, double* dx, do
         du[3 * size_t(G.idx[k / 3]) + k % 3] += ldu[size_t(k)];
       }
     }
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// assemble_mass (assembly.cpp:259-297) and assemble_geometric_stiffness (:299-330):
+// kind 0 consistent mass, 1 lumped mass, 2 geometric stiffness from gp_stress (per point,
+// element-major, 6 each). Scattered through the SPC congruence like the stiffness; the
+// load side effect is discarded, so rhs stays zero.
+int orc_assemble_scalar(const orc_model* m, int kind, const double* gp_stress, orc_system** out,
+                        char* err, int errlen) {
+  try {
+    auto* s = new orc_system();
+    DofMap d = dof_map(*m);
+    s->n_free = d.n_free;
+    s->rhs.assign(size_t(d.n_free), 0.0);
+    std::vector<double> dummy(size_t(d.n_free), 0.0);
+    KMap km;
+    Gathered G;
+    std::vector<double> Ke;
+    size_t poff = 0;
+    for (int64_t e = 0; e < m->n_elem; ++e) {
+      const int type = m->etype[e];
+      const size_t ng = size_t(ngp_of(type));
+      if (!is_active(*m, e)) {
+        poff += ng;
+        continue;
+      }
+      gather(*m, e, nullptr, G);
+      const int ndof = 3 * G.n;
+      Ke.resize(size_t(ndof) * ndof);
+      const double rho = m->rho ? m->rho[e] : 0.0;
+      if (kind == 0) mass(type, G.x.data(), rho, Ke.data());
+      else if (kind == 1) mass_lumped(type, G.x.data(), rho, Ke.data());
+      else geometric(type, G.x.data(), gp_stress ? gp_stress + 6 * poff : nullptr, Ke.data());
+      poff += ng;
+      scatter(Ke.data(), ndof, G.idx.data(), d, km, dummy);
+    }
+    flush(km, *s);
+    s->dof_eq = std::move(d.eq);
+    s->prescribed = std::move(d.prescribed);
+    *out = s;
     return 0;
   } catch (const std::exception& e) {
     set_err(err, errlen, e.what());

@@ -45,7 +45,7 @@ class OrcModel(C.Structure):
                 ("j2", C.c_void_p), ("active", C.c_void_p), ("n_spc", C.c_int32),
                 ("spc_node", C.c_void_p), ("spc_comp", C.c_void_p), ("spc_value", C.c_void_p),
                 ("n_cload", C.c_int32), ("cload_node", C.c_void_p), ("cload_comp", C.c_void_p),
-                ("cload_value", C.c_void_p)]
+                ("cload_value", C.c_void_p), ("rho", C.c_void_p)]
 
 
 def build(ref: bool = False) -> None:
@@ -110,6 +110,9 @@ class _ModelArrays:
         m.spc_node, m.spc_comp, m.spc_value = (_p(a) for a in self.spc)
         m.n_cload = len(self.cl[0])
         m.cload_node, m.cload_comp, m.cload_value = (_p(a) for a in self.cl)
+        rho = getattr(model, "rho", None)
+        self.rho = None if rho is None else np.ascontiguousarray(rho, np.float64)
+        m.rho = _p(self.rho)
         self.struct = m
 
 
@@ -305,6 +308,31 @@ class Oracle:
             self._check(status)
             return t1 - t0
         return self._with_model(model, run)
+
+    def _assemble_scalar(self, model, kind, gp_stress=None) -> System:
+        g = None if gp_stress is None else np.ascontiguousarray(gp_stress, np.float64)
+
+        def run(m, arrays):
+            h = C.c_void_p()
+            f = self._f("assemble_scalar")
+            f.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p), C.c_char_p,
+                          C.c_int]
+            self._check(f(m, int(kind), None if g is None else _p(g), C.byref(h), self._err, 512))
+            try:
+                nf, rows, cols, vals, rhs = self._system(h)
+                return System(nf, rows, cols, vals, rhs, self._dof_eq(h, model.n_nodes))
+            finally:
+                self._f("system_free")(h)
+        return self._with_model(model, run)
+
+    def assemble_mass(self, model, lumped=False) -> System:
+        """assemble_mass (consistent, or row-sum lumped) with model.rho."""
+        return self._assemble_scalar(model, 1 if lumped else 0)
+
+    def assemble_geometric_stiffness(self, model, gp_stress) -> System:
+        """assemble_geometric_stiffness; gp_stress (n_points, 6) in model element order
+        (rows of removed elements are ignored)."""
+        return self._assemble_scalar(model, 2, gp_stress)
 
     def recover_fields(self, model, u, state=None, gauss_stress=False):
         """recover_fields (state None) or its material-point form (state: (n_points, 26)

@@ -72,6 +72,7 @@ typedef struct {
   const int32_t* cload_node; /* node index */
   const int32_t* cload_comp; /* 1..3 */
   const double* cload_value;
+  const double* rho;         /* n_elem density (*DENSITY); may be NULL (no mass) */
 } orc_model;
 
 #ifdef __cplusplus

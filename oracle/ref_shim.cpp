@@ -14,6 +14,8 @@
 //   fem::assemble_material_tangent   (src/fem/assembly.cpp:385)
 //   fem::internal_force              (src/fem/stress.cpp:125)
 //   fem::recover_fields              (src/fem/stress.cpp:189, 275)
+//   fem::assemble_mass               (src/fem/assembly.cpp:259)
+//   fem::assemble_geometric_stiffness (src/fem/assembly.cpp:299)
 //   fem::recover_gauss_stress        (src/fem/stress.cpp:147)
 #include <cstring>
 #include <map>
@@ -92,18 +94,18 @@ std::vector<Vec3> as_vec3(const double* p, size_t n) {
 }
 
 // Build a reference Model: node ids = index+1, element ids = index+1, one
-// material + elset per distinct (E, nu, J2 material) triple.
+// material + elset per distinct (E, nu, J2 material, density).
 std::unique_ptr<Model> build_model(const orc_model& m) {
   auto M = std::make_unique<Model>();
   for (int32_t i = 0; i < m.n_nodes; ++i)
     M->mesh.add_node(i + 1, {m.coords[3 * i], m.coords[3 * i + 1], m.coords[3 * i + 2]});
-  std::map<std::tuple<double, double, int32_t>, std::vector<Index>> groups;
+  std::map<std::tuple<double, double, int32_t, double>, std::vector<Index>> groups;
   for (int32_t e = 0; e < m.n_elem; ++e) {
     std::vector<Index> ids;
     for (int64_t k = m.eptr[e]; k < m.eptr[e + 1]; ++k) ids.push_back(m.conn[k] + 1);
     M->mesh.add_element(e + 1, static_cast<ElementType>(m.etype[e]), std::move(ids));
     const int32_t jid = m.j2_id ? m.j2_id[e] : -1;
-    groups[{m.E[e], m.nu[e], jid}].push_back(e + 1);
+    groups[{m.E[e], m.nu[e], jid, m.rho ? m.rho[e] : 0.0}].push_back(e + 1);
     if (m.active && m.active[e] == 0) M->deactivated_elements.push_back(e + 1);
   }
   int g = 0;
@@ -112,6 +114,7 @@ std::unique_ptr<Model> build_model(const orc_model& m) {
     Material mat;
     mat.name = name;
     mat.elastic = ElasticIso{std::get<0>(key), std::get<1>(key)};
+    if (std::get<3>(key) != 0.0) mat.density = std::get<3>(key);
     if (std::get<2>(key) >= 0) mat.plastic = to_plastic(m.j2[std::get<2>(key)]);
     M->materials[name] = mat;
     M->mesh.add_elset(name, ids);
@@ -327,6 +330,38 @@ int ref_recover_fields(const ref_model* h, const double* u, const double* state,
         off += ng;
       }
     }
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// assemble_mass (kind 0 consistent, 1 lumped) or assemble_geometric_stiffness (kind 2,
+// gp_stress flattened per point in element order; removed elements' points skipped).
+int ref_assemble_scalar(const ref_model* h, int kind, const double* gp_stress, ref_system** out,
+                        char* err, int errlen) {
+  try {
+    const Model& M = *h->model;
+    auto* s = new ref_system();
+    if (kind < 2) {
+      s->sys = impl::assemble_mass(M, kind == 1);
+    } else {
+      std::vector<std::vector<Voigt6>> gp(M.mesh.num_elements());
+      const std::vector<bool> active = M.element_active_mask();
+      size_t off = 0;
+      for (size_t e = 0; e < gp.size(); ++e) {
+        const size_t ng = fem::gauss_rule(M.mesh.elements()[e].type).size();
+        if (active[e] && gp_stress) {
+          gp[e].resize(ng);
+          for (size_t q = 0; q < ng; ++q)
+            for (size_t c = 0; c < 6; ++c) gp[e][q][c] = gp_stress[6 * (off + q) + c];
+        }
+        off += ng;
+      }
+      s->sys = impl::assemble_geometric_stiffness(M, gp);
+    }
+    *out = s;
     return 0;
   } catch (const std::exception& e) {
     set_err(err, errlen, e.what());

@@ -7,7 +7,7 @@ oracle/ref_shim.cpp; inputs are seeded. The fixtures are data (inputs + expected
 outputs), committed so that tests pin the oracle without the reference present.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py recover    # one fixture group
+    python tests/golden/make_golden.py recover    # one fixture group (or mass_geometric)
 """
 import os
 import sys
@@ -78,10 +78,30 @@ def recover(R):
     save("recover", **rec)
 
 
+def mass_geometric(R):
+    """7. assemble_mass (consistent, lumped) and assemble_geometric_stiffness on the
+    2x2x2 blocks with per-element densities and seeded Gauss-point stresses."""
+    out = {}
+    rng = np.random.default_rng(45)
+    for name, t in FAMILIES.items():
+        b = meshgen.block(2, t, seed_geo=21, seed_E=22)
+        rho = rng.uniform(1e-9, 8e-9, b.n_elem)
+        gs = rng.normal(scale=100.0, size=(b.n_points, 6))
+        m = Model(b.coords, b.blocks, b.E, b.nu, rho=rho)
+        out[name + "_rho"], out[name + "_gp_stress"] = rho, gs
+        for key, s in (("mass", R.assemble_mass(m)), ("lumped", R.assemble_mass(m, True)),
+                       ("geo", R.assemble_geometric_stiffness(m, gs))):
+            out.update(system_arrays(s, f"{name}_{key}_"))
+    save("mass_geometric", **out)
+
+
 def main():
     R = Oracle("reference")
     if sys.argv[1:] == ["recover"]:
         recover(R)
+        return
+    if sys.argv[1:] == ["mass_geometric"]:
+        mass_geometric(R)
         return
     # 1. config 1 cantilever: K_ff, rhs; the reference key set equals the structural set
     m = meshgen.cantilever()
@@ -146,6 +166,7 @@ def main():
     save("j2_mesh_c3d8", u=u, history=hist, f_int=fi, state_out=st, **arr)
 
     recover(R)
+    mass_geometric(R)
 
 
 if __name__ == "__main__":

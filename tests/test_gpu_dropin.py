@@ -171,3 +171,22 @@ def test_recover_fields(pair, fam_i):
     a, b = gpu.recover_fields(m, u, h), ref.recover_fields(m, u, h)
     for k in ("strain", "stress", "reaction"):
         assert rel(a[k], b[k]) <= REL, k
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_mass_and_geometric(pair, fam_i):
+    """assemble_mass (consistent, lumped) and assemble_geometric_stiffness with
+    *DENSITY materials and prescribed DOFs."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    ref, gpu = pair
+    fam = [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6][fam_i]
+    b = meshgen.block(3, fam, seed_geo=181, seed_E=182, ny=2, nz=2)
+    x0 = np.nonzero(b.coords[:, 0] < 1e-12)[0]
+    rng = np.random.default_rng(183)
+    m = Model(b.coords, b.blocks, b.E, b.nu, spc_node=x0, spc_comp=np.full(len(x0), 1),
+              spc_value=np.full(len(x0), 1e-3), rho=rng.uniform(1e-9, 8e-9, b.n_elem))
+    gs = rng.normal(scale=80.0, size=(m.n_points, 6))
+    same_system(gpu.assemble_mass(m), ref.assemble_mass(m))
+    same_system(gpu.assemble_mass(m, True), ref.assemble_mass(m, True))
+    same_system(gpu.assemble_geometric_stiffness(m, gs), ref.assemble_geometric_stiffness(m, gs))

@@ -1,0 +1,95 @@
+"""Mass and geometric stiffness on the device (cxg_assemble_mass, cxg_assemble_geometric
+via fem.assemble_mass / fem.assemble_geometric_stiffness) against the oracle, which is
+pinned bit for bit to the reference core (tests/test_oracle.py):
+  CSR row_ptr / col_idx   bit-exact (the reference's key set: only same-direction
+                          couplings, entries with every contribution 0.0 left out)
+  values                  rel-L2 <= 1e-12
+  rhs                     zero (the eliminated-column side effect is discarded)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def rel_l2(a, b):
+    a, b = np.ravel(a), np.ravel(b)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from calculixpp_amd import fem
+    return fem
+
+
+def _families():
+    from calculixpp_amd import _lib
+    return [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6]
+
+
+def same(sys_g, sys_e):
+    rp, ci, v = sys_e.csr()
+    assert sys_g.n_free == sys_e.n_free
+    assert np.array_equal(sys_g.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(sys_g.col_idx.cpu().numpy(), ci)
+    assert rel_l2(sys_g.vals.cpu().numpy(), v) <= REL, rel_l2(sys_g.vals.cpu().numpy(), v)
+    assert not sys_g.rhs.cpu().numpy().any()
+
+
+def _model(fam, seed, spc=False):
+    from calculixpp_amd import meshgen
+    from calculixpp_amd.model import Model
+    b = meshgen.block(4, fam, seed_geo=seed, seed_E=seed + 1, ny=3, nz=2)
+    kw = {}
+    if spc:
+        x0 = np.nonzero(b.coords[:, 0] < 1e-12)[0]
+        kw = dict(spc_node=np.repeat(x0, 2), spc_comp=np.tile([1, 3], len(x0)),
+                  spc_value=np.tile([0.0, 1e-3], len(x0)))
+    rng = np.random.default_rng(seed + 2)
+    return Model(b.coords, b.blocks, b.E, b.nu, rho=rng.uniform(1e-9, 8e-9, b.n_elem), **kw)
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+@pytest.mark.parametrize("fam_i", range(4))
+def test_mass(gpu, port, fam_i, lumped):
+    m = _model(_families()[fam_i], 141, spc=True)
+    same(gpu.assemble_mass(m, lumped), port.assemble_mass(m, lumped))
+
+
+@pytest.mark.parametrize("fam_i", range(4))
+def test_geometric_stiffness(gpu, port, fam_i):
+    m = _model(_families()[fam_i], 151, spc=True)
+    gs = np.random.default_rng(152).normal(scale=80.0, size=(m.n_points, 6))
+    same(gpu.assemble_geometric_stiffness(m, gs), port.assemble_geometric_stiffness(m, gs))
+
+
+def test_geometric_from_recovered_stress_and_removed_elements(gpu, port):
+    """The buckling prestress chain: recover_gauss_stress of a displacement feeds
+    assemble_geometric_stiffness, on a model with removed elements (their points are
+    empty and they add no mass or K_geo)."""
+    from calculixpp_amd import meshgen, recover
+    m = meshgen.cantilever()
+    m.rho = np.full(m.n_elem, 7.85e-9)
+    m.active = np.ones(m.n_elem, bool)
+    m.active[1::4] = False
+    u = meshgen.smooth_field(m.coords, 161)
+    gs = recover.recover_gauss_stress(m, u)
+    exp_gs = port.recover_fields(m, u, gauss_stress=True)["gauss_stress"]
+    same(gpu.assemble_geometric_stiffness(m, gs), port.assemble_geometric_stiffness(m, exp_gs))
+    same(gpu.assemble_mass(m), port.assemble_mass(m))
+
+
+def test_zero_density_and_zero_stress_give_empty_systems(gpu):
+    from calculixpp_amd import _lib, meshgen
+    m = meshgen.block(2, _lib.C3D20, seed_geo=171)
+    s = gpu.assemble_mass(m)  # no *DENSITY
+    assert s.vals.numel() == 0 and s.row_ptr.cpu().numpy().max() == 0
+    s = gpu.assemble_geometric_stiffness(m, np.zeros((m.n_points, 6)))
+    assert s.vals.numel() == 0

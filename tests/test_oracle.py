@@ -455,3 +455,92 @@ def test_recover_node_average_weights(port):
     assert np.allclose(o["stress"][shared], 0.5 * (s_e[0] + s_e[1]), rtol=1e-14, atol=0)
     assert np.allclose(o["stress"][only0], s_e[0], rtol=1e-14, atol=0)
     assert np.all(o["stress"][shared, 0] > s_e[0][0]) and np.all(o["stress"][shared, 0] < s_e[1][0])
+
+
+# ---- mass and geometric stiffness (assemble_mass, assemble_geometric_stiffness) -------
+
+def _same_system(a, exp_prefix, g):
+    rp, ci, v = a.csr()
+    assert np.array_equal(rp, g[exp_prefix + "row_ptr"])
+    assert np.array_equal(ci, g[exp_prefix + "col_idx"])
+    assert np.array_equal(v, g[exp_prefix + "vals"])
+    assert np.array_equal(a.rhs, g[exp_prefix + "rhs"])
+
+
+def test_golden_mass_geometric(port):
+    g = gold("mass_geometric")
+    for name, t in FAMS.items():
+        b = meshgen.block(2, t, seed_geo=21, seed_E=22)
+        m = Model(b.coords, b.blocks, b.E, b.nu, rho=g[name + "_rho"])
+        _same_system(port.assemble_mass(m), name + "_mass_", g)
+        _same_system(port.assemble_mass(m, True), name + "_lumped_", g)
+        _same_system(port.assemble_geometric_stiffness(m, g[name + "_gp_stress"]), name + "_geo_", g)
+
+
+@pytest.mark.parametrize("name", list(FAMS))
+def test_reference_bitwise_mass_geometric(port, ref, name):
+    b = meshgen.cantilever(4, 2, 2) if name == "C3D8" else meshgen.block(3, FAMS[name], seed_geo=131)
+    rng = np.random.default_rng(132)
+    m = Model(b.coords, b.blocks, b.E, b.nu, b.spc_node, b.spc_comp, b.spc_value,
+              rho=rng.uniform(1e-9, 8e-9, b.n_elem))
+    gs = rng.normal(scale=50.0, size=(m.n_points, 6))
+    for a, r in ((port.assemble_mass(m), ref.assemble_mass(m)),
+                 (port.assemble_mass(m, True), ref.assemble_mass(m, True)),
+                 (port.assemble_geometric_stiffness(m, gs), ref.assemble_geometric_stiffness(m, gs))):
+        for x, y in zip(a.csr(), r.csr()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(a.rhs, r.rhs) and not a.rhs.any()
+
+
+@pytest.mark.parametrize("name", list(FAMS))
+def test_mass_closed_forms(port, name):
+    """Total mass rho V on each direction (partition of unity), only same-direction
+    couplings, and the lumped diagonal equal to the consistent row sums. (The reduced
+    C3D20R rule integrates det J exactly only on undistorted elements.)"""
+    b = meshgen.block(3, FAMS[name], seed_geo=14, ny=2, nz=2,
+                      jitter_frac=0.0 if name == "C3D20R" else 0.1)  # boundary stays put
+    L = b.coords.max(axis=0) - b.coords.min(axis=0)
+    rho = 7.85e-9
+    m = Model(b.coords, b.blocks, b.E, b.nu, rho=rho)
+    M = port.assemble_mass(m).dense()
+    V = float(np.prod(L))
+    for d in range(3):
+        assert abs(M[d::3, d::3].sum() - rho * V) <= 1e-13 * rho * V
+        for d2 in range(3):
+            if d2 != d:
+                assert not M[d::3, d2::3].any()
+    Ml = port.assemble_mass(m, True).dense()
+    assert np.array_equal(Ml, np.diag(np.diag(Ml)))
+    assert np.allclose(np.diag(Ml), M.sum(axis=1), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("name", list(FAMS))
+def test_geometric_closed_forms(port, name):
+    """K_geo annihilates rigid translations, is symmetric and linear in the stress,
+    and couples only equal directions."""
+    b = meshgen.block(3, FAMS[name], seed_geo=15, ny=2, nz=2)
+    m = Model(b.coords, b.blocks, b.E, b.nu)
+    rng = np.random.default_rng(16)
+    s1, s2 = rng.normal(size=(m.n_points, 6)), rng.normal(size=(m.n_points, 6))
+    K1 = port.assemble_geometric_stiffness(m, s1).dense()
+    K2 = port.assemble_geometric_stiffness(m, s2).dense()
+    K12 = port.assemble_geometric_stiffness(m, s1 + 2.0 * s2).dense()
+    scale = np.abs(K12).max()
+    assert np.abs(K12 - (K1 + 2.0 * K2)).max() <= 1e-12 * scale
+    assert np.array_equal(K1, K1.T)
+    for d in range(3):
+        t = np.zeros(K1.shape[0])
+        t[d::3] = 1.0
+        assert np.abs(K1 @ t).max() <= 1e-12 * np.abs(K1).max()
+    assert not K1[0::3, 1::3].any()
+
+
+def test_geometric_unit_cube_entry(port):
+    """One undistorted unit C3D8 cube under s_xx: K_geo[0, 0] = s_xx * integral of
+    (dN_1/dx)^2 = s_xx (1/3)(1/3)."""
+    c, conn = meshgen.hex_grid(1, 1, 1, (1.0, 1.0, 1.0), _lib.C3D8)
+    m = Model(c, [(_lib.C3D8, conn)], 2.0e5, 0.3)
+    s = np.zeros((8, 6))
+    s[:, 0] = 90.0
+    K = port.assemble_geometric_stiffness(m, s).dense()
+    assert abs(K[0, 0] - 10.0) <= 1e-12
