@@ -316,6 +316,68 @@ def recover_leg(model, u, family: int, reps: int = 5, cpu_n: int = 16, cpu: bool
     return out
 
 
+def mass_geometric_leg(asm, family: int, reps: int = 3, cpu_n: int = 16, cpu: bool = True) -> dict:
+    """Consistent mass, lumped mass and geometric stiffness into the CSR values of the
+    assembly workload's pattern (density 7.85e-9, seeded Gauss-point stress), from HIP
+    events on the library's stream. Both matrices are k_ab I3 blocks written through
+    the same element arena and row gather as K; algorithmic bytes are the CSR values
+    (8 nnz) plus coordinates, connectivity and one value per element or point."""
+    import statistics
+    import numpy as np
+    import torch
+    from calculixpp_amd.model import GAUSS, NODES
+    dev = asm.ctx.device
+    model = asm.model
+    rho = torch.full((model.n_elem,), 7.85e-9, dtype=torch.float64, device=dev)
+    gs = torch.from_numpy(np.random.default_rng(45).normal(scale=100.0, size=(model.n_points, 6))).to(dev)
+    vals = asm.new_values()
+    ctx = asm.ctx
+    runs = {"mass": lambda: asm.assemble_mass(vals, False, rho),
+            "mass_lumped": lambda: asm.assemble_mass(vals, True, rho),
+            "geometric": lambda: asm.assemble_geometric(vals, gs)}
+    nn, ngp = NODES[family], GAUSS[family]
+    nnz = asm.pattern.nnz
+    out = {"workload": "mass (consistent, lumped) and geometric stiffness on the assembly "
+                       "block's CSR pattern", "elements": model.n_elem, "nnz": nnz}
+    for key, fn in runs.items():
+        fn()
+        times = []
+        for _ in range(reps):
+            ctx.profile_reset()
+            ctx.profile(True)
+            fn()
+            ctx.profile(False)
+            times.append(sum(v[1] for v in ctx.profile_stats().values()))
+        ms = statistics.median(times)
+        extra = 48 * ngp * model.n_elem if key == "geometric" else 8 * model.n_elem
+        byt = 8 * nnz + 24 * model.n_nodes + 4 * nn * model.n_elem + extra
+        t_att = byt / (PEAK_HBM * 1e9)
+        out[key] = {"ms": ms, "elements_per_s": model.n_elem / (ms * 1e-3),
+                    "kernels_ms": {k: round(v[1], 4) for k, v in ctx.profile_stats().items()},
+                    "roofline": {"bound": "hbm", "algorithmic_bytes": byt,
+                                 "t_attainable_ms": round(t_att * 1e3, 4),
+                                 "frac": round(t_att / (ms * 1e-3), 4)}}
+    del vals
+    torch.cuda.synchronize()
+    if cpu:
+        from calculixpp_amd import meshgen
+        from calculixpp_amd.model import Model
+        from oracle.oracle import Oracle, ref_available
+        kind = "reference" if ref_available() else "port"
+        b = meshgen.block(cpu_n, family)
+        cm = Model(b.coords, b.blocks, b.E, b.nu, rho=7.85e-9)
+        o = Oracle(kind)
+        t0 = time.perf_counter()
+        o.assemble_mass(cm)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cm.n_elem / sec, "unit": "elements/s", "cores": 1,
+                               "kind": kind, "sample": f"{cpu_n}^3 block of the same family "
+                               f"({cm.n_elem} elements, assemble_mass, {sec:.1f} s on one "
+                               "host thread)"}
+        out["gpu_over_cpu"] = out["mass"]["elements_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
     """Reference CPU assembly (+ internal force) on an n^3 sample of the same
     workload, one thread (the reference is single-threaded)."""
@@ -346,6 +408,8 @@ def main() -> int:
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
     ap.add_argument("--no-recover", action="store_true", help="skip the field-recovery leg")
+    ap.add_argument("--no-mass", action="store_true",
+                    help="skip the mass / geometric-stiffness leg")
     ap.add_argument("--batch-meshes", type=int, default=512)
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="extra cxg_set_option for the library context (repeatable)")
@@ -484,6 +548,12 @@ def main() -> int:
         print(f"[bench] recover_fields {rec['ms']:.3f} ms ({rec['elements_per_s']:.4g} "
               f"elements/s)", file=sys.stderr, flush=True)
 
+    mass = None if args.no_mass else mass_geometric_leg(asm, family, cpu_n=args.cpu_n,
+                                                        cpu=not args.no_cpu)
+    if mass:
+        print(f"[bench] mass {mass['mass']['ms']:.3f} ms, lumped {mass['mass_lumped']['ms']:.3f} "
+              f"ms, geometric {mass['geometric']['ms']:.3f} ms", file=sys.stderr, flush=True)
+
     batch = None if args.no_batch else batch_step(
         args.batch_meshes, cpu=not args.no_cpu, log=lambda m: print(m, file=sys.stderr, flush=True))
     if batch:
@@ -515,6 +585,7 @@ def main() -> int:
         "adjoint": adjoint,
         "j2_newton": j2,
         "recover": rec,
+        "mass_geometric": mass,
         "batch": batch,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
