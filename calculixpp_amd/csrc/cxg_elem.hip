@@ -583,6 +583,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_assemble_iso2(
 // one wave within a short time, which the L2 merges into full lines. No colouring,
 // no read-modify-write.
 constexpr int kStageNodes = 4;  // element-matrix node rows staged in LDS per store turn
+typedef double f64x2 __attribute__((ext_vector_type(2)));  // one 16-byte store
 
 // J^-1 (row-major, cofactors) and sqrt(det J * w) of one Gauss point from the node
 // coordinates x[3n] and the natural derivatives d[n][3]; raises kFlagJacobian when
@@ -623,7 +624,7 @@ struct ElemLds {
   double Ai[NGP * 9];
   double rs[NGP];
   double h[QC * NN * 3];
-  double stage[kStageNodes * 3 * 3 * NN];
+  alignas(16) double stage[kStageNodes * 3 * 3 * NN];
 };
 
 // One element: ce its connectivity row, kel its 3n x 3n block of the arena. Every
@@ -738,8 +739,12 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
     }
     __syncthreads();
     const int nrow = 3 * (NN - g0 < kStageNodes ? NN - g0 : kStageNodes);
-    double* dst = kel + int64_t(3 * g0) * ND;
-    for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = L.stage[idx];
+    // 16-byte streaming stores: the arena is read once, by the gather, much later
+    // (nrow * ND is even and every staged run starts 16-byte aligned)
+    f64x2* dst = reinterpret_cast<f64x2*>(kel + int64_t(3 * g0) * ND);
+    const f64x2* src = reinterpret_cast<const f64x2*>(L.stage);
+    for (int idx = lane; idx < nrow * ND / 2; idx += kWave)
+      __builtin_nontemporal_store(src[idx], dst + idx);
     __syncthreads();
   }
 }
@@ -968,9 +973,9 @@ __device__ __forceinline__ void row_gather_node(
           fm[k] = sl >> kSlotFreeShift;
           const double* ke = kp + r.rows + 3 * bl + jl;
           const int ndof = 3 * r.n;
-          v[k][0] = ke[0];
-          v[k][1] = ke[ndof];
-          v[k][2] = ke[2 * ndof];
+          v[k][0] = __builtin_nontemporal_load(ke);
+          v[k][1] = __builtin_nontemporal_load(ke + ndof);
+          v[k][2] = __builtin_nontemporal_load(ke + 2 * ndof);
           if constexpr (NC > 0)
             if (u) {
               const int32_t nd = conn[r.conn + bl];
@@ -1024,7 +1029,8 @@ __device__ __forceinline__ void row_gather_node(
   if (in_lds)
     for (int i = 0; i < 3; ++i) {
       if (rs[i] < 0) continue;
-      for (int64_t c = lane; c < Lrow; c += kWave) vals[rs[i] + c] = buf[i * cap + c];
+      for (int64_t c = lane; c < Lrow; c += kWave)
+        __builtin_nontemporal_store(buf[i * cap + c], vals + rs[i] + c);
       if constexpr (LIVE)
         for (int64_t c = lane; c < Lrow; c += kWave) live[rs[i] + c] = lbuf[i * cap + c];
     }
@@ -1381,7 +1387,7 @@ struct ElemJ2Lds {
       double h[QC * NN * 3];
       double t[QC * NN * 3];
     } c;
-    double stage[kStageNodes * 3 * 3 * NN];
+    alignas(16) double stage[kStageNodes * 3 * 3 * NN];
   } w;
 };
 
@@ -1574,8 +1580,10 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
     }
     __syncthreads();
     const int nrow = 3 * (NN - g0 < kStageNodes ? NN - g0 : kStageNodes);
-    double* dst = kel + int64_t(3 * g0) * ND;
-    for (int idx = lane; idx < nrow * ND; idx += kWave) dst[idx] = w.stage[idx];
+    f64x2* dst = reinterpret_cast<f64x2*>(kel + int64_t(3 * g0) * ND);
+    const f64x2* src = reinterpret_cast<const f64x2*>(w.stage);
+    for (int idx = lane; idx < nrow * ND / 2; idx += kWave)
+      __builtin_nontemporal_store(src[idx], dst + idx);
     __syncthreads();
   }
 }
