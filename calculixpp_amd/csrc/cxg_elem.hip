@@ -1437,13 +1437,15 @@ __device__ __forceinline__ void j2_gauss_point(
   double e[6], sg[6];
   voigt_strain(H, e);
   J2Tangent T;
+  // the history planes stream: read once and written once per iteration (no reuse
+  // that a cache could serve)
   if (mat) {
     double hc[13], ht[13];
 #pragma unroll
-    for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
+    for (int c = 0; c < 13; ++c) hc[c] = __builtin_nontemporal_load(committed + c * n_points + pt);
     j2_return(*mat, e, hc, ht, sg, T);
 #pragma unroll
-    for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = ht[c];
+    for (int c = 0; c < 13; ++c) __builtin_nontemporal_store(ht[c], trial + c * n_points + pt);
   } else {
     lame(Ee, nue, T.lam, T.mu);
     iso_stress(T.lam, T.mu, e, sg);
@@ -1451,7 +1453,9 @@ __device__ __forceinline__ void j2_gauss_point(
 #pragma unroll
     for (int i = 0; i < 6; ++i) T.n[i] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 13; ++c) trial[c * n_points + pt] = committed[c * n_points + pt];
+    for (int c = 0; c < 13; ++c)
+      __builtin_nontemporal_store(__builtin_nontemporal_load(committed + c * n_points + pt),
+                                  trial + c * n_points + pt);
   }
   double* Tq = P.T + 9 * q;
   Tq[0] = T.lam;
@@ -2063,7 +2067,7 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
       if constexpr (KIND == kMassLumped) v = a == b ? L.row[a] : 0.0;
       else v = L.k[a * NN + b];
     }
-    ke[idx] = v;
+    __builtin_nontemporal_store(v, ke + idx);
   }
 }
 
