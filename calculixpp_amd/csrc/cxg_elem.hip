@@ -902,7 +902,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // One node's three rows. HALVES = 2: two elements per step, one per half-wave, for
 // nodes whose incident elements all have <= 10 nodes (3n <= 30 columns fit in 32 lanes).
-template <int NC, bool LIVE, int HALVES>
+// SCALAR: the arena holds k_ab per node pair (n^2 per element, at a ninth of the full
+// matrix's offsets) for a k_ab I3 block matrix, expanded here.
+template <int NC, bool LIVE, int HALVES, bool SCALAR>
 __device__ __forceinline__ void row_gather_node(
     int32_t a, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
     const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
@@ -971,11 +973,18 @@ __device__ __forceinline__ void row_gather_node(
           const int32_t sl = slot_all[r.slot + bl];
           colp[k] = sl & kSlotMask;
           fm[k] = sl >> kSlotFreeShift;
-          const double* ke = kp + r.rows + 3 * bl + jl;
-          const int ndof = 3 * r.n;
-          v[k][0] = __builtin_nontemporal_load(ke);
-          v[k][1] = __builtin_nontemporal_load(ke + ndof);
-          v[k][2] = __builtin_nontemporal_load(ke + 2 * ndof);
+          if constexpr (SCALAR) {
+            const double s = __builtin_nontemporal_load(kp + r.rows / 9 + bl);
+            v[k][0] = jl == 0 ? s : 0.0;
+            v[k][1] = jl == 1 ? s : 0.0;
+            v[k][2] = jl == 2 ? s : 0.0;
+          } else {
+            const double* ke = kp + r.rows + 3 * bl + jl;
+            const int ndof = 3 * r.n;
+            v[k][0] = __builtin_nontemporal_load(ke);
+            v[k][1] = __builtin_nontemporal_load(ke + ndof);
+            v[k][2] = __builtin_nontemporal_load(ke + 2 * ndof);
+          }
           if constexpr (NC > 0)
             if (u) {
               const int32_t nd = conn[r.conn + bl];
@@ -1056,8 +1065,9 @@ __device__ __forceinline__ void row_gather_node(
 // NC displacement fields (load cases) at u + c*n_dof give NC internal forces at
 // f_int + c*n_dof from the same pass over the element rows (u may be null: K only).
 // NC = 0: f_int is the sum of the element force vectors fe instead. LIVE: also flag the
-// entries that received a contribution other than 0.0. One wave per node.
-template <int NC, bool LIVE>
+// entries that received a contribution other than 0.0. SCALAR: the k_ab I3 arena of
+// the mass and geometric stiffness. One wave per node.
+template <int NC, bool LIVE, bool SCALAR = false>
 __global__ __launch_bounds__(kWave) void k_row_gather(
     const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
@@ -1078,10 +1088,10 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   const int32_t a = int32_t(ai);
   // nfm bit 3: every incident element has at most 10 nodes (wave-uniform)
   if (__builtin_amdgcn_readfirstlane(int(nfm[a])) & 8)
-    row_gather_node<NC, LIVE, 2>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
+    row_gather_node<NC, LIVE, 2, SCALAR>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
                                  row_ptr, kp, u, n_dof, fe, vals, f_int, live);
   else
-    row_gather_node<NC, LIVE, 1>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
+    row_gather_node<NC, LIVE, 1, SCALAR>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
                                  row_ptr, kp, u, n_dof, fe, vals, f_int, live);
 }
 
@@ -1996,8 +2006,9 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
 // (element_mass), the lumped mass its row sums on the diagonal (element_mass_lumped),
 // the geometric stiffness k_ab = sum_q detJ w g_a^T sigma_q g_b (element_geometric_
 // stiffness with accumulate_kgeo). One element per wave, lanes over the upper node
-// pairs. The full (3n)^2 matrix goes to the arena with exact zeros off the block
-// diagonals, so the row gather's liveness flags give the reference's key set.
+// pairs. The arena holds only k_ab (n^2 per element); the row gather expands the I3
+// blocks, so the entries off the block diagonals never receive a contribution and the
+// liveness flags give the reference's key set.
 enum : int { kMassConsistent = 0, kMassLumped = 1, kGeometric = 2 };
 
 template <int NN, int NGP>
@@ -2057,17 +2068,17 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
     }
     __syncthreads();
   }
-  constexpr int ND = 3 * NN;
-  double* ke = kp + le * ND * ND;
-  for (int idx = lane; idx < ND * ND; idx += kWave) {
-    const int i = idx / ND, j = idx - ND * (idx / ND);
-    const int a = i / 3, b = j / 3;
-    double v = 0.0;
-    if (i - 3 * a == j - 3 * b) {
-      if constexpr (KIND == kMassLumped) v = a == b ? L.row[a] : 0.0;
-      else v = L.k[a * NN + b];
+  // the element's n x n scalars, row-major, at a ninth of its full-matrix arena offset
+  double* ks = kp + le * NN * NN;
+  for (int idx = lane; idx < NN * NN; idx += kWave) {
+    double v;
+    if constexpr (KIND == kMassLumped) {
+      const int a = idx / NN;
+      v = idx == a * NN + a ? L.row[a] : 0.0;
+    } else {
+      v = L.k[idx];
     }
-    __builtin_nontemporal_store(v, ke + idx);
+    __builtin_nontemporal_store(v, ks + idx);
   }
 }
 
@@ -2081,7 +2092,7 @@ unsigned xcd_grid(const cxg_context* ctx, int64_t n) {
 
 // Phase B over every node of the pattern's mesh (see k_row_gather).
 void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
-                   double* vals, double* f_int, uint8_t* live, int n_cases) {
+                   double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false) {
   const cxg_mesh* m = p->mesh;
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
   const int32_t slab = int32_t(xcd_slab(ctx, m->n_nodes));
@@ -2089,12 +2100,15 @@ void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, cons
   const int32_t cap = int32_t(std::min<int64_t>((std::max<int64_t>(p->max_row, 1) + 7) / 8 * 8,
                                                 std::min(kRowCap, ctx->gather_lds_cols)));
   const size_t lds = size_t(3 * cap) * (sizeof(double) + (live ? 1 : 0));
-#define GATHER(NC, LV)                                                                          \
-  hipLaunchKernelGGL((k_row_gather<NC, LV>), dim3(grid), dim3(kWave), lds, ctx->stream,         \
+#define GATHER(NC, LV, ...)                                                                     \
+  hipLaunchKernelGGL((k_row_gather<NC, LV, ##__VA_ARGS__>), dim3(grid), dim3(kWave), lds,        \
+                     ctx->stream,                                                               \
                      m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
                      p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
                      live, m->n_nodes, slab, cap)
-  if (fe) {
+  if (scalar) {
+    if (live) GATHER(1, true, true); else GATHER(1, false, true);
+  } else if (fe) {
     if (live) GATHER(0, true); else GATHER(0, false);
   } else if (n_cases == 4) {
     if (live) GATHER(4, true); else GATHER(4, false);
@@ -2444,7 +2458,7 @@ void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double
     if (B.n_elem == 0) continue;
     const ElemTable& T = ctx->tables[B.type];
     const int32_t* conn = m->conn.p + B.conn_offset;
-    double* kp = pm->ke_arena.p + p->kp_base[bi];
+    double* kp = pm->ke_arena.p + p->kp_base[bi] / 9;  // scalar arena: n^2 per element
     LaunchTimer timer(ctx, std::string(names[kind]) + type_name(B.type), B.n_elem);
 #define LAUNCH_K(NN, NGP, KIND)                                                                 \
   hipLaunchKernelGGL((k_elem_scalar<NN, NGP, KIND>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,  \
@@ -2462,7 +2476,7 @@ void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double
   if (m->n_nodes > 0) {
     LaunchTimer timer(ctx, kind == kGeometric ? "row_gather_geometric" : "row_gather_mass",
                       m->n_elem);
-    launch_gather(ctx, p, nullptr, nullptr, vals, nullptr, live, 1);
+    launch_gather(ctx, p, nullptr, nullptr, vals, nullptr, live, 1, true);
     CXG_HIP(hipGetLastError());
   }
 }
