@@ -1888,15 +1888,26 @@ __global__ __launch_bounds__(kWave) void k_vjp(
 // order, f_int from the arena, RF = f_int - f_ext.
 template <int NN, int NGP>
 struct RecoverLds {
-  Geo<NN, NGP> G;
+  double x[NN * 3];
   double u[NN * 3];
+  double Ai[NGP * 9];   // J^-1 per Gauss point (row-major)
   double eps[NGP * 6];
   double sig[NGP * 6];
-  double S[NGP * 6];  // det J * w * sigma
+  double S[NGP * 6];    // det J * w * sigma
   int32_t node[NN];
 };
 
-template <int NN, int NGP>
+// Physical gradient g_k = J^-T dN_k at one Gauss point (A = J^-1 row-major).
+__device__ __forceinline__ void grad_at(const double* A, const double* d, double g[3]) {
+  g[0] = A[0] * d[0] + A[3] * d[1] + A[6] * d[2];
+  g[1] = A[1] * d[0] + A[4] * d[1] + A[7] * d[2];
+  g[2] = A[2] * d[0] + A[5] * d[1] + A[8] * d[2];
+}
+
+// J2: the radial return for elements with a J2 material (else every element elastic).
+// Gradients are formed on the fly from J^-1 and the natural derivatives, which keeps the
+// LDS at the element's coordinates, displacements, J^-1 and the point fields.
+template <int NN, int NGP, bool J2>
 __global__ __launch_bounds__(kWave) void k_gauss_fields(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
     int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
@@ -1910,22 +1921,47 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
   const int64_t le = blockIdx.x;
   const int64_t eg = elem_offset + le;
   const int32_t* ce = conn + le * NN;
-  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
-  stage_vec<NN>(L.u, ce, u, lane);
-  __syncthreads();
-  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
-  __syncthreads();
-  gradients<NN, NGP>(L.G, dN, lane);
+  if (lane < NN) {
+    const int32_t nd = ce[lane];
+    L.node[lane] = nd;
+    for (int c = 0; c < 3; ++c) {
+      L.x[3 * lane + c] = coords[3 * int64_t(nd) + c];
+      L.u[3 * lane + c] = u[3 * int64_t(nd) + c];
+    }
+  }
   __syncthreads();
   double lam, mu;
   lame(E[eg], nu[eg], lam, mu);
-  const int32_t mid = (mats && mat_id) ? mat_id[eg] : -1;
+  const int32_t mid = J2 ? mat_id[eg] : -1;
   for (int q = lane; q < NGP; q += kWave) {
-    double H[3][3], e[6], sg[6];
-    grad_field<NN, NGP>(L.G, L.u, q, H);
+    const double* d = dN + q * NN * 3;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < NN; ++k)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) J[i][j] += L.x[3 * k + i] * d[3 * k + j];
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
+    const double r = 1.0 / det;
+    double A[9] = {c00 * r, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r,
+                   (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r, c01 * r,
+                   (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r,
+                   (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r, c02 * r,
+                   (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r,
+                   (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r};
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < NN; ++k) {
+      double g[3];
+      grad_at(A, d + 3 * k, g);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) H[i][j] += L.u[3 * k + i] * g[j];
+    }
+    double e[6], sg[6];
     voigt_strain(H, e);
     const int64_t pt = gp_offset + le * NGP + q;
-    if (mid >= 0) {
+    if (J2 && mid >= 0) {
       // the committed history is read, never advanced (the reference evaluates a copy)
       double hc[13], ht[13];
       for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
@@ -1934,10 +1970,12 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
     } else {
       iso_stress(lam, mu, e, sg);
     }
+    const double sc = det * wq[q];
+    for (int i = 0; i < 9; ++i) L.Ai[9 * q + i] = A[i];
     for (int i = 0; i < 6; ++i) {
       L.eps[6 * q + i] = e[i];
       L.sig[6 * q + i] = sg[i];
-      L.S[6 * q + i] = L.G.sc[q] * sg[i];
+      L.S[6 * q + i] = sc * sg[i];
     }
     if (gauss_stress)
       for (int i = 0; i < 6; ++i) gauss_stress[6 * pt + i] = sg[i];
@@ -1952,7 +1990,8 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
   for (int k = lane; k < NN; k += kWave) {
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = 0; q < NGP; ++q) {
-      const double* g = L.G.g + (q * NN + k) * 3;
+      double g[3];
+      grad_at(L.Ai + 9 * q, dN + (q * NN + k) * 3, g);
       const double* sq = L.S + 6 * q;
       f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
       f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
@@ -2495,13 +2534,17 @@ void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, c
     const int32_t* conn = m->conn.p + B.conn_offset;
     double* feb = m->rec_fe.p + 3 * B.conn_offset;
     LaunchTimer timer(ctx, "gauss_fields", B.n_elem);
-#define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_gauss_fields<NN, NGP>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,       \
+#define LAUNCH_R(NN, NGP, J2)                                                                   \
+  hipLaunchKernelGGL((k_gauss_fields<NN, NGP, J2>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,   \
                      ctx->stream, conn, B.elem_offset, B.gp_offset, m->n_points, T.dN, T.w,     \
                      coords, E, nu, u, mat_id, dmats, committed, m->rec_mean.p, feb,            \
                      gauss_stress, ctx->flag_dev)
+#define LAUNCH(NN, NGP, UP, QC)                                  \
+  if (dmats && mat_id) LAUNCH_R(NN, NGP, true);                  \
+  else LAUNCH_R(NN, NGP, false);
     CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
+#undef LAUNCH_R
     CXG_HIP(hipGetLastError());
   }
   if (m->n_nodes > 0 && (strain || stress || reaction || f_int)) {
