@@ -1366,11 +1366,11 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
 // ---- K2, two-phase variant: return map, K_t and f_e to the element arenas ------------
 // Phase 1, one lane per Gauss point: Jacobian, strain (gradients formed on the fly),
 // radial return (j2_return), trial history to the SoA state, and the compact tangent
-// (lam, mu, c, n) and rs*sigma to LDS. The per-family kernel runs EPW elements per wave
-// here, so that up to all 64 lanes carry a Gauss point (8 C3D8 elements of 8 points,
-// 2 C3D20 of 27, 32 C3D6 of 2: measured faster than 4 C3D8 elements per wave, whose
-// smaller LDS record doubles the occupancy); the persistent kernel runs one element
-// per wave.
+// (lam, mu, c, n) and rs*sigma to LDS. Both the per-family and the persistent kernel run
+// EPW elements per wave here, so that most lanes carry a Gauss point (8 C3D8 elements of
+// 8 points, 2 C3D20 of 27, 4 C3D20R of 8, 24 C3D6 of 2; 8 C3D8 measured faster than 4,
+// whose smaller LDS record raises the occupancy; C3D20R and C3D6 stay below the C3D20
+// record, which sets the LDS of the persistent kernel's union).
 // Phase 2, one element at a time on the whole wave: QC Gauss points at a time,
 // h = rs g and t = B(h)^T n per node go to LDS, and each lane accumulates its upper
 // node-pair blocks
@@ -1401,8 +1401,10 @@ struct ElemJ2Lds {
   } w;
 };
 
-// Elements per wave of the per-family J2 kernel.
-constexpr int j2_epw(int ngp) { return ngp >= 27 ? 2 : (64 / ngp > 32 ? 32 : 64 / ngp); }
+// Elements per wave of the J2 kernels (see above).
+constexpr int j2_epw(int nn, int ngp) {
+  return ngp >= 27 ? 2 : (nn == 20 ? 4 : (64 / ngp > 24 ? 24 : 64 / ngp));
+}
 
 // Node k of an element into its LDS record.
 template <int NN, int NGP>
@@ -1644,7 +1646,7 @@ __global__ __launch_bounds__(kWave) void k_elem_j2(
     const J2Dev* __restrict__ mats, const double* __restrict__ u,
     const double* __restrict__ committed, double* __restrict__ trial, double* __restrict__ kp,
     double* __restrict__ fe, int* __restrict__ flag, int64_t slab) {
-  constexpr int EPW = j2_epw(NGP);
+  constexpr int EPW = j2_epw(NN, NGP);
   __shared__ ElemJ2Lds<NN, NGP, QC, EPW> L;
   const int lane = threadIdx.x;
   const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
@@ -1670,10 +1672,10 @@ union IsoLdsAll {
   ElemLds<6, 2, 2> c3d6;
 };
 union J2LdsAll {
-  ElemJ2Lds<8, 8, 8, j2_epw(8)> c3d8;
-  ElemJ2Lds<20, 27, 3, j2_epw(27)> c3d20;
-  ElemJ2Lds<20, 8, 3, j2_epw(8)> c3d20r;
-  ElemJ2Lds<6, 2, 2, j2_epw(2)> c3d6;
+  ElemJ2Lds<8, 8, 8, j2_epw(8, 8)> c3d8;
+  ElemJ2Lds<20, 27, 3, j2_epw(20, 27)> c3d20;
+  ElemJ2Lds<20, 8, 3, j2_epw(20, 8)> c3d20r;
+  ElemJ2Lds<6, 2, 2, j2_epw(6, 2)> c3d6;
 };
 
 // Next work item for the whole wave (lane 0 takes it from the counter).
@@ -1743,11 +1745,11 @@ __global__ __launch_bounds__(kWave, 2) void k_elem_j2_persistent(
     const WorkChunk c = work[w];
     const BlockDesc B = blocks[c.block];
 #define CXG_J2_RUN(NN, NGP, QC, LDS)                                                        \
-  for (int32_t i = 0; i < c.count; i += j2_epw(NGP)) {                                    \
-    const int ne = c.count - i < j2_epw(NGP) ? c.count - i : j2_epw(NGP);                 \
+  for (int32_t i = 0; i < c.count; i += j2_epw(NN, NGP)) {                                    \
+    const int ne = c.count - i < j2_epw(NN, NGP) ? c.count - i : j2_epw(NN, NGP);                 \
     int ln = lane;                                                                        \
     asm volatile("" : "+v"(ln));                                                          \
-    j2_group<NN, NGP, QC, j2_epw(NGP)>(LDS, ln, ne, int64_t(c.first) + i,                 \
+    j2_group<NN, NGP, QC, j2_epw(NN, NGP)>(LDS, ln, ne, int64_t(c.first) + i,                 \
                                        conn + B.conn_offset, B.elem_offset, B.gp_offset,  \
                                        n_points, B.dN, B.w, coords, E, nu, mat_id, mats,  \
                                        u, committed, trial, kp + B.kp_base,               \
@@ -2401,7 +2403,7 @@ void prepare_work(cxg_context* ctx, const cxg_pattern* p) {
   std::vector<WorkChunk> items;
   for (size_t bi : order) {
     const int64_t n = m->blocks[bi].n_elem;
-    const int64_t chunk = int64_t(ctx->chunk_elems) * j2_epw(m->blocks[bi].ngp);
+    const int64_t chunk = int64_t(ctx->chunk_elems) * j2_epw(m->blocks[bi].nn, m->blocks[bi].ngp);
     for (int64_t f = 0; f < n; f += chunk)
       items.push_back(WorkChunk{int32_t(bi), int32_t(f), int32_t(std::min<int64_t>(chunk, n - f)), 0});
   }
@@ -2449,10 +2451,10 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
       LaunchTimer timer(ctx, std::string("elem_j2_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
   hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC>),                                                  \
-                     dim3(xcd_grid(ctx, (B.n_elem + j2_epw(NGP) - 1) / j2_epw(NGP))), dim3(kWave), \
+                     dim3(xcd_grid(ctx, (B.n_elem + j2_epw(NN, NGP) - 1) / j2_epw(NN, NGP))), dim3(kWave), \
                      0, ctx->stream, conn, B.elem_offset, B.n_elem, B.gp_offset, m->n_points,   \
                      T.dN, T.w, coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe,      \
-                     ctx->flag_dev, xcd_slab(ctx, (B.n_elem + j2_epw(NGP) - 1) / j2_epw(NGP)))
+                     ctx->flag_dev, xcd_slab(ctx, (B.n_elem + j2_epw(NN, NGP) - 1) / j2_epw(NN, NGP)))
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
       CXG_HIP(hipGetLastError());

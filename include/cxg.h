@@ -101,7 +101,7 @@ const char* cxg_last_error(const cxg_context* ctx);
  *                 6 = two-phase with the element matrices on the FP64 matrix cores
  *                 (M = H^T H by MFMA)
  *   "chunk_elems"      work item of the persistent launch, in groups of the J2 kernel's
- *                      elements per wave (8 C3D8, 2 C3D20, 32 C3D6) (1..4096, 4)
+ *                      elements per wave (8 C3D8, 2 C3D20, 4 C3D20R, 24 C3D6) (1..4096, 4)
  *   "persistent_waves" resident waves per compute unit of that launch (1..32, 8)
  *   "xcd_remap"   1: the two-phase kernels give each of the 8 XCDs one contiguous range
  *                 of nodes / elements (its L2 then sees neighbouring data again); 0: the
