@@ -5,7 +5,10 @@ Two knobs decide how the stiffness assembly maps onto the GPU:
                    gradients in LDS), 2/3 = M-identity kernel with row-coalesced
                    scatter at 4/3 waves per SIMD, 4 = two-phase (element matrices
                    to an arena, rows gathered by their node), 6 = two-phase with
-                   M = H^T H on the FP64 matrix cores;
+                   M = H^T H on the FP64 matrix cores, 7 = two-phase with only the
+                   upper node-pair blocks in the arena (half the bytes), gathered
+                   in a locality order of the nodes so that both readers of a
+                   block find it in L2;
   patch_size   consecutive elements one workgroup walks (patch colouring keeps
                patches of one launch node-disjoint; longer patches reuse the CSR
                rows of shared nodes while they are still in cache, shorter ones
@@ -33,7 +36,7 @@ from . import fem
 from .model import Model
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "calculixpp_amd", "autotune.json")
-CANDIDATES = [(4, 8), (6, 8), (2, 8), (1, 8)]
+CANDIDATES = [(7, 8), (4, 8), (6, 8), (2, 8), (1, 8)]
 
 
 def _key(model: Model) -> str:

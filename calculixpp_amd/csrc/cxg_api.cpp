@@ -350,7 +350,7 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->patch_size = int(value);
     return CXG_OK;
   }
-  if (k == "asm_variant" && value >= 1 && value <= 6) {
+  if (k == "asm_variant" && value >= 1 && value <= 7) {
     ctx->asm_variant = int(value);
     return CXG_OK;
   }
@@ -364,6 +364,14 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
   }
   if (k == "persistent_waves" && value >= 1 && value <= 32) {
     ctx->persistent_waves = int(value);
+    return CXG_OK;
+  }
+  if (k == "elem_tiling" && (value == 0 || value == 1)) {
+    ctx->elem_tiling = int(value);
+    return CXG_OK;
+  }
+  if (k == "gather_tile" && value >= 0 && value <= 4096) {
+    ctx->gather_tile = int(value);
     return CXG_OK;
   }
   if (k == "gather_lds_cols" && value >= 8 && value <= 256) {

@@ -22,6 +22,7 @@
 //   scatter                src/fem/assembly.cpp:76-110 (SPC-only transform)
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 
 #include "cxg_internal.hpp"
@@ -623,13 +624,27 @@ struct ElemLds {
   double x[NN * 3];
   double Ai[NGP * 9];
   double rs[NGP];
-  double h[QC * NN * 3];
+  alignas(16) double h[QC * NN * 3];
   alignas(16) double stage[kStageNodes * 3 * 3 * NN];
 };
 
-// One element: ce its connectivity row, kel its 3n x 3n block of the arena. Every
-// lane of the wave calls it; it ends with a barrier, so L can be reused at once.
-template <int NN, int NGP, int QC>
+// Packed index of the upper node-pair block (a, b), b >= a, of an n-node element: row a
+// of the block triangle holds (a, a) .. (a, n-1) after the rows above it.
+__host__ __device__ __forceinline__ int up_block(int a, int b, int n) {
+  return a * n - a * (a - 1) / 2 + (b - a);
+}
+
+// One element: ce its connectivity row, kel its arena slot. UP = false: the full 3n x 3n
+// matrix, row-major. UP = true: only the upper node-pair blocks K_ab (b >= a), packed in
+// up_block order, each 3x3 row-major; the element matrix is symmetric (K_ba = K_ab^T),
+// so the row gather reads the lower blocks transposed. Every lane of the wave calls it;
+// it ends with a barrier, so L can be reused at once.
+//
+// TL = 1 (PPL = 4 only): a lane owns the 2x2 node pairs {2s, 2s+1} x {2t, 2t+1} of one
+// upper pair of node couples s <= t (of the lower pair (2s+1, 2s) of a diagonal couple
+// nothing is stored), and loads the two couples' gradients with 16-byte LDS reads: 6
+// reads of 96 bytes per Gauss point against 15 of 120 bytes for TL = 0.
+template <int NN, int NGP, int QC, bool UP = false, int TL = 0>
 __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int lane,
                                               const int32_t* __restrict__ ce, const double Ee,
                                               const double nue, const double* __restrict__ dN,
@@ -638,16 +653,33 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
                                               double* __restrict__ kel, int* __restrict__ flag) {
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
-  // A lane owns node a and up to PPL consecutive partners b (pairs (a, b0..b0+PPL-1)),
-  // so that h_a is loaded once per Gauss point for all of them.
+  static_assert(TL == 0 || (PPL == 4 && NN % 2 == 0), "couple tiling needs four pairs per lane");
+  // TL = 0: a lane owns node a and up to PPL consecutive partners b (pairs
+  // (a, b0..b0+PPL-1)), so that h_a is loaded once per Gauss point for all of them.
   int pa[PPL], pb[PPL];
   int ga, gb0;
-  decode_group<NN, PPL>(lane, ga, gb0);
+  int cs = 0, ct = 0;  // TL = 1: the lane's node couples (clamped for idle lanes)
+  if constexpr (TL == 0) {
+    decode_group<NN, PPL>(lane, ga, gb0);
 #pragma unroll
-  for (int t = 0; t < PPL; ++t) {
-    const bool ok = ga >= 0 && gb0 + t < NN;
-    pa[t] = ok ? ga : -1;
-    pb[t] = ok ? gb0 + t : -1;
+    for (int t = 0; t < PPL; ++t) {
+      const bool ok = ga >= 0 && gb0 + t < NN;
+      pa[t] = ok ? ga : -1;
+      pb[t] = ok ? gb0 + t : -1;
+    }
+  } else {
+    int s_ = -1, t_ = -1;
+    decode_group<NN / 2, 1>(lane, s_, t_);
+    ga = gb0 = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int a = 2 * s_ + t / 2, b = 2 * t_ + t % 2;
+      const bool ok = s_ >= 0 && a <= b;
+      pa[t] = ok ? a : -1;
+      pb[t] = ok ? b : -1;
+    }
+    cs = s_ >= 0 ? s_ : 0;
+    ct = s_ >= 0 ? t_ : 0;
   }
   const int ga_c = ga >= 0 ? ga : 0;  // clamped: unowned slots compute unused values
   if (lane < NN) {
@@ -682,17 +714,34 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
     }
     __syncthreads();
 #pragma unroll 1
-    for (int qq = 0; qq < QC; ++qq) {
+    for (int qq = 0; qq < QC && TL == 1; ++qq) {
+      // couples: 6 contiguous doubles, 16-byte aligned (h is, and 2 nodes are 48 bytes)
+      const f64x2* hs = reinterpret_cast<const f64x2*>(L.h + (qq * NN + 2 * cs) * 3);
+      const f64x2* ht = reinterpret_cast<const f64x2*>(L.h + (qq * NN + 2 * ct) * 3);
+      const f64x2 s0 = hs[0], s1 = hs[1], s2 = hs[2];
+      const f64x2 t0 = ht[0], t1 = ht[1], t2 = ht[2];
+      const double A[2][3] = {{s0.x, s0.y, s1.x}, {s1.y, s2.x, s2.y}};
+      const double B[2][3] = {{t0.x, t0.y, t1.x}, {t1.y, t2.x, t2.y}};
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) M[t][3 * i + j] = __builtin_fma(A[t / 2][i], B[t % 2][j], M[t][3 * i + j]);
+    }
+#pragma unroll 1
+    for (int qq = 0; qq < QC && TL == 0; ++qq) {
       const double* ha = L.h + (qq * NN + ga_c) * 3;
       const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
 #pragma unroll
       for (int t = 0; t < PPL; ++t) {
         const int b = gb0 + t < NN ? gb0 + t : NN - 1;
         const double* hb = L.h + (qq * NN + (b >= 0 ? b : 0)) * 3;
-        const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
-        M[t][0] += a0 * b0; M[t][1] += a0 * b1; M[t][2] += a0 * b2;
-        M[t][3] += a1 * b0; M[t][4] += a1 * b1; M[t][5] += a1 * b2;
-        M[t][6] += a2 * b0; M[t][7] += a2 * b1; M[t][8] += a2 * b2;
+        const double av[3] = {a0, a1, a2}, bv[3] = {hb[0], hb[1], hb[2]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) M[t][3 * i + j] = __builtin_fma(av[i], bv[j], M[t][3 * i + j]);
       }
     }
     __syncthreads();
@@ -710,9 +759,51 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        K[3 * i + j] = lam * m[3 * i + j] + mu * m[3 * j + i] + (i == j ? tr : 0.0);
+        K[3 * i + j] = __builtin_fma(lam, m[3 * i + j], __builtin_fma(mu, m[3 * j + i], i == j ? tr : 0.0));
 #pragma unroll
     for (int i = 0; i < 9; ++i) m[i] = K[i];
+  }
+  if constexpr (UP) {
+    // Staged store of the upper blocks, the block rows of kStageNodes nodes at a time:
+    // a lane's blocks all lie in the rows of its node a; the turn's blocks are one
+    // contiguous run of the packed arena (up_block(g0, g0) is even for g0 % 4 == 0, so
+    // the run starts 16-byte aligned).
+    for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
+      const int g1 = NN - g0 < kStageNodes ? NN : g0 + kStageNodes;
+#pragma unroll
+      for (int t = 0; t < PPL; ++t) {
+        if (pa[t] < g0 || pa[t] >= g1) continue;
+        const double* K = M[t];
+        double* ra = L.stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[3 * i + j];
+      }
+      __syncthreads();
+      const int k0 = up_block(g0, g0, NN);
+      const int k1 = g1 < NN ? up_block(g1, g1, NN) : NP;
+      const int cnt = 9 * (k1 - k0);
+      // packed double k0*9 + idx -> its place in the staged rows
+      auto staged = [&](int idx) {
+        const int bk = k0 + idx / 9, r = idx - 9 * (idx / 9);
+        int a = g0;
+        while (a + 1 < g1 && up_block(a + 1, a + 1, NN) <= bk) ++a;
+        const int b = a + (bk - up_block(a, a, NN));
+        const int i = r / 3, j = r - 3 * (r / 3);
+        return L.stage[(3 * (a - g0) + i) * ND + 3 * b + j];
+      };
+      f64x2* dst = reinterpret_cast<f64x2*>(kel + 9 * k0);
+      for (int idx = lane; idx < cnt / 2; idx += kWave) {
+        f64x2 v;
+        v.x = staged(2 * idx);
+        v.y = staged(2 * idx + 1);
+        __builtin_nontemporal_store(v, dst + idx);
+      }
+      if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(staged(cnt - 1), kel + 9 * k0 + cnt - 1);
+      __syncthreads();
+    }
+    return;
   }
   // Staged store: rows of kStageNodes nodes at a time. Block (a, b) goes to the rows of
   // a, its transpose to the rows of b; the staged rows are then one contiguous run of
@@ -749,7 +840,7 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
   }
 }
 
-template <int NN, int NGP, int QC, int WAVES>
+template <int NN, int NGP, int QC, int WAVES, bool UP = false, int TL = 0>
 __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
     const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ coords,
@@ -758,7 +849,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
   __shared__ ElemLds<NN, NGP, QC> L;
   const int64_t le = xcd_index(blockIdx.x, n_elem, slab);
   if (le < 0) return;
-  elem_iso_body<NN, NGP, QC>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
+  elem_iso_body<NN, NGP, QC, UP, TL>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
 }
 
@@ -904,7 +995,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // nodes whose incident elements all have <= 10 nodes (3n <= 30 columns fit in 32 lanes).
 // SCALAR: the arena holds k_ab per node pair (n^2 per element, at a ninth of the full
 // matrix's offsets) for a k_ab I3 block matrix, expanded here.
-template <int NC, bool LIVE, int HALVES, bool SCALAR>
+// UP: the arena holds the upper node-pair blocks of each element (elem_iso_body with
+// UP = true) at the element's slot; the blocks of partners b < a are read transposed.
+template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false>
 __device__ __forceinline__ void row_gather_node(
     int32_t a, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
     const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
@@ -978,6 +1071,21 @@ __device__ __forceinline__ void row_gather_node(
             v[k][0] = jl == 0 ? s : 0.0;
             v[k][1] = jl == 1 ? s : 0.0;
             v[k][2] = jl == 2 ? s : 0.0;
+          } else if constexpr (UP) {
+            // plain (cached) loads: the partner node's wave reads the same block
+            const int la = r.la, n = r.n;
+            const double* ke = kp + (r.rows - int64_t(9) * la * n);
+            if (bl >= la) {
+              ke += 9 * up_block(la, bl, n) + jl;  // K_ab[i][j] at 3i + j
+              v[k][0] = ke[0];
+              v[k][1] = ke[3];
+              v[k][2] = ke[6];
+            } else {
+              ke += 9 * up_block(bl, la, n) + 3 * jl;  // K_ab[i][j] = K_ba[j][i]
+              v[k][0] = ke[0];
+              v[k][1] = ke[1];
+              v[k][2] = ke[2];
+            }
           } else {
             const double* ke = kp + r.rows + 3 * bl + jl;
             const int ndof = 3 * r.n;
@@ -1067,7 +1175,9 @@ __device__ __forceinline__ void row_gather_node(
 // NC = 0: f_int is the sum of the element force vectors fe instead. LIVE: also flag the
 // entries that received a contribution other than 0.0. SCALAR: the k_ab I3 arena of
 // the mass and geometric stiffness. One wave per node.
-template <int NC, bool LIVE, bool SCALAR = false>
+// UP: upper-block arena, nodes taken in the order given by `order` (a locality order,
+// so that the two nodes reading one block do so close together in time).
+template <int NC, bool LIVE, bool SCALAR = false, bool UP = false>
 __global__ __launch_bounds__(kWave) void k_row_gather(
     const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
@@ -1075,7 +1185,7 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    int32_t n_nodes, int32_t slab, int32_t cap) {
+    int32_t n_nodes, int32_t slab, int32_t cap, const int32_t* __restrict__ order) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
   // rows accumulate in vals) and, for LIVE, their flags
   extern __shared__ double row_lds[];
@@ -1085,14 +1195,14 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // data again
   const int64_t ai = xcd_index(blockIdx.x, n_nodes, slab);
   if (ai < 0) return;
-  const int32_t a = int32_t(ai);
+  const int32_t a = order ? __builtin_amdgcn_readfirstlane(order[ai]) : int32_t(ai);
   // nfm bit 3: every incident element has at most 10 nodes (wave-uniform)
   if (__builtin_amdgcn_readfirstlane(int(nfm[a])) & 8)
-    row_gather_node<NC, LIVE, 2, SCALAR>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
-                                 row_ptr, kp, u, n_dof, fe, vals, f_int, live);
+    row_gather_node<NC, LIVE, 2, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
+                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live);
   else
-    row_gather_node<NC, LIVE, 1, SCALAR>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm, dof_eq,
-                                 row_ptr, kp, u, n_dof, fe, vals, f_int, live);
+    row_gather_node<NC, LIVE, 1, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
+                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live);
 }
 
 // ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
@@ -2132,8 +2242,10 @@ unsigned xcd_grid(const cxg_context* ctx, int64_t n) {
 }
 
 // Phase B over every node of the pattern's mesh (see k_row_gather).
+// up: the arena holds upper blocks (asm_variant 7), nodes in the pattern's locality order.
 void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
-                   double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false) {
+                   double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false,
+                   bool up = false) {
   const cxg_mesh* m = p->mesh;
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
   const int32_t slab = int32_t(xcd_slab(ctx, m->n_nodes));
@@ -2146,8 +2258,17 @@ void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, cons
                      ctx->stream,                                                               \
                      m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
                      p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
-                     live, m->n_nodes, slab, cap)
-  if (scalar) {
+                     live, m->n_nodes, slab, cap, up ? p->gorder.p : nullptr)
+  if (up) {
+    if (fe || scalar) throw Error(CXG_ERR_UNSUPPORTED, "upper-block arena: stiffness gathers only");
+    if (n_cases == 4) {
+      if (live) GATHER(4, true, false, true); else GATHER(4, false, false, true);
+    } else if (n_cases == 2) {
+      if (live) GATHER(2, true, false, true); else GATHER(2, false, false, true);
+    } else {
+      if (live) GATHER(1, true, false, true); else GATHER(1, false, false, true);
+    }
+  } else if (scalar) {
     if (live) GATHER(1, true, true); else GATHER(1, false, true);
   } else if (fe) {
     if (live) GATHER(0, true); else GATHER(0, false);
@@ -2258,24 +2379,37 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 #undef LAUNCH
       CXG_HIP(hipGetLastError());
     }
-    for (size_t bi = 0; ctx->asm_variant == 4 && bi < m->blocks.size(); ++bi) {
+    const bool up = ctx->asm_variant == 7;
+    if (up) prepare_order(ctx, p, coords);
+    for (size_t bi = 0; (ctx->asm_variant == 4 || up) && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
       const ElemTable& T = ctx->tables[B.type];
       const int32_t* conn = m->conn.p + B.conn_offset;
       double* kp = pm->ke_arena.p + p->kp_base[bi];
-      LaunchTimer timer(ctx, std::string("elem_iso_") + type_name(B.type), B.n_elem);
-#define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4>), dim3(xcd_grid(ctx, B.n_elem)), dim3(kWave), \
-                     0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E, nu, kp,         \
-                     ctx->flag_dev, B.n_elem, xcd_slab(ctx, B.n_elem))
+      LaunchTimer timer(ctx, std::string(up ? "elem_iso_up_" : "elem_iso_") + type_name(B.type), B.n_elem);
+#define ISO(NN, NGP, QC, UPV, TLV)                                                              \
+  hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4, UPV, TLV>), dim3(xcd_grid(ctx, B.n_elem)),    \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
+                     nu, kp, ctx->flag_dev, B.n_elem, xcd_slab(ctx, B.n_elem))
+#define LAUNCH(NN, NGP, UPPER, QC)                                                              \
+  if constexpr (NN == 20) {                                                                     \
+    if (ctx->elem_tiling == 1) {                                                                \
+      if (up) ISO(NN, NGP, QC, true, 1); else ISO(NN, NGP, QC, false, 1);                      \
+    } else {                                                                                    \
+      if (up) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                      \
+    }                                                                                           \
+  } else {                                                                                      \
+    if (up) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                        \
+  }
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
+#undef ISO
       CXG_HIP(hipGetLastError());
     }
     if (m->n_nodes > 0) {
-      LaunchTimer timer(ctx, "row_gather", m->n_elem);
-      launch_gather(ctx, p, f_int ? u : nullptr, nullptr, vals, f_int, live, n_cases);
+      LaunchTimer timer(ctx, up ? "row_gather_up" : "row_gather", m->n_elem);
+      launch_gather(ctx, p, f_int ? u : nullptr, nullptr, vals, f_int, live, n_cases, false, up);
       CXG_HIP(hipGetLastError());
     }
     return;
@@ -2379,6 +2513,76 @@ void prepare_arenas(const cxg_pattern* p, bool element_forces) {
   if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
   const size_t fe_n = size_t(3 * p->mesh->conn.n);
   if (element_forces && pm->fe_arena.n < fe_n) pm->fe_arena.alloc(fe_n);
+}
+
+// Columns of tile x tile elements over the two shorter extents of the nodes' bounding
+// box, each swept along the longest extent in layers a quarter element thick: the nodes
+// of one element, which read each other's upper blocks, come close together in the
+// order, so the second read of a block finds it in L2. The element size is estimated as
+// the edge of the bounding box volume shared out over the elements. Any order gives the
+// same values (each node's rows are summed by one wave in element order).
+void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords) {
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  const int tile = ctx->gather_tile;
+  if (pm->gorder_tile == tile) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CXG_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    throw Error(CXG_ERR_STATE, "the gather order is built before capture (assemble once first)");
+  const int32_t n = p->mesh->n_nodes;
+  if (tile <= 0 || n == 0) {
+    pm->gorder.reset();
+    pm->gorder_tile = tile;
+    return;
+  }
+  std::vector<double> x(size_t(3) * size_t(n));
+  CXG_HIP(hipStreamSynchronize(ctx->stream));
+  CXG_HIP(hipMemcpy(x.data(), coords, x.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double lo[3], hi[3];
+  for (int c = 0; c < 3; ++c) lo[c] = hi[c] = x[size_t(c)];
+  for (int32_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = std::min(lo[c], x[3 * size_t(i) + c]);
+      hi[c] = std::max(hi[c], x[3 * size_t(i) + c]);
+    }
+  int ax[3] = {0, 1, 2};  // the sweep axis last
+  std::sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] < hi[b] - lo[b]; });
+  const double ext = hi[ax[2]] - lo[ax[2]];
+  double vol = 1.0;
+  int dims = 0;
+  for (int c = 0; c < 3; ++c)
+    if (hi[c] - lo[c] > 1e-12 * ext) {
+      vol *= hi[c] - lo[c];
+      ++dims;
+    }
+  const double ne = double(std::max<int64_t>(p->mesh->n_elem, 1));
+  double h = dims > 0 ? std::pow(vol / ne, 1.0 / dims) : 1.0;
+  if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
+  const double w = tile * h, q = 0.25 * h;
+  struct Key {
+    int64_t t, s, v, u;
+    int32_t i;
+  };
+  const auto cell = [](double d, double step) { return int64_t(std::floor(d / step)); };
+  std::vector<Key> key(static_cast<size_t>(n));
+  const int64_t ntv = cell(hi[ax[1]] - lo[ax[1]], w) + 1;
+  for (int32_t i = 0; i < n; ++i) {
+    const double* xi = &x[3 * size_t(i)];
+    const double du = xi[ax[0]] - lo[ax[0]], dv = xi[ax[1]] - lo[ax[1]], ds = xi[ax[2]] - lo[ax[2]];
+    key[size_t(i)] = Key{cell(du, w) * ntv + cell(dv, w), cell(ds, q), cell(dv, q), cell(du, q), i};
+  }
+  std::sort(key.begin(), key.end(), [](const Key& a, const Key& b) {
+    if (a.t != b.t) return a.t < b.t;
+    if (a.s != b.s) return a.s < b.s;
+    if (a.v != b.v) return a.v < b.v;
+    if (a.u != b.u) return a.u < b.u;
+    return a.i < b.i;
+  });
+  std::vector<int32_t> ord(static_cast<size_t>(n));
+  for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = key[size_t(i)].i;
+  pm->gorder.alloc(size_t(n));
+  CXG_HIP(hipMemcpy(pm->gorder.p, ord.data(), ord.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  pm->gorder_tile = tile;
 }
 
 void prepare_work(cxg_context* ctx, const cxg_pattern* p) {

@@ -114,7 +114,7 @@ struct cxg_context {
   // Tunables (cxg_set_option): elements per patch for meshes created afterwards,
   // and the stiffness-kernel variant.
   int patch_size = 8;
-  int asm_variant = 4;
+  int asm_variant = 7;
   // Persistent element kernels (asm_variant 5): elements per work item, and resident
   // waves per compute unit of the persistent grid.
   int chunk_elems = 4;
@@ -122,6 +122,12 @@ struct cxg_context {
   int n_cu = 256;  // compute units of the device (hipDeviceAttributeMultiprocessorCount)
   int xcd_remap = 1;  // two-phase kernels: one contiguous work range per XCD
   int gather_lds_cols = 256;  // longest row the row gather accumulates in LDS
+  // asm_variant 7: columns of gather_tile x gather_tile elements in the gather's node
+  // order (0: node index order)
+  int gather_tile = 8;
+  // C3D20/C3D20R element kernels (asm_variant 4, 7): lane -> node-pair tiling (0: a node
+  // and four consecutive partners, 1: 2x2 pairs of two node couples)
+  int elem_tiling = 1;
   std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
 };
 
@@ -230,6 +236,10 @@ struct cxg_pattern {
   cxg::DevBuf<int32_t> queue;
   int32_t n_work = 0;
   int32_t work_chunk = 0;
+  // Node order of the upper-block gather (asm_variant 7), built for gorder_tile (-1: not
+  // built; empty for tile 0, node index order).
+  cxg::DevBuf<int32_t> gorder;
+  int gorder_tile = -1;
 };
 
 namespace cxg {
@@ -249,6 +259,9 @@ const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32
 // for asm_variant 5 the work queue of the persistent kernels.
 void prepare_arenas(const cxg_pattern* p, bool element_forces);
 void prepare_work(cxg_context* ctx, const cxg_pattern* p);
+// The upper-block gather's node order from the node coordinates (asm_variant 7; a
+// no-op when built for the context's gather_tile). Synchronizes the stream.
+void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords);
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
                         const void* j2_table, const double* u, const double* committed,

@@ -58,7 +58,7 @@ class Context:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(None, self.lib.cxg_create(self.device.index or 0, C.c_void_p(stream), C.byref(h)))
         self.h = h
-        self.options = {"asm_variant": 4, "patch_size": 8}
+        self.options = {"asm_variant": 7, "patch_size": 8}
 
     def sync_stream(self):
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -79,7 +79,7 @@ class Context:
     @property
     def tracks_live(self) -> bool:
         """Whether assemblies can report entry liveness (the two-phase kernels)."""
-        return self.options.get("asm_variant", 4) >= 4
+        return self.options.get("asm_variant", 7) >= 4
 
     def profile(self, enable: bool) -> None:
         """Bracket every library kernel launch with HIP events on its stream."""

@@ -96,10 +96,19 @@ const char* cxg_last_error(const cxg_context* ctx);
  *   "asm_variant" linear stiffness kernel: 1 = per-pair scatter, 2 = M-identity with
  *                 row-coalesced scatter at 4 waves/SIMD, 3 = the same at 3, 4 =
  *                 two-phase: element matrices to an arena, then each CSR row gathered
- *                 and written once by its node (default), 5 = two-phase with one
- *                 persistent element launch over all families, fed by a work queue,
- *                 6 = two-phase with the element matrices on the FP64 matrix cores
- *                 (M = H^T H by MFMA)
+ *                 and written once by its node, 5 = two-phase with one persistent
+ *                 element launch over all families, fed by a work queue, 6 = two-phase
+ *                 with the element matrices on the FP64 matrix cores (M = H^T H by
+ *                 MFMA), 7 = two-phase with only the upper node-pair blocks in the
+ *                 arena, the lower ones read transposed by the gather, which takes the
+ *                 nodes in a locality order built from the coordinates at the first
+ *                 assembly of a pattern (default; same values as 4 bit for bit)
+ *   "gather_tile" asm_variant 7: the locality order groups nodes into columns of
+ *                 gather_tile x gather_tile elements across the two shorter extents,
+ *                 swept along the longest (0: node index order; 0..4096, default 8)
+ *   "elem_tiling" 20-node element kernels of asm_variant 4 and 7: 0 = a lane takes a
+ *                 node and four consecutive partners, 1 = the 2x2 pairs of two node
+ *                 couples with 16-byte LDS reads (default 1; same values bit for bit)
  *   "chunk_elems"      work item of the persistent launch, in groups of the J2 kernel's
  *                      elements per wave (8 C3D8, 2 C3D20, 4 C3D20R, 24 C3D6) (1..4096, 4)
  *   "persistent_waves" resident waves per compute unit of that launch (1..32, 8)

@@ -97,7 +97,7 @@ def test_persistent_batch_matches_oracle(gpu, records, expected, chunk, waves):
         step.run()
         _check(step, records, expected)
     finally:
-        ctx.set_option("asm_variant", 4)
+        ctx.set_option("asm_variant", 7)
         ctx.set_option("chunk_elems", 4)
         ctx.set_option("persistent_waves", 8)
 
@@ -110,7 +110,7 @@ def test_batch_per_family_kernels_match_oracle(gpu, records, expected):
         step.run()
         _check(step, records, expected)
     finally:
-        ctx.set_option("asm_variant", 4)
+        ctx.set_option("asm_variant", 7)
 
 
 def test_batch_graph_replays_match_eager(gpu, records):
@@ -137,4 +137,4 @@ def test_batch_graph_replays_match_eager(gpu, records):
                 assert torch.equal(v, v2) and torch.equal(f, f2) and torch.equal(t, mp.trial)
             _ = (step.lin_f * 2).sum().item()
     finally:
-        ctx.set_option("asm_variant", 4)
+        ctx.set_option("asm_variant", 7)

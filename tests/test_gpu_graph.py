@@ -31,7 +31,7 @@ def _same(a, b):
     return np.array_equal(a.cpu().numpy(), b.cpu().numpy())
 
 
-@pytest.mark.parametrize("variant", [4, 1])
+@pytest.mark.parametrize("variant", [4, 1, 7])
 def test_linear_graph_replays_match_eager(gpu, port, variant):
     import torch
     from calculixpp_amd import _lib, meshgen
@@ -65,7 +65,7 @@ def test_linear_graph_replays_match_eager(gpu, port, variant):
                 assert rel_l2(f.cpu().numpy(), port.internal_force(m, uu.cpu().numpy())) <= 1e-12
             _ = (ve * 2.0).sum().item()  # unrelated torch work between replays
     finally:
-        ctx.set_option("asm_variant", 4)
+        ctx.set_option("asm_variant", 7)
 
 
 @pytest.mark.parametrize("fam_i", [0, 1])

@@ -129,7 +129,7 @@ def test_j2_tangent_force_state(gpu, port, fam_i, variant):
         mp = gpu.MaterialPoints(m.n_points, committed=hist)
         sys_g, f_g = gpu.assemble_material_tangent(m, u, mp)
     finally:
-        ctx.set_option("asm_variant", 4)
+        ctx.set_option("asm_variant", 7)
     rp, ci, vals = sys_e.csr()
     assert np.array_equal(sys_g.row_ptr.cpu().numpy(), rp)
     assert np.array_equal(sys_g.col_idx.cpu().numpy(), ci)
@@ -189,7 +189,7 @@ def test_deterministic_bitwise(gpu):
 
 
 @pytest.mark.parametrize("variant,patch", [(1, 1), (1, 5), (2, 1), (2, 3), (2, 8), (3, 16), (4, 1), (4, 8),
-                                           (5, 8), (6, 8)])
+                                           (5, 8), (6, 8), (7, 8)])
 @pytest.mark.parametrize("fam_i", range(4))
 def test_kernel_variants_and_patches(gpu, port, variant, patch, fam_i):
     """Every stiffness kernel variant and patch length gives the same K and f_int."""
@@ -208,7 +208,7 @@ def test_kernel_variants_and_patches(gpu, port, variant, patch, fam_i):
     assert np.array_equal(a.pattern.col_idx.cpu().numpy(), ci)
     assert rel_l2(v[: a.pattern.nnz].cpu().numpy(), vals) <= REL_K
     assert rel_l2(f.cpu().numpy(), fexp) <= REL_K
-    a.ctx.set_option("asm_variant", 4)  # restore the defaults
+    a.ctx.set_option("asm_variant", 7)  # restore the defaults
     a.ctx.set_option("patch_size", 8)
 
 
@@ -307,3 +307,67 @@ def test_gather_long_rows_match_lds_rows(gpu, port, fam_i):
     assert rel_l2(out[8][2], port.internal_force(m, u_np)) <= REL_K
     _, f_e, _ = port.assemble_material_tangent(m, u_np, aos_state(hist))
     assert rel_l2(out[8][5], f_e) <= REL_K
+
+
+def _spc_model(coords, blocks, seed):
+    from calculixpp_amd.model import Model
+    rng = np.random.default_rng(seed)
+    n_el = sum(len(c) for _, c in blocks)
+    x0 = np.nonzero(coords[:, 0] < 1e-12)[0]
+    return Model(coords, blocks, rng.uniform(1e5, 2e5, n_el), 0.3, spc_node=x0,
+                 spc_comp=np.full(len(x0), 2), spc_value=np.zeros(len(x0)))
+
+
+@pytest.mark.parametrize("tile", [8, 0, 1, 3])
+@pytest.mark.parametrize("mesh", ["mixed", "c3d20r"])
+def test_upper_arena_equals_full_arena_bitwise(gpu, port, tile, mesh):
+    """asm_variant 7 stores only the upper node-pair blocks and reads the lower ones
+    transposed, in a locality order of the nodes; the element matrices are bitwise
+    symmetric and every row is still summed in element order, so K, the live flags and
+    f_int (1, 2 and 4 load cases, LDS and long-row paths) equal asm_variant 4's bit for
+    bit, with either lane tiling of the 20-node kernels, on mixed C3D8/C3D20/C3D6 meshes
+    and C3D20R blocks with prescribed DOFs."""
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.fem import _ptr
+    if mesh == "mixed":
+        m = _spc_model(*meshgen.mixed_box(300, 51), 52)
+    else:
+        b = meshgen.block(3, _lib.C3D20R, seed_geo=54, seed_E=55, ny=2, nz=2)
+        m = _spc_model(b.coords, b.blocks, 56)
+    ctx = gpu.context()
+    a = gpu.Assembler(m, ctx)
+    dev = ctx.device
+    n3 = 3 * m.n_nodes
+    u = torch.from_numpy(np.stack([meshgen.smooth_field(m.coords, 53 + c).ravel()
+                                   for c in range(4)])).to(dev)
+    nnz = a.pattern.nnz
+    out = {}
+    try:
+        ctx.set_option("gather_tile", tile)
+        for variant, tiling in ((4, 0), (4, 1), (7, 0), (7, 1)):
+            ctx.set_option("asm_variant", variant)
+            ctx.set_option("elem_tiling", tiling)
+            for cols in (256, 8):
+                ctx.set_option("gather_lds_cols", cols)
+                for nc in (1, 2, 4):
+                    v, lv = a.new_values(), a.new_live()
+                    f = torch.empty(nc * n3, dtype=torch.float64, device=dev)
+                    ctx.check(ctx.lib.cxg_assemble_linear_cases(
+                        ctx.h, a.pattern.h, _ptr(a.mesh.coords), _ptr(a.mesh.E), _ptr(a.mesh.nu),
+                        _ptr(v), nc, _ptr(u), _ptr(f), _ptr(lv)))
+                    out[variant, tiling, cols, nc] = (v[:nnz].cpu().numpy(), lv[:nnz].cpu().numpy(),
+                                                      f.cpu().numpy())
+        ctx.set_option("gather_lds_cols", 256)
+        check_linear(gpu, port, m)  # variant 7 through the one-shot key-set path
+    finally:
+        ctx.set_option("asm_variant", 7)
+        ctx.set_option("elem_tiling", 1)
+        ctx.set_option("gather_lds_cols", 256)
+        ctx.set_option("gather_tile", 8)
+    for key, res in out.items():
+        for x, y in zip(out[4, 0, key[2], key[3]], res):
+            assert np.array_equal(x, y), key
+    for c in range(4):
+        f_exp = port.internal_force(m, u[c].cpu().numpy())
+        assert rel_l2(out[7, 1, 256, 4][2][c * n3:(c + 1) * n3], f_exp) <= REL_K
