@@ -840,15 +840,26 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
   }
 }
 
+// With ue given, the element's nodal displacements of n_cases fields (u + c * n_dof) are
+// copied to ue + c * ue_stride + 3 * (connectivity offset), where the row gather reads
+// them by its incidence records without a connectivity load.
 template <int NN, int NGP, int QC, int WAVES, bool UP = false, int TL = 0>
 __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
     const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ coords,
     const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
-    int* __restrict__ flag, int64_t n_elem, int64_t slab) {
+    int* __restrict__ flag, int64_t n_elem, int64_t slab, const double* __restrict__ u,
+    int64_t n_dof, int32_t n_cases, double* __restrict__ ue, int64_t ue_stride) {
   __shared__ ElemLds<NN, NGP, QC> L;
   const int64_t le = xcd_index(blockIdx.x, n_elem, slab);
   if (le < 0) return;
+  if (ue && threadIdx.x < NN) {
+    const int64_t k = le * NN + threadIdx.x;
+    const int64_t nd = conn[k];
+    for (int c = 0; c < n_cases; ++c)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ue[c * ue_stride + 3 * k + j] = u[c * n_dof + 3 * nd + j];
+  }
   elem_iso_body<NN, NGP, QC, UP, TL>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
 }
@@ -1005,7 +1016,8 @@ __device__ __forceinline__ void row_gather_node(
     const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
-    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live) {
+    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
+    const double* __restrict__ ue, int64_t ue_stride) {
   const int lane = threadIdx.x;
   constexpr int halves = HALVES;
   const int h = HALVES == 2 ? lane / 32 : 0;
@@ -1095,9 +1107,16 @@ __device__ __forceinline__ void row_gather_node(
           }
           if constexpr (NC > 0)
             if (u) {
-              const int32_t nd = conn[r.conn + bl];
+              if constexpr (UP) {
+                // the element's displacements, copied by the element kernel in
+                // connectivity order: no connectivity load ahead of this one
 #pragma unroll
-              for (int c = 0; c < NC; ++c) uv[k][c] = u[c * n_dof + 3 * int64_t(nd) + jl];
+                for (int c = 0; c < NC; ++c) uv[k][c] = ue[c * ue_stride + 3 * (r.conn + bl) + jl];
+              } else {
+                const int32_t nd = conn[r.conn + bl];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) uv[k][c] = u[c * n_dof + 3 * int64_t(nd) + jl];
+              }
             }
         }
       }
@@ -1185,7 +1204,8 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    int32_t n_nodes, int32_t slab, int32_t cap, const int32_t* __restrict__ order) {
+    int32_t n_nodes, int32_t slab, int32_t cap, const int32_t* __restrict__ order,
+    const double* __restrict__ ue, int64_t ue_stride) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
   // rows accumulate in vals) and, for LIVE, their flags
   extern __shared__ double row_lds[];
@@ -1199,10 +1219,12 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // nfm bit 3: every incident element has at most 10 nodes (wave-uniform)
   if (__builtin_amdgcn_readfirstlane(int(nfm[a])) & 8)
     row_gather_node<NC, LIVE, 2, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
-                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live);
+                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live,
+                                             ue, ue_stride);
   else
     row_gather_node<NC, LIVE, 1, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
-                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live);
+                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live,
+                                             ue, ue_stride);
 }
 
 // ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
@@ -2258,7 +2280,8 @@ void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, cons
                      ctx->stream,                                                               \
                      m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
                      p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
-                     live, m->n_nodes, slab, cap, up ? p->gorder.p : nullptr)
+                     live, m->n_nodes, slab, cap, up ? p->gorder.p : nullptr,                  \
+                     up ? p->ue_arena.p : nullptr, int64_t(3) * int64_t(m->conn.n))
   if (up) {
     if (fe || scalar) throw Error(CXG_ERR_UNSUPPORTED, "upper-block arena: stiffness gathers only");
     if (n_cases == 4) {
@@ -2381,6 +2404,14 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
     }
     const bool up = ctx->asm_variant == 7;
     if (up) prepare_order(ctx, p, coords);
+    // variant 7 with f_int: the element kernels copy u per element for the gather
+    double* ue = nullptr;
+    if (up && u && f_int) {
+      prepare_displacement_arena(p, n_cases);
+      ue = pm->ue_arena.p;
+    }
+    const int64_t n_dof = 3 * int64_t(m->n_nodes);
+    const int64_t ue_stride = int64_t(3) * int64_t(m->conn.n);
     for (size_t bi = 0; (ctx->asm_variant == 4 || up) && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
@@ -2391,7 +2422,8 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 #define ISO(NN, NGP, QC, UPV, TLV)                                                              \
   hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4, UPV, TLV>), dim3(xcd_grid(ctx, B.n_elem)),    \
                      dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
-                     nu, kp, ctx->flag_dev, B.n_elem, xcd_slab(ctx, B.n_elem))
+                     nu, kp, ctx->flag_dev, B.n_elem, xcd_slab(ctx, B.n_elem), u, n_dof,        \
+                     n_cases, ue ? ue + 3 * B.conn_offset : nullptr, ue_stride)
 #define LAUNCH(NN, NGP, UPPER, QC)                                                              \
   if constexpr (NN == 20) {                                                                     \
     if (ctx->elem_tiling == 1) {                                                                \
@@ -2506,6 +2538,12 @@ const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32
     ctx->j2_host = h;
   }
   return h.empty() ? nullptr : ctx->j2_dev.p;
+}
+
+void prepare_displacement_arena(const cxg_pattern* p, int n_cases) {
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  const size_t n = size_t(n_cases) * 3 * p->mesh->conn.n;
+  if (pm->ue_arena.n < n) pm->ue_arena.alloc(n);
 }
 
 void prepare_arenas(const cxg_pattern* p, bool element_forces) {

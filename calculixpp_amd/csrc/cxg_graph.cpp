@@ -97,7 +97,10 @@ cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, cons
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7) cxg::prepare_order(ctx, p, coords);
+    if (ctx->asm_variant == 7) {
+      cxg::prepare_order(ctx, p, coords);
+      if (with_f) cxg::prepare_displacement_arena(p, 1);
+    }
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
                                   with_f ? f_int : nullptr, nullptr);
@@ -121,7 +124,10 @@ cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7) cxg::prepare_order(ctx, p, coords);
+    if (ctx->asm_variant == 7) {
+      cxg::prepare_order(ctx, p, coords);
+      cxg::prepare_displacement_arena(p, int(n_cases));
+    }
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, u, f_int, nullptr, int(n_cases));
     });

@@ -224,6 +224,9 @@ struct cxg_pattern {
   // (3n x 3n, row-major) at kpptr[e].
   cxg::DevBuf<double> ke_arena;
   cxg::DevBuf<double> fe_arena;               // element force vectors (3n at 3*eptr[e])
+  // asm_variant 7 with f_int: each element's nodal displacements of every load case,
+  // 3 per connectivity entry, case c at c * 3 * conn.n
+  cxg::DevBuf<double> ue_arena;
   cxg::DevBuf<int64_t> kpptr;                 // n_elem+1, in doubles
   cxg::DevBuf<cxg::IncRecord> inc_rec;        // per mesh incidence entry (inc_elem order)
   cxg::DevBuf<int8_t> nfm;                    // per node: bit c set if DOF c is free
@@ -259,6 +262,9 @@ const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32
 // for asm_variant 5 the work queue of the persistent kernels.
 void prepare_arenas(const cxg_pattern* p, bool element_forces);
 void prepare_work(cxg_context* ctx, const cxg_pattern* p);
+// The per-element displacement arena of asm_variant 7 for n_cases fields (no-op when
+// large enough); before capture, as it allocates.
+void prepare_displacement_arena(const cxg_pattern* p, int n_cases);
 // The upper-block gather's node order from the node coordinates (asm_variant 7; a
 // no-op when built for the context's gather_tile). Synchronizes the stream.
 void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords);
