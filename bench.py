@@ -433,6 +433,13 @@ def main() -> int:
     if args.only:
         import torch
         torch.cuda.set_device(0)
+        if args.option or args.variant:
+            from calculixpp_amd import fem
+            if args.variant:
+                fem.context().set_option("asm_variant", args.variant)
+            for opt in args.option:
+                key, val = opt.split("=", 1)
+                fem.context().set_option(key, int(val))
         if args.only == "j2":
             res = j2_newton(reps=args.steps, cpu=not args.no_cpu)
         elif args.only == "batch":

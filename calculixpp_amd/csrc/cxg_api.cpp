@@ -366,6 +366,10 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->persistent_waves = int(value);
     return CXG_OK;
   }
+  if (k == "up_min_nodes" && value >= 0 && value <= 64) {
+    ctx->up_min_nodes = int(value);
+    return CXG_OK;
+  }
   if (k == "elem_tiling" && (value == 0 || value == 1)) {
     ctx->elem_tiling = int(value);
     return CXG_OK;

@@ -634,6 +634,55 @@ __host__ __device__ __forceinline__ int up_block(int a, int b, int n) {
   return a * n - a * (a - 1) / 2 + (b - a);
 }
 
+// Staged store of a lane's upper node-pair blocks K[t] = K_(pa[t], pb[t]) (pb >= pa; pa <
+// 0: none) into the element's packed upper-block arena kel, the block rows of
+// kStageNodes nodes at a time through stage (kStageNodes * 9 * NN doubles, 16-byte
+// aligned). A lane's blocks all lie in the rows of its node pa; a turn's blocks are one
+// contiguous run of the packed arena (up_block(g0, g0) is even for g0 % 4 == 0, so the
+// run starts 16-byte aligned), written with 16-byte streaming stores. Every lane of the
+// wave calls it; it ends with a barrier.
+template <int NN, int PPL>
+__device__ __forceinline__ void store_upper_blocks(double* __restrict__ stage, const int* pa,
+                                                   const int* pb, const double (*K)[9],
+                                                   double* __restrict__ kel, const int lane) {
+  constexpr int NP = NN * (NN + 1) / 2;
+  constexpr int ND = 3 * NN;
+  for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
+    const int g1 = NN - g0 < kStageNodes ? NN : g0 + kStageNodes;
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      if (pa[t] < g0 || pa[t] >= g1) continue;
+      double* ra = stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[t][3 * i + j];
+    }
+    __syncthreads();
+    const int k0 = up_block(g0, g0, NN);
+    const int k1 = g1 < NN ? up_block(g1, g1, NN) : NP;
+    const int cnt = 9 * (k1 - k0);
+    // packed double k0*9 + idx -> its place in the staged rows
+    auto staged = [&](int idx) {
+      const int bk = k0 + idx / 9, r = idx - 9 * (idx / 9);
+      int a = g0;
+      while (a + 1 < g1 && up_block(a + 1, a + 1, NN) <= bk) ++a;
+      const int b = a + (bk - up_block(a, a, NN));
+      const int i = r / 3, j = r - 3 * (r / 3);
+      return stage[(3 * (a - g0) + i) * ND + 3 * b + j];
+    };
+    f64x2* dst = reinterpret_cast<f64x2*>(kel + 9 * k0);
+    for (int idx = lane; idx < cnt / 2; idx += kWave) {
+      f64x2 v;
+      v.x = staged(2 * idx);
+      v.y = staged(2 * idx + 1);
+      __builtin_nontemporal_store(v, dst + idx);
+    }
+    if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(staged(cnt - 1), kel + 9 * k0 + cnt - 1);
+    __syncthreads();
+  }
+}
+
 // One element: ce its connectivity row, kel its arena slot. UP = false: the full 3n x 3n
 // matrix, row-major. UP = true: only the upper node-pair blocks K_ab (b >= a), packed in
 // up_block order, each 3x3 row-major; the element matrix is symmetric (K_ba = K_ab^T),
@@ -764,45 +813,7 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
     for (int i = 0; i < 9; ++i) m[i] = K[i];
   }
   if constexpr (UP) {
-    // Staged store of the upper blocks, the block rows of kStageNodes nodes at a time:
-    // a lane's blocks all lie in the rows of its node a; the turn's blocks are one
-    // contiguous run of the packed arena (up_block(g0, g0) is even for g0 % 4 == 0, so
-    // the run starts 16-byte aligned).
-    for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
-      const int g1 = NN - g0 < kStageNodes ? NN : g0 + kStageNodes;
-#pragma unroll
-      for (int t = 0; t < PPL; ++t) {
-        if (pa[t] < g0 || pa[t] >= g1) continue;
-        const double* K = M[t];
-        double* ra = L.stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[3 * i + j];
-      }
-      __syncthreads();
-      const int k0 = up_block(g0, g0, NN);
-      const int k1 = g1 < NN ? up_block(g1, g1, NN) : NP;
-      const int cnt = 9 * (k1 - k0);
-      // packed double k0*9 + idx -> its place in the staged rows
-      auto staged = [&](int idx) {
-        const int bk = k0 + idx / 9, r = idx - 9 * (idx / 9);
-        int a = g0;
-        while (a + 1 < g1 && up_block(a + 1, a + 1, NN) <= bk) ++a;
-        const int b = a + (bk - up_block(a, a, NN));
-        const int i = r / 3, j = r - 3 * (r / 3);
-        return L.stage[(3 * (a - g0) + i) * ND + 3 * b + j];
-      };
-      f64x2* dst = reinterpret_cast<f64x2*>(kel + 9 * k0);
-      for (int idx = lane; idx < cnt / 2; idx += kWave) {
-        f64x2 v;
-        v.x = staged(2 * idx);
-        v.y = staged(2 * idx + 1);
-        __builtin_nontemporal_store(v, dst + idx);
-      }
-      if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(staged(cnt - 1), kel + 9 * k0 + cnt - 1);
-      __syncthreads();
-    }
+    store_upper_blocks<NN, PPL>(L.stage, pa, pb, M, kel, lane);
     return;
   }
   // Staged store: rows of kStageNodes nodes at a time. Block (a, b) goes to the rows of
@@ -1006,8 +1017,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // nodes whose incident elements all have <= 10 nodes (3n <= 30 columns fit in 32 lanes).
 // SCALAR: the arena holds k_ab per node pair (n^2 per element, at a ninth of the full
 // matrix's offsets) for a k_ab I3 block matrix, expanded here.
-// UP: the arena holds the upper node-pair blocks of each element (elem_iso_body with
-// UP = true) at the element's slot; the blocks of partners b < a are read transposed.
+// UP: the arena holds the upper node-pair blocks (elem_iso_body with UP = true) of the
+// elements of at least up_min nodes at their slots, the blocks of partners b < a read
+// transposed, and the full matrix of the others.
 template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false>
 __device__ __forceinline__ void row_gather_node(
     int32_t a, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
@@ -1017,7 +1029,7 @@ __device__ __forceinline__ void row_gather_node(
     const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    const double* __restrict__ ue, int64_t ue_stride) {
+    const double* __restrict__ ue, int64_t ue_stride, int up_min) {
   const int lane = threadIdx.x;
   constexpr int halves = HALVES;
   const int h = HALVES == 2 ? lane / 32 : 0;
@@ -1084,20 +1096,20 @@ __device__ __forceinline__ void row_gather_node(
             v[k][1] = jl == 1 ? s : 0.0;
             v[k][2] = jl == 2 ? s : 0.0;
           } else if constexpr (UP) {
-            // plain (cached) loads: the partner node's wave reads the same block
+            // Branch-free, so that the batch's loads issue together: an element of at
+            // least up_min nodes has its upper blocks in the arena (K_ab[i][j] at 3i + j
+            // of block (a, b) for b >= a, at 3j + i of block (b, a) for b < a), the
+            // others their full rows. Plain (cached) loads: the partner node's wave reads
+            // the same block.
             const int la = r.la, n = r.n;
-            const double* ke = kp + (r.rows - int64_t(9) * la * n);
-            if (bl >= la) {
-              ke += 9 * up_block(la, bl, n) + jl;  // K_ab[i][j] at 3i + j
-              v[k][0] = ke[0];
-              v[k][1] = ke[3];
-              v[k][2] = ke[6];
-            } else {
-              ke += 9 * up_block(bl, la, n) + 3 * jl;  // K_ab[i][j] = K_ba[j][i]
-              v[k][0] = ke[0];
-              v[k][1] = ke[1];
-              v[k][2] = ke[2];
-            }
+            const bool upr = n >= up_min, dir = bl >= la;
+            const int64_t base = r.rows - int64_t(9) * la * n;
+            const int64_t off_up = dir ? 9 * up_block(la, bl, n) + jl : 9 * up_block(bl, la, n) + 3 * jl;
+            const double* ke = kp + (upr ? base + off_up : r.rows + 3 * bl + jl);
+            const int64_t st = upr ? (dir ? 3 : 1) : 3 * n;
+            v[k][0] = ke[0];
+            v[k][1] = ke[st];
+            v[k][2] = ke[2 * st];
           } else {
             const double* ke = kp + r.rows + 3 * bl + jl;
             const int ndof = 3 * r.n;
@@ -1205,7 +1217,7 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
     int32_t n_nodes, int32_t slab, int32_t cap, const int32_t* __restrict__ order,
-    const double* __restrict__ ue, int64_t ue_stride) {
+    const double* __restrict__ ue, int64_t ue_stride, int32_t up_min) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
   // rows accumulate in vals) and, for LIVE, their flags
   extern __shared__ double row_lds[];
@@ -1220,11 +1232,11 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   if (__builtin_amdgcn_readfirstlane(int(nfm[a])) & 8)
     row_gather_node<NC, LIVE, 2, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
                                              dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live,
-                                             ue, ue_stride);
+                                             ue, ue_stride, up_min);
   else
     row_gather_node<NC, LIVE, 1, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
                                              dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live,
-                                             ue, ue_stride);
+                                             ue, ue_stride, up_min);
 }
 
 // ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
@@ -1612,8 +1624,10 @@ __device__ __forceinline__ void j2_gauss_point(
   }
 }
 
-// Phase 2 for one element: every lane of the wave calls it; ends with a barrier.
-template <int NN, int NGP, int QC, class W>
+// Phase 2 for one element: every lane of the wave calls it; ends with a barrier. UP: K_t
+// (symmetric) as its packed upper node-pair blocks (store_upper_blocks) instead of full
+// rows.
+template <int NN, int NGP, int QC, class W, bool UP = false>
 __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w, const int lane,
                                                   const double* __restrict__ dN,
                                                   double* __restrict__ kel,
@@ -1707,6 +1721,10 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
     fp[1] = f1;
     fp[2] = f2;
   }
+  if constexpr (UP) {
+    store_upper_blocks<NN, PPL>(w.stage, pa, pb, K, kel, lane);
+    return;
+  }
   for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
 #pragma unroll
     for (int t = 0; t < PPL; ++t) {
@@ -1739,7 +1757,7 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
 // ne <= EPW consecutive elements of one block (first block-local le0) on one wave:
 // phase 1 for all their Gauss points at once, then phase 2 element by element. kp and
 // fe point at the block's arenas. Ends with a barrier.
-template <int NN, int NGP, int QC, int EPW>
+template <int NN, int NGP, int QC, int EPW, bool UP = false>
 __device__ __forceinline__ void j2_group(
     ElemJ2Lds<NN, NGP, QC, EPW>& L, const int lane, const int ne, const int64_t le0,
     const int32_t* __restrict__ conn, const int64_t elem_offset, const int64_t gp_offset,
@@ -1764,12 +1782,13 @@ __device__ __forceinline__ void j2_group(
   }
   __syncthreads();
   for (int e = 0; e < ne; ++e)
-    j2_element_matrix<NN, NGP, QC>(L.e[e], L.w, lane, dN, kp + (le0 + e) * (9 * NN * NN),
-                                   fe + (le0 + e) * (3 * NN));
+    j2_element_matrix<NN, NGP, QC, decltype(L.w), UP>(L.e[e], L.w, lane, dN,
+                                                      kp + (le0 + e) * (9 * NN * NN),
+                                                      fe + (le0 + e) * (3 * NN));
 }
 
-// EPW elements per wave (see j2_group).
-template <int NN, int NGP, int QC>
+// EPW elements per wave (see j2_group). UP: upper-block arena (asm_variant 7).
+template <int NN, int NGP, int QC, bool UP = false>
 __global__ __launch_bounds__(kWave) void k_elem_j2(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t n_elem, int64_t gp_offset,
     int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
@@ -1785,8 +1804,9 @@ __global__ __launch_bounds__(kWave) void k_elem_j2(
   if (grp < 0) return;
   const int64_t le0 = grp * EPW;
   const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
-  j2_group<NN, NGP, QC, EPW>(L, lane, ne, le0, conn, elem_offset, gp_offset, n_points, dN, wq,
-                             coords, E, nu, mat_id, mats, u, committed, trial, kp, fe, flag);
+  j2_group<NN, NGP, QC, EPW, UP>(L, lane, ne, le0, conn, elem_offset, gp_offset, n_points, dN,
+                                 wq, coords, E, nu, mat_id, mats, u, committed, trial, kp, fe,
+                                 flag);
 }
 
 // ---- Persistent element kernels (asm_variant 5) ------------------------------------
@@ -2281,10 +2301,13 @@ void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, cons
                      m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
                      p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
                      live, m->n_nodes, slab, cap, up ? p->gorder.p : nullptr,                  \
-                     up ? p->ue_arena.p : nullptr, int64_t(3) * int64_t(m->conn.n))
+                     up ? p->ue_arena.p : nullptr, int64_t(3) * int64_t(m->conn.n),            \
+                     int32_t(ctx->up_min_nodes))
   if (up) {
-    if (fe || scalar) throw Error(CXG_ERR_UNSUPPORTED, "upper-block arena: stiffness gathers only");
-    if (n_cases == 4) {
+    if (scalar) throw Error(CXG_ERR_UNSUPPORTED, "upper-block arena: stiffness gathers only");
+    if (fe) {
+      if (live) GATHER(0, true, false, true); else GATHER(0, false, false, true);
+    } else if (n_cases == 4) {
       if (live) GATHER(4, true, false, true); else GATHER(4, false, false, true);
     } else if (n_cases == 2) {
       if (live) GATHER(2, true, false, true); else GATHER(2, false, false, true);
@@ -2402,7 +2425,7 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 #undef LAUNCH
       CXG_HIP(hipGetLastError());
     }
-    const bool up = ctx->asm_variant == 7;
+    const bool up = ctx->asm_variant == 7 && uses_upper_blocks(ctx, m);
     if (up) prepare_order(ctx, p, coords);
     // variant 7 with f_int: the element kernels copy u per element for the gather
     double* ue = nullptr;
@@ -2412,13 +2435,14 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
     }
     const int64_t n_dof = 3 * int64_t(m->n_nodes);
     const int64_t ue_stride = int64_t(3) * int64_t(m->conn.n);
-    for (size_t bi = 0; (ctx->asm_variant == 4 || up) && bi < m->blocks.size(); ++bi) {
+    for (size_t bi = 0; (ctx->asm_variant == 4 || ctx->asm_variant == 7) && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
       const ElemTable& T = ctx->tables[B.type];
       const int32_t* conn = m->conn.p + B.conn_offset;
       double* kp = pm->ke_arena.p + p->kp_base[bi];
-      LaunchTimer timer(ctx, std::string(up ? "elem_iso_up_" : "elem_iso_") + type_name(B.type), B.n_elem);
+      const bool bup = up && B.nn >= ctx->up_min_nodes;  // this family's arena layout
+      LaunchTimer timer(ctx, std::string(bup ? "elem_iso_up_" : "elem_iso_") + type_name(B.type), B.n_elem);
 #define ISO(NN, NGP, QC, UPV, TLV)                                                              \
   hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4, UPV, TLV>), dim3(xcd_grid(ctx, B.n_elem)),    \
                      dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
@@ -2427,12 +2451,12 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 #define LAUNCH(NN, NGP, UPPER, QC)                                                              \
   if constexpr (NN == 20) {                                                                     \
     if (ctx->elem_tiling == 1) {                                                                \
-      if (up) ISO(NN, NGP, QC, true, 1); else ISO(NN, NGP, QC, false, 1);                      \
+      if (bup) ISO(NN, NGP, QC, true, 1); else ISO(NN, NGP, QC, false, 1);                     \
     } else {                                                                                    \
-      if (up) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                      \
+      if (bup) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                     \
     }                                                                                           \
   } else {                                                                                      \
-    if (up) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                        \
+    if (bup) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                       \
   }
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
@@ -2538,6 +2562,12 @@ const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32
     ctx->j2_host = h;
   }
   return h.empty() ? nullptr : ctx->j2_dev.p;
+}
+
+bool uses_upper_blocks(const cxg_context* ctx, const cxg_mesh* m) {
+  for (const cxg_block_dev& B : m->blocks)
+    if (B.n_elem > 0 && B.nn >= ctx->up_min_nodes) return true;
+  return false;
 }
 
 void prepare_displacement_arena(const cxg_pattern* p, int n_cases) {
@@ -2683,6 +2713,8 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
                          pm->ke_arena.p, pm->fe_arena.p, ctx->flag_dev);
       CXG_HIP(hipGetLastError());
     }
+    const bool up = ctx->asm_variant == 7 && uses_upper_blocks(ctx, m);
+    if (up) prepare_order(ctx, p, coords);
     for (size_t bi = 0; ctx->asm_variant != 5 && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
@@ -2690,20 +2722,24 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
       const int32_t* conn = m->conn.p + B.conn_offset;
       double* kp = pm->ke_arena.p + p->kp_base[bi];
       double* fe = pm->fe_arena.p + 3 * B.conn_offset;
-      LaunchTimer timer(ctx, std::string("elem_j2_") + type_name(B.type), B.n_elem);
-#define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC>),                                                  \
+      const bool bup = up && B.nn >= ctx->up_min_nodes;  // this family's arena layout
+      LaunchTimer timer(ctx, std::string(bup ? "elem_j2_up_" : "elem_j2_") + type_name(B.type), B.n_elem);
+#define J2K(NN, NGP, QC, UPV)                                                                   \
+  hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC, UPV>),                                             \
                      dim3(xcd_grid(ctx, (B.n_elem + j2_epw(NN, NGP) - 1) / j2_epw(NN, NGP))), dim3(kWave), \
                      0, ctx->stream, conn, B.elem_offset, B.n_elem, B.gp_offset, m->n_points,   \
                      T.dN, T.w, coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe,      \
                      ctx->flag_dev, xcd_slab(ctx, (B.n_elem + j2_epw(NN, NGP) - 1) / j2_epw(NN, NGP)))
+#define LAUNCH(NN, NGP, UPPER, QC) \
+  if (bup) J2K(NN, NGP, QC, true); else J2K(NN, NGP, QC, false);
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
+#undef J2K
       CXG_HIP(hipGetLastError());
     }
     if (m->n_nodes > 0) {
-      LaunchTimer timer(ctx, "row_gather_j2", m->n_elem);
-      launch_gather(ctx, p, nullptr, pm->fe_arena.p, vals, f_int, live, 1);
+      LaunchTimer timer(ctx, up ? "row_gather_j2_up" : "row_gather_j2", m->n_elem);
+      launch_gather(ctx, p, nullptr, pm->fe_arena.p, vals, f_int, live, 1, false, up);
       CXG_HIP(hipGetLastError());
     }
     return;

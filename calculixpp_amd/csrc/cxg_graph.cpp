@@ -97,7 +97,7 @@ cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, cons
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7) {
+    if (ctx->asm_variant == 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
       cxg::prepare_order(ctx, p, coords);
       if (with_f) cxg::prepare_displacement_arena(p, 1);
     }
@@ -124,7 +124,7 @@ cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7) {
+    if (ctx->asm_variant == 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
       cxg::prepare_order(ctx, p, coords);
       cxg::prepare_displacement_arena(p, int(n_cases));
     }
@@ -158,6 +158,8 @@ cxg_status cxg_graph_capture_j2(cxg_context* ctx, const cxg_pattern* p, const do
     }
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, true);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
+    if (ctx->asm_variant == 7 && cxg::uses_upper_blocks(ctx, p->mesh))
+      cxg::prepare_order(ctx, p, coords);
     const int32_t* mid = n_mats > 0 ? mat_id : nullptr;
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_j2(ctx, p, coords, E, nu, mid, g->j2_table.p, u, committed, trial,

@@ -125,6 +125,9 @@ struct cxg_context {
   // asm_variant 7: columns of gather_tile x gather_tile elements in the gather's node
   // order (0: node index order)
   int gather_tile = 8;
+  // asm_variant 7: element families of at least this many nodes store the upper node-pair
+  // blocks, the others the full matrix (the gather reads each element in its layout)
+  int up_min_nodes = 0;
   // C3D20/C3D20R element kernels (asm_variant 4, 7): lane -> node-pair tiling (0: a node
   // and four consecutive partners, 1: 2x2 pairs of two node couples)
   int elem_tiling = 1;
@@ -262,6 +265,9 @@ const void* upload_j2_table(cxg_context* ctx, const cxg_j2_material* mats, int32
 // for asm_variant 5 the work queue of the persistent kernels.
 void prepare_arenas(const cxg_pattern* p, bool element_forces);
 void prepare_work(cxg_context* ctx, const cxg_pattern* p);
+// asm_variant 7 stores upper blocks for the families of at least up_min_nodes nodes; a
+// mesh without such a family runs the full-matrix kernels and gather of asm_variant 4.
+bool uses_upper_blocks(const cxg_context* ctx, const cxg_mesh* m);
 // The per-element displacement arena of asm_variant 7 for n_cases fields (no-op when
 // large enough); before capture, as it allocates.
 void prepare_displacement_arena(const cxg_pattern* p, int n_cases);

@@ -106,6 +106,9 @@ const char* cxg_last_error(const cxg_context* ctx);
  *   "gather_tile" asm_variant 7: the locality order groups nodes into columns of
  *                 gather_tile x gather_tile elements across the two shorter extents,
  *                 swept along the longest (0: node index order; 0..4096, default 8)
+ *   "up_min_nodes" asm_variant 7: element families of at least this many nodes store
+ *                 the upper node-pair blocks, the others their full matrix; the gather
+ *                 reads each element in its own layout (0..64, default 0: every family)
  *   "elem_tiling" 20-node element kernels of asm_variant 4 and 7: 0 = a lane takes a
  *                 node and four consecutive partners, 1 = the 2x2 pairs of two node
  *                 couples with 16-byte LDS reads (default 1; same values bit for bit)

@@ -111,11 +111,12 @@ def test_internal_force_fused_and_matrix_free(gpu, port, fam_i):
     assert rel_l2(f.cpu().numpy(), exp) <= REL_K
 
 
-@pytest.mark.parametrize("variant", [4, 1, 5])
+@pytest.mark.parametrize("variant", [7, 4, 1, 5])
 @pytest.mark.parametrize("fam_i", range(4))
 def test_j2_tangent_force_state(gpu, port, fam_i, variant):
-    """K_t, f_int and trial history of the two-phase (4), colour-scatter (1) and
-    persistent (5) paths; every 5th element stays isotropic elastic."""
+    """K_t, f_int and trial history of the two-phase paths with the upper-block (7) and
+    full (4) arena, the colour-scatter (1) and persistent (5) paths; every 5th element
+    stays isotropic elastic."""
     from calculixpp_amd import meshgen
     from oracle.oracle import aos_state
     fam = _families()[fam_i]
@@ -318,15 +319,15 @@ def _spc_model(coords, blocks, seed):
                  spc_comp=np.full(len(x0), 2), spc_value=np.zeros(len(x0)))
 
 
-@pytest.mark.parametrize("tile", [8, 0, 1, 3])
+@pytest.mark.parametrize("tile,up_min", [(8, 20), (8, 0), (0, 0), (1, 20), (3, 0)])
 @pytest.mark.parametrize("mesh", ["mixed", "c3d20r"])
-def test_upper_arena_equals_full_arena_bitwise(gpu, port, tile, mesh):
-    """asm_variant 7 stores only the upper node-pair blocks and reads the lower ones
-    transposed, in a locality order of the nodes; the element matrices are bitwise
-    symmetric and every row is still summed in element order, so K, the live flags and
-    f_int (1, 2 and 4 load cases, LDS and long-row paths) equal asm_variant 4's bit for
-    bit, with either lane tiling of the 20-node kernels, on mixed C3D8/C3D20/C3D6 meshes
-    and C3D20R blocks with prescribed DOFs."""
+def test_upper_arena_equals_full_arena_bitwise(gpu, port, tile, up_min, mesh):
+    """asm_variant 7 stores only the upper node-pair blocks (of the families of at least
+    up_min nodes) and reads the lower ones transposed, in a locality order of the nodes;
+    the element matrices are bitwise symmetric and every row is still summed in element
+    order, so K, the live flags and f_int (1, 2 and 4 load cases, LDS and long-row paths)
+    equal asm_variant 4's bit for bit, with either lane tiling of the 20-node kernels, on
+    mixed C3D8/C3D20/C3D6 meshes and C3D20R blocks with prescribed DOFs."""
     import torch
     from calculixpp_amd import _lib, meshgen
     from calculixpp_amd.fem import _ptr
@@ -345,6 +346,7 @@ def test_upper_arena_equals_full_arena_bitwise(gpu, port, tile, mesh):
     out = {}
     try:
         ctx.set_option("gather_tile", tile)
+        ctx.set_option("up_min_nodes", up_min)
         for variant, tiling in ((4, 0), (4, 1), (7, 0), (7, 1)):
             ctx.set_option("asm_variant", variant)
             ctx.set_option("elem_tiling", tiling)
@@ -365,9 +367,43 @@ def test_upper_arena_equals_full_arena_bitwise(gpu, port, tile, mesh):
         ctx.set_option("elem_tiling", 1)
         ctx.set_option("gather_lds_cols", 256)
         ctx.set_option("gather_tile", 8)
+        ctx.set_option("up_min_nodes", 0)
     for key, res in out.items():
         for x, y in zip(out[4, 0, key[2], key[3]], res):
             assert np.array_equal(x, y), key
     for c in range(4):
         f_exp = port.internal_force(m, u[c].cpu().numpy())
         assert rel_l2(out[7, 1, 256, 4][2][c * n3:(c + 1) * n3], f_exp) <= REL_K
+
+
+@pytest.mark.parametrize("up_min", [20, 0])
+@pytest.mark.parametrize("fam_i", range(4))
+def test_j2_upper_arena_equals_full_arena_bitwise(gpu, fam_i, up_min):
+    """The J2 tangent is symmetric and stored as upper node-pair blocks by asm_variant 7
+    (families of at least up_min nodes); K_t, the live flags, f_int and the trial history
+    equal asm_variant 4's bit for bit."""
+    import torch
+    from calculixpp_amd import meshgen
+    fam = _families()[fam_i]
+    m = meshgen.j2_block(4, 3, 3, (1.0, 0.8, 0.6), family=fam, elastic_every=5, jitter_frac=0.1)
+    hist = meshgen.seeded_history(m.n_points, 3)
+    u = torch.from_numpy(meshgen.smooth_field(m.coords, 7, 1.5e-4))
+    ctx = gpu.context()
+    a = gpu.TangentAssembler(m, ctx)
+    out = {}
+    try:
+        ctx.set_option("up_min_nodes", up_min)
+        for variant in (4, 7):
+            ctx.set_option("asm_variant", variant)
+            mp = gpu.MaterialPoints(m.n_points, committed=hist)
+            v, lv = a.new_values(), a.new_live()
+            f = torch.empty(3 * m.n_nodes, dtype=torch.float64, device=ctx.device)
+            a.assemble_tangent(u.to(ctx.device), mp, v, f, lv)
+            nnz = a.pattern.nnz
+            out[variant] = [v[:nnz].cpu().numpy(), lv[:nnz].cpu().numpy(), f.cpu().numpy(),
+                            mp.trial.cpu().numpy()]
+    finally:
+        ctx.set_option("asm_variant", 7)
+        ctx.set_option("up_min_nodes", 0)
+    for x, y in zip(out[4], out[7]):
+        assert np.array_equal(x, y)
