@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 (vector = matrix)
 PEAK_HBM = 8000.0   # GB/s, MI355X HBM3E
+J2_FLOP_PER_POINT = 300  # radial return, stress and compact tangent of a yielding point
 
 
 def flops_per_element(nn: int, ngp: int, residual: bool) -> float:
@@ -44,9 +45,10 @@ def bytes_per_assembly(nnz: int, n_nodes: int, n_elem: int, nn: int, residual: b
     return b + (48 * n_nodes if residual else 0)
 
 
-def load_traffic(workload: str, variant: int):
-    """HBM bytes per assembly measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    copy-calibrated, as committed in profiles/traffic.json; None if not measured."""
+def load_traffic(workload: str, variant: int, key: str = "hbm_bytes_per_step"):
+    """HBM bytes per assembly (or Newton iteration) measured by rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE), copy-calibrated, as committed in profiles/traffic.json;
+    None if not measured for this variant."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -55,7 +57,7 @@ def load_traffic(workload: str, variant: int):
         return None
     if not rec or rec.get("asm_variant") != variant:
         return None
-    return rec.get("hbm_bytes_per_step")
+    return rec.get(key)
 
 
 def adjoint_ratio(reps: int = 20) -> dict:
@@ -162,6 +164,18 @@ def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
            "kernel_ms": statistics.median(per),
            "kernels": {k: round(statistics.median(v), 4) for k, v in kern.items()},
            "elements_per_s": m.n_elem / (wall_ms * 1e-3)}
+    # roofline of the iteration's launches: K_t + f_int per element and 300 FLOP per
+    # Gauss point of return map; bytes as the assembly plus the history read and written
+    flop = m.n_elem * flops_per_element(8, 8, True) + J2_FLOP_PER_POINT * m.n_points
+    byt = bytes_per_assembly(a.pattern.nnz, m.n_nodes, m.n_elem, 8, True) + 208 * m.n_points
+    t_att = max(flop / (PEAK_FP64 * 1e12), byt / (PEAK_HBM * 1e9))
+    out["roofline"] = {"bound": "mfma" if flop / (PEAK_FP64 * 1e12) >= byt / (PEAK_HBM * 1e9)
+                       else "hbm", "flop": flop, "algorithmic_bytes": byt,
+                       "t_attainable_ms": round(t_att * 1e3, 4),
+                       "frac": round(t_att * 1e3 / out["kernel_ms"], 4),
+                       "traffic": load_traffic("J2-C3D8-128x128x64",
+                                               ctx.options.get("asm_variant"),
+                                               "hbm_bytes_per_iteration")}
     if cpu:
         from oracle.oracle import Oracle, aos_state, ref_available
         kind = "reference" if ref_available() else "port"
