@@ -586,13 +586,13 @@ __global__ __launch_bounds__(kWave, WAVES) void k_assemble_iso2(
 constexpr int kStageNodes = 4;  // element-matrix node rows staged in LDS per store turn
 typedef double f64x2 __attribute__((ext_vector_type(2)));  // one 16-byte store
 
-// J^-1 (row-major, cofactors) and sqrt(det J * w) of one Gauss point from the node
-// coordinates x[3n] and the natural derivatives d[n][3]; raises kFlagJacobian when
+// det J * w of one Gauss point from the node coordinates x[3n] and the natural derivatives
+// d[n][3], and with INV also J^-1 (row-major, cofactors) into A; raises kFlagJacobian when
 // det J <= 0 (element.cpp:86-87).
-template <int NN>
-__device__ __forceinline__ void jacobian_inverse(const double* x, const double* __restrict__ d,
-                                                 const double w, double* A, double& rs,
-                                                 int* __restrict__ flag) {
+template <int NN, bool INV>
+__device__ __forceinline__ double jacobian_detw(const double* x, const double* __restrict__ d,
+                                                const double w, double* A,
+                                                int* __restrict__ flag) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (int k = 0; k < NN; ++k) {
     const double x0 = x[3 * k], x1 = x[3 * k + 1], x2 = x[3 * k + 2];
@@ -606,17 +606,27 @@ __device__ __forceinline__ void jacobian_inverse(const double* x, const double* 
   const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
   const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
   if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
-  const double r = 1.0 / det;
-  A[0] = c00 * r;
-  A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
-  A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
-  A[3] = c01 * r;
-  A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
-  A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
-  A[6] = c02 * r;
-  A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
-  A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
-  rs = sqrt(det * w);
+  if constexpr (INV) {
+    const double r = 1.0 / det;
+    A[0] = c00 * r;
+    A[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+    A[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+    A[3] = c01 * r;
+    A[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+    A[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+    A[6] = c02 * r;
+    A[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+    A[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+  }
+  return det * w;
+}
+
+// J^-1 into A and sqrt(det J * w) of one Gauss point (see jacobian_detw).
+template <int NN>
+__device__ __forceinline__ void jacobian_inverse(const double* x, const double* __restrict__ d,
+                                                 const double w, double* A, double& rs,
+                                                 int* __restrict__ flag) {
+  rs = sqrt(jacobian_detw<NN, true>(x, d, w, A, flag));
 }
 
 template <int NN, int NGP, int QC>
@@ -2204,60 +2214,120 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
 // liveness flags give the reference's key set.
 enum : int { kMassConsistent = 0, kMassLumped = 1, kGeometric = 2 };
 
-template <int NN, int NGP>
+// LDS of k_elem_scalar: node coordinates, det J * w per point, for the mass the shape-value
+// table (read 5 times per point and lane: from LDS, not through the vector memory path)
+// and, for the geometric stiffness, J^-1 per point and the gradients of QC points at a
+// time; the element's n x n scalars.
+template <int NN, int NGP, int QC, int KIND>
 struct ScalarLds {
-  Geo<NN, NGP> G;
-  int32_t node[NN];
+  double x[NN * 3];
+  double sc[NGP];
+  double n[KIND == kGeometric ? 1 : NGP * NN];  // shape values N[q][k] (mass)
+  double Ai[KIND == kGeometric ? NGP * 9 : 1];
+  double g[KIND == kGeometric ? QC * NN * 3 : 1];
   double k[NN * NN];
   double row[NN];
 };
 
-template <int NN, int NGP, int KIND>
+// One element per wave (XCD-aware like k_elem_iso). A lane owns node a and up to four
+// consecutive upper partners b (decode_group): k_ab = rho sum_q det J w N_a N_b (mass) or
+// sum_q det J w g_a^T sigma_q g_b (geometric, with t_a = sigma_q g_a formed once per point
+// for the lane's partners); the upper values are mirrored in LDS, so k is bitwise
+// symmetric, and written row-major with coalesced streaming stores.
+template <int NN, int NGP, int QC, int KIND>
 __global__ __launch_bounds__(kWave) void k_elem_scalar(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
     const double* __restrict__ dN, const double* __restrict__ wq, const double* __restrict__ Nq,
     const double* __restrict__ coords, const double* __restrict__ rho,
-    const double* __restrict__ sig, double* __restrict__ kp, int* __restrict__ flag) {
-  __shared__ ScalarLds<NN, NGP> L;
+    const double* __restrict__ sig, double* __restrict__ kp, int* __restrict__ flag,
+    int64_t n_elem, int64_t slab) {
+  __shared__ ScalarLds<NN, NGP, QC, KIND> L;
   const int lane = threadIdx.x;
-  const int64_t le = blockIdx.x;
-  stage_coords<NN, NGP>(L.G, L.node, conn + le * NN, coords, lane);
-  __syncthreads();
-  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
-  __syncthreads();
-  if constexpr (KIND == kGeometric) {
-    gradients<NN, NGP>(L.G, dN, lane);
-    __syncthreads();
+  const int64_t le = xcd_index(blockIdx.x, n_elem, slab);
+  if (le < 0) return;
+  if (lane < NN) {
+    const double* xp = coords + 3 * int64_t(conn[le * NN + lane]);
+    L.x[3 * lane + 0] = xp[0];
+    L.x[3 * lane + 1] = xp[1];
+    L.x[3 * lane + 2] = xp[2];
   }
-  const double r = KIND == kGeometric ? 0.0 : rho[elem_offset + le];
+  if constexpr (KIND != kGeometric)
+    for (int idx = lane; idx < NGP * NN; idx += kWave) L.n[idx] = Nq[idx];
+  __syncthreads();
+  for (int q = lane; q < NGP; q += kWave)
+    L.sc[q] = jacobian_detw<NN, KIND == kGeometric>(L.x, dN + q * NN * 3, wq[q], L.Ai + 9 * q, flag);
   constexpr int NP = NN * (NN + 1) / 2;
-  for (int p = lane; p < NP; p += kWave) {
-    int a, b;
-    decode_pair<NN, true>(p, a, b);
-    double k = 0.0;
-    for (int q = 0; q < NGP; ++q) {
-      if constexpr (KIND == kGeometric) {
-        const double* s = sig + 6 * (gp_offset + le * NGP + q);
-        const double* ga = L.G.g + (q * NN + a) * 3;
-        const double* gb = L.G.g + (q * NN + b) * 3;
-        const double se = s[0] * ga[0] * gb[0] + s[1] * ga[1] * gb[1] + s[2] * ga[2] * gb[2] +
-                          s[3] * (ga[0] * gb[1] + ga[1] * gb[0]) +
-                          s[4] * (ga[0] * gb[2] + ga[2] * gb[0]) +
-                          s[5] * (ga[1] * gb[2] + ga[2] * gb[1]);
-        k += L.G.sc[q] * se;
-      } else {
-        k += r * L.G.sc[q] * Nq[q * NN + a] * Nq[q * NN + b];
-      }
-    }
-    L.k[a * NN + b] = k;
-    L.k[b * NN + a] = k;
+  constexpr int PPL = (NP + kWave - 1) / kWave;
+  int ga, gb0;
+  decode_group<NN, PPL>(lane, ga, gb0);
+  const int a = ga >= 0 ? ga : 0;  // clamped: unowned slots compute unused values
+  int b[PPL];
+  double k[PPL];
+#pragma unroll
+  for (int t = 0; t < PPL; ++t) {
+    b[t] = gb0 + t < NN ? (gb0 + t >= 0 ? gb0 + t : 0) : NN - 1;
+    k[t] = 0.0;
   }
+  __syncthreads();
+  if constexpr (KIND != kGeometric) {
+    const double r = rho[elem_offset + le];
+#pragma unroll 3
+    for (int q = 0; q < NGP; ++q) {
+      const double* n = L.n + q * NN;
+      const double c = r * L.sc[q] * n[a];
+#pragma unroll
+      for (int t = 0; t < PPL; ++t) k[t] += c * n[b[t]];
+    }
+  } else {
+#pragma unroll 1
+    for (int q0 = 0; q0 < NGP; q0 += QC) {
+      for (int idx = lane; idx < QC * NN; idx += kWave) {
+        const int q = q0 + idx / NN;
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        if (q < NGP) {
+          const double* A = L.Ai + 9 * q;
+          const double* d = dN + (q * NN + idx % NN) * 3;
+          g0 = A[0] * d[0] + A[3] * d[1] + A[6] * d[2];
+          g1 = A[1] * d[0] + A[4] * d[1] + A[7] * d[2];
+          g2 = A[2] * d[0] + A[5] * d[1] + A[8] * d[2];
+        }
+        L.g[3 * idx + 0] = g0;
+        L.g[3 * idx + 1] = g1;
+        L.g[3 * idx + 2] = g2;
+      }
+      __syncthreads();
+      const int qn = NGP - q0 < QC ? NGP - q0 : QC;
+#pragma unroll 1
+      for (int qq = 0; qq < qn; ++qq) {
+        const int q = q0 + qq;
+        const double* s = sig + 6 * (gp_offset + le * NGP + q);  // Voigt xx yy zz xy xz yz
+        const double s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3], s4 = s[4], s5 = s[5];
+        const double* gA = L.g + (qq * NN + a) * 3;
+        const double sc = L.sc[q];
+        const double t0 = sc * (s0 * gA[0] + s3 * gA[1] + s4 * gA[2]);
+        const double t1 = sc * (s3 * gA[0] + s1 * gA[1] + s5 * gA[2]);
+        const double t2 = sc * (s4 * gA[0] + s5 * gA[1] + s2 * gA[2]);
+#pragma unroll
+        for (int t = 0; t < PPL; ++t) {
+          const double* gB = L.g + (qq * NN + b[t]) * 3;
+          k[t] += t0 * gB[0] + t1 * gB[1] + t2 * gB[2];
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < PPL; ++t)
+    if (ga >= 0 && gb0 + t < NN) {
+      L.k[a * NN + gb0 + t] = k[t];
+      L.k[(gb0 + t) * NN + a] = k[t];
+    }
   __syncthreads();
   if constexpr (KIND == kMassLumped) {
-    for (int a = lane; a < NN; a += kWave) {
+    for (int r = lane; r < NN; r += kWave) {
       double sum = 0.0;
-      for (int b = 0; b < NN; ++b) sum += L.k[a * NN + b];
-      L.row[a] = sum;
+      for (int c = 0; c < NN; ++c) sum += L.k[r * NN + c];
+      L.row[r] = sum;
     }
     __syncthreads();
   }
@@ -2266,8 +2336,8 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
   for (int idx = lane; idx < NN * NN; idx += kWave) {
     double v;
     if constexpr (KIND == kMassLumped) {
-      const int a = idx / NN;
-      v = idx == a * NN + a ? L.row[a] : 0.0;
+      const int r = idx / NN;
+      v = idx == r * NN + r ? L.row[r] : 0.0;
     } else {
       v = L.k[idx];
     }
@@ -2779,14 +2849,15 @@ void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double
     const int32_t* conn = m->conn.p + B.conn_offset;
     double* kp = pm->ke_arena.p + p->kp_base[bi] / 9;  // scalar arena: n^2 per element
     LaunchTimer timer(ctx, std::string(names[kind]) + type_name(B.type), B.n_elem);
-#define LAUNCH_K(NN, NGP, KIND)                                                                 \
-  hipLaunchKernelGGL((k_elem_scalar<NN, NGP, KIND>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,  \
-                     ctx->stream, conn, B.elem_offset, B.gp_offset, T.dN, T.w, T.N, coords,     \
-                     rho, sig, kp, ctx->flag_dev)
+#define LAUNCH_K(NN, NGP, QC, KIND)                                                             \
+  hipLaunchKernelGGL((k_elem_scalar<NN, NGP, QC, KIND>), dim3(xcd_grid(ctx, B.n_elem)),         \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, B.gp_offset, T.dN, T.w,  \
+                     T.N, coords, rho, sig, kp, ctx->flag_dev, B.n_elem,                        \
+                     xcd_slab(ctx, B.n_elem))
 #define LAUNCH(NN, NGP, UP, QC)                                          \
-  if (kind == kMassConsistent) LAUNCH_K(NN, NGP, kMassConsistent);       \
-  else if (kind == kMassLumped) LAUNCH_K(NN, NGP, kMassLumped);          \
-  else LAUNCH_K(NN, NGP, kGeometric);
+  if (kind == kMassConsistent) LAUNCH_K(NN, NGP, QC, kMassConsistent);   \
+  else if (kind == kMassLumped) LAUNCH_K(NN, NGP, QC, kMassLumped);      \
+  else LAUNCH_K(NN, NGP, QC, kGeometric);
     CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
 #undef LAUNCH_K
