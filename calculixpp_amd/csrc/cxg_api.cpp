@@ -160,14 +160,16 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
   for (int32_t v : conn) ++inc_ptr[size_t(v) + 1];
   for (int32_t a = 0; a < n_nodes; ++a) inc_ptr[size_t(a) + 1] += inc_ptr[size_t(a)];
   std::vector<int32_t> inc_elem(static_cast<size_t>(n_conn));
+  std::vector<int64_t> inc_at(static_cast<size_t>(n_conn));
   {
     std::vector<int64_t> cur(inc_ptr.begin(), inc_ptr.end() - 1);
     for (int64_t e = 0; e < n_elem; ++e)
       for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
         const int32_t v = conn[size_t(k)];
-        // a node repeated inside one element is listed once
+        // a node repeated inside one element is listed once, at its first occurrence
         if (cur[size_t(v)] > inc_ptr[size_t(v)] && inc_elem[size_t(cur[size_t(v)] - 1)] == int32_t(e))
           continue;
+        inc_at[size_t(cur[size_t(v)])] = k;
         inc_elem[size_t(cur[size_t(v)]++)] = int32_t(e);
       }
     // compact (duplicates left holes at the end of a node's range)
@@ -175,11 +177,15 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
     int64_t w = 0;
     for (int32_t a = 0; a < n_nodes; ++a) {
       np[size_t(a)] = w;
-      for (int64_t i = inc_ptr[size_t(a)]; i < cur[size_t(a)]; ++i) inc_elem[size_t(w++)] = inc_elem[size_t(i)];
+      for (int64_t i = inc_ptr[size_t(a)]; i < cur[size_t(a)]; ++i, ++w) {
+        inc_elem[size_t(w)] = inc_elem[size_t(i)];
+        inc_at[size_t(w)] = inc_at[size_t(i)];
+      }
     }
     np[size_t(n_nodes)] = w;
     inc_ptr.swap(np);
     inc_elem.resize(size_t(w));
+    inc_at.resize(size_t(w));
   }
   // patches: up to patch_size consecutive elements of one block
   const int64_t P = std::max(1, ctx->patch_size);
@@ -225,14 +231,18 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
   M->eptr.alloc(eptr.size());
   M->inc_ptr.alloc(inc_ptr.size());
   M->inc_elem.alloc(std::max<size_t>(inc_elem.size(), 1));
+  M->inc_at.alloc(std::max<size_t>(inc_at.size(), 1));
   if (n_conn)
     CXG_HIP(hipMemcpyAsync(M->conn.p, conn.data(), conn.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
   CXG_HIP(hipMemcpyAsync(M->eptr.p, eptr.data(), eptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
   CXG_HIP(hipMemcpyAsync(M->inc_ptr.p, inc_ptr.data(), inc_ptr.size() * sizeof(int64_t),
                          hipMemcpyHostToDevice, s));
-  if (!inc_elem.empty())
+  if (!inc_elem.empty()) {
     CXG_HIP(hipMemcpyAsync(M->inc_elem.p, inc_elem.data(), inc_elem.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, s));
+    CXG_HIP(hipMemcpyAsync(M->inc_at.p, inc_at.data(), inc_at.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, s));
+  }
   CXG_HIP(hipStreamSynchronize(s));
 }
 

@@ -2057,7 +2057,7 @@ struct RecoverLds {
   double Ai[NGP * 9];   // J^-1 per Gauss point (row-major)
   double eps[NGP * 6];
   double sig[NGP * 6];
-  double S[NGP * 6];    // det J * w * sigma
+  double sc[NGP];       // det J * w
   int32_t node[NN];
 };
 
@@ -2068,9 +2068,16 @@ __device__ __forceinline__ void grad_at(const double* A, const double* d, double
   g[2] = A[2] * d[0] + A[5] * d[1] + A[8] * d[2];
 }
 
+// Elements per wave of the recovery kernel: one lane per (element, Gauss point) and per
+// (element, node) (2 C3D20, 3 C3D20R, 8 C3D8, 10 C3D6).
+constexpr int rec_epw(int nn, int ngp) {
+  return (64 / ngp < 64 / nn ? 64 / ngp : 64 / nn) > 0 ? (64 / ngp < 64 / nn ? 64 / ngp : 64 / nn) : 1;
+}
+
 // J2: the radial return for elements with a J2 material (else every element elastic).
 // Gradients are formed on the fly from J^-1 and the natural derivatives, which keeps the
-// LDS at the element's coordinates, displacements, J^-1 and the point fields.
+// LDS at the elements' coordinates, displacements, J^-1 and the point fields. EPW
+// consecutive elements per wave.
 template <int NN, int NGP, bool J2>
 __global__ __launch_bounds__(kWave) void k_gauss_fields(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
@@ -2079,128 +2086,124 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
     const double* __restrict__ nu, const double* __restrict__ u,
     const int32_t* __restrict__ mat_id, const J2Dev* __restrict__ mats,
     const double* __restrict__ committed, double* __restrict__ emean,
-    double* __restrict__ fe, double* __restrict__ gauss_stress, int* __restrict__ flag) {
-  __shared__ RecoverLds<NN, NGP> L;
+    double* __restrict__ fe, double* __restrict__ gauss_stress, int* __restrict__ flag,
+    int64_t n_elem, int64_t slab) {
+  constexpr int EPW = rec_epw(NN, NGP);
+  __shared__ RecoverLds<NN, NGP> L[EPW];
   const int lane = threadIdx.x;
-  const int64_t le = blockIdx.x;
-  const int64_t eg = elem_offset + le;
-  const int32_t* ce = conn + le * NN;
-  if (lane < NN) {
-    const int32_t nd = ce[lane];
-    L.node[lane] = nd;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  for (int idx = lane; idx < ne * NN; idx += kWave) {
+    const int e = idx / NN, k = idx - e * NN;
+    const int32_t nd = conn[(le0 + e) * NN + k];
+    L[e].node[k] = nd;
     for (int c = 0; c < 3; ++c) {
-      L.x[3 * lane + c] = coords[3 * int64_t(nd) + c];
-      L.u[3 * lane + c] = u[3 * int64_t(nd) + c];
+      L[e].x[3 * k + c] = coords[3 * int64_t(nd) + c];
+      L[e].u[3 * k + c] = u[3 * int64_t(nd) + c];
     }
   }
   __syncthreads();
-  double lam, mu;
-  lame(E[eg], nu[eg], lam, mu);
-  const int32_t mid = J2 ? mat_id[eg] : -1;
-  for (int q = lane; q < NGP; q += kWave) {
+  static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
+  if (lane < ne * NGP) {
+    const int e = lane / NGP, q = lane - e * NGP;
+    RecoverLds<NN, NGP>& R = L[e];
+    const int64_t le = le0 + e, eg = elem_offset + le;
     const double* d = dN + q * NN * 3;
-    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int k = 0; k < NN; ++k)
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) J[i][j] += L.x[3 * k + i] * d[3 * k + j];
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    if (!(det > 0.0)) atomicOr(flag, kFlagJacobian);
-    const double r = 1.0 / det;
-    double A[9] = {c00 * r, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r,
-                   (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r, c01 * r,
-                   (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r,
-                   (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r, c02 * r,
-                   (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r,
-                   (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r};
+    double A[9];
+    const double sc = jacobian_detw<NN, true>(R.x, d, wq[q], A, flag);
     double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < NN; ++k) {
       double g[3];
       grad_at(A, d + 3 * k, g);
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) H[i][j] += L.u[3 * k + i] * g[j];
+        for (int j = 0; j < 3; ++j) H[i][j] += R.u[3 * k + i] * g[j];
     }
-    double e[6], sg[6];
-    voigt_strain(H, e);
+    double e6[6], sg[6];
+    voigt_strain(H, e6);
     const int64_t pt = gp_offset + le * NGP + q;
+    const int32_t mid = J2 ? mat_id[eg] : -1;
     if (J2 && mid >= 0) {
       // the committed history is read, never advanced (the reference evaluates a copy)
       double hc[13], ht[13];
       for (int c = 0; c < 13; ++c) hc[c] = committed[c * n_points + pt];
       J2Tangent T;
-      j2_return(mats[mid], e, hc, ht, sg, T);
+      j2_return(mats[mid], e6, hc, ht, sg, T);
     } else {
-      iso_stress(lam, mu, e, sg);
+      double lam, mu;
+      lame(E[eg], nu[eg], lam, mu);
+      iso_stress(lam, mu, e6, sg);
     }
-    const double sc = det * wq[q];
-    for (int i = 0; i < 9; ++i) L.Ai[9 * q + i] = A[i];
+    for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
     for (int i = 0; i < 6; ++i) {
-      L.eps[6 * q + i] = e[i];
-      L.sig[6 * q + i] = sg[i];
-      L.S[6 * q + i] = sc * sg[i];
+      R.eps[6 * q + i] = e6[i];
+      R.sig[6 * q + i] = sg[i];
     }
+    R.sc[q] = sc;
     if (gauss_stress)
       for (int i = 0; i < 6; ++i) gauss_stress[6 * pt + i] = sg[i];
   }
   __syncthreads();
-  if (lane < 12) {  // Gauss-point means: sum in point order, times 1/NGP
-    const double* v = lane < 6 ? L.eps + lane : L.sig + (lane - 6);
+  for (int idx = lane; idx < ne * 12; idx += kWave) {
+    // Gauss-point means: sum in point order, times 1/NGP
+    const int e = idx / 12, c = idx - e * 12;
+    const double* v = c < 6 ? L[e].eps + c : L[e].sig + (c - 6);
     double m = 0.0;
     for (int q = 0; q < NGP; ++q) m += v[6 * q];
-    emean[12 * eg + lane] = m * (1.0 / NGP);
+    emean[12 * (elem_offset + le0 + e) + c] = m * (1.0 / NGP);
   }
-  for (int k = lane; k < NN; k += kWave) {
+  if (lane < ne * NN) {
+    const int e = lane / NN, k = lane - e * NN;
+    const RecoverLds<NN, NGP>& R = L[e];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = 0; q < NGP; ++q) {
       double g[3];
-      grad_at(L.Ai + 9 * q, dN + (q * NN + k) * 3, g);
-      const double* sq = L.S + 6 * q;
-      f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
-      f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
-      f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
+      grad_at(R.Ai + 9 * q, dN + (q * NN + k) * 3, g);
+      const double* sg = R.sig + 6 * q;
+      const double sc = R.sc[q];
+      const double s0 = sc * sg[0], s1 = sc * sg[1], s2 = sc * sg[2];
+      const double s3 = sc * sg[3], s4 = sc * sg[4], s5 = sc * sg[5];
+      f0 += g[0] * s0 + g[1] * s3 + g[2] * s4;
+      f1 += g[1] * s1 + g[0] * s3 + g[2] * s5;
+      f2 += g[2] * s2 + g[0] * s4 + g[1] * s5;
     }
-    double* fp = fe + 3 * (le * NN + k);
+    double* fp = fe + 3 * ((le0 + e) * NN + k);
     fp[0] = f0;
     fp[1] = f1;
     fp[2] = f2;
   }
 }
 
-// Phase 2: one thread per node. fe holds the element forces at 3*(conn offset + local
-// node) + component, as the J2 element-force arena.
+// Phase 2: 16 lanes per node (4 nodes per wave). fe holds the element forces at
+// 3*(conn offset) + component, as the J2 element-force arena; inc_at gives the node's conn
+// offset in each incident element. Lanes 0-11 sum the incident elements' Gauss-point
+// means of one component (strain, then stress), lanes 12-14 one component of the force, in
+// ascending element order; the means are divided by the number of incident elements.
 __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
                               const int32_t* __restrict__ inc_elem,
-                              const int64_t* __restrict__ eptr, const int32_t* __restrict__ conn,
+                              const int64_t* __restrict__ inc_at,
                               const double* __restrict__ emean, const double* __restrict__ fe,
                               const double* __restrict__ f_ext, double* __restrict__ strain,
                               double* __restrict__ stress, double* __restrict__ reaction,
                               double* __restrict__ f_int) {
-  const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n_nodes) return;
-  double m[12], f[3] = {0.0, 0.0, 0.0};
-  for (int c = 0; c < 12; ++c) m[c] = 0.0;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int32_t a = int32_t(t / 16);
+  const int c = int(t % 16);
+  if (a >= n_nodes || c >= 15) return;
   const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
-  for (int64_t it = i0; it < i1; ++it) {
-    const int32_t e = inc_elem[it];
-    for (int c = 0; c < 12; ++c) m[c] += emean[12 * int64_t(e) + c];
-    const int64_t b = eptr[e], n = eptr[e + 1] - b;
-    int64_t la = 0;
-    while (la < n && conn[b + la] != a) ++la;
-    if (la < n)  // always: the incidence lists hold only elements containing the node
-      for (int c = 0; c < 3; ++c) f[c] += fe[3 * (b + la) + c];
-  }
-  const int64_t cnt = i1 - i0;
-  if (cnt > 0)
-    for (int c = 0; c < 12; ++c) m[c] /= double(cnt);
-  for (int c = 0; c < 6; ++c) {
-    if (strain) strain[6 * int64_t(a) + c] = m[c];
-    if (stress) stress[6 * int64_t(a) + c] = m[6 + c];
-  }
-  for (int c = 0; c < 3; ++c) {
-    if (f_int) f_int[3 * int64_t(a) + c] = f[c];
-    if (reaction) reaction[3 * int64_t(a) + c] = f[c] - (f_ext ? f_ext[3 * int64_t(a) + c] : 0.0);
+  double v = 0.0;
+  for (int64_t it = i0; it < i1; ++it)
+    v += c < 12 ? emean[12 * int64_t(inc_elem[it]) + c] : fe[3 * inc_at[it] + (c - 12)];
+  if (c < 12) {
+    const int64_t cnt = i1 - i0;
+    if (cnt > 0) v /= double(cnt);
+    double* dst = c < 6 ? strain : stress;
+    if (dst) dst[6 * int64_t(a) + (c < 6 ? c : c - 6)] = v;
+  } else {
+    const int64_t k = 3 * int64_t(a) + (c - 12);
+    if (f_int) f_int[k] = v;
+    if (reaction) reaction[k] = v - (f_ext ? f_ext[k] : 0.0);
   }
 }
 
@@ -2886,10 +2889,12 @@ void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, c
     double* feb = m->rec_fe.p + 3 * B.conn_offset;
     LaunchTimer timer(ctx, "gauss_fields", B.n_elem);
 #define LAUNCH_R(NN, NGP, J2)                                                                   \
-  hipLaunchKernelGGL((k_gauss_fields<NN, NGP, J2>), dim3(unsigned(B.n_elem)), dim3(kWave), 0,   \
-                     ctx->stream, conn, B.elem_offset, B.gp_offset, m->n_points, T.dN, T.w,     \
-                     coords, E, nu, u, mat_id, dmats, committed, m->rec_mean.p, feb,            \
-                     gauss_stress, ctx->flag_dev)
+  hipLaunchKernelGGL((k_gauss_fields<NN, NGP, J2>),                                             \
+                     dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, B.gp_offset,             \
+                     m->n_points, T.dN, T.w, coords, E, nu, u, mat_id, dmats, committed,        \
+                     m->rec_mean.p, feb, gauss_stress, ctx->flag_dev, B.n_elem,                 \
+                     xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
 #define LAUNCH(NN, NGP, UP, QC)                                  \
   if (dmats && mat_id) LAUNCH_R(NN, NGP, true);                  \
   else LAUNCH_R(NN, NGP, false);
@@ -2900,9 +2905,9 @@ void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, c
   }
   if (m->n_nodes > 0 && (strain || stress || reaction || f_int)) {
     LaunchTimer timer(ctx, "node_fields", m->n_nodes);
-    hipLaunchKernelGGL(k_node_fields, dim3(unsigned((m->n_nodes + 255) / 256)), dim3(256), 0,
-                       ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_elem.p, m->eptr.p,
-                       m->conn.p, m->rec_mean.p, m->rec_fe.p, f_ext, strain, stress, reaction,
+    hipLaunchKernelGGL(k_node_fields, dim3(unsigned((int64_t(m->n_nodes) * 16 + 255) / 256)),
+                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_elem.p,
+                       m->inc_at.p, m->rec_mean.p, m->rec_fe.p, f_ext, strain, stress, reaction,
                        f_int);
     CXG_HIP(hipGetLastError());
   }

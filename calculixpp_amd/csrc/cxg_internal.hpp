@@ -172,6 +172,8 @@ struct cxg_mesh {
   cxg::DevBuf<int64_t> eptr;      // model element -> offset into conn (n_elem+1)
   cxg::DevBuf<int64_t> inc_ptr;   // node -> incident elements (n_nodes+1)
   cxg::DevBuf<int32_t> inc_elem;  // incident model element numbers, ascending
+  cxg::DevBuf<int64_t> inc_at;    // per incidence: conn offset of the node in the element
+                                  // (its first occurrence)
   int32_t max_candidates = 0;     // max over nodes of sum of incident element sizes
   // Field recovery arenas, allocated on first use: per element the Gauss-point means of
   // strain and stress (12), per element node its force vector (3, at 3 * conn offset).
