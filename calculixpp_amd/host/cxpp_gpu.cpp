@@ -92,10 +92,33 @@ int gauss_points(ElementType t) {
   return n;
 }
 
-// A model's active elements on the device, as runs of one family in model order.
+// The mesh part of a Model as arrays (coordinates, element runs, elasticity, active mask);
+// a node id the mesh does not know becomes -1, an error only for an active element.
+ArrayModel mesh_arrays(const Model& model) {
+  const Mesh& m = model.mesh;
+  ArrayModel a;
+  a.coords.resize(m.num_nodes() * kDof);
+  for (size_t i = 0; i < m.num_nodes(); ++i)
+    for (int k = 0; k < kDof; ++k) a.coords[kDof * i + size_t(k)] = m.nodes()[i].x[size_t(k)];
+  const std::vector<ElasticIso> el = model.element_elastic();
+  a.active = model.element_active_mask();
+  a.E.resize(m.num_elements());
+  a.nu.resize(m.num_elements());
+  for (size_t e = 0; e < m.num_elements(); ++e) {
+    const Element& elem_e = m.elements()[e];
+    if (a.blocks.empty() || a.blocks.back().type != elem_e.type)
+      a.blocks.push_back(ArrayModel::Block{elem_e.type, {}});
+    for (Index id : elem_e.nodes) a.blocks.back().conn.push_back(m.node_index(id));
+    a.E[e] = el[e].E;
+    a.nu[e] = el[e].nu;
+  }
+  return a;
+}
+
+// An array model's active elements on the device, as runs of one family in model order.
 struct DeviceMesh {
   cxg_context* c;
-  std::vector<size_t> elem;       // reference element index of each device element
+  std::vector<size_t> elem;       // model element index of each device element
   std::vector<int64_t> gp_first;  // first device Gauss point of each device element
   int32_t n_nodes = 0;
   int64_t n_points = 0;
@@ -103,48 +126,55 @@ struct DeviceMesh {
   std::unique_ptr<Buf<int32_t>> conn;
   cxg_mesh* mesh = nullptr;
 
-  DeviceMesh(cxg_context* ctx, const Model& model) : c(ctx) {
-    const Mesh& m = model.mesh;
-    n_nodes = static_cast<int32_t>(m.num_nodes());
-    std::vector<double> x(m.num_nodes() * kDof);
-    for (size_t i = 0; i < m.num_nodes(); ++i)
-      for (int k = 0; k < kDof; ++k) x[kDof * i + size_t(k)] = m.nodes()[i].x[size_t(k)];
-    const std::vector<ElasticIso> el = model.element_elastic();
-    const std::vector<bool> active = model.element_active_mask();
+  DeviceMesh(cxg_context* ctx, const ArrayModel& am) : c(ctx) {
+    const size_t ne_all = am.num_elements();
+    if (am.coords.size() % kDof != 0) throw std::runtime_error("cxpp::gpu: coords must hold 3 values per node");
+    if (am.E.size() != ne_all || am.nu.size() != ne_all)
+      throw std::runtime_error("cxpp::gpu: E and nu must hold one value per element");
+    if (!am.active.empty() && am.active.size() != ne_all)
+      throw std::runtime_error("cxpp::gpu: active must hold one flag per element");
+    if (am.num_nodes() > size_t(INT32_MAX)) throw std::runtime_error("cxpp::gpu: too many nodes");
+    n_nodes = static_cast<int32_t>(am.num_nodes());
     std::vector<int32_t> cn;
     std::vector<double> ev, nv;
     std::vector<cxg_block> blocks;
     std::vector<size_t> block_conn;
-    for (size_t e = 0; e < m.num_elements(); ++e) {
-      if (!active[e]) continue;  // *MODEL CHANGE, REMOVE: no stiffness (assembly.cpp:238)
-      const Element& elem_e = m.elements()[e];
-      if (!on_gpu_path(elem_e.type))
-        throw std::runtime_error("cxpp::gpu: element type outside the GPU path (C3D8, C3D20, "
-                                 "C3D20R, C3D6)");
-      const int32_t t = static_cast<int32_t>(elem_e.type);
-      if (blocks.empty() || blocks.back().type != t) {
-        blocks.push_back(cxg_block{t, 0, int64_t(elem.size()), n_points, nullptr});
-        block_conn.push_back(cn.size());
+    size_t e = 0;
+    for (const ArrayModel::Block& B : am.blocks) {
+      const size_t nn = size_t(nodes_per_element(B.type));
+      if (nn == 0 || B.conn.size() % nn != 0)
+        throw std::runtime_error("cxpp::gpu: connectivity does not match the element type");
+      for (size_t k = 0; k < B.conn.size(); k += nn, ++e) {
+        if (!am.active.empty() && !am.active[e]) continue;  // *MODEL CHANGE, REMOVE (assembly.cpp:238)
+        if (!on_gpu_path(B.type))
+          throw std::runtime_error("cxpp::gpu: element type outside the GPU path (C3D8, C3D20, "
+                                   "C3D20R, C3D6)");
+        const int32_t t = static_cast<int32_t>(B.type);
+        if (blocks.empty() || blocks.back().type != t) {
+          blocks.push_back(cxg_block{t, 0, int64_t(elem.size()), n_points, nullptr});
+          block_conn.push_back(cn.size());
+        }
+        ++blocks.back().n_elem;
+        for (size_t i = 0; i < nn; ++i) {
+          const Index ni = B.conn[k + i];
+          if (ni < 0 || ni >= n_nodes) throw std::runtime_error("element references unknown node");
+          cn.push_back(ni);
+        }
+        elem.push_back(e);
+        gp_first.push_back(n_points);
+        n_points += gauss_points(B.type);
+        ev.push_back(am.E[e]);
+        nv.push_back(am.nu[e]);
       }
-      ++blocks.back().n_elem;
-      for (Index id : elem_e.nodes) {
-        const Index ni = m.node_index(id);
-        if (ni < 0) throw std::runtime_error("element references unknown node");
-        cn.push_back(ni);
-      }
-      elem.push_back(e);
-      gp_first.push_back(n_points);
-      n_points += gauss_points(elem_e.type);
-      ev.push_back(el[e].E);
-      nv.push_back(el[e].nu);
     }
-    coords = std::make_unique<Buf<double>>(c, x);
+    coords = std::make_unique<Buf<double>>(c, am.coords);
     conn = std::make_unique<Buf<int32_t>>(c, cn);
     E = std::make_unique<Buf<double>>(c, ev);
     nu = std::make_unique<Buf<double>>(c, nv);
     for (size_t b = 0; b < blocks.size(); ++b) blocks[b].conn = conn->get() + block_conn[b];
     check(c, cxg_mesh_create(c, n_nodes, int32_t(blocks.size()), blocks.data(), &mesh));
   }
+  DeviceMesh(cxg_context* ctx, const Model& model) : DeviceMesh(ctx, mesh_arrays(model)) {}
   DeviceMesh(const DeviceMesh&) = delete;
   DeviceMesh& operator=(const DeviceMesh&) = delete;
   ~DeviceMesh() {
@@ -206,6 +236,53 @@ void dof_map(const Model& model, fem::LinearSystem& sys) {
     if (!single)
       throw std::runtime_error("cxpp::gpu: *EQUATION / MPC / rigid / tie constraints are "
                                "outside the GPU path");
+  }
+}
+
+// build_dof_map for single point constraints only: free equations in ascending global DOF
+// order, constrained DOFs -1 with their prescribed value; the transform expands a free
+// DOF to itself and a constrained one to its value (constraint_transform.cpp:25-131 for
+// a model without equations).
+void dof_map(const ArrayModel& am, fem::LinearSystem& sys) {
+  const size_t n_dof = am.num_nodes() * kDof;
+  sys.prescribed.assign(n_dof, 0.0);
+  sys.prescribed_amp.assign(n_dof, std::string{});
+  std::vector<bool> constrained(n_dof, false);
+  for (const ArrayModel::NodalValue& spc : am.spc) {
+    if (spc.node < 0 || size_t(spc.node) >= am.num_nodes())
+      throw std::runtime_error("*BOUNDARY references unknown node");
+    if (spc.comp < 1 || spc.comp > kDof) throw std::runtime_error("*BOUNDARY dof out of range");
+    const size_t g = size_t(spc.node) * kDof + size_t(spc.comp - 1);
+    constrained[g] = true;
+    sys.prescribed[g] = spc.value;
+  }
+  fem::ConstraintTransform& T = sys.transform;
+  T.dof_eq.assign(n_dof, -1);
+  T.expansion.assign(n_dof, fem::DofExpansion{});
+  T.is_slave.assign(n_dof, false);
+  Index n_free = 0;
+  for (size_t g = 0; g < n_dof; ++g) {
+    if (constrained[g]) {
+      T.expansion[g].constant = sys.prescribed[g];
+    } else {
+      T.dof_eq[g] = n_free;
+      T.expansion[g].terms.push_back(fem::DofTerm{n_free, 1.0});
+      ++n_free;
+    }
+  }
+  T.n_free = n_free;
+  sys.dof_eq = T.dof_eq;
+  sys.n_free = n_free;
+  sys.rhs.assign(size_t(n_free), 0.0);
+}
+
+void add_external_load(const ArrayModel& am, fem::LinearSystem& sys) {
+  for (const ArrayModel::NodalValue& f : am.load) {
+    if (f.node < 0 || size_t(f.node) >= am.num_nodes())
+      throw std::runtime_error("*CLOAD references unknown node");
+    if (f.comp < 1 || f.comp > kDof) throw std::runtime_error("*CLOAD dof out of range");
+    const Index r = sys.dof_eq[size_t(f.node) * kDof + size_t(f.comp - 1)];
+    if (r >= 0) sys.rhs[size_t(r)] += f.value;
   }
 }
 
@@ -453,6 +530,61 @@ std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u)
   return fd.fetch(uh.size());
 }
 
+size_t ArrayModel::num_elements() const {
+  size_t n = 0;
+  for (const Block& b : blocks) {
+    const int nn = nodes_per_element(b.type);
+    n += nn > 0 ? b.conn.size() / size_t(nn) : 0;
+  }
+  return n;
+}
+
+ArrayModel to_arrays(const Model& model) {
+  ArrayModel a = mesh_arrays(model);
+  const Mesh& mesh = model.mesh;
+  for (const ArrayModel::Block& b : a.blocks)
+    for (Index ni : b.conn)
+      if (ni < 0) throw std::runtime_error("element references unknown node");
+  a.rho = model.element_density();
+  for (const Spc& spc : model.spcs) {
+    const Index ni = mesh.node_index(spc.node_id);
+    if (ni < 0) throw std::runtime_error("*BOUNDARY references unknown node");
+    a.spc.push_back(ArrayModel::NodalValue{ni, spc.comp, spc.value});
+  }
+  const std::vector<Real> f = fem::external_load_vector(model);
+  for (size_t g = 0; g < f.size(); ++g)
+    if (f[g] != 0.0)
+      a.load.push_back(ArrayModel::NodalValue{Index(g / kDof), int(g % kDof) + 1, f[g]});
+  return a;
+}
+
+fem::LinearSystem assemble_linear_static(const ArrayModel& model) {
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
+  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
+  Buf<double> vals(c, nnz);
+  Buf<uint8_t> live(c, nnz);
+  check(c, cxg_assemble_linear(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(), vals.get(),
+                               nullptr, nullptr, live.get()));
+  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  add_external_load(model, sys);
+  return sys;
+}
+
+std::vector<Real> internal_force(const ArrayModel& model, const std::vector<Vec3>& u) {
+  if (u.size() != model.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  const std::vector<double> uh = flat(u);
+  Buf<double> ud(c, uh), fd(c, uh.size());
+  check(c, cxg_internal_force(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(), ud.get(),
+                              fd.get()));
+  return fd.fetch(uh.size());
+}
+
 std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
                                     const ElasticIso& elastic) {
   if (!on_gpu_path(type))
@@ -576,21 +708,18 @@ namespace {
 
 // assemble_mass / assemble_geometric_stiffness: the k_ab I3 matrices on the all-free
 // pattern, reduced like the stiffness with the eliminated-column load discarded.
-fem::LinearSystem assemble_scalar(const Model& model, int kind,
+// sys holds the DOF map; rho per model element (mass) or gp_stress per model element
+// (geometric stiffness).
+fem::LinearSystem assemble_scalar(fem::LinearSystem sys, cxg_context* c, const DeviceMesh& dm,
+                                  int kind, const std::vector<Real>& rho,
                                   const std::vector<std::vector<Voigt6>>* gp_stress) {
-  const Mesh& mesh = model.mesh;
-  fem::LinearSystem sys;
-  dof_map(model, sys);
-  cxg_context* c = context();
-  const DeviceMesh dm(c, model);
   const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
   const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
   Buf<double> vals(c, nnz);
   Buf<uint8_t> live(c, nnz);
   if (kind < 2) {
-    const std::vector<Real> rho = model.element_density();
     std::vector<double> r(std::max<size_t>(dm.elem.size(), 1), 0.0);
-    for (size_t k = 0; k < dm.elem.size(); ++k) r[k] = rho[dm.elem[k]];
+    for (size_t k = 0; k < dm.elem.size(); ++k) r[k] = dm.elem[k] < rho.size() ? rho[dm.elem[k]] : 0.0;
     Buf<double> rd(c, r);
     check(c, cxg_assemble_mass(c, pat.p, dm.coords->get(), rd.get(), kind == 1, vals.get(),
                                live.get()));
@@ -601,7 +730,8 @@ fem::LinearSystem assemble_scalar(const Model& model, int kind,
       const size_t e = dm.elem[k];
       if (e >= gp_stress->size()) continue;
       const std::vector<Voigt6>& es = (*gp_stress)[e];
-      const size_t ng = size_t(gauss_points(mesh.elements()[e].type));
+      const size_t ng = size_t(k + 1 < dm.elem.size() ? dm.gp_first[k + 1] : dm.n_points) -
+                        size_t(dm.gp_first[k]);
       for (size_t q = 0; q < ng && q < es.size(); ++q)
         for (size_t i = 0; i < 6; ++i) g[6 * (size_t(dm.gp_first[k]) + q) + i] = es[q][i];
     }
@@ -619,13 +749,29 @@ fem::LinearSystem assemble_mass(const Model& model, bool lumped) {
   require_mechanical(model);
   if (!model.point_masses.empty())
     throw std::runtime_error("cxpp::gpu: *MASS point masses are outside the GPU path");
-  return assemble_scalar(model, lumped ? 1 : 0, nullptr);
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  return assemble_scalar(std::move(sys), c, dm, lumped ? 1 : 0, model.element_density(), nullptr);
+}
+
+fem::LinearSystem assemble_mass(const ArrayModel& model, bool lumped) {
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  return assemble_scalar(std::move(sys), c, dm, lumped ? 1 : 0, model.rho, nullptr);
 }
 
 fem::LinearSystem assemble_geometric_stiffness(const Model& model,
                                                const std::vector<std::vector<Voigt6>>& gp_stress) {
   require_mechanical(model);
-  return assemble_scalar(model, 2, &gp_stress);
+  fem::LinearSystem sys;
+  dof_map(model, sys);
+  cxg_context* c = context();
+  const DeviceMesh dm(c, model);
+  return assemble_scalar(std::move(sys), c, dm, 2, {}, &gp_stress);
 }
 
 }  // namespace cxpp::gpu

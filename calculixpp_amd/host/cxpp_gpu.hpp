@@ -52,12 +52,54 @@ namespace cxpp::gpu {
 // HIP device of the calling thread's context (takes effect on the next call).
 void set_device(int device);
 
+// Array-native model of the GPU path: what a cxpp::Model carries for it, as arrays, so
+// that a mesh with per-element elasticity at scale (262k elements, each its own E) needs
+// no *MATERIAL and *ELSET per element. It replaces the section -> material resolution of
+// Model::element_elastic / element_density / element_active_mask
+// (src/core/model.cpp:33-72, 200-207, 267-280) and the *BOUNDARY / *CLOAD lists of the
+// deck (io/inp_parser.cpp:1631-1645); node and element numbers are indices (node ids of
+// a Model map to their insertion order, mesh.cpp:29-32).
+struct ArrayModel {
+  struct Block {
+    ElementType type{};
+    std::vector<Index> conn;  // node indices, nodes_per_element(type) per element, CalculiX order
+  };
+  struct NodalValue {
+    Index node{0};  // node index
+    int comp{1};    // 1..3
+    Real value{0.0};
+  };
+  std::vector<Real> coords;     // x, y, z per node
+  std::vector<Block> blocks;    // runs of one family; element numbers follow block order
+  std::vector<Real> E, nu;      // per element
+  std::vector<Real> rho;        // per element (*DENSITY); empty: 0
+  std::vector<bool> active;     // per element (*MODEL CHANGE, REMOVE); empty: all active
+  std::vector<NodalValue> spc;  // single point constraints (prescribed values)
+  std::vector<NodalValue> load; // nodal loads
+  size_t num_nodes() const { return coords.size() / 3; }
+  size_t num_elements() const;
+};
+
+// The arrays of a reference Model: coordinates in node order, element runs in model
+// order, each element's elasticity and density through its section, the active mask,
+// the *BOUNDARY values and the nonzero entries of fem::external_load_vector. Throws
+// std::runtime_error with the reference's messages (unknown node, elset, material).
+ArrayModel to_arrays(const Model& model);
+
 fem::LinearSystem assemble_linear_static(const Model& model);
 
 fem::LinearSystem assemble_material_tangent(const Model& model, const std::vector<Vec3>& u,
                                             fem::MaterialPoints& mp, std::vector<Real>& f_int);
 
 std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u);
+
+// The same three entry points on an ArrayModel: single point constraints only, numbered
+// as build_dof_map does (ascending global DOF, constrained DOFs skipped); the transform of
+// the returned system is the SPC-only one (a free DOF expands to itself, a constrained DOF
+// to its prescribed value).
+fem::LinearSystem assemble_linear_static(const ArrayModel& model);
+std::vector<Real> internal_force(const ArrayModel& model, const std::vector<Vec3>& u);
+fem::LinearSystem assemble_mass(const ArrayModel& model, bool lumped = false);
 
 std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
                                     const ElasticIso& elastic);

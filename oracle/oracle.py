@@ -231,6 +231,38 @@ class Oracle:
                 self._f("system_free")(h)
         return self._with_model(model, run)
 
+    # ---- cxpp::gpu::ArrayModel entry points (drop-in build only) ----------------------
+    def _array_system(self, model, fn):
+        arrays = _ModelArrays(model)
+        h = C.c_void_p()
+        self._check(fn(C.byref(arrays.struct), C.byref(h)))
+        try:
+            nf, rows, cols, vals, rhs = self._system(h)
+            return System(nf, rows, cols, vals, rhs, self._dof_eq(h, model.n_nodes))
+        finally:
+            self._f("system_free")(h)
+
+    def array_assemble_linear_static(self, model, via_model: bool = False) -> System:
+        """cxpp::gpu::assemble_linear_static on an ArrayModel built straight from the arrays
+        (via_model: on cxpp::gpu::to_arrays of the reference Model built from them)."""
+        f = self.L.ref_gpu_array_linear_static
+        f.argtypes = [C.POINTER(OrcModel), C.c_int, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
+        return self._array_system(model, lambda m, h: f(m, 0 if via_model else 1, h, self._err, 512))
+
+    def array_assemble_mass(self, model, lumped: bool = False) -> System:
+        f = self.L.ref_gpu_array_mass
+        f.argtypes = [C.POINTER(OrcModel), C.c_int, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
+        return self._array_system(model, lambda m, h: f(m, int(lumped), h, self._err, 512))
+
+    def array_internal_force(self, model, u):
+        arrays = _ModelArrays(model)
+        uu = np.ascontiguousarray(u, np.float64).ravel()
+        out = np.zeros(3 * model.n_nodes)
+        f = self.L.ref_gpu_array_internal_force
+        f.argtypes = [C.POINTER(OrcModel), C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        self._check(f(C.byref(arrays.struct), _p(uu), _p(out), self._err, 512))
+        return out
+
     def assemble_material_tangent(self, model, u, state):
         """state: (n_points, 26) AoS history, updated in place copy; returns
         (System, f_int, new_state)."""

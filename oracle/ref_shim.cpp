@@ -389,6 +389,70 @@ void ref_system_dof_eq(const ref_system* s, int32_t* dof_eq) {
 void ref_system_free(ref_system* s) { delete s; }
 
 #ifdef CXPP_DROPIN
+// The orc_model arrays as a cxpp::gpu::ArrayModel: no Model, no materials or elsets.
+static cxpp::gpu::ArrayModel array_model(const orc_model& m) {
+  cxpp::gpu::ArrayModel a;
+  a.coords.assign(m.coords, m.coords + 3 * int64_t(m.n_nodes));
+  for (int32_t e = 0; e < m.n_elem; ++e) {
+    const ElementType t = static_cast<ElementType>(m.etype[e]);
+    if (a.blocks.empty() || a.blocks.back().type != t) a.blocks.push_back({t, {}});
+    a.blocks.back().conn.insert(a.blocks.back().conn.end(), m.conn + m.eptr[e], m.conn + m.eptr[e + 1]);
+  }
+  a.E.assign(m.E, m.E + m.n_elem);
+  a.nu.assign(m.nu, m.nu + m.n_elem);
+  if (m.rho) a.rho.assign(m.rho, m.rho + m.n_elem);
+  if (m.active) a.active.assign(m.active, m.active + m.n_elem);
+  for (int32_t i = 0; i < m.n_spc; ++i) a.spc.push_back({m.spc_node[i], m.spc_comp[i], m.spc_value[i]});
+  for (int32_t i = 0; i < m.n_cload; ++i)
+    a.load.push_back({m.cload_node[i], m.cload_comp[i], m.cload_value[i]});
+  return a;
+}
+
+// cxpp::gpu::assemble_linear_static on the ArrayModel of the arrays (arrays = 1) or on
+// to_arrays of the reference Model built from them (arrays = 0).
+int ref_gpu_array_linear_static(const orc_model* m, int arrays, ref_system** out, char* err,
+                                int errlen) {
+  try {
+    auto* s = new ref_system();
+    if (arrays) {
+      s->sys = cxpp::gpu::assemble_linear_static(array_model(*m));
+    } else {
+      const std::unique_ptr<Model> M = build_model(*m);
+      s->sys = cxpp::gpu::assemble_linear_static(cxpp::gpu::to_arrays(*M));
+    }
+    *out = s;
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int ref_gpu_array_internal_force(const orc_model* m, const double* u, double* f_int, char* err,
+                                 int errlen) {
+  try {
+    const std::vector<Real> f =
+        cxpp::gpu::internal_force(array_model(*m), as_vec3(u, size_t(m->n_nodes)));
+    std::memcpy(f_int, f.data(), f.size() * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int ref_gpu_array_mass(const orc_model* m, int lumped, ref_system** out, char* err, int errlen) {
+  try {
+    auto* s = new ref_system();
+    s->sys = cxpp::gpu::assemble_mass(array_model(*m), lumped != 0);
+    *out = s;
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
 // COO system -> cxpp::gpu::solve_sparse (drop-in build only).
 int ref_solve_sparse(int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
                      const double* vals, const double* rhs, double* x, char* err, int errlen) {

@@ -190,3 +190,77 @@ def test_mass_and_geometric(pair, fam_i):
     same_system(gpu.assemble_mass(m), ref.assemble_mass(m))
     same_system(gpu.assemble_mass(m, True), ref.assemble_mass(m, True))
     same_system(gpu.assemble_geometric_stiffness(m, gs), ref.assemble_geometric_stiffness(m, gs))
+
+
+def _bitwise(a, b):
+    assert a.n_free == b.n_free and np.array_equal(a.dof_eq, b.dof_eq)
+    for x, y in zip(a.csr(), b.csr()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.rhs, b.rhs)
+
+
+def _loaded_mixed_model():
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, hexes = meshgen.hex_grid(4, 3, 2, (1.0, 0.75, 0.5), _lib.C3D8)
+    coords = meshgen.jitter(coords, 0.25, 0.1, 213, (1.0, 0.75, 0.5))
+    wed = np.concatenate([hexes[12:][:, [0, 1, 2, 4, 5, 6]], hexes[12:][:, [0, 2, 3, 4, 6, 7]]])
+    rng = np.random.default_rng(214)
+    x0 = np.nonzero(coords[:, 0] < 1e-12)[0]
+    cl = rng.choice(len(coords), 5, replace=False)
+    n_el = 12 + len(wed)
+    return Model(coords, [(_lib.C3D8, hexes[:12]), (_lib.C3D6, wed)],
+                 rng.uniform(1e5, 2e5, n_el), rng.uniform(0.25, 0.35, n_el),
+                 spc_node=np.concatenate([x0, x0[:3]]),
+                 spc_comp=np.concatenate([np.full(len(x0), 1), np.full(3, 2)]),
+                 spc_value=rng.uniform(-1e-3, 1e-3, len(x0) + 3),
+                 cload_node=cl, cload_comp=rng.integers(1, 4, 5), cload_value=rng.normal(size=5),
+                 rho=rng.uniform(1e-9, 8e-9, n_el))
+
+
+@pytest.mark.parametrize("case", ["loaded_mixed", "c3d20", "c3d20r"])
+def test_array_model_ingestion(pair, case):
+    """cxpp::gpu::ArrayModel: per-element E, nu and density given as arrays (no material
+    or elset per element) against the reference on the same model built with one
+    material per element; to_arrays of that reference Model reproduces the array path
+    bit for bit, and both equal the Model-path drop-in bit for bit."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    ref, gpu = pair
+    if case == "loaded_mixed":
+        m = _loaded_mixed_model()
+    else:
+        fam = _lib.C3D20 if case == "c3d20" else _lib.C3D20R
+        b = meshgen.block(3, fam, seed_geo=221, seed_E=222, ny=2, nz=2)
+        x0 = np.nonzero(b.coords[:, 0] < 1e-12)[0]
+        m = Model(b.coords, b.blocks, b.E, b.nu, spc_node=x0, spc_comp=np.full(len(x0), 3),
+                  spc_value=np.full(len(x0), -2e-4),
+                  rho=np.random.default_rng(223).uniform(1e-9, 8e-9, b.n_elem))
+    arr = gpu.array_assemble_linear_static(m)
+    same_system(arr, ref.assemble_linear_static(m))
+    _bitwise(arr, gpu.array_assemble_linear_static(m, via_model=True))
+    _bitwise(arr, gpu.assemble_linear_static(m))
+    u = meshgen.smooth_field(m.coords, 224)
+    assert rel(gpu.array_internal_force(m, u), ref.internal_force(m, u)) <= REL
+    for lumped in (False, True):
+        same_system(gpu.array_assemble_mass(m, lumped), ref.assemble_mass(m, lumped))
+
+
+def test_array_model_errors(pair):
+    """Unknown nodes and out-of-range constraint components raise the reference's texts."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    from oracle.oracle import OracleError
+    ref, gpu = pair
+    coords, conn = meshgen.hex_grid(2, 1, 1, (1.0, 0.5, 0.5), _lib.C3D8)
+    m = Model(coords, [(_lib.C3D8, conn)], 210000.0, 0.3, spc_node=np.array([0], np.int32),
+              spc_comp=np.array([4], np.int32), spc_value=np.zeros(1))
+    with pytest.raises(OracleError, match=r"\*BOUNDARY dof out of range"):
+        gpu.array_assemble_linear_static(m)
+    bad = conn.copy()
+    bad[1, 3] = len(coords)  # a node index past the last node
+    m = Model.__new__(Model)
+    m.__dict__.update(Model(coords, [(_lib.C3D8, conn)], 210000.0, 0.3).__dict__)
+    m.blocks = [(_lib.C3D8, bad)]
+    with pytest.raises(OracleError, match="element references unknown node"):
+        gpu.array_assemble_linear_static(m)
