@@ -1521,10 +1521,11 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
 // Phase 1, one lane per Gauss point: Jacobian, strain (gradients formed on the fly),
 // radial return (j2_return), trial history to the SoA state, and the compact tangent
 // (lam, mu, c, n) and rs*sigma to LDS. Both the per-family and the persistent kernel run
-// EPW elements per wave here, so that most lanes carry a Gauss point (8 C3D8 elements of
-// 8 points, 2 C3D20 of 27, 4 C3D20R of 8, 24 C3D6 of 2; 8 C3D8 measured faster than 4,
-// whose smaller LDS record raises the occupancy; C3D20R and C3D6 stay below the C3D20
-// record, which sets the LDS of the persistent kernel's union).
+// EPW elements per wave here, so that most lanes carry a Gauss point (6 C3D8 elements of
+// 8 points, 2 C3D20 of 27, 4 C3D20R of 8, 24 C3D6 of 2; for C3D8, 6 measured faster than
+// 8 and 4 (see j2_epw); C3D20R and C3D6 stay below the C3D20 record, which sets the LDS
+// of the persistent kernel's union). The coordinates and displacements, read only here,
+// share the LDS with phase 2's workspace.
 // Phase 2, one element at a time on the whole wave: QC Gauss points at a time,
 // h = rs g and t = B(h)^T n per node go to LDS, and each lane accumulates its upper
 // node-pair blocks
@@ -1533,36 +1534,48 @@ __global__ __launch_bounds__(kWave) void k_assemble_j2(
 // element force f_a += B(h_a)^T (rs sigma). K_t is written as full rows through the
 // staged, coalesced stores of k_elem_iso; f_e (3n) to its own arena. The row gather
 // then sums K_t rows and f_e entries per node in element order.
+// What phase 2 needs of an element (kept from phase 1) ...
 template <int NN, int NGP>
 struct J2Elem {
-  double x[NN * 3];
-  double u[NN * 3];
   double Ai[NGP * 9];
   double rs[NGP];
   double T[NGP * 9];  // lam, mu, c, n[6] per Gauss point
   double S[NGP * 6];  // rs * sigma
+};
+// ... and what only phase 1 reads: it shares the LDS with phase 2's workspace.
+template <int NN>
+struct J2Geo {
+  double x[NN * 3];
+  double u[NN * 3];
 };
 
 template <int NN, int NGP, int QC, int EPW>
 struct ElemJ2Lds {
   J2Elem<NN, NGP> e[EPW];
   union {
+    J2Geo<NN> g[EPW];  // phase 1
     struct {
       double h[QC * NN * 3];
       double t[QC * NN * 3];
-    } c;
+    } c;               // phase 2
     alignas(16) double stage[kStageNodes * 3 * 3 * NN];
   } w;
 };
 
-// Elements per wave of the J2 kernels (see above).
+// Resident waves per SIMD the J2 kernel is compiled for: the 8-node kernel fits 3 (168
+// VGPRs, 2 spilled), the others keep the compiler's choice.
+constexpr int j2_min_waves(int nn) { return nn == 8 ? 3 : 1; }
+
+// Elements per wave of the J2 kernels (see above). C3D8: 6, whose 12.7 KB of LDS lets the
+// 3 waves per SIMD of its registers reside (8 per wave, 15.9 KB, allow 10 waves per CU:
+// 2.43 against 2.22 ms at 128x128x64).
 constexpr int j2_epw(int nn, int ngp) {
-  return ngp >= 27 ? 2 : (nn == 20 ? 4 : (64 / ngp > 24 ? 24 : 64 / ngp));
+  return ngp >= 27 ? 2 : (nn == 20 ? 4 : (nn == 8 ? 6 : (64 / ngp > 24 ? 24 : 64 / ngp)));
 }
 
 // Node k of an element into its LDS record.
-template <int NN, int NGP>
-__device__ __forceinline__ void j2_stage(J2Elem<NN, NGP>& P, const int32_t* __restrict__ ce,
+template <int NN>
+__device__ __forceinline__ void j2_stage(J2Geo<NN>& P, const int32_t* __restrict__ ce,
                                          const double* __restrict__ coords,
                                          const double* __restrict__ u, int k) {
   const int64_t n3 = 3 * int64_t(ce[k]);
@@ -1573,18 +1586,19 @@ __device__ __forceinline__ void j2_stage(J2Elem<NN, NGP>& P, const int32_t* __re
   }
 }
 
-// Phase 1 for Gauss point q (x, u staged). mat == nullptr: isotropic elastic with
+// Phase 1 for Gauss point q (x, u staged in G). mat == nullptr: isotropic elastic with
 // Ee, nue. pt: the point's index in the SoA state.
 template <int NN, int NGP>
 __device__ __forceinline__ void j2_gauss_point(
-    J2Elem<NN, NGP>& P, const int q, const double Ee, const double nue, const J2Dev* mat,
+    J2Elem<NN, NGP>& P, const J2Geo<NN>& G, const int q, const double Ee, const double nue,
+    const J2Dev* mat,
     const int64_t pt, const int64_t n_points, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ committed,
     double* __restrict__ trial, int* __restrict__ flag) {
   const double* d = dN + q * NN * 3;
   double* A = P.Ai + 9 * q;
   double rs;
-  jacobian_inverse<NN>(P.x, d, wq[q], A, rs, flag);
+  jacobian_inverse<NN>(G.x, d, wq[q], A, rs, flag);
   P.rs[q] = rs;
   double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (int k = 0; k < NN; ++k) {
@@ -1594,7 +1608,7 @@ __device__ __forceinline__ void j2_gauss_point(
     const double g2 = A[2] * d0 + A[5] * d1 + A[8] * d2;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const double v = P.u[3 * k + i];
+      const double v = G.u[3 * k + i];
       H[i][0] += v * g0;
       H[i][1] += v * g1;
       H[i][2] += v * g2;
@@ -1779,14 +1793,14 @@ __device__ __forceinline__ void j2_group(
     double* __restrict__ fe, int* __restrict__ flag) {
   for (int idx = lane; idx < ne * NN; idx += kWave) {
     const int e = idx / NN;
-    j2_stage<NN, NGP>(L.e[e], conn + (le0 + e) * NN, coords, u, idx - e * NN);
+    j2_stage<NN>(L.w.g[e], conn + (le0 + e) * NN, coords, u, idx - e * NN);
   }
   __syncthreads();
   for (int idx = lane; idx < ne * NGP; idx += kWave) {
     const int e = idx / NGP, q = idx - e * NGP;
     const int64_t ge = elem_offset + le0 + e;
     const int32_t mid = mat_id ? mat_id[ge] : -1;
-    j2_gauss_point<NN, NGP>(L.e[e], q, E[ge], nu[ge], mid >= 0 ? mats + mid : nullptr,
+    j2_gauss_point<NN, NGP>(L.e[e], L.w.g[e], q, E[ge], nu[ge], mid >= 0 ? mats + mid : nullptr,
                             gp_offset + (le0 + e) * NGP + q, n_points, dN, wq, committed, trial,
                             flag);
   }
@@ -1799,7 +1813,7 @@ __device__ __forceinline__ void j2_group(
 
 // EPW elements per wave (see j2_group). UP: upper-block arena (asm_variant 7).
 template <int NN, int NGP, int QC, bool UP = false>
-__global__ __launch_bounds__(kWave) void k_elem_j2(
+__global__ __launch_bounds__(kWave, j2_min_waves(NN)) void k_elem_j2(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t n_elem, int64_t gp_offset,
     int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
