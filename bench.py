@@ -330,6 +330,59 @@ def recover_leg(model, u, family: int, reps: int = 5, cpu_n: int = 16, cpu: bool
     return out
 
 
+def pcg_leg(n: int = 64, iters: int = 96) -> dict:
+    """The device solve (cxg_pcg, Jacobi-preconditioned CG) on the n^3 C3D20 block of the
+    assembly workload, clamped on x = 0 with a z load on x = 1: `iters` iterations (a
+    multiple of the 16 between host residual checks) timed by HIP events. Bytes per
+    iteration: the CSR product (8 + 4 per nonzero, 8 per row) and the vector work (x, r,
+    z, p, q, 1/diag: 128 per row). No CPU baseline: the reference solves with SciPP,
+    which is not available."""
+    import numpy as np
+    import torch
+    from calculixpp_amd import _lib, fem, meshgen, solve
+    from calculixpp_amd.model import Model
+    b = meshgen.block(n, _lib.C3D20)
+    x0 = np.nonzero(b.coords[:, 0] < 1e-12)[0].astype(np.int32)
+    x1 = np.nonzero(b.coords[:, 0] > 1.0 - 1e-12)[0].astype(np.int32)
+    m = Model(b.coords, b.blocks, b.E, b.nu, spc_node=np.repeat(x0, 3),
+              spc_comp=np.tile(np.arange(1, 4, dtype=np.int32), len(x0)),
+              spc_value=np.zeros(3 * len(x0)), cload_node=x1,
+              cload_comp=np.full(len(x1), 3, np.int32), cload_value=np.full(len(x1), -1e-3))
+    ctx = fem.context()
+    a = fem.Assembler(m, ctx)
+    vals = a.new_values()
+    a.assemble(vals)
+    rhs = a.rhs()
+    nnz, nf = a.pattern.nnz, a.n_free
+    rp, ci, va = a.pattern.row_ptr, a.pattern.col_idx, vals[:nnz]
+    torch.cuda.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    rel = None
+    try:
+        _, _, rel = solve.pcg(ctx, rp, ci, va, rhs, tol=0.0, max_iter=iters)
+    except _lib.CxgError as e:  # tol 0 is never reached: x holds the last iterate
+        if "did not reach" not in str(e):
+            raise
+    ctx.profile(False)
+    st = ctx.profile_stats()
+    ms_iter = st["pcg_iterations"][1] / iters
+    byt = 12 * nnz + 8 * (nf + 1) + 128 * nf
+    t_att = byt / (PEAK_HBM * 1e9)
+    out = {"workload": f"{n}^3 C3D20 block clamped on x = 0, z load on x = 1: Jacobi-PCG",
+           "n_free": nf, "nnz": nnz, "iterations": iters, "ms_per_iteration": round(ms_iter, 4),
+           "setup_ms": round(st["pcg_setup"][1], 4),
+           "roofline": {"bound": "hbm", "algorithmic_bytes_per_iteration": byt,
+                        "t_attainable_ms": round(t_att * 1e3, 4),
+                        "frac": round(t_att * 1e3 / ms_iter, 4),
+                        "achieved_GBps": round(byt / (ms_iter * 1e-3) / 1e9, 1)},
+           "cpu_baseline": None,
+           "note": "the reference's solve (SciPP spsolve / cg) is absent: no CPU baseline"}
+    del a, vals
+    torch.cuda.empty_cache()
+    return out
+
+
 def mass_geometric_leg(asm, family: int, reps: int = 3, cpu_n: int = 16, cpu: bool = True) -> dict:
     """Consistent mass, lumped mass and geometric stiffness into the CSR values of the
     assembly workload's pattern (density 7.85e-9, seeded Gauss-point stress), from HIP
@@ -424,10 +477,11 @@ def main() -> int:
     ap.add_argument("--no-recover", action="store_true", help="skip the field-recovery leg")
     ap.add_argument("--no-mass", action="store_true",
                     help="skip the mass / geometric-stiffness leg")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the device-solve leg")
     ap.add_argument("--batch-meshes", type=int, default=512)
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="extra cxg_set_option for the library context (repeatable)")
-    ap.add_argument("--only", choices=["j2", "batch", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["j2", "batch", "adjoint", "pcg"], default=None,
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
     ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
@@ -456,6 +510,8 @@ def main() -> int:
                 fem.context().set_option(key, int(val))
         if args.only == "j2":
             res = j2_newton(reps=args.steps, cpu=not args.no_cpu)
+        elif args.only == "pcg":
+            res = pcg_leg(args.n)
         elif args.only == "batch":
             res = batch_step(args.batch_meshes, reps=args.steps, cpu=not args.no_cpu,
                              log=lambda m: print(m, file=sys.stderr, flush=True))
@@ -576,6 +632,11 @@ def main() -> int:
         print(f"[bench] mass {mass['mass']['ms']:.3f} ms, lumped {mass['mass_lumped']['ms']:.3f} "
               f"ms, geometric {mass['geometric']['ms']:.3f} ms", file=sys.stderr, flush=True)
 
+    pcg = None if args.no_pcg else pcg_leg(args.n)
+    if pcg:
+        print(f"[bench] pcg {pcg['ms_per_iteration']:.3f} ms per iteration "
+              f"({pcg['roofline']['achieved_GBps']:.0f} GB/s)", file=sys.stderr, flush=True)
+
     batch = None if args.no_batch else batch_step(
         args.batch_meshes, cpu=not args.no_cpu, log=lambda m: print(m, file=sys.stderr, flush=True))
     if batch:
@@ -608,6 +669,7 @@ def main() -> int:
         "j2_newton": j2,
         "recover": rec,
         "mass_geometric": mass,
+        "pcg": pcg,
         "batch": batch,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
