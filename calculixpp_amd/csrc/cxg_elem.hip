@@ -1249,51 +1249,6 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
                                              ue, ue_stride, up_min);
 }
 
-// ---- K5: matrix-free internal force sum_e Ke ue ------------------------------
-template <int NN, int NGP>
-struct ForceLds {
-  Geo<NN, NGP> G;
-  double u[NN * 3];
-  double S[NGP * 6];
-  int32_t node[NN];
-};
-
-template <int NN, int NGP>
-__global__ __launch_bounds__(kWave) void k_internal_force(
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
-    const int32_t* __restrict__ pptr, int64_t elem_offset,
-    const double* __restrict__ dN, const double* __restrict__ wq,
-    const double* __restrict__ coords, const double* __restrict__ E,
-    const double* __restrict__ nu, const double* __restrict__ u, double* __restrict__ f_int,
-    int* __restrict__ flag) {
-  __shared__ ForceLds<NN, NGP> L;
-  const int lane = threadIdx.x;
-  const int32_t it_end = pptr[blockIdx.x + 1];
-  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
-  const int64_t le = elist[it];
-  const int32_t* ce = conn + le * NN;
-  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
-  stage_vec<NN>(L.u, ce, u, lane);
-  __syncthreads();
-  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
-  __syncthreads();
-  gradients<NN, NGP>(L.G, dN, lane);
-  __syncthreads();
-  double lam, mu;
-  lame(E[elem_offset + le], nu[elem_offset + le], lam, mu);
-  for (int q = lane; q < NGP; q += kWave) {
-    double H[3][3], e[6], s[6];
-    grad_field<NN, NGP>(L.G, L.u, q, H);
-    voigt_strain(H, e);
-    iso_stress(lam, mu, e, s);
-    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = L.G.sc[q] * s[i];
-  }
-  __syncthreads();
-  node_forces<NN, NGP>(L.G, L.S, L.node, f_int, lane);
-  __syncthreads();
-  }
-}
-
 // ---- K2: J2 radial return + consistent tangent + internal force --------------
 // Material constants prepared on the host (J2PlasticMaterial ctor,
 // src/fem/plasticity.cpp:42-65): kinematic modulus derived from the table slope
@@ -1963,98 +1918,6 @@ __global__ void k_reduce_free(int64_t n_dof, const int32_t* __restrict__ dof_eq,
   }
 }
 
-// ---- K6: adjoint of r = K(x, E) u -------------------------------------------
-// Per Gauss point, with s = det J * w, sigma = D eps:
-//   phi_q = s sigma_u : H_w             (w . r restricted to the element)
-//   dphi/dx_m = sum_q P_q g_m,  P_q = phi_q I - s (sigma_u H_w + sigma_w H_u)^T
-//   dphi/dE   = phi / E,   dphi/du_m = sum_q s B_m^T sigma_w
-// B is recomputed at the Gauss points (never stored), exactly as the forward pass.
-template <int NN, int NGP>
-struct VjpLds {
-  Geo<NN, NGP> G;
-  double u[NN * 3];
-  double w[NN * 3];
-  double P[NGP * 9];
-  double S[NGP * 6];
-  double phi[NGP];
-  int32_t node[NN];
-};
-
-template <int NN, int NGP>
-__global__ __launch_bounds__(kWave) void k_vjp(
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ elist,
-    const int32_t* __restrict__ pptr, int64_t elem_offset,
-    const double* __restrict__ dN, const double* __restrict__ wq,
-    const double* __restrict__ coords, const double* __restrict__ E,
-    const double* __restrict__ nu, const double* __restrict__ u, const double* __restrict__ w,
-    double* __restrict__ dx, double* __restrict__ dE, double* __restrict__ du,
-    int* __restrict__ flag) {
-  __shared__ VjpLds<NN, NGP> L;
-  const int lane = threadIdx.x;
-  const int32_t it_end = pptr[blockIdx.x + 1];
-  for (int32_t it = pptr[blockIdx.x]; it < it_end; ++it) {
-  const int64_t le = elist[it];
-  const int64_t ge = elem_offset + le;
-  const int32_t* ce = conn + le * NN;
-  stage_coords<NN, NGP>(L.G, L.node, ce, coords, lane);
-  stage_vec<NN>(L.u, ce, u, lane);
-  stage_vec<NN>(L.w, ce, w, lane);
-  __syncthreads();
-  jacobians<NN, NGP>(L.G, dN, wq, lane, flag);
-  __syncthreads();
-  gradients<NN, NGP>(L.G, dN, lane);
-  __syncthreads();
-  const double Ee = E[ge];
-  double lam, mu;
-  lame(Ee, nu[ge], lam, mu);
-  for (int q = lane; q < NGP; q += kWave) {
-    double Hu[3][3], Hw[3][3], eu[6], ew[6], su[6], sw[6];
-    grad_field<NN, NGP>(L.G, L.u, q, Hu);
-    grad_field<NN, NGP>(L.G, L.w, q, Hw);
-    voigt_strain(Hu, eu);
-    voigt_strain(Hw, ew);
-    iso_stress(lam, mu, eu, su);
-    iso_stress(lam, mu, ew, sw);
-    const double s = L.G.sc[q];
-    double pq = 0.0;
-    for (int i = 0; i < 6; ++i) pq += ew[i] * su[i];
-    pq *= s;
-    const double Su[3][3] = {{su[0], su[3], su[4]}, {su[3], su[1], su[5]}, {su[4], su[5], su[2]}};
-    const double Sw[3][3] = {{sw[0], sw[3], sw[4]}, {sw[3], sw[1], sw[5]}, {sw[4], sw[5], sw[2]}};
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        double m = 0.0;
-        for (int l = 0; l < 3; ++l) m += Su[i][l] * Hw[l][j] + Sw[i][l] * Hu[l][j];
-        // P[j][i] = pq delta_ij - s M[i][j]
-        L.P[9 * q + 3 * j + i] = (i == j ? pq : 0.0) - s * m;
-      }
-    for (int i = 0; i < 6; ++i) L.S[6 * q + i] = s * sw[i];
-    L.phi[q] = pq;
-  }
-  __syncthreads();
-  if (lane == 0) {
-    double phi = 0.0;
-    for (int q = 0; q < NGP; ++q) phi += L.phi[q];
-    dE[ge] = phi / Ee;
-  }
-  for (int k = lane; k < NN; k += kWave) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int q = 0; q < NGP; ++q) {
-      const double* g = L.G.g + (q * NN + k) * 3;
-      const double* P = L.P + 9 * q;
-      a0 += P[0] * g[0] + P[1] * g[1] + P[2] * g[2];
-      a1 += P[3] * g[0] + P[4] * g[1] + P[5] * g[2];
-      a2 += P[6] * g[0] + P[7] * g[1] + P[8] * g[2];
-    }
-    double* xp = dx + 3 * int64_t(L.node[k]);
-    xp[0] += a0;
-    xp[1] += a1;
-    xp[2] += a2;
-  }
-  node_forces<NN, NGP>(L.G, L.S, L.node, du, lane);
-  __syncthreads();
-  }
-}
 
 // ---- K7: field recovery (stress.cpp:147-353) ----------------------------------------
 // Phase 1, one wave per element: Gauss-point strain (B u, engineering shear) and stress
@@ -2159,7 +2022,7 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
       for (int i = 0; i < 6; ++i) gauss_stress[6 * pt + i] = sg[i];
   }
   __syncthreads();
-  for (int idx = lane; idx < ne * 12; idx += kWave) {
+  for (int idx = lane; emean && idx < ne * 12; idx += kWave) {
     // Gauss-point means: sum in point order, times 1/NGP
     const int e = idx / 12, c = idx - e * 12;
     const double* v = c < 6 ? L[e].eps + c : L[e].sig + (c - 6);
@@ -2218,6 +2081,141 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
     const int64_t k = 3 * int64_t(a) + (c - 12);
     if (f_int) f_int[k] = v;
     if (reaction) reaction[k] = v - (f_ext ? f_ext[k] : 0.0);
+  }
+}
+
+// Sums of per-element node vectors over each node's incident elements, in ascending
+// element order: out0[3a + c] = sum_e a0[3 * inc_at + c] (and out1 from a1). 8 lanes per
+// node: lanes 0-2 the components of out0, 3-5 those of out1.
+__global__ void k_node_sum(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
+                           const int64_t* __restrict__ inc_at, const double* __restrict__ a0,
+                           const double* __restrict__ a1, double* __restrict__ out0,
+                           double* __restrict__ out1) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int32_t a = int32_t(t / 8);
+  const int c = int(t % 8);
+  if (a >= n_nodes || c >= 6 || (c >= 3 && !a1)) return;
+  const double* src = c < 3 ? a0 : a1;
+  const int cc = c < 3 ? c : c - 3;
+  double v = 0.0;
+  for (int64_t it = inc_ptr[a]; it < inc_ptr[a + 1]; ++it) v += src[3 * inc_at[it] + cc];
+  (c < 3 ? out0 : out1)[3 * int64_t(a) + cc] = v;
+}
+
+// ---- K6: adjoint of r = K(x, E) u -------------------------------------------
+// Per Gauss point, with s = det J * w, sigma = D eps:
+//   phi_q = s sigma_u : H_w             (w . r restricted to the element)
+//   dphi/dx_m = sum_q P_q g_m,  P_q = phi_q I - s (sigma_u H_w + sigma_w H_u)^T
+//   dphi/dE   = phi / E,   dphi/du_m = sum_q s B_m^T sigma_w
+// B is recomputed at the Gauss points (never stored), exactly as the forward pass. EPW
+// elements per wave as in the recovery kernel: one lane per (element, Gauss point), then
+// one per (element, node) writing the element's dx and du vectors to arenas (3 per
+// connectivity entry) that k_node_sum adds up per node in ascending element order.
+template <int NN, int NGP>
+struct VjpLds {
+  double x[NN * 3];
+  double u[NN * 3];
+  double w[NN * 3];
+  double Ai[NGP * 9];  // J^-1 per Gauss point
+  double P[NGP * 9];
+  double S[NGP * 6];   // s sigma_w
+  double phi[NGP];
+};
+
+template <int NN, int NGP>
+__global__ __launch_bounds__(kWave) void k_vjp(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, const double* __restrict__ u,
+    const double* __restrict__ w, double* __restrict__ dxe, double* __restrict__ dE,
+    double* __restrict__ due, int* __restrict__ flag, int64_t n_elem, int64_t slab) {
+  constexpr int EPW = rec_epw(NN, NGP);
+  static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
+  __shared__ VjpLds<NN, NGP> L[EPW];
+  const int lane = threadIdx.x;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  for (int idx = lane; idx < ne * NN; idx += kWave) {
+    const int e = idx / NN, k = idx - e * NN;
+    const int64_t nd = conn[(le0 + e) * NN + k];
+    for (int c = 0; c < 3; ++c) {
+      L[e].x[3 * k + c] = coords[3 * nd + c];
+      L[e].u[3 * k + c] = u[3 * nd + c];
+      L[e].w[3 * k + c] = w[3 * nd + c];
+    }
+  }
+  __syncthreads();
+  if (lane < ne * NGP) {
+    const int e = lane / NGP, q = lane - e * NGP;
+    VjpLds<NN, NGP>& R = L[e];
+    const int64_t ge = elem_offset + le0 + e;
+    const double* d = dN + q * NN * 3;
+    double A[9];
+    const double s = jacobian_detw<NN, true>(R.x, d, wq[q], A, flag);
+    double Hu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Hw[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < NN; ++k) {
+      double g[3];
+      grad_at(A, d + 3 * k, g);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          Hu[i][j] += R.u[3 * k + i] * g[j];
+          Hw[i][j] += R.w[3 * k + i] * g[j];
+        }
+    }
+    double eu[6], ew[6], su[6], sw[6], lam, mu;
+    lame(E[ge], nu[ge], lam, mu);
+    voigt_strain(Hu, eu);
+    voigt_strain(Hw, ew);
+    iso_stress(lam, mu, eu, su);
+    iso_stress(lam, mu, ew, sw);
+    double pq = 0.0;
+    for (int i = 0; i < 6; ++i) pq += ew[i] * su[i];
+    pq *= s;
+    const double Su[3][3] = {{su[0], su[3], su[4]}, {su[3], su[1], su[5]}, {su[4], su[5], su[2]}};
+    const double Sw[3][3] = {{sw[0], sw[3], sw[4]}, {sw[3], sw[1], sw[5]}, {sw[4], sw[5], sw[2]}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double m = 0.0;
+        for (int l = 0; l < 3; ++l) m += Su[i][l] * Hw[l][j] + Sw[i][l] * Hu[l][j];
+        // P[j][i] = pq delta_ij - s M[i][j]
+        R.P[9 * q + 3 * j + i] = (i == j ? pq : 0.0) - s * m;
+      }
+    for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
+    for (int i = 0; i < 6; ++i) R.S[6 * q + i] = s * sw[i];
+    R.phi[q] = pq;
+  }
+  __syncthreads();
+  if (lane < ne) {
+    double phi = 0.0;
+    for (int q = 0; q < NGP; ++q) phi += L[lane].phi[q];
+    const int64_t ge = elem_offset + le0 + lane;
+    dE[ge] = phi / E[ge];
+  }
+  if (lane < ne * NN) {
+    const int e = lane / NN, k = lane - e * NN;
+    const VjpLds<NN, NGP>& R = L[e];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      double g[3];
+      grad_at(R.Ai + 9 * q, dN + (q * NN + k) * 3, g);
+      const double* P = R.P + 9 * q;
+      a0 += P[0] * g[0] + P[1] * g[1] + P[2] * g[2];
+      a1 += P[3] * g[0] + P[4] * g[1] + P[5] * g[2];
+      a2 += P[6] * g[0] + P[7] * g[1] + P[8] * g[2];
+      const double* S = R.S + 6 * q;
+      f0 += g[0] * S[0] + g[1] * S[3] + g[2] * S[4];
+      f1 += g[1] * S[1] + g[0] * S[3] + g[2] * S[5];
+      f2 += g[2] * S[2] + g[0] * S[4] + g[1] * S[5];
+    }
+    const int64_t o = 3 * ((le0 + e) * NN + k);
+    dxe[o] = a0;
+    dxe[o + 1] = a1;
+    dxe[o + 2] = a2;
+    due[o] = f0;
+    due[o + 1] = f1;
+    due[o + 2] = f2;
   }
 }
 
@@ -2584,19 +2582,36 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
   });
 }
 
+// Matrix-free f_int = sum_e Ke ue (stress.cpp:125-145): the element forces
+// sum_q det J w B^T sigma of k_gauss_fields (means off) into the element-force arena,
+// then summed per node in ascending element order.
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
                            const double* E, const double* nu, const double* u, double* f_int) {
-  fill_f64(ctx->stream, f_int, int64_t(m->n_nodes) * 3, 0.0);
-  for_each_colour_block(ctx, "internal_force", m, [&](const LaunchSet& S) {
-    const ElemTable& T = ctx->tables[S.B.type];
-    const int32_t* conn = m->conn.p + S.B.conn_offset;
-#define LAUNCH(NN, NGP, UP, QC)                                                               \
-  hipLaunchKernelGGL((k_internal_force<NN, NGP>), dim3(S.patches), dim3(kWave), 0,            \
-                     ctx->stream, conn, S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords,  \
-                     E, nu, u, f_int, ctx->flag_dev)
-    CXG_DISPATCH(S.B.type, LAUNCH)
+  m->rec_fe.ensure(std::max<size_t>(3 * m->conn.n, 1));
+  for (const cxg_block_dev& B : m->blocks) {
+    if (B.n_elem == 0) continue;
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+    double* feb = m->rec_fe.p + 3 * B.conn_offset;
+    LaunchTimer timer(ctx, std::string("internal_force_") + type_name(B.type), B.n_elem);
+#define LAUNCH(NN, NGP, UP, QC)                                                                 \
+  hipLaunchKernelGGL((k_gauss_fields<NN, NGP, false>),                                          \
+                     dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, B.gp_offset,             \
+                     m->n_points, T.dN, T.w, coords, E, nu, u, nullptr, nullptr, nullptr,       \
+                     nullptr, feb, nullptr, ctx->flag_dev, B.n_elem,                            \
+                     xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
+    CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
-  });
+    CXG_HIP(hipGetLastError());
+  }
+  if (m->n_nodes > 0) {
+    LaunchTimer timer(ctx, "internal_force_nodes", m->n_nodes);
+    hipLaunchKernelGGL(k_node_sum, dim3(unsigned((int64_t(m->n_nodes) * 8 + 255) / 256)),
+                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_at.p,
+                       m->rec_fe.p, nullptr, f_int, nullptr);
+    CXG_HIP(hipGetLastError());
+  }
 }
 
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
@@ -2930,18 +2945,31 @@ void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, c
 void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
                 const double* nu, const double* u, const double* w, double* dx, double* dE,
                 double* du) {
-  fill_f64(ctx->stream, dx, int64_t(m->n_nodes) * 3, 0.0);
-  fill_f64(ctx->stream, du, int64_t(m->n_nodes) * 3, 0.0);
-  for_each_colour_block(ctx, "vjp", m, [&](const LaunchSet& S) {
-    const ElemTable& T = ctx->tables[S.B.type];
-    const int32_t* conn = m->conn.p + S.B.conn_offset;
+  m->rec_fe.ensure(std::max<size_t>(3 * m->conn.n, 1));
+  m->vjp_dx.ensure(std::max<size_t>(3 * m->conn.n, 1));
+  for (const cxg_block_dev& B : m->blocks) {
+    if (B.n_elem == 0) continue;
+    const ElemTable& T = ctx->tables[B.type];
+    const int32_t* conn = m->conn.p + B.conn_offset;
+    LaunchTimer timer(ctx, std::string("vjp_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_vjp<NN, NGP>), dim3(S.patches), dim3(kWave), 0, ctx->stream, conn,      \
-                     S.elist, S.pptr, S.B.elem_offset, T.dN, T.w, coords, E, nu, u, w, dx, dE,  \
-                     du, ctx->flag_dev)
-    CXG_DISPATCH(S.B.type, LAUNCH)
+  hipLaunchKernelGGL((k_vjp<NN, NGP>),                                                          \
+                     dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
+                     nu, u, w, m->vjp_dx.p + 3 * B.conn_offset, dE,                             \
+                     m->rec_fe.p + 3 * B.conn_offset, ctx->flag_dev, B.n_elem,                  \
+                     xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
+    CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
-  });
+    CXG_HIP(hipGetLastError());
+  }
+  if (m->n_nodes > 0) {
+    LaunchTimer timer(ctx, "vjp_nodes", m->n_nodes);
+    hipLaunchKernelGGL(k_node_sum, dim3(unsigned((int64_t(m->n_nodes) * 8 + 255) / 256)),
+                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_at.p,
+                       m->vjp_dx.p, m->rec_fe.p, dx, du);
+    CXG_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace cxg

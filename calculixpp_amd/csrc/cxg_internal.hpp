@@ -179,6 +179,8 @@ struct cxg_mesh {
   // strain and stress (12), per element node its force vector (3, at 3 * conn offset).
   mutable cxg::DevBuf<double> rec_mean;
   mutable cxg::DevBuf<double> rec_fe;
+  // The adjoint's per-element dx vectors (3 per connectivity entry; du goes to rec_fe).
+  mutable cxg::DevBuf<double> vjp_dx;
 };
 
 namespace cxg {
