@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 (vector = matrix)
 PEAK_HBM = 8000.0   # GB/s, MI355X HBM3E
 J2_FLOP_PER_POINT = 300  # radial return, stress and compact tangent of a yielding point
+ADJ_FWD_FLOP_C3D8 = 5586  # matrix-free r = K(x, rho) u per C3D8 element
 
 
 def flops_per_element(nn: int, ngp: int, residual: bool) -> float:
@@ -91,9 +92,18 @@ def adjoint_ratio(reps: int = 20) -> dict:
             ctx.profile(False)
             acc.append(sum(v[1] for k, v in ctx.profile_stats().items()))
     f_ms, v_ms = statistics.median(fwd), statistics.median(vjp)
+    # forward roofline: 5,586 FLOP per C3D8 element (Jacobian, gradients, strain, stress,
+    # B^T sigma); bytes: coordinates, u and r per node, connectivity and E per element
+    flop = ADJ_FWD_FLOP_C3D8 * m.n_elem
+    byt = 72 * m.n_nodes + (4 * 8 + 8) * m.n_elem
+    t_att = max(flop / (PEAK_FP64 * 1e12), byt / (PEAK_HBM * 1e9))
     return {"workload": "96x48x48 C3D8 SIMP block, r = K(x,rho)u and its VJP (dx, drho, du)",
             "elements": m.n_elem, "forward_ms": f_ms, "vjp_ms": v_ms, "ratio": v_ms / f_ms,
-            "reps": reps}
+            "reps": reps,
+            "forward_roofline": {"bound": "mfma" if flop / (PEAK_FP64 * 1e12) >= byt / (PEAK_HBM * 1e9)
+                                 else "hbm", "flop": flop, "algorithmic_bytes": byt,
+                                 "t_attainable_ms": round(t_att * 1e3, 4),
+                                 "frac": round(t_att * 1e3 / f_ms, 4)}}
 
 
 def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
