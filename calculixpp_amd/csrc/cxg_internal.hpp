@@ -203,7 +203,8 @@ struct BlockDesc {
 // rows of the element matrix start in the arena, where its slot row starts, the
 // element's connectivity offset and node count.
 struct IncRecord {
-  int64_t rows;   // arena offset of row 3*la of the element matrix
+  int64_t rows;   // arena offset of row 3*la of the element matrix (full layout; the
+                  // element's slot starts 9*la*n doubles earlier)
   int64_t slot;   // offset of slot row la of the element
   int64_t conn;   // offset of the element in the mesh connectivity
   int32_t n;      // nodes of the element
@@ -227,8 +228,10 @@ struct cxg_pattern {
   cxg::DevBuf<int64_t> sptr;                  // n_elem+1
   std::vector<int64_t> slot_base;
   int64_t slot_total = 0;                     // sum over elements of n^2
-  // Two-phase assembly arena, allocated on first use: each element's full Ke
-  // (3n x 3n, row-major) at kpptr[e].
+  // Two-phase assembly arena, allocated on first use: a slot of 9n^2 doubles per element
+  // at kpptr[e], holding its full Ke (3n x 3n, row-major), its packed upper node-pair
+  // blocks (asm_variant 7: the first 9n(n+1)/2 doubles), or its n^2 scalars k_ab (mass,
+  // geometric stiffness: at kpptr[e] / 9).
   cxg::DevBuf<double> ke_arena;
   cxg::DevBuf<double> fe_arena;               // element force vectors (3n at 3*eptr[e])
   // asm_variant 7 with f_int: each element's nodal displacements of every load case,
