@@ -4,8 +4,10 @@
 // Replaces ComputeBackend::solve_sparse (include/calculixpp/compute/backend.hpp:51-56)
 // for the symmetric positive definite systems of linear and Newton steps (the CPU
 // backend hands them to SciPP, src/compute/cpu_backend.cpp:33-46). Everything stays
-// on the device: the CSR product (one 64-lane wave per row, fixed-order wave sum), the
-// vector updates, and the dot products, reduced in a fixed order (per-thread strided
+// on the device: the CSR product (one 64-lane wave per group of consecutive rows with the
+// same columns, such as the three rows of a node, so that the column indices are read and
+// x gathered once per group; a fixed-order wave sum per row), the vector updates, and the
+// dot products, reduced in a fixed order (per-thread strided
 // sums, a fixed LDS tree per block, then one block over the block sums), so the result
 // is bitwise reproducible. The CG scalars (alpha, beta, r.z) live in device memory;
 // the host reads the residual norm only every kCheck iterations.
@@ -58,6 +60,59 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int32_t n, const int64_t* __res
   for (int64_t k = rp[row] + lane; k < rp[row + 1]; k += kWave) s += va[k] * x[ci[k]];
   s = wave_sum(s);
   if (lane == 0) y[row] = s;
+}
+
+constexpr int kGroupRows = 3;  // rows per group of the grouped product (a node's DOFs)
+
+// same[r] = 1 when row r has the length and the columns of row r - 1 (one wave per row).
+__global__ __launch_bounds__(kBlock) void k_same_columns(int32_t n, const int64_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ ci,
+                                                         uint8_t* __restrict__ same) {
+  const int64_t row = int64_t(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (row >= n) return;
+  bool eq = row > 0 && rp[row + 1] - rp[row] == rp[row] - rp[row - 1];
+  if (eq) {
+    const int64_t len = rp[row + 1] - rp[row], a = rp[row - 1], b = rp[row];
+    bool mine = true;
+    for (int64_t k = lane; k < len; k += kWave) mine = mine && ci[a + k] == ci[b + k];
+    eq = __all(mine);
+  }
+  if (lane == 0) same[row] = eq ? 1 : 0;
+}
+
+// y = A x over groups of up to kGroupRows consecutive rows with the same columns, one wave
+// per group (group g: rows first[g] .. first[g + 1] - 1): column k of the first row is
+// read once and x[ci[k]] gathered once for all rows of the group. Lane l takes offsets
+// l, l + 64, ... of every row and the wave sums in the order of k_spmv, so y is bitwise
+// k_spmv's.
+__global__ __launch_bounds__(kBlock) void k_spmv_groups(int32_t n_groups,
+                                                        const int32_t* __restrict__ first,
+                                                        const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci,
+                                                        const double* __restrict__ va,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+  const int64_t g = int64_t(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (g >= n_groups) return;
+  const int32_t r0 = first[g], nr = first[g + 1] - r0;
+  const int64_t b0 = rp[r0], len = rp[r0 + 1] - b0;
+  double s[kGroupRows] = {0.0, 0.0, 0.0};
+  int64_t base[kGroupRows];
+#pragma unroll
+  for (int r = 0; r < kGroupRows; ++r) base[r] = r < nr ? rp[r0 + r] : b0;
+  for (int64_t k = lane; k < len; k += kWave) {
+    const double xv = x[ci[b0 + k]];
+#pragma unroll
+    for (int r = 0; r < kGroupRows; ++r)
+      if (r < nr) s[r] += va[base[r] + k] * xv;
+  }
+#pragma unroll
+  for (int r = 0; r < kGroupRows; ++r) {
+    const double t = wave_sum(s[r]);
+    if (lane == 0 && r < nr) y[r0 + r] = t;
+  }
 }
 
 // dinv[i] = 1 / K_ii (1 where the diagonal is missing or not positive).
@@ -180,6 +235,27 @@ void pcg(cxg_context* ctx, int32_t n, const int64_t* rp, const int32_t* ci, cons
   double* p3 = part.p + 2 * kParts;
   const unsigned g_rows = unsigned((int64_t(n) + kBlock / kWave - 1) / (kBlock / kWave));
   const unsigned g_vec = unsigned((int64_t(n) + kBlock - 1) / kBlock);
+  // groups of consecutive rows with the same columns (up to kGroupRows each)
+  DevBuf<int32_t> first;
+  int32_t n_groups = 0;
+  {
+    LaunchTimer t(ctx, "pcg_groups", 0);
+    DevBuf<uint8_t> same(nn);
+    hipLaunchKernelGGL(k_same_columns, dim3(g_rows), dim3(kBlock), 0, st, n, rp, ci, same.p);
+    CXG_HIP(hipGetLastError());
+    std::vector<uint8_t> hs(nn);
+    CXG_HIP(hipMemcpyAsync(hs.data(), same.p, nn, hipMemcpyDeviceToHost, st));
+    CXG_HIP(hipStreamSynchronize(st));
+    std::vector<int32_t> hf;
+    hf.reserve(nn / 2 + 2);
+    for (int32_t r = 0; r < n; ++r)
+      if (r == 0 || !hs[size_t(r)] || r - hf.back() >= kGroupRows) hf.push_back(r);
+    n_groups = int32_t(hf.size());
+    hf.push_back(n);
+    first.alloc(hf.size());
+    CXG_HIP(hipMemcpyAsync(first.p, hf.data(), hf.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  }
+  const unsigned g_groups = unsigned((int64_t(n_groups) + kBlock / kWave - 1) / (kBlock / kWave));
   {
     LaunchTimer t(ctx, "pcg_setup", 0);
     hipLaunchKernelGGL(k_diag_inv, dim3(g_vec), dim3(kBlock), 0, st, n, rp, ci, va, dinv.p);
@@ -204,7 +280,8 @@ void pcg(cxg_context* ctx, int32_t n, const int64_t* rp, const int32_t* ci, cons
   while (rel_res > tol && iters < max_iter) {
     LaunchTimer t(ctx, "pcg_iterations", kCheck);
     for (int k = 0; k < kCheck && iters < max_iter; ++k, ++iters) {
-      hipLaunchKernelGGL(k_spmv, dim3(g_rows), dim3(kBlock), 0, st, n, rp, ci, va, p.p, q.p);
+      hipLaunchKernelGGL(k_spmv_groups, dim3(g_groups), dim3(kBlock), 0, st, n_groups, first.p,
+                         rp, ci, va, p.p, q.p);
       hipLaunchKernelGGL(k_dot2, dim3(kParts), dim3(kBlock), 0, st, n, p.p, q.p, nullptr, nullptr, p1, nullptr);
       hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, st, p1, nullptr, nullptr, s.p, 1);
       hipLaunchKernelGGL(k_update, dim3(g_vec), dim3(kBlock), 0, st, n, s.p, p.p, q.p, dinv.p, x, r.p, z.p);
