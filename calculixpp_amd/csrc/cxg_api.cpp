@@ -43,6 +43,7 @@ LaunchTimer::~LaunchTimer() {
 void collect_timing(cxg_context* ctx) {
   for (const auto& p : ctx->pending) {
     float ms = 0.f;
+    CXG_HIP(hipEventSynchronize(p.b));
     CXG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
     auto it = std::find_if(ctx->stats.begin(), ctx->stats.end(),
                            [&](const cxg_context::Stat& s) { return s.name == p.name; });

@@ -278,18 +278,20 @@ void pcg(cxg_context* ctx, int32_t n, const int64_t* rp, const int32_t* ci, cons
   }
   rel_res = std::sqrt(h.rr) / bnorm;
   while (rel_res > tol && iters < max_iter) {
-    LaunchTimer t(ctx, "pcg_iterations", kCheck);
-    for (int k = 0; k < kCheck && iters < max_iter; ++k, ++iters) {
-      hipLaunchKernelGGL(k_spmv_groups, dim3(g_groups), dim3(kBlock), 0, st, n_groups, first.p,
-                         rp, ci, va, p.p, q.p);
-      hipLaunchKernelGGL(k_dot2, dim3(kParts), dim3(kBlock), 0, st, n, p.p, q.p, nullptr, nullptr, p1, nullptr);
-      hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, st, p1, nullptr, nullptr, s.p, 1);
-      hipLaunchKernelGGL(k_update, dim3(g_vec), dim3(kBlock), 0, st, n, s.p, p.p, q.p, dinv.p, x, r.p, z.p);
-      hipLaunchKernelGGL(k_dot2, dim3(kParts), dim3(kBlock), 0, st, n, r.p, z.p, r.p, r.p, p1, p2);
-      hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, st, p1, p2, nullptr, s.p, 2);
-      hipLaunchKernelGGL(k_direction, dim3(g_vec), dim3(kBlock), 0, st, n, s.p, z.p, p.p);
+    {  // the timer's end event precedes the synchronization below
+      LaunchTimer t(ctx, "pcg_iterations", kCheck);
+      for (int k = 0; k < kCheck && iters < max_iter; ++k, ++iters) {
+        hipLaunchKernelGGL(k_spmv_groups, dim3(g_groups), dim3(kBlock), 0, st, n_groups, first.p,
+                           rp, ci, va, p.p, q.p);
+        hipLaunchKernelGGL(k_dot2, dim3(kParts), dim3(kBlock), 0, st, n, p.p, q.p, nullptr, nullptr, p1, nullptr);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, st, p1, nullptr, nullptr, s.p, 1);
+        hipLaunchKernelGGL(k_update, dim3(g_vec), dim3(kBlock), 0, st, n, s.p, p.p, q.p, dinv.p, x, r.p, z.p);
+        hipLaunchKernelGGL(k_dot2, dim3(kParts), dim3(kBlock), 0, st, n, r.p, z.p, r.p, r.p, p1, p2);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, st, p1, p2, nullptr, s.p, 2);
+        hipLaunchKernelGGL(k_direction, dim3(g_vec), dim3(kBlock), 0, st, n, s.p, z.p, p.p);
+      }
+      CXG_HIP(hipGetLastError());
     }
-    CXG_HIP(hipGetLastError());
     CXG_HIP(hipMemcpyAsync(&h, s.p, sizeof(h), hipMemcpyDeviceToHost, st));
     CXG_HIP(hipStreamSynchronize(st));
     rel_res = std::sqrt(h.rr) / bnorm;

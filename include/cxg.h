@@ -186,7 +186,7 @@ cxg_status cxg_pattern_compact(cxg_context* ctx, const cxg_pattern* p, const uin
  * Optional live[nnz]: set to 1 where at least one element contributed a value other
  * than 0.0, else 0. The reference leaves entries whose every contribution is 0.0 out
  * of its COO map (scatter_block, fem/assembly.cpp:90); cxg_pattern_compact applies
- * that rule. Needs the two-phase kernels (option "asm_variant" 4, the default, or 5). */
+ * that rule. Needs a two-phase assembly (option "asm_variant" 4 to 7; 7 is the default). */
 cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                const double* E, const double* nu, double* vals,
                                const double* u, double* f_int, uint8_t* live);
@@ -194,7 +194,7 @@ cxg_status cxg_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const dou
 /* cxg_assemble_linear for n_cases displacement fields at once (load cases of one
  * mesh or batch): u and f_int hold n_cases consecutive vectors of 3*n_nodes; K is
  * formed once and every f_int[c] = K_full u[c] comes from the same pass over the
- * element rows. n_cases 1, 2 or 4 (2 and 4 need asm_variant 4 or 5). */
+ * element rows. n_cases 1, 2 or 4 (2 and 4 need a two-phase asm_variant, 4 to 7). */
 cxg_status cxg_assemble_linear_cases(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                      const double* E, const double* nu, double* vals,
                                      int32_t n_cases, const double* u, double* f_int,
