@@ -646,49 +646,35 @@ __host__ __device__ __forceinline__ int up_block(int a, int b, int n) {
 
 // Staged store of a lane's upper node-pair blocks K[t] = K_(pa[t], pb[t]) (pb >= pa; pa <
 // 0: none) into the element's packed upper-block arena kel, the block rows of
-// kStageNodes nodes at a time through stage (kStageNodes * 9 * NN doubles, 16-byte
-// aligned). A lane's blocks all lie in the rows of its node pa; a turn's blocks are one
-// contiguous run of the packed arena (up_block(g0, g0) is even for g0 % 4 == 0, so the
-// run starts 16-byte aligned), written with 16-byte streaming stores. Every lane of the
-// wave calls it; it ends with a barrier.
+// kStageNodes nodes at a time. A turn's blocks are one contiguous run of the packed
+// arena; each lane writes its blocks into stage at their packed place (9 consecutive
+// doubles each), and the wave then copies the run out with 16-byte streaming stores
+// (up_block(g0, g0) is even for g0 % 4 == 0, so the run starts 16-byte aligned). stage
+// holds kStageNodes * NN blocks, 16-byte aligned. Every lane of the wave calls it; it
+// ends with a barrier.
 template <int NN, int PPL>
 __device__ __forceinline__ void store_upper_blocks(double* __restrict__ stage, const int* pa,
                                                    const int* pb, const double (*K)[9],
                                                    double* __restrict__ kel, const int lane) {
   constexpr int NP = NN * (NN + 1) / 2;
-  constexpr int ND = 3 * NN;
+  static_assert(kStageNodes % 4 == 0, "turns must start at even packed blocks");
   for (int g0 = 0; g0 < NN; g0 += kStageNodes) {
     const int g1 = NN - g0 < kStageNodes ? NN : g0 + kStageNodes;
+    const int k0 = up_block(g0, g0, NN);
+    const int k1 = g1 < NN ? up_block(g1, g1, NN) : NP;
 #pragma unroll
     for (int t = 0; t < PPL; ++t) {
       if (pa[t] < g0 || pa[t] >= g1) continue;
-      double* ra = stage + (3 * (pa[t] - g0)) * ND + 3 * pb[t];
+      double* sb = stage + 9 * (up_block(pa[t], pb[t], NN) - k0);
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) ra[i * ND + j] = K[t][3 * i + j];
+      for (int r = 0; r < 9; ++r) sb[r] = K[t][r];
     }
     __syncthreads();
-    const int k0 = up_block(g0, g0, NN);
-    const int k1 = g1 < NN ? up_block(g1, g1, NN) : NP;
     const int cnt = 9 * (k1 - k0);
-    // packed double k0*9 + idx -> its place in the staged rows
-    auto staged = [&](int idx) {
-      const int bk = k0 + idx / 9, r = idx - 9 * (idx / 9);
-      int a = g0;
-      while (a + 1 < g1 && up_block(a + 1, a + 1, NN) <= bk) ++a;
-      const int b = a + (bk - up_block(a, a, NN));
-      const int i = r / 3, j = r - 3 * (r / 3);
-      return stage[(3 * (a - g0) + i) * ND + 3 * b + j];
-    };
     f64x2* dst = reinterpret_cast<f64x2*>(kel + 9 * k0);
-    for (int idx = lane; idx < cnt / 2; idx += kWave) {
-      f64x2 v;
-      v.x = staged(2 * idx);
-      v.y = staged(2 * idx + 1);
-      __builtin_nontemporal_store(v, dst + idx);
-    }
-    if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(staged(cnt - 1), kel + 9 * k0 + cnt - 1);
+    const f64x2* src = reinterpret_cast<const f64x2*>(stage);
+    for (int idx = lane; idx < cnt / 2; idx += kWave) __builtin_nontemporal_store(src[idx], dst + idx);
+    if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(stage[cnt - 1], kel + 9 * k0 + cnt - 1);
     __syncthreads();
   }
 }
@@ -755,23 +741,40 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
 #pragma unroll
     for (int i = 0; i < 9; ++i) M[t][i] = 0.0;
   __syncthreads();
+  // A chunk's (Gauss point, node) gradients take one pass of the wave; lane cq * NN + cn
+  // forms those of point q0 + cq, node cn, from the reference gradients it loaded during
+  // the previous chunk.
+  static_assert(QC * NN <= kWave, "one pass per chunk");
+  const int cq = lane / NN, cn = lane - NN * (lane / NN);
+  const auto load_dn = [&](int q0, double (&d)[3]) {
+    const int q = q0 + cq;
+    d[0] = d[1] = d[2] = 0.0;
+    if (lane < QC * NN && q < NGP) {
+      const double* p = dN + (q * NN + cn) * 3;
+      d[0] = p[0];
+      d[1] = p[1];
+      d[2] = p[2];
+    }
+  };
+  double d[3];
+  load_dn(0, d);
   for (int q0 = 0; q0 < NGP; q0 += QC) {
-    for (int idx = lane; idx < QC * NN; idx += kWave) {
-      const int q = q0 + idx / NN;
+    if (lane < QC * NN) {
+      const int q = q0 + cq;
       double h0 = 0.0, h1 = 0.0, h2 = 0.0;
       if (q < NGP) {
         const double* A = L.Ai + 9 * q;
-        const double* d = dN + (q * NN + idx % NN) * 3;
         const double s = L.rs[q];
         h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
         h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
         h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
       }
-      L.h[3 * idx + 0] = h0;
-      L.h[3 * idx + 1] = h1;
-      L.h[3 * idx + 2] = h2;
+      L.h[3 * lane + 0] = h0;
+      L.h[3 * lane + 1] = h1;
+      L.h[3 * lane + 2] = h2;
     }
     __syncthreads();
+    load_dn(q0 + QC, d);  // in flight during this chunk's products
 #pragma unroll 1
     for (int qq = 0; qq < QC && TL == 1; ++qq) {
       // couples: 6 contiguous doubles, 16-byte aligned (h is, and 2 nodes are 48 bytes)
