@@ -1020,6 +1020,12 @@ __global__ __launch_bounds__(kWave) void k_elem_iso_mfma(
 constexpr int kRowCap = 256;
 constexpr int kBatch = 4;  // incident elements whose loads are issued together
 
+// buf[0] += v in LDS without a read-back (DS_ADD_F64). The rows belong to one wave, whose
+// LDS operations execute in issue order, so every entry is still summed in element order.
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -1164,9 +1170,9 @@ __device__ __forceinline__ void row_gather_node(
       if ((fm[k] >> jl) & 1) {
         const int64_t pos = int64_t(colp[k]) + __popc(fm[k] & ((1 << jl) - 1));
         if (in_lds) {
-          if (rs[0] >= 0) buf[pos] += v[k][0];
-          if (rs[1] >= 0) buf[cap + pos] += v[k][1];
-          if (rs[2] >= 0) buf[2 * cap + pos] += v[k][2];
+          if (rs[0] >= 0) lds_add(buf + pos, v[k][0]);
+          if (rs[1] >= 0) lds_add(buf + cap + pos, v[k][1]);
+          if (rs[2] >= 0) lds_add(buf + 2 * cap + pos, v[k][2]);
           if constexpr (LIVE) {
             if (v[k][0] != 0.0) lbuf[pos] = 1;
             if (v[k][1] != 0.0) lbuf[cap + pos] = 1;
