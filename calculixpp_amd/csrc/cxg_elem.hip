@@ -1938,10 +1938,14 @@ __global__ void k_reduce_free(int64_t n_dof, const int32_t* __restrict__ dof_eq,
 // order, f_int from the arena, RF = f_int - f_ext.
 template <int NN, int NGP>
 struct RecoverLds {
-  double x[NN * 3];
-  double u[NN * 3];
+  union {
+    struct {
+      double x[NN * 3];
+      double u[NN * 3];
+    };
+    double eps[NGP * 6];  // written once the points' x and u reads are done
+  };
   double Ai[NGP * 9];   // J^-1 per Gauss point (row-major)
-  double eps[NGP * 6];
   double sig[NGP * 6];
   double sc[NGP];       // det J * w
   int32_t node[NN];
@@ -1965,7 +1969,7 @@ constexpr int rec_epw(int nn, int ngp) {
 // LDS at the elements' coordinates, displacements, J^-1 and the point fields. EPW
 // consecutive elements per wave.
 template <int NN, int NGP, bool J2>
-__global__ __launch_bounds__(kWave) void k_gauss_fields(
+__global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
     int64_t n_points, const double* __restrict__ dN, const double* __restrict__ wq,
     const double* __restrict__ coords, const double* __restrict__ E,
@@ -1992,13 +1996,14 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
   }
   __syncthreads();
   static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
-  if (lane < ne * NGP) {
-    const int e = lane / NGP, q = lane - e * NGP;
-    RecoverLds<NN, NGP>& R = L[e];
+  const bool pt_lane = lane < ne * NGP;
+  const int e = pt_lane ? lane / NGP : 0, q = pt_lane ? lane - e * NGP : 0;
+  RecoverLds<NN, NGP>& R = L[e];
+  double A[9], e6[6], sg[6], sc = 0.0;
+  if (pt_lane) {
     const int64_t le = le0 + e, eg = elem_offset + le;
     const double* d = dN + q * NN * 3;
-    double A[9];
-    const double sc = jacobian_detw<NN, true>(R.x, d, wq[q], A, flag);
+    sc = jacobian_detw<NN, true>(R.x, d, wq[q], A, flag);
     double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < NN; ++k) {
       double g[3];
@@ -2006,7 +2011,6 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) H[i][j] += R.u[3 * k + i] * g[j];
     }
-    double e6[6], sg[6];
     voigt_strain(H, e6);
     const int64_t pt = gp_offset + le * NGP + q;
     const int32_t mid = J2 ? mat_id[eg] : -1;
@@ -2021,14 +2025,17 @@ __global__ __launch_bounds__(kWave) void k_gauss_fields(
       lame(E[eg], nu[eg], lam, mu);
       iso_stress(lam, mu, e6, sg);
     }
+    if (gauss_stress)
+      for (int i = 0; i < 6; ++i) gauss_stress[6 * pt + i] = sg[i];
+  }
+  __syncthreads();  // x and u are read: eps takes their place
+  if (pt_lane) {
     for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
     for (int i = 0; i < 6; ++i) {
       R.eps[6 * q + i] = e6[i];
       R.sig[6 * q + i] = sg[i];
     }
     R.sc[q] = sc;
-    if (gauss_stress)
-      for (int i = 0; i < 6; ++i) gauss_stress[6 * pt + i] = sg[i];
   }
   __syncthreads();
   for (int idx = lane; emean && idx < ne * 12; idx += kWave) {
