@@ -200,29 +200,126 @@ __global__ void k_inc_records(int32_t n_nodes, const int64_t* __restrict__ inc_p
 }
 
 // Entries kept per row (live != 0); one thread per row.
-__global__ void k_live_count(int32_t n_free, const int64_t* __restrict__ row_ptr,
-                             const uint8_t* __restrict__ live, int64_t* __restrict__ count) {
-  const int32_t r = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= n_free) return;
-  int64_t c = 0;
-  for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) c += live[k] ? 1 : 0;
-  count[r] = c;
+// Key-set compaction, one 64-lane wave per row (four rows per 256-thread block): the
+// lanes take 64 consecutive entries at a time and count or place the live ones with a
+// ballot, so every access is coalesced.
+constexpr int kRowsPerBlock = 4;
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) {
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Copies the kept entries of each row, in column order; one thread per row.
-__global__ void k_compact(int32_t n_free, const int64_t* __restrict__ row_ptr,
-                          const int32_t* __restrict__ col_idx, const double* __restrict__ vals,
-                          const uint8_t* __restrict__ live, const int64_t* __restrict__ row_out,
-                          int32_t* __restrict__ col_out, double* __restrict__ vals_out) {
-  const int32_t r = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_live_count(int32_t n_free,
+                                                    const int64_t* __restrict__ row_ptr,
+                                                    const uint8_t* __restrict__ live,
+                                                    int64_t* __restrict__ count) {
+  const int64_t r = int64_t(blockIdx.x) * kRowsPerBlock + threadIdx.x / kWave;
   if (r >= n_free) return;
+  const int lane = threadIdx.x % kWave;
+  const int64_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  int64_t c = 0;
+  for (int64_t k = k0; k < k1; k += kWave) {
+    const bool on = k + lane < k1 && live[k + lane];
+    c += __popcll(__ballot(on));
+  }
+  if (lane == 0) count[r] = c;
+}
+
+// Exclusive scan of count[0..n) into out[0..n] (out[n] = the total), in three passes:
+// per-block scans of kScanItems entries with the block totals, a scan of the block totals
+// by one block, and the block offsets added back.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr int kScanItems = kScanThreads * kScanPer;
+
+__device__ int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t& total) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int64_t add = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  total = lds[kScanThreads - 1];
+  const int64_t excl = lds[t] - v;
+  __syncthreads();
+  return excl;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int64_t n,
+                                                              const int64_t* __restrict__ count,
+                                                              int64_t* __restrict__ out,
+                                                              int64_t* __restrict__ block_sum) {
+  __shared__ int64_t lds[kScanThreads];
+  const int64_t base = int64_t(blockIdx.x) * kScanItems + int64_t(threadIdx.x) * kScanPer;
+  int64_t v[kScanPer];
+  int64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    v[i] = base + i < n ? count[base + i] : 0;
+    sum += v[i];
+  }
+  int64_t total = 0;
+  int64_t run = block_exclusive_scan(sum, lds, total);
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 0) block_sum[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the n_blocks block totals in place; out[n] = the total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_totals(int64_t n_blocks,
+                                                              int64_t* __restrict__ block_sum,
+                                                              int64_t* __restrict__ out, int64_t n) {
+  __shared__ int64_t lds[kScanThreads];
+  int64_t carry = 0;
+  for (int64_t b0 = 0; b0 < n_blocks; b0 += kScanThreads) {
+    const int64_t b = b0 + threadIdx.x;
+    const int64_t v = b < n_blocks ? block_sum[b] : 0;
+    int64_t total = 0;
+    const int64_t excl = block_exclusive_scan(v, lds, total);
+    if (b < n_blocks) block_sum[b] = carry + excl;
+    carry += total;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(int64_t n,
+                                                           const int64_t* __restrict__ block_sum,
+                                                           int64_t* __restrict__ out) {
+  const int64_t base = int64_t(blockIdx.x) * kScanItems;
+  const int64_t add = block_sum[blockIdx.x];
+  for (int i = threadIdx.x; i < kScanItems; i += kScanThreads)
+    if (base + i < n) out[base + i] += add;
+}
+
+// Copies the kept entries of each row, in column order.
+__global__ __launch_bounds__(256) void k_compact(int32_t n_free, const int64_t* __restrict__ row_ptr,
+                                                 const int32_t* __restrict__ col_idx,
+                                                 const double* __restrict__ vals,
+                                                 const uint8_t* __restrict__ live,
+                                                 const int64_t* __restrict__ row_out,
+                                                 int32_t* __restrict__ col_out,
+                                                 double* __restrict__ vals_out) {
+  const int64_t r = int64_t(blockIdx.x) * kRowsPerBlock + threadIdx.x / kWave;
+  if (r >= n_free) return;
+  const int lane = threadIdx.x % kWave;
+  const int64_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
   int64_t o = row_out[r];
-  for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
-    if (live[k]) {
-      col_out[o] = col_idx[k];
-      vals_out[o] = vals[k];
-      ++o;
+  for (int64_t k = k0; k < k1; k += kWave) {
+    const bool on = k + lane < k1 && live[k + lane];
+    const uint64_t m = __ballot(on);
+    if (on) {
+      const int64_t at = o + __popcll(m & lanes_below(lane));
+      col_out[at] = col_idx[k + lane];
+      vals_out[at] = vals[k + lane];
     }
+    o += __popcll(m);
+  }
 }
 
 }  // namespace
@@ -235,27 +332,31 @@ int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* l
                         double* vals_out) {
   hipStream_t s = ctx->stream;
   const int32_t n = p->n_free;
-  std::vector<int64_t> hrow(size_t(n) + 1, 0);
+  int64_t kept = 0;
   if (n > 0) {
     DevBuf<int64_t> count(static_cast<size_t>(n));
-    const unsigned blocks = unsigned((n + 255) / 256);
-    hipLaunchKernelGGL(k_live_count, dim3(blocks), dim3(256), 0, s, n, p->row_ptr.p, live,
+    const unsigned rblocks = unsigned((int64_t(n) + kRowsPerBlock - 1) / kRowsPerBlock);
+    hipLaunchKernelGGL(k_live_count, dim3(rblocks), dim3(256), 0, s, n, p->row_ptr.p, live,
                        count.p);
     CXG_HIP(hipGetLastError());
-    CXG_HIP(hipMemcpyAsync(hrow.data() + 1, count.p, size_t(n) * sizeof(int64_t),
-                           hipMemcpyDeviceToHost, s));
-    CXG_HIP(hipStreamSynchronize(s));
-    for (size_t r = 0; r < size_t(n); ++r) hrow[r + 1] += hrow[r];
-    CXG_HIP(hipMemcpyAsync(row_ptr_out, hrow.data(), hrow.size() * sizeof(int64_t),
-                           hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(256), 0, s, n, p->row_ptr.p, p->col_idx.p,
+    const int64_t sblocks = (int64_t(n) + kScanItems - 1) / kScanItems;
+    DevBuf<int64_t> bsum(static_cast<size_t>(sblocks));
+    hipLaunchKernelGGL(k_scan_blocks, dim3(unsigned(sblocks)), dim3(kScanThreads), 0, s,
+                       int64_t(n), count.p, row_ptr_out, bsum.p);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, s, sblocks, bsum.p,
+                       row_ptr_out, int64_t(n));
+    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(sblocks)), dim3(kScanThreads), 0, s, int64_t(n),
+                       bsum.p, row_ptr_out);
+    CXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_compact, dim3(rblocks), dim3(256), 0, s, n, p->row_ptr.p, p->col_idx.p,
                        vals, live, row_ptr_out, col_idx_out, vals_out);
     CXG_HIP(hipGetLastError());
+    CXG_HIP(hipMemcpyAsync(&kept, row_ptr_out + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   } else {
-    CXG_HIP(hipMemcpyAsync(row_ptr_out, hrow.data(), sizeof(int64_t), hipMemcpyHostToDevice, s));
+    CXG_HIP(hipMemcpyAsync(row_ptr_out, &kept, sizeof(int64_t), hipMemcpyHostToDevice, s));
   }
-  CXG_HIP(hipStreamSynchronize(s));  // hrow and count may be released afterwards
-  return hrow.back();
+  CXG_HIP(hipStreamSynchronize(s));  // kept is read, count and bsum may be released
+  return kept;
 }
 
 void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_dev,
