@@ -1169,7 +1169,21 @@ __device__ __forceinline__ void row_gather_node(
       }
       if ((fm[k] >> jl) & 1) {
         const int64_t pos = int64_t(colp[k]) + __popc(fm[k] & ((1 << jl) - 1));
-        if (in_lds) {
+        if constexpr (SCALAR) {
+          // k_ab I3: of the three rows only row jl receives a value (the others' zeros
+          // would leave them unchanged)
+          const double sv = jl == 0 ? v[k][0] : (jl == 1 ? v[k][1] : v[k][2]);
+          const int64_t rsj = jl == 0 ? rs[0] : (jl == 1 ? rs[1] : rs[2]);
+          if (in_lds) {
+            if (rsj >= 0) lds_add(buf + jl * cap + pos, sv);
+            if constexpr (LIVE)
+              if (sv != 0.0) lbuf[jl * cap + pos] = 1;
+          } else {
+            if (rsj >= 0) vals[rsj + pos] += sv;
+            if constexpr (LIVE)
+              if (rsj >= 0 && sv != 0.0) live[rsj + pos] = 1;
+          }
+        } else if (in_lds) {
           if (rs[0] >= 0) lds_add(buf + pos, v[k][0]);
           if (rs[1] >= 0) lds_add(buf + cap + pos, v[k][1]);
           if (rs[2] >= 0) lds_add(buf + 2 * cap + pos, v[k][2]);
