@@ -2759,27 +2759,73 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
   double h = dims > 0 ? std::pow(vol / ne, 1.0 / dims) : 1.0;
   if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
   const double w = tile * h, q = 0.25 * h;
-  struct Key {
-    int64_t t, s, v, u;
-    int32_t i;
-  };
   const auto cell = [](double d, double step) { return int64_t(std::floor(d / step)); };
-  std::vector<Key> key(static_cast<size_t>(n));
+  // key (tile column, layer, v, u) of every node; the cells are >= 0 and at most those
+  // of the bounding box's far corner
   const int64_t ntv = cell(hi[ax[1]] - lo[ax[1]], w) + 1;
-  for (int32_t i = 0; i < n; ++i) {
+  const int64_t nt = (cell(hi[ax[0]] - lo[ax[0]], w) + 1) * ntv;
+  const int64_t ns = cell(hi[ax[2]] - lo[ax[2]], q) + 1;
+  const int64_t nv = cell(hi[ax[1]] - lo[ax[1]], q) + 1;
+  const int64_t nu = cell(hi[ax[0]] - lo[ax[0]], q) + 1;
+  const auto cells = [&](int32_t i, int64_t c[4]) {
     const double* xi = &x[3 * size_t(i)];
     const double du = xi[ax[0]] - lo[ax[0]], dv = xi[ax[1]] - lo[ax[1]], ds = xi[ax[2]] - lo[ax[2]];
-    key[size_t(i)] = Key{cell(du, w) * ntv + cell(dv, w), cell(ds, q), cell(dv, q), cell(du, q), i};
-  }
-  std::sort(key.begin(), key.end(), [](const Key& a, const Key& b) {
-    if (a.t != b.t) return a.t < b.t;
-    if (a.s != b.s) return a.s < b.s;
-    if (a.v != b.v) return a.v < b.v;
-    if (a.u != b.u) return a.u < b.u;
-    return a.i < b.i;
-  });
+    c[0] = cell(du, w) * ntv + cell(dv, w);
+    c[1] = cell(ds, q);
+    c[2] = cell(dv, q);
+    c[3] = cell(du, q);
+  };
   std::vector<int32_t> ord(static_cast<size_t>(n));
-  for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = key[size_t(i)].i;
+  const unsigned __int128 span = static_cast<unsigned __int128>(nt) * uint64_t(ns) * uint64_t(nv) * uint64_t(nu);
+  if (span < (static_cast<unsigned __int128>(1) << 63)) {
+    // one 64-bit key per node, ordered by a stable LSD radix sort over 16-bit digits: ties
+    // keep ascending node order
+    std::vector<uint64_t> k(static_cast<size_t>(n)), k2(static_cast<size_t>(n));
+    std::vector<int32_t> o2(static_cast<size_t>(n));
+    uint64_t kmax = 0;
+    for (int32_t i = 0; i < n; ++i) {
+      int64_t c[4];
+      cells(i, c);
+      k[size_t(i)] = ((uint64_t(c[0]) * uint64_t(ns) + uint64_t(c[1])) * uint64_t(nv) + uint64_t(c[2])) *
+                         uint64_t(nu) + uint64_t(c[3]);
+      kmax = std::max(kmax, k[size_t(i)]);
+      ord[size_t(i)] = i;
+    }
+    std::vector<size_t> hist(size_t(1) << 16);
+    for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += 16) {
+      std::fill(hist.begin(), hist.end(), 0);
+      for (uint64_t v : k) ++hist[(v >> shift) & 0xffff];
+      size_t run = 0;
+      for (size_t& hcount : hist) {
+        const size_t c = hcount;
+        hcount = run;
+        run += c;
+      }
+      for (size_t j = 0; j < k.size(); ++j) {
+        const size_t at = hist[(k[j] >> shift) & 0xffff]++;
+        k2[at] = k[j];
+        o2[at] = ord[j];
+      }
+      k.swap(k2);
+      ord.swap(o2);
+    }
+  } else {
+    struct Key {
+      int64_t c[4];
+      int32_t i;
+    };
+    std::vector<Key> key(static_cast<size_t>(n));
+    for (int32_t i = 0; i < n; ++i) {
+      cells(i, key[size_t(i)].c);
+      key[size_t(i)].i = i;
+    }
+    std::sort(key.begin(), key.end(), [](const Key& a, const Key& b) {
+      for (int j = 0; j < 4; ++j)
+        if (a.c[j] != b.c[j]) return a.c[j] < b.c[j];
+      return a.i < b.i;
+    });
+    for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = key[size_t(i)].i;
+  }
   pm->gorder.alloc(size_t(n));
   CXG_HIP(hipMemcpy(pm->gorder.p, ord.data(), ord.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   pm->gorder_tile = tile;
