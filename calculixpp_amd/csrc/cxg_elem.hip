@@ -2087,6 +2087,8 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
 // offset in each incident element. Lanes 0-11 sum the incident elements' Gauss-point
 // means of one component (strain, then stress), lanes 12-14 one component of the force, in
 // ascending element order; the means are divided by the number of incident elements.
+constexpr int kNodeBatch = 8;
+
 __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
                               const int32_t* __restrict__ inc_elem,
                               const int64_t* __restrict__ inc_at,
@@ -2100,8 +2102,20 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
   if (a >= n_nodes || c >= 15) return;
   const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
   double v = 0.0;
-  for (int64_t it = i0; it < i1; ++it)
-    v += c < 12 ? emean[12 * int64_t(inc_elem[it]) + c] : fe[3 * inc_at[it] + (c - 12)];
+  // kNodeBatch incidences at a time: their index and value loads issue together, the sum
+  // keeps ascending element order
+  for (int64_t it = i0; it < i1; it += kNodeBatch) {
+    int64_t at[kNodeBatch];
+    double x[kNodeBatch];
+#pragma unroll
+    for (int k = 0; k < kNodeBatch; ++k)
+      at[k] = it + k < i1 ? (c < 12 ? 12 * int64_t(inc_elem[it + k]) + c : 3 * inc_at[it + k] + (c - 12)) : -1;
+#pragma unroll
+    for (int k = 0; k < kNodeBatch; ++k) x[k] = at[k] >= 0 ? (c < 12 ? emean : fe)[at[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kNodeBatch; ++k)
+      if (at[k] >= 0) v += x[k];
+  }
   if (c < 12) {
     const int64_t cnt = i1 - i0;
     if (cnt > 0) v /= double(cnt);
@@ -2127,8 +2141,19 @@ __global__ void k_node_sum(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
   if (a >= n_nodes || c >= 6 || (c >= 3 && !a1)) return;
   const double* src = c < 3 ? a0 : a1;
   const int cc = c < 3 ? c : c - 3;
+  const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
   double v = 0.0;
-  for (int64_t it = inc_ptr[a]; it < inc_ptr[a + 1]; ++it) v += src[3 * inc_at[it] + cc];
+  for (int64_t it = i0; it < i1; it += kNodeBatch) {  // as k_node_fields
+    int64_t at[kNodeBatch];
+    double x[kNodeBatch];
+#pragma unroll
+    for (int k = 0; k < kNodeBatch; ++k) at[k] = it + k < i1 ? 3 * inc_at[it + k] + cc : -1;
+#pragma unroll
+    for (int k = 0; k < kNodeBatch; ++k) x[k] = at[k] >= 0 ? src[at[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kNodeBatch; ++k)
+      if (at[k] >= 0) v += x[k];
+  }
   (c < 3 ? out0 : out1)[3 * int64_t(a) + cc] = v;
 }
 
