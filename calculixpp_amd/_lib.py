@@ -77,6 +77,8 @@ _SIGS = {
     "cxg_assemble_j2": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
                                   _P, _P, _P, _P, _P, _P]),
     "cxg_commit": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "cxg_material_evaluate": (C.c_int, [_P, C.c_int64, _P, C.POINTER(J2Material), C.c_int32,
+                                        _P, _P, _P, _P, _P, _P, _P]),
     "cxg_vjp": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_assemble_mass": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P]),
     "cxg_assemble_geometric": (C.c_int, [_P, _P, _P, _P, _P, _P]),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "cxg_internal.hpp"
@@ -156,6 +157,17 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
   CXG_HIP(hipStreamSynchronize(s));
   for (int32_t v : conn)
     if (v < 0 || v >= n_nodes) throw Error(CXG_ERR_ARG, "element references unknown node");
+  // A node listed twice by one element (a collapsed hex or wedge) would need one incidence
+  // per occurrence in the row gather and the node sums; such meshes are outside the GPU
+  // path and are refused here, before anything is assembled.
+  for (int64_t e = 0; e < n_elem; ++e)
+    for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k)
+      for (int64_t j = eptr[size_t(e)]; j < k; ++j)
+        if (conn[size_t(j)] == conn[size_t(k)])
+          throw Error(CXG_ERR_UNSUPPORTED,
+                      "element " + std::to_string(e) + " lists node " +
+                          std::to_string(conn[size_t(k)]) +
+                          " twice (collapsed element): outside the GPU path");
   // node -> element incidence, ascending element order
   std::vector<int64_t> inc_ptr(size_t(n_nodes) + 1, 0);
   for (int32_t v : conn) ++inc_ptr[size_t(v) + 1];
@@ -167,26 +179,9 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
     for (int64_t e = 0; e < n_elem; ++e)
       for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
         const int32_t v = conn[size_t(k)];
-        // a node repeated inside one element is listed once, at its first occurrence
-        if (cur[size_t(v)] > inc_ptr[size_t(v)] && inc_elem[size_t(cur[size_t(v)] - 1)] == int32_t(e))
-          continue;
         inc_at[size_t(cur[size_t(v)])] = k;
         inc_elem[size_t(cur[size_t(v)]++)] = int32_t(e);
       }
-    // compact (duplicates left holes at the end of a node's range)
-    std::vector<int64_t> np(size_t(n_nodes) + 1, 0);
-    int64_t w = 0;
-    for (int32_t a = 0; a < n_nodes; ++a) {
-      np[size_t(a)] = w;
-      for (int64_t i = inc_ptr[size_t(a)]; i < cur[size_t(a)]; ++i, ++w) {
-        inc_elem[size_t(w)] = inc_elem[size_t(i)];
-        inc_at[size_t(w)] = inc_at[size_t(i)];
-      }
-    }
-    np[size_t(n_nodes)] = w;
-    inc_ptr.swap(np);
-    inc_elem.resize(size_t(w));
-    inc_at.resize(size_t(w));
   }
   // patches: up to patch_size consecutive elements of one block
   const int64_t P = std::max(1, ctx->patch_size);
@@ -614,6 +609,21 @@ cxg_status cxg_recover_fields(cxg_context* ctx, const cxg_mesh* mesh, const doub
     cxg::launch_recover(ctx, mesh, coords, E, nu, u, n_mats > 0 ? mat_id : nullptr, table,
                         committed, f_ext, strain, stress, reaction, f_int, gauss_stress);
     cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_material_evaluate(cxg_context* ctx, int64_t n_points, const int32_t* mat_id,
+                                 const cxg_j2_material* mats, int32_t n_mats, const double* E,
+                                 const double* nu, const double* strain, const double* committed,
+                                 double* trial, double* stress, double* tangent) {
+  if (n_points < 0 || n_mats < 0 || (n_points > 0 && (!strain || !stress || !tangent)) ||
+      (mat_id && (n_mats == 0 || !mats || !committed || !trial)) || (!mat_id && (!E || !nu)))
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad material_evaluate arguments");
+  return cxg::guarded(ctx, [&] {
+    const void* table = mat_id ? cxg::upload_j2_table(ctx, mats, n_mats) : nullptr;
+    cxg::launch_material_points(ctx, n_points, mat_id, table, E, nu, strain, committed, trial,
+                                stress, tangent);
+    CXG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 

@@ -1920,6 +1920,42 @@ __global__ __launch_bounds__(kWave, 2) void k_elem_j2_persistent(
   }
 }
 
+// MaterialModel::evaluate at independent material points (one thread per point, grid
+// stride): J2PlasticMaterial::evaluate (plasticity.cpp:90-236) for points with a J2
+// material, ElasticIsoMaterial::evaluate (element.cpp:856-871) for the others, whose
+// trial history is left untouched as the reference leaves it.
+__global__ void k_material_points(int64_t n, const int32_t* __restrict__ mat_id,
+                                  const J2Dev* __restrict__ mats, const double* __restrict__ E,
+                                  const double* __restrict__ nu, const double* __restrict__ strain,
+                                  const double* __restrict__ committed, double* __restrict__ trial,
+                                  double* __restrict__ stress, double* __restrict__ tangent) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride) {
+    double e[6], sg[6], D[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) e[i] = strain[6 * p + i];
+    const int32_t id = mat_id ? mat_id[p] : -1;
+    if (id >= 0) {
+      double hc[13], ht[13];
+#pragma unroll
+      for (int c = 0; c < 13; ++c) hc[c] = committed[c * n + p];
+      j2_point(mats[id], e, hc, ht, sg, D);
+#pragma unroll
+      for (int c = 0; c < 13; ++c) trial[c * n + p] = ht[c];
+    } else {
+      double lam, mu;
+      lame(E[p], nu[p], lam, mu);
+      iso_stress(lam, mu, e, sg);
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+          D[6 * i + j] = i < 3 && j < 3 ? lam + (i == j ? 2.0 * mu : 0.0) : (i == j ? mu : 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) stress[6 * p + i] = sg[i];
+    for (int k = 0; k < 36; ++k) tangent[36 * p + k] = D[k];
+  }
+}
+
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -2641,13 +2677,17 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 // sum_q det J w B^T sigma of k_gauss_fields (means off) into the element-force arena,
 // then summed per node in ascending element order.
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
-                           const double* E, const double* nu, const double* u, double* f_int) {
-  m->rec_fe.ensure(std::max<size_t>(3 * m->conn.n, 1));
+                           const double* E, const double* nu, const double* u, double* f_int,
+                           double* arena) {
+  if (!arena) {
+    m->rec_fe.ensure(std::max<size_t>(3 * m->conn.n, 1));
+    arena = m->rec_fe.p;
+  }
   for (const cxg_block_dev& B : m->blocks) {
     if (B.n_elem == 0) continue;
     const ElemTable& T = ctx->tables[B.type];
     const int32_t* conn = m->conn.p + B.conn_offset;
-    double* feb = m->rec_fe.p + 3 * B.conn_offset;
+    double* feb = arena + 3 * B.conn_offset;
     LaunchTimer timer(ctx, std::string("internal_force_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
   hipLaunchKernelGGL((k_gauss_fields<NN, NGP, false>),                                          \
@@ -2664,10 +2704,11 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
     LaunchTimer timer(ctx, "internal_force_nodes", m->n_nodes);
     hipLaunchKernelGGL(k_node_sum, dim3(unsigned((int64_t(m->n_nodes) * 8 + 255) / 256)),
                        dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_at.p,
-                       m->rec_fe.p, nullptr, f_int, nullptr);
+                       arena, nullptr, f_int, nullptr);
     CXG_HIP(hipGetLastError());
   }
 }
+
 
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
                         const double* f_ext, const double* f_c, double* rhs) {
@@ -2730,14 +2771,23 @@ bool uses_upper_blocks(const cxg_context* ctx, const cxg_mesh* m) {
 void prepare_displacement_arena(const cxg_pattern* p, int n_cases) {
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
   const size_t n = size_t(n_cases) * 3 * p->mesh->conn.n;
-  if (pm->ue_arena.n < n) pm->ue_arena.alloc(n);
+  if (pm->ue_arena.n < n) {
+    guard_realloc(p, pm->ue_arena.p != nullptr, "displacement arena (more load cases)");
+    pm->ue_arena.alloc(n);
+  }
 }
 
 void prepare_arenas(const cxg_pattern* p, bool element_forces) {
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
-  if (pm->ke_arena.n < size_t(p->kp_total)) pm->ke_arena.alloc(size_t(p->kp_total));
+  if (pm->ke_arena.n < size_t(p->kp_total)) {
+    guard_realloc(p, pm->ke_arena.p != nullptr, "element-matrix arena");
+    pm->ke_arena.alloc(size_t(p->kp_total));
+  }
   const size_t fe_n = size_t(3 * p->mesh->conn.n);
-  if (element_forces && pm->fe_arena.n < fe_n) pm->fe_arena.alloc(fe_n);
+  if (element_forces && pm->fe_arena.n < fe_n) {
+    guard_realloc(p, pm->fe_arena.p != nullptr, "element-force arena");
+    pm->fe_arena.alloc(fe_n);
+  }
 }
 
 // Columns of tile x tile elements over the two shorter extents of the nodes' bounding
@@ -2750,6 +2800,7 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
   const int tile = ctx->gather_tile;
   if (pm->gorder_tile == tile) return;
+  guard_realloc(p, pm->gorder.p != nullptr, "gather order (option gather_tile changed)");
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   CXG_HIP(hipStreamIsCapturing(ctx->stream, &cs));
   if (cs != hipStreamCaptureStatusNone)
@@ -2859,6 +2910,7 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
 void prepare_work(cxg_context* ctx, const cxg_pattern* p) {
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
   if (pm->work_chunk == ctx->chunk_elems && pm->queue.p) return;
+  guard_realloc(p, pm->work.p != nullptr, "work queue (option chunk_elems changed)");
   const cxg_mesh* m = p->mesh;
   std::vector<BlockDesc> desc(m->blocks.size());
   std::vector<size_t> order(m->blocks.size());
@@ -2960,6 +3012,18 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
     CXG_DISPATCH(S.B.type, LAUNCH)
 #undef LAUNCH
   });
+}
+
+void launch_material_points(cxg_context* ctx, int64_t n_points, const int32_t* mat_id,
+                            const void* j2_table, const double* E, const double* nu,
+                            const double* strain, const double* committed, double* trial,
+                            double* stress, double* tangent) {
+  if (n_points <= 0) return;
+  LaunchTimer timer(ctx, "material_points", n_points);
+  hipLaunchKernelGGL(k_material_points, dim3(grid_for(n_points)), dim3(256), 0, ctx->stream,
+                     n_points, mat_id, static_cast<const J2Dev*>(j2_table), E, nu, strain,
+                     committed, trial, stress, tangent);
+  CXG_HIP(hipGetLastError());
 }
 
 void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial) {

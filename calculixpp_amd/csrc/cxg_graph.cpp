@@ -21,6 +21,7 @@
 
 struct cxg_graph {
   cxg_context* ctx = nullptr;
+  const cxg_pattern* pattern = nullptr;  // whose buffers the graph's kernels address
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   cxg::DevBuf<unsigned char> j2_table;  // the graph's own copy of the material table
@@ -29,6 +30,7 @@ struct cxg_graph {
   ~cxg_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (pattern) --pattern->live_graphs;
   }
 };
 
@@ -78,6 +80,12 @@ void capture(cxg_context* ctx, cxg_graph* g, F&& enqueue) {
   CXG_HIP(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
 }
 
+// From here on the pattern keeps the buffers the graph's kernels address.
+void hold(cxg_graph* g, const cxg_pattern* p) {
+  g->pattern = p;
+  ++p->live_graphs;
+}
+
 }  // namespace
 }  // namespace cxg
 
@@ -105,6 +113,7 @@ cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, cons
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
                                   with_f ? f_int : nullptr, nullptr);
     });
+    cxg::hold(g.get(), p);
     *out = g.release();
   });
 }
@@ -131,6 +140,7 @@ cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, u, f_int, nullptr, int(n_cases));
     });
+    cxg::hold(g.get(), p);
     *out = g.release();
   });
 }
@@ -169,6 +179,7 @@ cxg_status cxg_graph_capture_j2(cxg_context* ctx, const cxg_pattern* p, const do
         cxg::launch_reduce_free(ctx, p->mesh->n_nodes, p->dof_eq.p, f_ext, f_int, rhs);
       }
     });
+    cxg::hold(g.get(), p);
     *out = g.release();
   });
 }

@@ -173,7 +173,7 @@ struct cxg_mesh {
   cxg::DevBuf<int64_t> inc_ptr;   // node -> incident elements (n_nodes+1)
   cxg::DevBuf<int32_t> inc_elem;  // incident model element numbers, ascending
   cxg::DevBuf<int64_t> inc_at;    // per incidence: conn offset of the node in the element
-                                  // (its first occurrence)
+                                  // (an element lists a node once; create_mesh checks)
   int32_t max_candidates = 0;     // max over nodes of sum of incident element sizes
   // Field recovery arenas, allocated on first use: per element the Gauss-point means of
   // strain and stress (12), per element node its force vector (3, at 3 * conn offset).
@@ -253,6 +253,9 @@ struct cxg_pattern {
   // built; empty for tile 0, node index order).
   cxg::DevBuf<int32_t> gorder;
   int gorder_tile = -1;
+  // Captured graphs (cxg_graph_capture_*) holding the addresses of the buffers above;
+  // while any is alive those buffers are never freed or moved (cxg::guard_realloc).
+  mutable int live_graphs = 0;
 };
 
 namespace cxg {
@@ -261,8 +264,11 @@ void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             const double* E, const double* nu, double* vals, const double* u,
                             double* f_int, uint8_t* live, int n_cases = 1);
+// arena: the element-force arena (3 doubles per connectivity entry); null: the mesh's own,
+// allocated on first use.
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
-                           const double* E, const double* nu, const double* u, double* f_int);
+                           const double* E, const double* nu, const double* u, double* f_int,
+                           double* arena = nullptr);
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
                         const double* f_ext, const double* f_c, double* rhs);
 // J2 material tables: host bytes of the device form, and the context's cached upload.
@@ -286,6 +292,11 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
                         const void* j2_table, const double* u, const double* committed,
                         double* trial, double* vals, double* f_int, uint8_t* live);
 void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const double* trial);
+// MaterialModel::evaluate at n independent points (cxg_material_evaluate).
+void launch_material_points(cxg_context* ctx, int64_t n_points, const int32_t* mat_id,
+                            const void* j2_table, const double* E, const double* nu,
+                            const double* strain, const double* committed, double* trial,
+                            double* stress, double* tangent);
 // Mass (kind 0 consistent, 1 lumped; rho per element) or geometric stiffness (kind 2;
 // sig per Gauss point) into the CSR values (K8 + the row gather).
 void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double* coords,
@@ -311,6 +322,16 @@ int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* l
                         double* vals_out);
 // Read back and clear the device status word; throws on a raised flag.
 void check_flags(cxg_context* ctx);
+
+// A captured graph replays with the addresses its pattern's arenas, gather order and work
+// queue had at capture. `frees`: the caller is about to free or move one of them (`what`);
+// refused while a graph of the pattern is alive.
+inline void guard_realloc(const cxg_pattern* p, bool frees, const char* what) {
+  if (frees && p->live_graphs > 0)
+    throw Error(CXG_ERR_STATE, std::string("a captured graph uses this pattern's ") + what +
+                                   "; destroy the pattern's graphs before a call that needs "
+                                   "it rebuilt");
+}
 
 // Status + last-error text for the C ABI.
 inline cxg_status fail(cxg_context* ctx, cxg_status st, const char* msg) {

@@ -276,11 +276,11 @@ void dof_map(const ArrayModel& am, fem::LinearSystem& sys) {
   sys.rhs.assign(size_t(n_free), 0.0);
 }
 
+// As fem::external_load_vector (src/fem/loads.cpp:578-584): a load on an unknown node or
+// an out-of-range component carries nothing.
 void add_external_load(const ArrayModel& am, fem::LinearSystem& sys) {
   for (const ArrayModel::NodalValue& f : am.load) {
-    if (f.node < 0 || size_t(f.node) >= am.num_nodes())
-      throw std::runtime_error("*CLOAD references unknown node");
-    if (f.comp < 1 || f.comp > kDof) throw std::runtime_error("*CLOAD dof out of range");
+    if (f.node < 0 || size_t(f.node) >= am.num_nodes() || f.comp < 1 || f.comp > kDof) continue;
     const Index r = sys.dof_eq[size_t(f.node) * kDof + size_t(f.comp - 1)];
     if (r >= 0) sys.rhs[size_t(r)] += f.value;
   }

@@ -78,6 +78,12 @@ class Model:
                 raise ValueError(f"element type {t} outside the GPU path")
             if c.ndim != 2 or c.shape[1] != NODES[t]:
                 raise ValueError("connectivity shape does not match the element type")
+            srt = np.sort(c, axis=1)
+            if c.shape[0] and bool((srt[:, 1:] == srt[:, :-1]).any()):
+                # one incidence per node and element is what the row gather and the
+                # node sums assume; a collapsed element repeats a node
+                raise ValueError("an element lists a node twice (collapsed element): "
+                                 "outside the GPU path")
         n = self.n_elem
         self.E = np.array(np.broadcast_to(np.asarray(self.E, np.float64), (n,)))
         self.nu = np.array(np.broadcast_to(np.asarray(self.nu, np.float64), (n,)))
@@ -126,7 +132,11 @@ class Model:
                 raise RuntimeError("*BOUNDARY dof out of range")
             g = 3 * self.spc_node.astype(np.int64) + self.spc_comp - 1
             fixed[g] = True
-            pres[g] = self.spc_value  # last card wins, as the sequential loop
+            # the last card of a DOF wins, as in the reference's sequential loop: the
+            # first occurrence of each DOF in the reversed card list
+            gr = g[::-1]
+            uniq, first = np.unique(gr, return_index=True)
+            pres[uniq] = self.spc_value[::-1][first]
         dof_eq = np.full(nd, -1, np.int32)
         free = ~fixed
         dof_eq[free] = np.arange(int(free.sum()), dtype=np.int32)

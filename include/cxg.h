@@ -231,6 +231,22 @@ cxg_status cxg_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double*
                            const double* committed, double* trial, double* vals,
                            double* f_int, uint8_t* live);
 
+/* ---- material points: MaterialModel::evaluate at n independent points --------
+ * Replaces the virtual MaterialModel::evaluate (fem/material_model.hpp:77) of
+ * J2PlasticMaterial (src/fem/plasticity.cpp:90-236) and ElasticIsoMaterial
+ * (src/fem/element.cpp:856-871), point by point, on the device.
+ * mat_id[n_points]: index into mats (host array of n_mats) or -1 for an isotropic
+ * elastic point with E[p], nu[p]; NULL: every point elastic (E and nu required; they
+ * may be NULL when every point has a J2 material). strain[6 n]: total strain (Voigt,
+ * engineering shear). committed, trial: SoA history, 13 planes of n_points; a J2 point
+ * reads committed and writes trial, an elastic point leaves trial untouched (as the
+ * reference). stress[6 n] and tangent[36 n] (row-major 6x6 D_t) are overwritten.
+ * Synchronizes the context's stream. */
+cxg_status cxg_material_evaluate(cxg_context* ctx, int64_t n_points, const int32_t* mat_id,
+                                 const cxg_j2_material* mats, int32_t n_mats, const double* E,
+                                 const double* nu, const double* strain, const double* committed,
+                                 double* trial, double* stress, double* tangent);
+
 /* MaterialPoints::commit (fem/assembly.hpp:84-87): committed <- trial. */
 cxg_status cxg_commit(cxg_context* ctx, int64_t n_points, double* committed,
                       const double* trial);
@@ -307,6 +323,11 @@ cxg_status cxg_pcg(cxg_context* ctx, int32_t n, const int64_t* row_ptr, const in
  * time (update u in place between iterations). cxg_graph_capture_linear does the
  * same for cxg_assemble_linear (no live flags in graphs). The graph keeps its own
  * copy of the material table; it must be destroyed before its pattern and mesh.
+ * While a graph of a pattern is alive, the pattern's element arenas, gather order and
+ * work queue stay where the graph found them: a later call that would have to rebuild
+ * one of them (more load cases than any capture or earlier call used, or a changed
+ * "gather_tile" or "chunk_elems" option) returns CXG_ERR_STATE instead; destroy the
+ * pattern's graphs first.
  * Kernel status (non-positive Jacobian) is reported by cxg_check after replays. */
 cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                     const double* E, const double* nu, double* vals,

@@ -142,3 +142,57 @@ def test_device_edge_models(port, kind, fam_i):
     f_ext = np.zeros((m.n_nodes, 3))
     np.add.at(f_ext, (m.cload_node, m.cload_comp - 1), m.cload_value)
     assert np.array_equal(o["reaction"].reshape(-1, 3)[unused], -f_ext[unused])
+
+
+def _collapsed_hex_model():
+    """A 2x1x1 C3D8 block whose second element has edge 2-3 collapsed (node 3 := node 2),
+    a wedge-shaped hex with a positive Jacobian at every Gauss point. Built valid, then
+    collapsed in place: the array model itself refuses such elements."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, conn = meshgen.hex_grid(2, 1, 1, (1.0, 0.5, 0.5), _lib.C3D8)
+    m = Model(coords, [(_lib.C3D8, conn)], 210000.0, 0.3)
+    m.blocks[0][1][1, 3] = m.blocks[0][1][1, 2]
+    return m
+
+
+def test_model_refuses_collapsed_element():
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import Model
+    coords, conn = meshgen.hex_grid(2, 1, 1, (1.0, 0.5, 0.5), _lib.C3D8)
+    conn[1, 7] = conn[1, 6]
+    with pytest.raises(ValueError, match="collapsed element"):
+        Model(coords, [(_lib.C3D8, conn)], 210000.0, 0.3)
+
+
+def test_reference_assembles_the_collapsed_element(ref):
+    """The reference accepts the collapsed hex (the mesh the device path refuses)."""
+    m = _collapsed_hex_model()
+    s = ref.assemble_linear_static(m)
+    assert s.n_free == 3 * m.n_nodes and np.isfinite(s.vals).all()
+
+
+@pytest.mark.gpu
+def test_device_refuses_collapsed_element():
+    """cxg_mesh_create refuses an element that lists a node twice (the row gather and the
+    node sums hold one incidence per node and element), and so does the C++ drop-in on the
+    reference's own Model, with the same message: no wrong numbers come back."""
+    import ctypes as C
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from calculixpp_amd import _lib, fem
+    m = _collapsed_hex_model()
+    ctx = fem.context()
+    conn = torch.from_numpy(m.blocks[0][1]).to(ctx.device)
+    blocks = (_lib.Block * 1)()
+    blocks[0].type, blocks[0].n_elem = _lib.C3D8, conn.shape[0]
+    blocks[0].elem_offset, blocks[0].gp_offset, blocks[0].conn = 0, 0, conn.data_ptr()
+    h = C.c_void_p()
+    st = ctx.lib.cxg_mesh_create(ctx.h, m.n_nodes, 1, blocks, C.byref(h))
+    assert st == _lib.ERR_UNSUPPORTED and not h.value
+    assert b"collapsed element" in ctx.lib.cxg_last_error(ctx.h)
+    from oracle.oracle import Oracle, OracleError, dropin_available
+    if dropin_available():
+        with pytest.raises(OracleError, match="collapsed element"):
+            Oracle("dropin").assemble_linear_static(m)

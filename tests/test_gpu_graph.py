@@ -137,3 +137,47 @@ def test_graph_reports_bad_jacobian_on_check(gpu):
     g.launch()
     ctx.check_kernels()
     torch.cuda.synchronize()
+
+
+def test_graph_keeps_its_buffers(gpu):
+    """While a graph of a pattern is alive, a call that would rebuild a buffer the graph
+    addresses (a larger displacement arena for more load cases, another gather order) is
+    refused with CXG_ERR_STATE; once the graph is destroyed the same call runs."""
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    m = meshgen.block(3, _lib.C3D8, seed_geo=51, seed_E=52)
+    ctx = gpu.context()
+    a = gpu.Assembler(m, ctx)
+    dev = ctx.device
+    u = torch.from_numpy(meshgen.smooth_field(m.coords, 53)).to(dev)
+    vals, f = a.new_values(), torch.empty(3 * m.n_nodes, dtype=torch.float64, device=dev)
+    g = a.capture(vals, u, f)  # one load case: the displacement arena holds one field
+    g.launch()
+    ctx.check_kernels()
+    u2 = torch.cat([u, 2.0 * u])
+    f2 = torch.empty(2 * 3 * m.n_nodes, dtype=torch.float64, device=dev)
+    p = gpu._ptr
+
+    def two_cases():
+        return ctx.lib.cxg_assemble_linear_cases(ctx.h, a.pattern.h, p(a.mesh.coords),
+                                                 p(a.mesh.E), p(a.mesh.nu), p(vals), 2, p(u2),
+                                                 p(f2), None)
+    assert two_cases() == _lib.ERR_STATE
+    assert b"captured graph" in ctx.lib.cxg_last_error(ctx.h)
+    tile = ctx.options.get("gather_tile", 8)
+    ctx.set_option("gather_tile", 4)
+    try:
+        with pytest.raises(_lib.CxgError, match="captured graph"):
+            a.assemble(vals, u, f)
+    finally:
+        ctx.set_option("gather_tile", tile)
+    g.launch()  # the graph still replays on its own buffers
+    ctx.check_kernels()
+    ve, fe = a.new_values(), torch.empty_like(f)
+    a.assemble(ve, u, fe)
+    torch.cuda.synchronize()
+    assert _same(vals[: a.pattern.nnz], ve[: a.pattern.nnz]) and _same(f, fe)
+    del g
+    assert two_cases() == _lib.OK
+    torch.cuda.synchronize()
+    assert rel_l2(f2[: 3 * m.n_nodes].cpu().numpy(), fe.cpu().numpy()) <= 1e-12
