@@ -73,6 +73,7 @@ _SIGS = {
     "cxg_assemble_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_assemble_linear_cases": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int32, _P, _P, _P]),
     "cxg_internal_force": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "cxg_internal_force_noalloc": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "cxg_reduce_free": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "cxg_assemble_j2": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
                                   _P, _P, _P, _P, _P, _P]),

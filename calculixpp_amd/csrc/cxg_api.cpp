@@ -264,6 +264,7 @@ cxg_status cxg_create(int device, void* stream, cxg_context** out) {
     CXG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->flag_host), sizeof(int), hipHostMallocDefault));
     *ctx->flag_host = 0;
     CXG_HIP(hipMemcpy(ctx->flag_dev, ctx->flag_host, sizeof(int), hipMemcpyHostToDevice));
+    ctx->scratch = cxg::scratch_device_ptr(device);
     int cus = 0;
     CXG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (cus > 0) ctx->n_cu = cus;
@@ -544,6 +545,18 @@ cxg_status cxg_internal_force(cxg_context* ctx, const cxg_mesh* mesh, const doub
   if (!mesh || !coords || !E || !nu || !u || !f_int) return cxg::fail(ctx, CXG_ERR_ARG, "bad internal_force arguments");
   return cxg::guarded(ctx, [&] {
     cxg::launch_internal_force(ctx, mesh, coords, E, nu, u, f_int);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_internal_force_noalloc(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                                      const double* E, const double* nu, const double* u,
+                                      double* f_int) {
+  if (!mesh || !coords || !E || !nu || !u || !f_int)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad internal_force_noalloc arguments");
+  if (mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "mesh belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_internal_force_noalloc(ctx, mesh, coords, E, nu, u, f_int);
     cxg::check_flags(ctx);
   });
 }

@@ -24,10 +24,34 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "cxg_internal.hpp"
 
+// Element-force arena of the allocation-free internal force (cxg_internal_force_noalloc):
+// a fixed device array, 3 doubles per connectivity entry, kScratchDoubles at most. Each
+// device that loads the library holds its own copy until the process ends. In host code
+// `scratch` names a host-side stand-in, never a device address: kernels receive the
+// address scratch_device_ptr() obtained from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[cxg::kScratchDoubles];
+
 namespace cxg {
+
+double* scratch_device_ptr(int device) {
+  // One lookup per device, with that device current (cxg_create calls this after
+  // hipSetDevice); every later context on the device reuses the address.
+  static std::mutex mu;
+  static std::vector<double*> table;
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0) throw Error(CXG_ERR_ARG, "bad device");
+  if (table.size() <= size_t(device)) table.resize(size_t(device) + 1, nullptr);
+  if (!table[size_t(device)]) {
+    void* ptr = nullptr;
+    CXG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    table[size_t(device)] = static_cast<double*>(ptr);
+  }
+  return table[size_t(device)];
+}
 namespace {
 
 constexpr int kWave = 64;
@@ -2709,6 +2733,17 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
   }
 }
 
+void launch_internal_force_noalloc(cxg_context* ctx, const cxg_mesh* m, const double* coords,
+                                   const double* E, const double* nu, const double* u,
+                                   double* f_int) {
+  const int64_t need = 3 * int64_t(m->conn.n);
+  if (need > kScratchDoubles)
+    throw Error(CXG_ERR_ARG, "cxg_internal_force_noalloc: the mesh needs " + std::to_string(need) +
+                                 " doubles of scratch (3 per connectivity entry), the maximum is " +
+                                 std::to_string(kScratchDoubles));
+  if (!ctx->scratch) throw Error(CXG_ERR_STATE, "no scratch address on this context");
+  launch_internal_force(ctx, m, coords, E, nu, u, f_int, ctx->scratch);
+}
 
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
                         const double* f_ext, const double* f_c, double* rhs) {

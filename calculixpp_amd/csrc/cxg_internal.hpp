@@ -12,6 +12,10 @@
 
 namespace cxg {
 
+// Doubles in the fixed device scratch array of the allocation-free entry point
+// (64 MiB; CXG_NOALLOC_SCRATCH_DOUBLES in cxg.h).
+constexpr int64_t kScratchDoubles = CXG_NOALLOC_SCRATCH_DOUBLES;
+
 // Error carried from the internals to the C ABI, where it becomes a status code and
 // the context's last-error text. Never crosses the ABI as an exception.
 struct Error : std::runtime_error {
@@ -132,6 +136,7 @@ struct cxg_context {
   // and four consecutive partners, 1: 2x2 pairs of two node couples)
   int elem_tiling = 1;
   std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
+  double* scratch = nullptr;  // device address of the fixed scratch array on `device`
 };
 
 namespace cxg {
@@ -269,6 +274,14 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* coords,
                            const double* E, const double* nu, const double* u, double* f_int,
                            double* arena = nullptr);
+// The same into the library's fixed scratch array (no allocation; CXG_ERR_ARG beyond
+// kScratchDoubles).
+void launch_internal_force_noalloc(cxg_context* ctx, const cxg_mesh* m, const double* coords,
+                                   const double* E, const double* nu, const double* u,
+                                   double* f_int);
+// Device address of the fixed scratch array on `device` (looked up once per device, with
+// the device current).
+double* scratch_device_ptr(int device);
 void launch_reduce_free(cxg_context* ctx, int32_t n_nodes, const int32_t* dof_eq,
                         const double* f_ext, const double* f_c, double* rhs);
 // J2 material tables: host bytes of the device form, and the context's cached upload.

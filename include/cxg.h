@@ -208,6 +208,23 @@ cxg_status cxg_internal_force(cxg_context* ctx, const cxg_mesh* mesh, const doub
                               const double* E, const double* nu, const double* u,
                               double* f_int);
 
+/* cxg_internal_force without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The element forces go to a
+ * fixed device array of CXG_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) that the library
+ * holds on every device it runs on until the process ends, 3 doubles per connectivity
+ * entry: meshes of at most 2,796,202 entries (349,525 C3D8 or 139,810 C3D20
+ * elements); a larger mesh returns CXG_ERR_ARG before anything is enqueued. The caller
+ * passes the mesh (created beforehand), inputs and outputs as for cxg_internal_force,
+ * with the same results bit for bit. Every call on one device shares that device's
+ * array, which keeps the last call's contents: calls that overlap on one device (two
+ * streams or two threads) race. Like cxg_internal_force it synchronizes the context's
+ * stream to report a non-positive Jacobian. */
+#define CXG_NOALLOC_SCRATCH_DOUBLES (8 * 1024 * 1024)
+cxg_status cxg_internal_force_noalloc(cxg_context* ctx, const cxg_mesh* mesh, const double* coords,
+                                      const double* E, const double* nu, const double* u,
+                                      double* f_int);
+
 /* rhs[eq(g)] += f_ext[g] - f_c[g] for every free g, where f_c = K_full u_c is the
  * internal force of the prescribed displacement field u_c (zero on free DOFs).
  * Together with cxg_internal_force this forms the reduced rhs of
