@@ -445,6 +445,35 @@ class MaterialPoints:
         ctx.check(ctx.lib.cxg_commit(ctx.h, self.n_points, _ptr(self.committed), _ptr(self.trial)))
 
 
+def evaluate_material_points(strain, mats=(), mat_id=None, E=None, nu=None, committed=None,
+                             device=None):
+    """MaterialModel::evaluate (fem/material_model.hpp:77) at n independent points on the
+    device (cxg_material_evaluate): J2PlasticMaterial::evaluate (plasticity.cpp:90-236)
+    where mat_id[p] >= 0 names one of `mats` (J2Params), ElasticIsoMaterial with E[p],
+    nu[p] where it is -1. strain (n, 6), committed (13, n). Returns host arrays
+    (stress (n, 6), tangent (n, 6, 6), trial (13, n)); trial holds the committed history
+    at elastic points."""
+    dev = _dev(device)
+    ctx = context(dev)
+    eps = torch.from_numpy(np.ascontiguousarray(strain, np.float64)).to(dev)
+    n = eps.shape[0]
+    hc = torch.zeros((_lib.STATE_COMPONENTS, n), dtype=torch.float64, device=dev) \
+        if committed is None else torch.from_numpy(np.ascontiguousarray(committed, np.float64)).to(dev)
+    trial = hc.clone()
+    table = (_lib.J2Material * max(len(mats), 1))()
+    for i, m in enumerate(mats):
+        table[i] = m.to_c()
+    mid = None if mat_id is None else torch.from_numpy(np.ascontiguousarray(mat_id, np.int32)).to(dev)
+    Ed = None if E is None else torch.from_numpy(np.broadcast_to(np.asarray(E, np.float64), (n,)).copy()).to(dev)
+    nud = None if nu is None else torch.from_numpy(np.broadcast_to(np.asarray(nu, np.float64), (n,)).copy()).to(dev)
+    sig = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    D = torch.empty((n, 6, 6), dtype=torch.float64, device=dev)
+    ctx.check(ctx.lib.cxg_material_evaluate(ctx.h, n, _ptr(mid), table, len(mats), _ptr(Ed),
+                                            _ptr(nud), _ptr(eps), _ptr(hc), _ptr(trial),
+                                            _ptr(sig), _ptr(D)))
+    return sig.cpu().numpy(), D.cpu().numpy(), trial.cpu().numpy()
+
+
 def make_material_points(model: Model, device=None) -> MaterialPoints:
     """fem::make_material_points (fem/assembly.cpp:357-383): virgin history."""
     return MaterialPoints(model.n_points, device)

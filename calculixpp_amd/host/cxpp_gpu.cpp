@@ -1,17 +1,26 @@
 // cxpp::gpu — the reference's assembly entry points on libcxg.so (see cxpp_gpu.hpp).
 //
-// Each call turns the Model into the C ABI's arrays (node coordinates in index order,
+// A call turns the Model into the C ABI's arrays (node coordinates in index order,
 // connectivity as node indices in runs of one element family, per-element E, nu and
 // J2 material), assembles on the device over the pattern in which every DOF is free,
 // and reduces on the host into the reference's LinearSystem: a free/free entry lands
 // at (dof_eq[row], dof_eq[col]) when some element contributed a value other than 0.0
 // to it (the key rule of scatter_block, src/fem/assembly.cpp:76-100); an entry in a
 // prescribed column moves to the load, rhs[dof_eq[row]] -= value * prescribed[col]
-// (assembly.cpp:95-96); rows of constrained DOFs are dropped (assembly.cpp:86). The
-// DOF map and the constraint transform are the reference's own (build_dof_map,
-// assembly.cpp:19-42, over fem::build_constraint_transform) and must reduce to single
-// point constraints; external loads come from fem::external_load_vector, as in
-// assemble_linear_static (assembly.cpp:250).
+// (assembly.cpp:95-96); rows of constrained DOFs are dropped (assembly.cpp:86).
+// External loads come from fem::external_load_vector, as in assemble_linear_static
+// (assembly.cpp:250).
+//
+// The numbering of LinearSystem (LinearSystem::dof_eq, assembly.hpp:25-27) for models
+// whose only constraints are *BOUNDARY cards: equations in ascending global DOF order
+// over the DOFs no card names, each prescribed DOF expanding to its value (the
+// transform the reference's fem::build_constraint_transform returns when a model has no
+// equations, constraint_transform.cpp:25-131). Models with *EQUATION / MPC / rigid /
+// coupling / tie constraints go through build_constraint_transform itself and are
+// refused unless it reduces to single point constraints.
+//
+// Residency (cxpp_gpu.hpp): the last model of the calling thread stays on the device,
+// keyed by its address and a fingerprint of the uploaded arrays.
 #include "cxpp_gpu.hpp"
 
 #include <algorithm>
@@ -30,13 +39,17 @@ namespace cxpp::gpu {
 namespace {
 
 constexpr int kDof = 3;
+constexpr size_t kHist = CXG_STATE_COMPONENTS;
 
 thread_local int t_device = 0;
 thread_local cxg_context* t_ctx = nullptr;  // kept until the process ends
 thread_local int t_ctx_device = -1;
 
+void drop_resident();
+
 cxg_context* context() {
   if (!t_ctx || t_ctx_device != t_device) {
+    drop_resident();  // device memory of the previous context
     if (t_ctx) cxg_destroy(t_ctx);
     t_ctx = nullptr;
     cxg_context* h = nullptr;
@@ -58,7 +71,7 @@ class Buf {
  public:
   Buf(cxg_context* c, size_t n) : c_(c), n_(n) {
     void* p = nullptr;
-    check(c, cxg_malloc(c, n * sizeof(T), &p));
+    check(c, cxg_malloc(c, std::max<size_t>(n, 1) * sizeof(T), &p));
     p_ = static_cast<T*>(p);
   }
   Buf(cxg_context* c, const std::vector<T>& host) : Buf(c, host.size()) { put(host.data()); }
@@ -68,17 +81,73 @@ class Buf {
     if (p_) cxg_free(c_, p_);
   }
   T* get() const { return p_; }
+  size_t size() const { return n_; }
   void put(const T* src) { check(c_, cxg_copy_to_device(c_, p_, src, n_ * sizeof(T))); }
+  void put(const std::vector<T>& v) {
+    if (v.size() != n_) throw std::runtime_error("cxpp::gpu: internal buffer size mismatch");
+    put(v.data());
+  }
   std::vector<T> fetch(size_t n) const {
     std::vector<T> out(n);
     check(c_, cxg_copy_to_host(c_, out.data(), p_, n * sizeof(T)));
     return out;
+  }
+  void fetch_into(std::vector<T>& out) const {
+    out.resize(n_);
+    check(c_, cxg_copy_to_host(c_, out.data(), p_, n_ * sizeof(T)));
   }
 
  private:
   cxg_context* c_;
   size_t n_;
   T* p_ = nullptr;
+};
+
+template <class T>
+using BufPtr = std::unique_ptr<Buf<T>>;
+
+// 64-bit fingerprint of host arrays (multiply-rotate per 8-byte word), used to tell
+// whether a model seen before still holds the same numbers.
+class Fingerprint {
+ public:
+  void word(uint64_t w) {
+    h_ ^= w * 0x9e3779b97f4a7c15ull;
+    h_ = ((h_ << 29) | (h_ >> 35)) * 0xbf58476d1ce4e5b9ull;
+  }
+  void bytes(const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, c + i, 8);
+      word(w);
+    }
+    uint64_t tail = uint64_t(n) << 56;
+    if (i < n) std::memcpy(&tail, c + i, n - i);
+    word(tail);
+  }
+  template <class T>
+  void vec(const std::vector<T>& v) {
+    word(v.size());
+    bytes(v.data(), v.size() * sizeof(T));
+  }
+  void flags(const std::vector<bool>& v) {
+    word(v.size());
+    uint64_t acc = 0;
+    for (size_t i = 0; i < v.size(); ++i) {
+      acc = (acc << 1) | uint64_t(v[i]);
+      if (i % 64 == 63) word(acc), acc = 0;
+    }
+    word(acc);
+  }
+  void text(const std::string& s) {
+    word(s.size());
+    bytes(s.data(), s.size());
+  }
+  uint64_t value() const { return h_; }
+
+ private:
+  uint64_t h_ = 0x243f6a8885a308d3ull;
 };
 
 bool on_gpu_path(ElementType t) {
@@ -115,6 +184,21 @@ ArrayModel mesh_arrays(const Model& model) {
   return a;
 }
 
+// Everything of an array model that goes to the device with its mesh.
+uint64_t mesh_fingerprint(const ArrayModel& a) {
+  Fingerprint f;
+  f.vec(a.coords);
+  f.word(a.blocks.size());
+  for (const ArrayModel::Block& b : a.blocks) {
+    f.word(uint64_t(b.type));
+    f.vec(b.conn);
+  }
+  f.vec(a.E);
+  f.vec(a.nu);
+  f.flags(a.active);
+  return f.value();
+}
+
 // An array model's active elements on the device, as runs of one family in model order.
 struct DeviceMesh {
   cxg_context* c;
@@ -122,8 +206,8 @@ struct DeviceMesh {
   std::vector<int64_t> gp_first;  // first device Gauss point of each device element
   int32_t n_nodes = 0;
   int64_t n_points = 0;
-  std::unique_ptr<Buf<double>> coords, E, nu;
-  std::unique_ptr<Buf<int32_t>> conn;
+  BufPtr<double> coords, E, nu;
+  BufPtr<int32_t> conn;
   cxg_mesh* mesh = nullptr;
 
   DeviceMesh(cxg_context* ctx, const ArrayModel& am) : c(ctx) {
@@ -174,7 +258,6 @@ struct DeviceMesh {
     for (size_t b = 0; b < blocks.size(); ++b) blocks[b].conn = conn->get() + block_conn[b];
     check(c, cxg_mesh_create(c, n_nodes, int32_t(blocks.size()), blocks.data(), &mesh));
   }
-  DeviceMesh(cxg_context* ctx, const Model& model) : DeviceMesh(ctx, mesh_arrays(model)) {}
   DeviceMesh(const DeviceMesh&) = delete;
   DeviceMesh& operator=(const DeviceMesh&) = delete;
   ~DeviceMesh() {
@@ -182,12 +265,15 @@ struct DeviceMesh {
   }
 };
 
-// The pattern in which every DOF of the mesh is a free equation.
+// The pattern in which every DOF of the mesh is a free equation, with a host copy of its
+// row pointers and column indices (what the reduction to the reference's COO walks).
 struct AllFreePattern {
   cxg_context* c;
   cxg_pattern* p = nullptr;
   int64_t nnz = 0;
   int32_t n = 0;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> ci;
   AllFreePattern(cxg_context* ctx, const cxg_mesh* mesh, int32_t n_nodes) : c(ctx) {
     const int64_t nd = int64_t(n_nodes) * kDof;
     if (nd > INT32_MAX) throw std::runtime_error("cxpp::gpu: model too large for int32 equations");
@@ -198,6 +284,11 @@ struct AllFreePattern {
     check(c, cxg_pattern_build(c, mesh, d.get(), n, &p));
     int32_t nf = 0;
     check(c, cxg_pattern_info(p, &nf, &nnz));
+    Buf<int64_t> rp_dev(c, size_t(n) + 1);
+    Buf<int32_t> ci_dev(c, size_t(nnz));
+    check(c, cxg_pattern_copy(c, p, rp_dev.get(), ci_dev.get()));
+    rp = rp_dev.fetch(size_t(n) + 1);
+    ci = ci_dev.fetch(size_t(nnz));
   }
   AllFreePattern(const AllFreePattern&) = delete;
   AllFreePattern& operator=(const AllFreePattern&) = delete;
@@ -206,74 +297,198 @@ struct AllFreePattern {
   }
 };
 
-// build_dof_map (assembly.cpp:19-42) through the public constraint transform; the GPU
-// path takes single point constraints only.
-void dof_map(const Model& model, fem::LinearSystem& sys) {
-  const Mesh& mesh = model.mesh;
-  const size_t n_dof = mesh.num_nodes() * kDof;
-  sys.prescribed.assign(n_dof, 0.0);
-  sys.prescribed_amp.assign(n_dof, std::string{});
-  std::vector<bool> constrained(n_dof, false);
-  for (const Spc& spc : model.spcs) {
-    const Index ni = mesh.node_index(spc.node_id);
-    if (ni < 0) throw std::runtime_error("*BOUNDARY references unknown node");
-    if (spc.comp < 1 || spc.comp > kDof) throw std::runtime_error("*BOUNDARY dof out of range");
-    const size_t g = size_t(ni) * kDof + size_t(spc.comp - 1);
-    constrained[g] = true;
-    sys.prescribed[g] = spc.value;
-    sys.prescribed_amp[g] = spc.amplitude;
-  }
-  sys.transform = fem::build_constraint_transform(model, constrained, sys.prescribed);
-  sys.dof_eq = sys.transform.dof_eq;
-  sys.n_free = sys.transform.n_free;
-  sys.rhs.assign(size_t(sys.n_free), 0.0);
-  for (size_t g = 0; g < n_dof; ++g) {
-    const fem::DofExpansion& x = sys.transform.expansion[g];
-    const bool single = sys.dof_eq[g] >= 0
-                            ? x.terms.size() == 1 && x.terms[0].eq == sys.dof_eq[g] &&
-                                  x.terms[0].coeff == 1.0 && x.constant == 0.0
-                            : x.terms.empty() && !sys.transform.is_slave[g];
-    if (!single)
-      throw std::runtime_error("cxpp::gpu: *EQUATION / MPC / rigid / tie constraints are "
-                               "outside the GPU path");
-  }
+// J2 inputs of the device elements: the material table (one entry per distinct
+// elastic + plastic pair) and each element's entry (-1: elastic).
+struct J2Table {
+  std::vector<cxg_j2_material> mats;
+  std::vector<int32_t> mat_id;
+};
+
+// The calling thread's resident model. Members are released in reverse order of
+// declaration, so buffers and the pattern go before the mesh they belong to.
+struct Resident {
+  const void* owner = nullptr;
+  uint64_t mesh_fp = 0;
+  std::unique_ptr<DeviceMesh> dm;
+  std::unique_ptr<AllFreePattern> pat;
+  BufPtr<double> vals;
+  BufPtr<uint8_t> live;
+  BufPtr<double> u, f;  // 3 doubles per node
+  // DOF map of the model's constraints
+  bool dof_valid = false;
+  uint64_t dof_fp = 0;
+  fem::LinearSystem dof;  // prescribed, prescribed_amp, transform, dof_eq, n_free
+  // J2 material table of the device elements
+  bool j2_valid = false;
+  uint64_t j2_fp = 0;
+  J2Table j2;
+  BufPtr<int32_t> mid;
+  // committed / trial history (13 planes of n_points) and host copies of what the
+  // device holds
+  BufPtr<double> committed, trial;
+  std::vector<double> committed_dev, trial_dev;
+  bool has_committed = false, has_trial = false;
+};
+
+thread_local Resident* t_res = nullptr;  // released by resident_reset or a model change
+thread_local ResidentStats t_stats;
+
+void drop_resident() {
+  delete t_res;
+  t_res = nullptr;
 }
 
-// build_dof_map for single point constraints only: free equations in ascending global DOF
-// order, constrained DOFs -1 with their prescribed value; the transform expands a free
-// DOF to itself and a constrained one to its value (constraint_transform.cpp:25-131 for
-// a model without equations).
-void dof_map(const ArrayModel& am, fem::LinearSystem& sys) {
-  const size_t n_dof = am.num_nodes() * kDof;
-  sys.prescribed.assign(n_dof, 0.0);
-  sys.prescribed_amp.assign(n_dof, std::string{});
-  std::vector<bool> constrained(n_dof, false);
-  for (const ArrayModel::NodalValue& spc : am.spc) {
-    if (spc.node < 0 || size_t(spc.node) >= am.num_nodes())
-      throw std::runtime_error("*BOUNDARY references unknown node");
-    if (spc.comp < 1 || spc.comp > kDof) throw std::runtime_error("*BOUNDARY dof out of range");
-    const size_t g = size_t(spc.node) * kDof + size_t(spc.comp - 1);
-    constrained[g] = true;
-    sys.prescribed[g] = spc.value;
+// The resident copy of the array model `am` seen through `owner` (the caller's Model or
+// ArrayModel): reused when both the owner and the fingerprint match, else uploaded.
+Resident& resident(const ArrayModel& am, const void* owner) {
+  cxg_context* c = context();
+  ++t_stats.calls;
+  const uint64_t fp = mesh_fingerprint(am);
+  if (!t_res || !t_res->dm || t_res->owner != owner || t_res->mesh_fp != fp) {
+    drop_resident();
+    auto r = std::make_unique<Resident>();
+    r->dm = std::make_unique<DeviceMesh>(c, am);
+    r->owner = owner;
+    r->mesh_fp = fp;
+    const size_t nd = size_t(r->dm->n_nodes) * kDof;
+    r->u = std::make_unique<Buf<double>>(c, nd);
+    r->f = std::make_unique<Buf<double>>(c, nd);
+    t_res = r.release();
+    ++t_stats.mesh_uploads;
   }
-  fem::ConstraintTransform& T = sys.transform;
+  return *t_res;
+}
+
+AllFreePattern& pattern(Resident& R) {
+  if (!R.pat) {
+    cxg_context* c = context();
+    R.pat = std::make_unique<AllFreePattern>(c, R.dm->mesh, R.dm->n_nodes);
+    R.vals = std::make_unique<Buf<double>>(c, size_t(R.pat->nnz));
+    R.live = std::make_unique<Buf<uint8_t>>(c, size_t(R.pat->nnz));
+    ++t_stats.pattern_builds;
+  }
+  return *R.pat;
+}
+
+// The SPC-only constraint transform: free equations in ascending global DOF order over
+// the DOFs no card names; a free DOF expands to itself, a prescribed one to its value
+// with itself as its one SPC master (what build_constraint_transform gives for a model
+// without equations).
+fem::ConstraintTransform spc_transform(const std::vector<char>& fixed,
+                                       const std::vector<Real>& value) {
+  const size_t n_dof = fixed.size();
+  fem::ConstraintTransform T;
   T.dof_eq.assign(n_dof, -1);
   T.expansion.assign(n_dof, fem::DofExpansion{});
   T.is_slave.assign(n_dof, false);
-  Index n_free = 0;
+  T.spc_terms.assign(n_dof, {});
+  Index next = 0;
   for (size_t g = 0; g < n_dof; ++g) {
-    if (constrained[g]) {
-      T.expansion[g].constant = sys.prescribed[g];
+    if (fixed[g]) {
+      T.expansion[g].constant = value[g];
+      T.spc_terms[g].push_back(fem::DofTerm{Index(g), 1.0});
     } else {
-      T.dof_eq[g] = n_free;
-      T.expansion[g].terms.push_back(fem::DofTerm{n_free, 1.0});
-      ++n_free;
+      T.expansion[g].terms.push_back(fem::DofTerm{next, 1.0});
+      T.dof_eq[g] = next++;
     }
   }
-  T.n_free = n_free;
-  sys.dof_eq = T.dof_eq;
-  sys.n_free = n_free;
-  sys.rhs.assign(size_t(n_free), 0.0);
+  T.n_free = next;
+  return T;
+}
+
+void adopt_transform(fem::LinearSystem& sys) {
+  sys.dof_eq = sys.transform.dof_eq;
+  sys.n_free = sys.transform.n_free;
+  sys.rhs.assign(size_t(sys.n_free), 0.0);
+}
+
+bool spc_only(const Model& model) {
+  return model.equations.empty() && model.mpcs.empty() && model.rigid_bodies.empty() &&
+         model.couplings.empty() && model.ties.empty();
+}
+
+// LinearSystem numbering of a reference Model (see the top of this file). The cards are
+// applied in deck order, so a DOF named twice takes its last value and amplitude.
+void number_dofs(const Model& model, fem::LinearSystem& sys) {
+  const Mesh& mesh = model.mesh;
+  const size_t n_dof = mesh.num_nodes() * kDof;
+  std::vector<char> fixed(n_dof, 0);
+  std::vector<Real> value(n_dof, 0.0);
+  std::vector<std::string> amp(n_dof);
+  for (const Spc& card : model.spcs) {
+    const Index node = mesh.node_index(card.node_id);
+    if (node < 0) throw std::runtime_error("*BOUNDARY references unknown node");
+    if (card.comp < 1 || card.comp > kDof) throw std::runtime_error("*BOUNDARY dof out of range");
+    const size_t dof = size_t(node) * kDof + size_t(card.comp - 1);
+    fixed[dof] = 1;
+    value[dof] = card.value;
+    amp[dof] = card.amplitude;
+  }
+  sys.prescribed = value;
+  sys.prescribed_amp = std::move(amp);
+  if (spc_only(model)) {
+    sys.transform = spc_transform(fixed, value);
+  } else {
+    // equations present: the reference's own transform, which must come out SPC-only
+    const std::vector<bool> as_bool(fixed.begin(), fixed.end());
+    sys.transform = fem::build_constraint_transform(model, as_bool, value);
+    for (size_t g = 0; g < n_dof; ++g) {
+      const fem::DofExpansion& x = sys.transform.expansion[g];
+      const Index eq = sys.transform.dof_eq[g];
+      const bool plain = eq >= 0 ? x.terms.size() == 1 && x.terms[0].eq == eq &&
+                                       x.terms[0].coeff == 1.0 && x.constant == 0.0
+                                 : x.terms.empty() && !sys.transform.is_slave[g];
+      if (!plain)
+        throw std::runtime_error("cxpp::gpu: *EQUATION / MPC / rigid / tie constraints are "
+                                 "outside the GPU path");
+    }
+  }
+  adopt_transform(sys);
+}
+
+// The DOF map of `model` through the resident cache (rebuilt when the constraint cards
+// change).
+void dof_map(const Model& model, Resident& R, fem::LinearSystem& sys) {
+  Fingerprint f;
+  f.word(model.mesh.num_nodes());
+  f.word(model.spcs.size());
+  for (const Spc& s : model.spcs) {
+    f.word(uint64_t(s.node_id));
+    f.word(uint64_t(s.comp));
+    f.bytes(&s.value, sizeof(Real));
+    f.text(s.amplitude);
+  }
+  f.word(model.equations.size() + model.mpcs.size() + model.rigid_bodies.size() +
+         model.couplings.size() + model.ties.size());
+  if (!spc_only(model) || !R.dof_valid || R.dof_fp != f.value()) {
+    number_dofs(model, R.dof);
+    R.dof_valid = spc_only(model);
+    R.dof_fp = f.value();
+    ++t_stats.dof_maps;
+  }
+  sys.prescribed = R.dof.prescribed;
+  sys.prescribed_amp = R.dof.prescribed_amp;
+  sys.transform = R.dof.transform;
+  adopt_transform(sys);
+}
+
+// The same numbering for an ArrayModel's single point constraints.
+void dof_map(const ArrayModel& am, fem::LinearSystem& sys) {
+  const size_t n_dof = am.num_nodes() * kDof;
+  std::vector<char> fixed(n_dof, 0);
+  std::vector<Real> value(n_dof, 0.0);
+  for (const ArrayModel::NodalValue& card : am.spc) {
+    if (card.node < 0 || size_t(card.node) >= am.num_nodes())
+      throw std::runtime_error("*BOUNDARY references unknown node");
+    if (card.comp < 1 || card.comp > kDof) throw std::runtime_error("*BOUNDARY dof out of range");
+    const size_t dof = size_t(card.node) * kDof + size_t(card.comp - 1);
+    fixed[dof] = 1;
+    value[dof] = card.value;
+  }
+  sys.prescribed = value;
+  sys.prescribed_amp.assign(n_dof, std::string{});
+  sys.transform = spc_transform(fixed, value);
+  adopt_transform(sys);
 }
 
 // As fem::external_load_vector (src/fem/loads.cpp:578-584): a load on an unknown node or
@@ -286,6 +501,12 @@ void add_external_load(const ArrayModel& am, fem::LinearSystem& sys) {
   }
 }
 
+void add_external_load(const Model& model, fem::LinearSystem& sys) {
+  const std::vector<Real> f = fem::external_load_vector(model);
+  for (size_t g = 0; g < f.size(); ++g)
+    if (f[g] != 0.0 && sys.dof_eq[g] >= 0) sys.rhs[size_t(sys.dof_eq[g])] += f[g];
+}
+
 void require_mechanical(const Model& model) {
   if (!model.springs.empty()) throw std::runtime_error("cxpp::gpu: *SPRING is outside the GPU path");
   for (Real v : fem::thermal_strain_load_vector(model))
@@ -295,20 +516,15 @@ void require_mechanical(const Model& model) {
 // All-free CSR + live flags -> the reference's reduced COO and load correction.
 void reduce(const AllFreePattern& pat, const std::vector<double>& vals,
             const std::vector<uint8_t>& live, fem::LinearSystem& sys) {
-  Buf<int64_t> rp_dev(pat.c, size_t(pat.n) + 1);
-  Buf<int32_t> ci_dev(pat.c, size_t(std::max<int64_t>(pat.nnz, 1)));
-  check(pat.c, cxg_pattern_copy(pat.c, pat.p, rp_dev.get(), ci_dev.get()));
-  const std::vector<int64_t> rp = rp_dev.fetch(size_t(pat.n) + 1);
-  const std::vector<int32_t> ci = ci_dev.fetch(size_t(pat.nnz));
   sys.rows.clear();
   sys.cols.clear();
   sys.vals.clear();
   for (int32_t g = 0; g < pat.n; ++g) {
     const Index r = sys.dof_eq[size_t(g)];
     if (r < 0) continue;  // constrained row
-    for (int64_t k = rp[size_t(g)]; k < rp[size_t(g) + 1]; ++k) {
+    for (int64_t k = pat.rp[size_t(g)]; k < pat.rp[size_t(g) + 1]; ++k) {
       if (!live[size_t(k)]) continue;  // every contribution was 0.0
-      const int32_t h = ci[size_t(k)];
+      const int32_t h = pat.ci[size_t(k)];
       const Index col = sys.dof_eq[size_t(h)];
       if (col >= 0) {
         sys.rows.push_back(r);
@@ -319,12 +535,6 @@ void reduce(const AllFreePattern& pat, const std::vector<double>& vals,
       }
     }
   }
-}
-
-void add_external_load(const Model& model, fem::LinearSystem& sys) {
-  const std::vector<Real> f = fem::external_load_vector(model);
-  for (size_t g = 0; g < f.size(); ++g)
-    if (f[g] != 0.0 && sys.dof_eq[g] >= 0) sys.rhs[size_t(sys.dof_eq[g])] += f[g];
 }
 
 cxg_j2_material to_cxg(const ElasticIso& el, const Plastic& pl) {
@@ -356,22 +566,13 @@ void require_material_points(const Model& model, const fem::MaterialPoints& mp) 
     throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
 }
 
-// J2 inputs of the device elements: the material table (one entry per distinct
-// elastic + plastic pair), each element's entry (-1: elastic) and the committed history
-// as SoA planes: plastic strain (6), eqplastic, back stress (6).
-struct J2Inputs {
-  std::vector<cxg_j2_material> mats;
-  std::vector<int32_t> mat_id;
-  std::vector<double> committed;
-};
-
-J2Inputs j2_inputs(const Model& model, const DeviceMesh& dm, const fem::MaterialPoints& mp) {
-  J2Inputs in;
+J2Table j2_table(const Model& model, const DeviceMesh& dm) {
+  J2Table t;
   const size_t ne = dm.elem.size();
   const std::vector<ElasticIso> el = model.element_elastic();
   const std::vector<std::optional<Plastic>> pl = model.element_plastic();
   std::map<std::vector<double>, int32_t> seen;
-  in.mat_id.assign(ne, -1);
+  t.mat_id.assign(ne, -1);
   for (size_t k = 0; k < ne; ++k) {
     const size_t e = dm.elem[k];
     if (!pl[e] || pl[e]->empty()) continue;
@@ -381,27 +582,70 @@ J2Inputs j2_inputs(const Model& model, const DeviceMesh& dm, const fem::Material
     key.insert(key.end(), m.yield, m.yield + m.n_table);
     auto it = seen.find(key);
     if (it == seen.end()) {
-      it = seen.emplace(key, int32_t(in.mats.size())).first;
-      in.mats.push_back(m);
+      it = seen.emplace(key, int32_t(t.mats.size())).first;
+      t.mats.push_back(m);
     }
-    in.mat_id[k] = it->second;
+    t.mat_id[k] = it->second;
   }
+  return t;
+}
+
+// The model's J2 table on the device (re-uploaded when the materials change).
+const J2Table& resident_j2(const Model& model, Resident& R) {
+  J2Table t = j2_table(model, *R.dm);
+  Fingerprint f;
+  f.vec(t.mat_id);
+  f.vec(t.mats);
+  if (!R.j2_valid || R.j2_fp != f.value() || R.j2.mat_id != t.mat_id) {
+    R.j2 = std::move(t);
+    R.mid = std::make_unique<Buf<int32_t>>(context(), R.j2.mat_id);
+    R.j2_fp = f.value();
+    R.j2_valid = true;
+  }
+  return R.j2;
+}
+
+// The committed history of mp on the device: compared with what the device holds and
+// copied only when it differs; when it equals the trial state the last call returned (the
+// caller committed), the commit is replayed on the device instead of copying. Only the
+// points of J2 elements are taken: the elastic law neither reads nor advances a history,
+// so the device holds zeros there whatever the host's slots hold.
+void resident_committed(const Model& model, Resident& R, const fem::MaterialPoints& mp,
+                        const J2Table& j2) {
+  cxg_context* c = context();
+  const DeviceMesh& dm = *R.dm;
   const size_t P = size_t(dm.n_points);
-  in.committed.assign(size_t(CXG_STATE_COMPONENTS) * P, 0.0);
-  for (size_t k = 0; k < ne; ++k) {
+  std::vector<double> cur(kHist * P, 0.0);
+  for (size_t k = 0; k < dm.elem.size(); ++k) {
+    if (j2.mat_id[k] < 0) continue;
     const std::vector<fem::MaterialState>& st = mp.state[dm.elem[k]];
     const size_t ng = size_t(gauss_points(model.mesh.elements()[dm.elem[k]].type));
     if (st.size() != ng) throw std::runtime_error("cxpp::gpu: material points do not match the mesh");
     for (size_t q = 0; q < ng; ++q) {
       const size_t pt = size_t(dm.gp_first[k]) + q;
       for (size_t i = 0; i < 6; ++i) {
-        in.committed[i * P + pt] = st[q].committed_plastic_strain[i];
-        in.committed[(7 + i) * P + pt] = st[q].committed_back_stress[i];
+        cur[i * P + pt] = st[q].committed_plastic_strain[i];
+        cur[(7 + i) * P + pt] = st[q].committed_back_stress[i];
       }
-      in.committed[6 * P + pt] = st[q].committed_eqplastic;
+      cur[6 * P + pt] = st[q].committed_eqplastic;
     }
   }
-  return in;
+  const size_t bytes = cur.size() * sizeof(double);
+  if (!R.committed) {
+    R.committed = std::make_unique<Buf<double>>(c, cur.size());
+    R.trial = std::make_unique<Buf<double>>(c, cur.size());
+  }
+  if (R.has_committed && std::memcmp(cur.data(), R.committed_dev.data(), bytes) == 0) return;
+  if (R.has_trial && std::memcmp(cur.data(), R.trial_dev.data(), bytes) == 0) {
+    check(c, cxg_commit(c, int64_t(P), R.committed->get(), R.trial->get()));
+    R.committed_dev = R.trial_dev;
+    ++t_stats.history_commits;
+  } else {
+    R.committed->put(cur);
+    R.committed_dev = std::move(cur);
+    ++t_stats.history_uploads;
+  }
+  R.has_committed = true;
 }
 
 std::vector<double> flat(const std::vector<Vec3>& v) {
@@ -420,22 +664,22 @@ void recover(const Model& model, StaticFields& fields, const fem::MaterialPoints
   if (fields.displacement.size() != nn)
     throw std::runtime_error("cxpp::gpu: the displacement does not match the mesh");
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  J2Inputs j2;
-  if (mp) j2 = j2_inputs(model, dm, *mp);
-  const bool plastic = !j2.mats.empty();
-  Buf<double> ud(c, flat(fields.displacement)), fx(c, fem::external_load_vector(model));
-  Buf<double> sd(c, 6 * nn), ed(c, 6 * nn), rd(c, kDof * nn);
-  Buf<double> cd(c, std::max<size_t>(j2.committed.size(), 1));
-  Buf<int32_t> mid(c, std::max<size_t>(j2.mat_id.size(), 1));
-  if (plastic) {
-    cd.put(j2.committed.data());
-    mid.put(j2.mat_id.data());
+  Resident& R = resident(mesh_arrays(model), &model);
+  const DeviceMesh& dm = *R.dm;
+  const J2Table* j2 = nullptr;
+  if (mp) {
+    j2 = &resident_j2(model, R);
+    if (!j2->mats.empty()) resident_committed(model, R, *mp, *j2);
   }
+  const bool plastic = j2 && !j2->mats.empty();
+  R.u->put(flat(fields.displacement));
+  Buf<double> fx(c, fem::external_load_vector(model));
+  Buf<double> sd(c, 6 * nn), ed(c, 6 * nn), rd(c, kDof * nn);
   check(c, cxg_recover_fields(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(),
-                              plastic ? mid.get() : nullptr, plastic ? j2.mats.data() : nullptr,
-                              int32_t(j2.mats.size()), ud.get(), plastic ? cd.get() : nullptr,
-                              fx.get(), ed.get(), sd.get(), rd.get(), nullptr, nullptr));
+                              plastic ? R.mid->get() : nullptr, plastic ? j2->mats.data() : nullptr,
+                              plastic ? int32_t(j2->mats.size()) : 0, R.u->get(),
+                              plastic ? R.committed->get() : nullptr, fx.get(), ed.get(), sd.get(),
+                              rd.get(), nullptr, nullptr));
   const std::vector<double> s = sd.fetch(6 * nn), e = ed.fetch(6 * nn), r = rd.fetch(kDof * nn);
   fields.stress.assign(nn, Voigt6{});
   fields.strain.assign(nn, Voigt6{});
@@ -449,23 +693,88 @@ void recover(const Model& model, StaticFields& fields, const fem::MaterialPoints
   }
 }
 
+// One element of a family with nodes 0..n-1 on the device: mesh and all-free pattern,
+// kept per family for the element-level entry points.
+struct OneElement {
+  std::unique_ptr<Buf<int32_t>> conn;
+  cxg_mesh* mesh = nullptr;
+  std::unique_ptr<AllFreePattern> pat;
+  ~OneElement() {
+    pat.reset();
+    if (mesh) cxg_mesh_destroy(mesh);
+  }
+};
+
+thread_local OneElement* t_one[8] = {};
+thread_local cxg_context* t_one_ctx = nullptr;
+
+const OneElement& one_element(ElementType type) {
+  cxg_context* c = context();
+  if (t_one_ctx != c) {  // a new context: the old objects belonged to the old one
+    for (OneElement*& o : t_one) o = nullptr;  // (their context is gone; not freed)
+    t_one_ctx = c;
+  }
+  const int t = static_cast<int>(type);
+  if (t < 0 || t >= 8) throw std::runtime_error("cxpp::gpu: unknown element type");
+  if (!t_one[t]) {
+    const int n = cxg_nodes_per_element(t);
+    auto o = std::make_unique<OneElement>();
+    std::vector<int32_t> cn(static_cast<size_t>(n));
+    for (int a = 0; a < n; ++a) cn[size_t(a)] = a;
+    o->conn = std::make_unique<Buf<int32_t>>(c, cn);
+    cxg_block b{t, 1, 0, 0, o->conn->get()};
+    check(c, cxg_mesh_create(c, n, 1, &b, &o->mesh));
+    o->pat = std::make_unique<AllFreePattern>(c, o->mesh, n);
+    t_one[t] = o.release();
+  }
+  return *t_one[t];
+}
+
+// Dense (3n)^2 row-major matrix of one element from its all-free CSR values.
+std::vector<Real> dense_element(const AllFreePattern& pat, const std::vector<double>& v) {
+  const size_t nd = size_t(pat.n);
+  std::vector<Real> K(nd * nd, 0.0);
+  for (size_t r = 0; r < nd; ++r)
+    for (int64_t k = pat.rp[r]; k < pat.rp[r + 1]; ++k) K[r * nd + size_t(pat.ci[size_t(k)])] = v[size_t(k)];
+  return K;
+}
+
+void require_element(ElementType type, size_t n_coords) {
+  if (!on_gpu_path(type))
+    throw std::runtime_error("cxpp::gpu: element type outside the GPU path (C3D8, C3D20, C3D20R, C3D6)");
+  if (n_coords != size_t(cxg_nodes_per_element(static_cast<int32_t>(type))))
+    throw std::runtime_error("cxpp::gpu: coords do not match the element type");
+}
+
+// E and nu of an isotropic elastic D (elastic_iso_D, element.cpp:410-422: lambda off the
+// normal diagonal, mu on the shear diagonal).
+ElasticIso elastic_of(const fem::D6& D) {
+  const Real lam = D[0][1], mu = D[3][3];
+  return ElasticIso{mu * (3.0 * lam + 2.0 * mu) / (lam + mu), lam / (2.0 * (lam + mu))};
+}
+
 }  // namespace
 
 void set_device(int device) { t_device = device; }
 
+ResidentStats resident_stats() { return t_stats; }
+
+void resident_reset() {
+  drop_resident();
+  t_stats = ResidentStats{};
+}
+
 fem::LinearSystem assemble_linear_static(const Model& model) {
   require_mechanical(model);
-  fem::LinearSystem sys;
-  dof_map(model, sys);
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
-  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
-  Buf<double> vals(c, nnz);
-  Buf<uint8_t> live(c, nnz);
-  check(c, cxg_assemble_linear(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(), vals.get(),
-                               nullptr, nullptr, live.get()));
-  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  Resident& R = resident(mesh_arrays(model), &model);
+  fem::LinearSystem sys;
+  dof_map(model, R, sys);
+  const AllFreePattern& pat = pattern(R);
+  const DeviceMesh& dm = *R.dm;
+  check(c, cxg_assemble_linear(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(),
+                               R.vals->get(), nullptr, nullptr, R.live->get()));
+  reduce(pat, R.vals->fetch(size_t(pat.nnz)), R.live->fetch(size_t(pat.nnz)), sys);
   add_external_load(model, sys);
   return sys;
 }
@@ -474,34 +783,32 @@ fem::LinearSystem assemble_material_tangent(const Model& model, const std::vecto
                                             fem::MaterialPoints& mp, std::vector<Real>& f_int) {
   require_mechanical(model);
   require_material_points(model, mp);
-  const Mesh& mesh = model.mesh;
-  if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
-  fem::LinearSystem sys;
-  dof_map(model, sys);
+  if (u.size() != model.mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  const size_t ne = dm.elem.size();
-  const J2Inputs j2 = j2_inputs(model, dm, mp);
-  const std::vector<cxg_j2_material>& mats = j2.mats;
-  const std::vector<int32_t>& mat_id = j2.mat_id;
-  const std::vector<double>& committed = j2.committed;
-  const size_t P = size_t(dm.n_points);
-  const std::vector<double> uh = flat(u);
-  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
-  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
-  Buf<double> vals(c, nnz), fd(c, uh.size()), ud(c, uh), cd(c, committed), td(c, committed.size());
-  Buf<uint8_t> live(c, nnz);
-  Buf<int32_t> mid(c, std::max<size_t>(ne, 1));
-  if (ne) mid.put(mat_id.data());
+  Resident& R = resident(mesh_arrays(model), &model);
+  fem::LinearSystem sys;
+  dof_map(model, R, sys);
+  const AllFreePattern& pat = pattern(R);
+  const DeviceMesh& dm = *R.dm;
+  const J2Table& j2 = resident_j2(model, R);
+  resident_committed(model, R, mp, j2);
+  R.u->put(flat(u));
+  const bool plastic = !j2.mats.empty();
   check(c, cxg_assemble_j2(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(),
-                           mats.empty() ? nullptr : mid.get(), mats.empty() ? nullptr : mats.data(),
-                           int32_t(mats.size()), ud.get(), cd.get(), td.get(), vals.get(),
-                           fd.get(), live.get()));
-  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+                           plastic ? R.mid->get() : nullptr, plastic ? j2.mats.data() : nullptr,
+                           int32_t(j2.mats.size()), R.u->get(), R.committed->get(), R.trial->get(),
+                           R.vals->get(), R.f->get(), R.live->get()));
+  reduce(pat, R.vals->fetch(size_t(pat.nnz)), R.live->fetch(size_t(pat.nnz)), sys);
   add_external_load(model, sys);
-  f_int = fd.fetch(uh.size());
-  const std::vector<double> trial = td.fetch(committed.size());
-  for (size_t k = 0; k < ne; ++k) {
+  f_int = R.f->fetch(R.f->size());
+  R.trial->fetch_into(R.trial_dev);
+  R.has_trial = true;
+  // trial slots of the J2 points; elastic points keep theirs, as ElasticIsoMaterial
+  // leaves the state alone (element.cpp:859-871)
+  const size_t P = size_t(dm.n_points);
+  const std::vector<double>& trial = R.trial_dev;
+  for (size_t k = 0; k < dm.elem.size(); ++k) {
+    if (j2.mat_id[k] < 0) continue;
     std::vector<fem::MaterialState>& st = mp.state[dm.elem[k]];
     for (size_t q = 0; q < st.size(); ++q) {
       const size_t pt = size_t(dm.gp_first[k]) + q;
@@ -517,17 +824,14 @@ fem::LinearSystem assemble_material_tangent(const Model& model, const std::vecto
 
 std::vector<Real> internal_force(const Model& model, const std::vector<Vec3>& u) {
   require_mechanical(model);
-  const Mesh& mesh = model.mesh;
-  if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
+  if (u.size() != model.mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  std::vector<double> uh(mesh.num_nodes() * kDof);
-  for (size_t i = 0; i < mesh.num_nodes(); ++i)
-    for (int k = 0; k < kDof; ++k) uh[kDof * i + size_t(k)] = u[i][size_t(k)];
-  Buf<double> ud(c, uh), fd(c, uh.size());
-  check(c, cxg_internal_force(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(), ud.get(),
-                              fd.get()));
-  return fd.fetch(uh.size());
+  Resident& R = resident(mesh_arrays(model), &model);
+  const DeviceMesh& dm = *R.dm;
+  R.u->put(flat(u));
+  check(c, cxg_internal_force(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(),
+                              R.u->get(), R.f->get()));
+  return R.f->fetch(R.f->size());
 }
 
 size_t ArrayModel::num_elements() const {
@@ -562,14 +866,12 @@ fem::LinearSystem assemble_linear_static(const ArrayModel& model) {
   fem::LinearSystem sys;
   dof_map(model, sys);
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
-  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
-  Buf<double> vals(c, nnz);
-  Buf<uint8_t> live(c, nnz);
-  check(c, cxg_assemble_linear(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(), vals.get(),
-                               nullptr, nullptr, live.get()));
-  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  Resident& R = resident(model, &model);
+  const AllFreePattern& pat = pattern(R);
+  const DeviceMesh& dm = *R.dm;
+  check(c, cxg_assemble_linear(c, pat.p, dm.coords->get(), dm.E->get(), dm.nu->get(),
+                               R.vals->get(), nullptr, nullptr, R.live->get()));
+  reduce(pat, R.vals->fetch(size_t(pat.nnz)), R.live->fetch(size_t(pat.nnz)), sys);
   add_external_load(model, sys);
   return sys;
 }
@@ -577,56 +879,127 @@ fem::LinearSystem assemble_linear_static(const ArrayModel& model) {
 std::vector<Real> internal_force(const ArrayModel& model, const std::vector<Vec3>& u) {
   if (u.size() != model.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  const std::vector<double> uh = flat(u);
-  Buf<double> ud(c, uh), fd(c, uh.size());
-  check(c, cxg_internal_force(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(), ud.get(),
-                              fd.get()));
-  return fd.fetch(uh.size());
+  Resident& R = resident(model, &model);
+  const DeviceMesh& dm = *R.dm;
+  R.u->put(flat(u));
+  check(c, cxg_internal_force(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(),
+                              R.u->get(), R.f->get()));
+  return R.f->fetch(R.f->size());
 }
 
 std::vector<Real> element_stiffness(ElementType type, std::span<const Vec3> coords,
                                     const ElasticIso& elastic) {
-  if (!on_gpu_path(type))
-    throw std::runtime_error("cxpp::gpu: element type outside the GPU path (C3D8, C3D20, C3D20R, C3D6)");
-  const int n = cxg_nodes_per_element(static_cast<int32_t>(type));
-  if (coords.size() != size_t(n)) throw std::runtime_error("cxpp::gpu: coords do not match the element type");
+  require_element(type, coords.size());
   cxg_context* c = context();
-  std::vector<double> x(size_t(n) * kDof);
-  std::vector<int32_t> cn(static_cast<size_t>(n));
-  for (int a = 0; a < n; ++a) {
-    cn[size_t(a)] = a;
-    for (int k = 0; k < kDof; ++k) x[size_t(kDof * a + k)] = coords[size_t(a)][size_t(k)];
-  }
-  Buf<double> xd(c, x), Ed(c, std::vector<double>{elastic.E}), nd(c, std::vector<double>{elastic.nu});
-  Buf<int32_t> cd(c, cn);
-  cxg_block b{static_cast<int32_t>(type), 1, 0, 0, cd.get()};
-  cxg_mesh* mesh = nullptr;
-  check(c, cxg_mesh_create(c, n, 1, &b, &mesh));
-  std::unique_ptr<cxg_mesh, void (*)(cxg_mesh*)> hold(mesh, cxg_mesh_destroy);
-  const AllFreePattern pat(c, mesh, n);
-  Buf<double> vals(c, size_t(pat.nnz));
-  check(c, cxg_assemble_linear(c, pat.p, xd.get(), Ed.get(), nd.get(), vals.get(), nullptr,
+  const OneElement& one = one_element(type);
+  Buf<double> xd(c, flat(std::vector<Vec3>(coords.begin(), coords.end())));
+  Buf<double> Ed(c, std::vector<double>{elastic.E}), nd(c, std::vector<double>{elastic.nu});
+  Buf<double> vals(c, size_t(one.pat->nnz));
+  check(c, cxg_assemble_linear(c, one.pat->p, xd.get(), Ed.get(), nd.get(), vals.get(), nullptr,
                                nullptr, nullptr));
-  Buf<int64_t> rp_dev(c, size_t(pat.n) + 1);
-  Buf<int32_t> ci_dev(c, size_t(pat.nnz));
-  check(c, cxg_pattern_copy(c, pat.p, rp_dev.get(), ci_dev.get()));
-  const std::vector<int64_t> rp = rp_dev.fetch(size_t(pat.n) + 1);
-  const std::vector<int32_t> ci = ci_dev.fetch(size_t(pat.nnz));
-  const std::vector<double> v = vals.fetch(size_t(pat.nnz));
-  const size_t nd3 = size_t(kDof * n);
-  std::vector<Real> K(nd3 * nd3, 0.0);
-  for (size_t r = 0; r < nd3; ++r)
-    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) K[r * nd3 + size_t(ci[size_t(k)])] = v[size_t(k)];
-  return K;
+  return dense_element(*one.pat, vals.fetch(size_t(one.pat->nnz)));
 }
 
-std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector<Index>& cols,
-                               const std::vector<Real>& vals, Index n,
-                               const std::vector<Real>& rhs, compute::SolverKind) {
-  if (rows.size() != cols.size() || rows.size() != vals.size() || rhs.size() != size_t(n) || n < 0)
+J2Material::J2Material(const ElasticIso& elastic, const Plastic& plastic)
+    : elastic_(elastic), plastic_(plastic) {
+  if (plastic.empty())
+    throw std::runtime_error("J2PlasticMaterial needs a non-empty *PLASTIC table");
+  (void)to_cxg(elastic_, plastic_);  // the table fits the device form
+}
+
+fem::MaterialResponse J2Material::evaluate(const Voigt6& strain, fem::MaterialState& state) const {
+  cxg_context* c = context();
+  const cxg_j2_material m = to_cxg(elastic_, plastic_);
+  std::vector<double> hc(kHist);
+  for (size_t i = 0; i < 6; ++i) {
+    hc[i] = state.committed_plastic_strain[i];
+    hc[7 + i] = state.committed_back_stress[i];
+  }
+  hc[6] = state.committed_eqplastic;
+  Buf<int32_t> mid(c, std::vector<int32_t>{0});
+  Buf<double> ed(c, std::vector<double>(strain.begin(), strain.end()));
+  Buf<double> cd(c, hc), td(c, kHist), sd(c, 6), dd(c, 36);
+  check(c, cxg_material_evaluate(c, 1, mid.get(), &m, 1, nullptr, nullptr, ed.get(), cd.get(),
+                                 td.get(), sd.get(), dd.get()));
+  const std::vector<double> ht = td.fetch(kHist), s = sd.fetch(6), D = dd.fetch(36);
+  for (size_t i = 0; i < 6; ++i) {
+    state.plastic_strain[i] = ht[i];
+    state.back_stress[i] = ht[7 + i];
+  }
+  state.eqplastic = ht[6];
+  fem::MaterialResponse r;
+  for (size_t i = 0; i < 6; ++i) {
+    r.stress[i] = s[i];
+    for (size_t j = 0; j < 6; ++j) r.tangent[i][j] = D[6 * i + j];
+  }
+  return r;
+}
+
+fem::ElementResponse element_tangent_force(ElementType type, std::span<const Vec3> coords,
+                                           std::span<const Vec3> ue,
+                                           const fem::MaterialModel& material,
+                                           std::span<fem::MaterialState> state) {
+  require_element(type, coords.size());
+  if (ue.size() != coords.size()) throw std::runtime_error("cxpp::gpu: ue does not match the element");
+  const size_t ng = size_t(gauss_points(type));
+  if (state.size() != ng) throw std::runtime_error("cxpp::gpu: state needs one entry per Gauss point");
+  const auto* iso = dynamic_cast<const fem::ElasticIsoMaterial*>(&material);
+  const auto* j2 = dynamic_cast<const J2Material*>(&material);
+  if (!iso && !j2)
+    throw std::runtime_error("cxpp::gpu::element_tangent_force: material outside the GPU path "
+                             "(fem::ElasticIsoMaterial or cxpp::gpu::J2Material)");
+  cxg_context* c = context();
+  const OneElement& one = one_element(type);
+  const ElasticIso el = iso ? elastic_of(iso->tangent()) : j2->elastic();
+  const cxg_j2_material m = j2 ? to_cxg(j2->elastic(), j2->plastic()) : cxg_j2_material{};
+  std::vector<double> hc(kHist * ng, 0.0);
+  for (size_t q = 0; q < ng; ++q) {
+    for (size_t i = 0; i < 6; ++i) {
+      hc[i * ng + q] = state[q].committed_plastic_strain[i];
+      hc[(7 + i) * ng + q] = state[q].committed_back_stress[i];
+    }
+    hc[6 * ng + q] = state[q].committed_eqplastic;
+  }
+  const size_t nd = kDof * coords.size();
+  Buf<double> xd(c, flat(std::vector<Vec3>(coords.begin(), coords.end())));
+  Buf<double> ud(c, flat(std::vector<Vec3>(ue.begin(), ue.end())));
+  Buf<double> Ed(c, std::vector<double>{el.E}), nud(c, std::vector<double>{el.nu});
+  Buf<int32_t> mid(c, std::vector<int32_t>{0});
+  Buf<double> cd(c, hc), td(c, hc.size()), fd(c, nd), vals(c, size_t(one.pat->nnz));
+  check(c, cxg_assemble_j2(c, one.pat->p, xd.get(), Ed.get(), nud.get(), j2 ? mid.get() : nullptr,
+                           j2 ? &m : nullptr, j2 ? 1 : 0, ud.get(), cd.get(), td.get(), vals.get(),
+                           fd.get(), nullptr));
+  fem::ElementResponse out;
+  out.Ke = dense_element(*one.pat, vals.fetch(size_t(one.pat->nnz)));
+  out.fe = fd.fetch(nd);
+  if (j2) {  // the elastic law leaves the state alone (element.cpp:859-871)
+    const std::vector<double> ht = td.fetch(hc.size());
+    for (size_t q = 0; q < ng; ++q) {
+      for (size_t i = 0; i < 6; ++i) {
+        state[q].plastic_strain[i] = ht[i * ng + q];
+        state[q].back_stress[i] = ht[(7 + i) * ng + q];
+      }
+      state[q].eqplastic = ht[6 * ng + q];
+    }
+  }
+  return out;
+}
+
+std::vector<Real> HipBackend::solve_sparse(const std::vector<Index>& rows,
+                                           const std::vector<Index>& cols,
+                                           const std::vector<Real>& vals, Index n,
+                                           const std::vector<Real>& rhs,
+                                           compute::SolverKind solver) const {
+  if (rows.size() != cols.size() || rows.size() != vals.size() || n < 0 || rhs.size() != size_t(n))
     throw std::runtime_error("cxpp::gpu::solve_sparse: inconsistent system sizes");
-  // COO -> CSR with duplicates summed in triplet order
+  double tol = 0.0;
+  switch (solver) {
+    case compute::SolverKind::CG: tol = 1e-12; break;
+    case compute::SolverKind::Direct: tol = 1e-13; break;
+    default: throw std::runtime_error("cxpp::gpu::solve_sparse: unknown SolverKind");
+  }
+  // COO -> CSR, each row's triplets in column order with duplicates summed in triplet
+  // order
   std::vector<int64_t> rp(size_t(n) + 1, 0);
   for (Index r : rows) {
     if (r < 0 || r >= n) throw std::runtime_error("cxpp::gpu::solve_sparse: row out of range");
@@ -634,31 +1007,25 @@ std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector
   }
   for (size_t i = 0; i < size_t(n); ++i) rp[i + 1] += rp[i];
   std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
-  std::vector<int32_t> ci(rows.size());
-  std::vector<double> va(rows.size());
+  std::vector<std::pair<int32_t, double>> ent(rows.size());
   for (size_t k = 0; k < rows.size(); ++k) {
     if (cols[k] < 0 || cols[k] >= n) throw std::runtime_error("cxpp::gpu::solve_sparse: column out of range");
-    const int64_t at = fill[size_t(rows[k])]++;
-    ci[size_t(at)] = cols[k];
-    va[size_t(at)] = vals[k];
+    ent[size_t(fill[size_t(rows[k])]++)] = {cols[k], vals[k]};
   }
   std::vector<int64_t> out_rp(size_t(n) + 1, 0);
   std::vector<int32_t> out_ci;
   std::vector<double> out_va;
-  out_ci.reserve(ci.size());
-  out_va.reserve(va.size());
-  std::vector<std::pair<int32_t, double>> row;
+  out_ci.reserve(ent.size());
+  out_va.reserve(ent.size());
   for (size_t r = 0; r < size_t(n); ++r) {
-    row.clear();
-    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) row.emplace_back(ci[size_t(k)], va[size_t(k)]);
-    std::stable_sort(row.begin(), row.end(),
-                     [](const auto& a, const auto& b) { return a.first < b.first; });
-    for (size_t k = 0; k < row.size(); ++k) {
-      if (k > 0 && row[k].first == out_ci.back()) {
-        out_va.back() += row[k].second;
+    auto b = ent.begin() + rp[r], e = ent.begin() + rp[r + 1];
+    std::stable_sort(b, e, [](const auto& x, const auto& y) { return x.first < y.first; });
+    for (auto it = b; it != e; ++it) {
+      if (it != b && it->first == out_ci.back()) {
+        out_va.back() += it->second;
       } else {
-        out_ci.push_back(row[k].first);
-        out_va.push_back(row[k].second);
+        out_ci.push_back(it->first);
+        out_va.push_back(it->second);
       }
     }
     out_rp[r + 1] = int64_t(out_ci.size());
@@ -667,11 +1034,32 @@ std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector
   Buf<int64_t> rpd(c, out_rp);
   Buf<int32_t> cid(c, out_ci);
   Buf<double> vad(c, out_va), bd(c, rhs), xd(c, std::vector<double>(size_t(n), 0.0));
+  const int32_t max_iter = int32_t(std::min<int64_t>(INT32_MAX, std::max<int64_t>(100, 20 * int64_t(n))));
   int32_t iters = 0;
   double rel = 0.0;
-  check(c, cxg_pcg(c, n, rpd.get(), cid.get(), vad.get(), bd.get(), xd.get(), 1e-13,
-                   std::max<int32_t>(100, 20 * n), &iters, &rel));
+  const cxg_status st = cxg_pcg(c, n, rpd.get(), cid.get(), vad.get(), bd.get(), xd.get(), tol,
+                                max_iter, &iters, &rel);
+  if (st == CXG_ERR_STATE) {  // the tolerance was not reached
+    if (solver == compute::SolverKind::CG)
+      throw std::runtime_error("CG did not converge (relative residual " + std::to_string(rel) +
+                               " after " + std::to_string(iters) + " iterations)");
+    throw std::runtime_error("Direct solve by device PCG did not reach 1e-13 (relative residual " +
+                             std::to_string(rel) + " after " + std::to_string(iters) +
+                             " iterations)");
+  }
+  check(c, st);
   return xd.fetch(size_t(n));
+}
+
+const compute::ComputeBackend& hip_backend() {
+  static const HipBackend backend;
+  return backend;
+}
+
+std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector<Index>& cols,
+                               const std::vector<Real>& vals, Index n,
+                               const std::vector<Real>& rhs, compute::SolverKind solver) {
+  return hip_backend().solve_sparse(rows, cols, vals, n, rhs, solver);
 }
 
 void recover_fields(const Model& model, StaticFields& fields) { recover(model, fields, nullptr); }
@@ -686,11 +1074,13 @@ std::vector<std::vector<Voigt6>> recover_gauss_stress(const Model& model,
   const Mesh& mesh = model.mesh;
   if (u.size() != mesh.num_nodes()) throw std::runtime_error("cxpp::gpu: u does not match the mesh");
   cxg_context* c = context();
-  const DeviceMesh dm(c, model);
+  Resident& R = resident(mesh_arrays(model), &model);
+  const DeviceMesh& dm = *R.dm;
   const size_t P = size_t(dm.n_points);
-  Buf<double> ud(c, flat(u)), gd(c, std::max<size_t>(6 * P, 1));
+  R.u->put(flat(u));
+  Buf<double> gd(c, 6 * P);
   check(c, cxg_recover_fields(c, dm.mesh, dm.coords->get(), dm.E->get(), dm.nu->get(), nullptr,
-                              nullptr, 0, ud.get(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, 0, R.u->get(), nullptr, nullptr, nullptr, nullptr, nullptr,
                               nullptr, gd.get()));
   const std::vector<double> g = gd.fetch(6 * P);
   std::vector<std::vector<Voigt6>> out(mesh.num_elements());  // removed elements: empty
@@ -710,22 +1100,21 @@ namespace {
 // pattern, reduced like the stiffness with the eliminated-column load discarded.
 // sys holds the DOF map; rho per model element (mass) or gp_stress per model element
 // (geometric stiffness).
-fem::LinearSystem assemble_scalar(fem::LinearSystem sys, cxg_context* c, const DeviceMesh& dm,
-                                  int kind, const std::vector<Real>& rho,
+fem::LinearSystem assemble_scalar(fem::LinearSystem sys, Resident& R, int kind,
+                                  const std::vector<Real>& rho,
                                   const std::vector<std::vector<Voigt6>>* gp_stress) {
-  const AllFreePattern pat(c, dm.mesh, dm.n_nodes);
-  const size_t nnz = size_t(std::max<int64_t>(pat.nnz, 1));
-  Buf<double> vals(c, nnz);
-  Buf<uint8_t> live(c, nnz);
+  cxg_context* c = context();
+  const AllFreePattern& pat = pattern(R);
+  const DeviceMesh& dm = *R.dm;
   if (kind < 2) {
-    std::vector<double> r(std::max<size_t>(dm.elem.size(), 1), 0.0);
+    std::vector<double> r(dm.elem.size(), 0.0);
     for (size_t k = 0; k < dm.elem.size(); ++k) r[k] = dm.elem[k] < rho.size() ? rho[dm.elem[k]] : 0.0;
     Buf<double> rd(c, r);
-    check(c, cxg_assemble_mass(c, pat.p, dm.coords->get(), rd.get(), kind == 1, vals.get(),
-                               live.get()));
+    check(c, cxg_assemble_mass(c, pat.p, dm.coords->get(), rd.get(), kind == 1, R.vals->get(),
+                               R.live->get()));
   } else {
     const size_t P = size_t(dm.n_points);
-    std::vector<double> g(std::max<size_t>(6 * P, 1), 0.0);  // missing stress: zero
+    std::vector<double> g(6 * P, 0.0);  // missing stress: zero
     for (size_t k = 0; k < dm.elem.size(); ++k) {
       const size_t e = dm.elem[k];
       if (e >= gp_stress->size()) continue;
@@ -736,9 +1125,10 @@ fem::LinearSystem assemble_scalar(fem::LinearSystem sys, cxg_context* c, const D
         for (size_t i = 0; i < 6; ++i) g[6 * (size_t(dm.gp_first[k]) + q) + i] = es[q][i];
     }
     Buf<double> gd(c, g);
-    check(c, cxg_assemble_geometric(c, pat.p, dm.coords->get(), gd.get(), vals.get(), live.get()));
+    check(c, cxg_assemble_geometric(c, pat.p, dm.coords->get(), gd.get(), R.vals->get(),
+                                    R.live->get()));
   }
-  reduce(pat, vals.fetch(size_t(pat.nnz)), live.fetch(size_t(pat.nnz)), sys);
+  reduce(pat, R.vals->fetch(size_t(pat.nnz)), R.live->fetch(size_t(pat.nnz)), sys);
   std::fill(sys.rhs.begin(), sys.rhs.end(), 0.0);  // mass and K_geo never drive a load
   return sys;
 }
@@ -749,29 +1139,26 @@ fem::LinearSystem assemble_mass(const Model& model, bool lumped) {
   require_mechanical(model);
   if (!model.point_masses.empty())
     throw std::runtime_error("cxpp::gpu: *MASS point masses are outside the GPU path");
+  Resident& R = resident(mesh_arrays(model), &model);
   fem::LinearSystem sys;
-  dof_map(model, sys);
-  cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  return assemble_scalar(std::move(sys), c, dm, lumped ? 1 : 0, model.element_density(), nullptr);
+  dof_map(model, R, sys);
+  return assemble_scalar(std::move(sys), R, lumped ? 1 : 0, model.element_density(), nullptr);
 }
 
 fem::LinearSystem assemble_mass(const ArrayModel& model, bool lumped) {
   fem::LinearSystem sys;
   dof_map(model, sys);
-  cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  return assemble_scalar(std::move(sys), c, dm, lumped ? 1 : 0, model.rho, nullptr);
+  Resident& R = resident(model, &model);
+  return assemble_scalar(std::move(sys), R, lumped ? 1 : 0, model.rho, nullptr);
 }
 
 fem::LinearSystem assemble_geometric_stiffness(const Model& model,
                                                const std::vector<std::vector<Voigt6>>& gp_stress) {
   require_mechanical(model);
+  Resident& R = resident(mesh_arrays(model), &model);
   fem::LinearSystem sys;
-  dof_map(model, sys);
-  cxg_context* c = context();
-  const DeviceMesh dm(c, model);
-  return assemble_scalar(std::move(sys), c, dm, 2, {}, &gp_stress);
+  dof_map(model, R, sys);
+  return assemble_scalar(std::move(sys), R, 2, {}, &gp_stress);
 }
 
 }  // namespace cxpp::gpu

@@ -21,9 +21,15 @@
 //                                          (fem/stress.hpp:24,43, stress.cpp:189-353)
 //   cxpp::gpu::recover_gauss_stress        fem::recover_gauss_stress
 //                                          (fem/stress.hpp:35, stress.cpp:147-187)
-//   cxpp::gpu::solve_sparse                compute::ComputeBackend::solve_sparse
-//                                          (compute/backend.hpp:51-56), as a function:
-//                                          a HIP backend class wraps it one to one
+//   cxpp::gpu::element_tangent_force       fem::element_tangent_force
+//                                          (fem/element.hpp:160, element.cpp:708-743)
+//   cxpp::gpu::J2Material::evaluate        fem::MaterialModel::evaluate of the J2 law
+//                                          (fem/material_model.hpp:77,
+//                                           plasticity.cpp:90-236), on the device
+//   cxpp::gpu::HipBackend                  compute::ComputeBackend
+//                                          (compute/backend.hpp:40-57): solve_sparse
+//                                          and kind(); hip_backend() is its instance
+//   cxpp::gpu::solve_sparse                HipBackend::solve_sparse as a free function
 //
 // This file and cxpp_gpu.cpp compile inside the reference's tree (they include its
 // headers) and link against libcxg.so. A model the GPU path does not cover (element
@@ -36,8 +42,19 @@
 // are left out, assembly.cpp:90); values within rel-L2 1e-12 of the reference's.
 // One libcxg context per host thread, on the device chosen by set_device (default
 // 0), created on first use and kept until the process ends.
+//
+// Residency: a Newton loop calls assemble_material_tangent with one Model many times
+// (src/numerics/nonlinear_static.cpp:94 and the line-search re-assemblies at :123-138).
+// Each thread keeps the device copy of the last model it saw: node coordinates,
+// connectivity, elasticity, the device mesh, the all-free CSR pattern (and its host
+// copy), the value buffers, the J2 material table and the committed history. The cache
+// is keyed by the Model's address and a 64-bit fingerprint of everything uploaded, so
+// a changed or different model is uploaded afresh; the committed history is compared
+// with the device copy on every call and uploaded only when the host changed it (a
+// commit() of the trial state the last call returned is replayed on the device).
 #pragma once
 
+#include <cstdint>
 #include <span>
 #include <vector>
 
@@ -46,6 +63,7 @@
 #include "calculixpp/core/results.hpp"
 #include "calculixpp/fem/assembly.hpp"
 #include "calculixpp/fem/element.hpp"
+#include "calculixpp/fem/material_model.hpp"
 
 namespace cxpp::gpu {
 
@@ -122,11 +140,75 @@ void recover_fields(const Model& model, StaticFields& fields, fem::MaterialPoint
 std::vector<std::vector<Voigt6>> recover_gauss_stress(const Model& model,
                                                       const std::vector<Vec3>& u);
 
-// A x = rhs for the SPD system given as COO triplets (duplicates summed), on the
-// device by Jacobi-preconditioned conjugate gradients to ||b - A x|| <= 1e-13 ||b||
-// (cxg_pcg) whichever SolverKind is asked for; throws if that is not reached.
+// J2 plasticity (*PLASTIC with its *ELASTIC) whose constants are readable, so that the
+// device path can take it: the reference's J2PlasticMaterial keeps them private. Built
+// from the same ElasticIso + Plastic as J2PlasticMaterial (and validated the same way);
+// evaluate() runs the radial return of one point on the device (cxg_material_evaluate).
+class J2Material final : public fem::MaterialModel {
+ public:
+  J2Material(const ElasticIso& elastic, const Plastic& plastic);
+  fem::MaterialResponse evaluate(const Voigt6& strain, fem::MaterialState& state) const override;
+  const ElasticIso& elastic() const { return elastic_; }
+  const Plastic& plastic() const { return plastic_; }
+
+ private:
+  ElasticIso elastic_;
+  Plastic plastic_;
+};
+
+// Ke = sum_q B^T D_t B det J w and fe = sum_q B^T sigma det J w of one element at the
+// nodal displacements ue, with the material's history advanced in the trial slots of
+// `state` (one entry per Gauss point). The material must be fem::ElasticIsoMaterial
+// (its D gives E and nu) or cxpp::gpu::J2Material; any other MaterialModel raises
+// std::runtime_error naming the types the device path takes.
+fem::ElementResponse element_tangent_force(ElementType type, std::span<const Vec3> coords,
+                                           std::span<const Vec3> ue,
+                                           const fem::MaterialModel& material,
+                                           std::span<fem::MaterialState> state);
+
+// The ComputeBackend of the device. kind() is kHipBackendKind, the value a maintainer
+// gives the enumerator BackendKind::HIP when adding it to compute/backend.hpp (the
+// reference lists CPU, CUDA, OpenCL and Metal only). solve_sparse runs
+// Jacobi-preconditioned conjugate gradients on the device (cxg_pcg) for either
+// SolverKind, each with its own contract:
+//   SolverKind::CG      ||b - A x|| <= 1e-12 ||b||; on failure std::runtime_error
+//                       "CG did not converge (relative residual R after N iterations)"
+//                       as the reference's CG path (src/compute/cpu_backend.cpp:55-62);
+//   SolverKind::Direct  there is no sparse factorization on the device: the request is
+//                       served by the same iteration to ||b - A x|| <= 1e-13 ||b|| and
+//                       raises "Direct solve by device PCG did not reach 1e-13 ..."
+//                       when that is not reached, never returning a looser result.
+inline constexpr compute::BackendKind kHipBackendKind = static_cast<compute::BackendKind>(4);
+
+class HipBackend final : public compute::ComputeBackend {
+ public:
+  compute::BackendKind kind() const override { return kHipBackendKind; }
+  std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector<Index>& cols,
+                                 const std::vector<Real>& vals, Index n,
+                                 const std::vector<Real>& rhs,
+                                 compute::SolverKind solver) const override;
+};
+
+// The process-wide HipBackend (stateless; the device context is per calling thread).
+const compute::ComputeBackend& hip_backend();
+
+// HipBackend::solve_sparse as a free function.
 std::vector<Real> solve_sparse(const std::vector<Index>& rows, const std::vector<Index>& cols,
                                const std::vector<Real>& vals, Index n,
                                const std::vector<Real>& rhs, compute::SolverKind solver);
+
+// Counters of the calling thread's residency cache (see the top of this header).
+struct ResidentStats {
+  int64_t calls = 0;             // model-level calls that went through the cache
+  int64_t mesh_uploads = 0;      // device meshes built (coordinates, connectivity, E, nu)
+  int64_t pattern_builds = 0;    // all-free CSR patterns built
+  int64_t dof_maps = 0;          // DOF maps built from the model's constraints
+  int64_t history_uploads = 0;   // committed histories copied to the device
+  int64_t history_commits = 0;   // commits replayed on the device (trial -> committed)
+};
+ResidentStats resident_stats();
+// Drops the calling thread's cached model (device memory is released) and zeroes the
+// counters.
+void resident_reset();
 
 }  // namespace cxpp::gpu

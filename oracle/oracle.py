@@ -205,7 +205,41 @@ class Oracle:
         self._f("system_dof_eq")(h, _p(d))
         return d
 
+    def open(self, model) -> "ModelHandle":
+        """The model built once and kept across calls (a Newton loop's Model)."""
+        return ModelHandle(self, model)
+
+    def resident_stats(self, reset: bool = False) -> dict:
+        """Residency counters of the drop-in's calling thread (drop-in build only)."""
+        out = np.zeros(6, np.int64)
+        f = self.L.ref_gpu_resident_stats
+        f.argtypes = [C.c_int, C.c_void_p]
+        f.restype = None
+        f(int(reset), _p(out))
+        keys = ("calls", "mesh_uploads", "pattern_builds", "dof_maps", "history_uploads",
+                "history_commits")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def solve_sparse(self, n, rows, cols, vals, rhs, direct: bool = False):
+        """cxpp::gpu::hip_backend().solve_sparse on COO triplets (drop-in build only):
+        (x, backend kind)."""
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64)
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros(n)
+        kind = C.c_int(-1)
+        f = self.L.ref_solve_sparse
+        f.argtypes = [C.c_int32, C.c_int64] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p,
+                                                                  C.POINTER(C.c_int), C.c_char_p,
+                                                                  C.c_int]
+        self._check(f(int(n), len(vals), _p(rows), _p(cols), _p(vals), _p(rhs), int(direct),
+                      _p(x), C.byref(kind), self._err, 512))
+        return x, kind.value
+
     def _with_model(self, model, fn):
+        if isinstance(model, ModelHandle):
+            return fn(model.ref, model.arrays)
         arrays = _ModelArrays(model)
         if self.kind == "port":
             return fn(C.byref(arrays.struct), arrays)
@@ -404,6 +438,40 @@ class Oracle:
         self._check(f(C.byref(arrays.struct), _p(uu), _p(ww), _p(dx), _p(dE), _p(du), _p(r),
                       self._err, 512))
         return dx.reshape(-1, 3), dE, du, r
+
+
+class ModelHandle:
+    """One model built once by an Oracle and passed to its methods in place of the array
+    model: for the reference and drop-in libraries a cxpp::Model that lives until close(),
+    as the Model a Newton loop holds across its calls. Attributes of the array model
+    (n_nodes, n_points, ...) read through."""
+
+    def __init__(self, oracle: Oracle, model):
+        self.model = model
+        self.arrays = _ModelArrays(model)
+        self._oracle = oracle
+        if oracle.kind == "port":
+            self.ref = C.byref(self.arrays.struct)
+            return
+        h = C.c_void_p()
+        create = oracle.L.ref_model_create
+        create.argtypes = [C.POINTER(OrcModel), C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
+        oracle._check(create(C.byref(self.arrays.struct), C.byref(h), oracle._err, 512))
+        self.ref = h
+
+    def __getattr__(self, name):
+        return getattr(self.model, name)
+
+    def close(self):
+        if self._oracle.kind != "port" and self.ref is not None and self.ref.value:
+            self._oracle.L.ref_model_free(self.ref)
+        self.ref = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def aos_state(soa: np.ndarray) -> np.ndarray:

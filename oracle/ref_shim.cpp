@@ -167,7 +167,11 @@ int ref_element_stiffness(int type, const double* coords, double E, double nu, d
 int ref_j2_evaluate(const orc_j2* p, const double* strain, double* state, double* stress,
                     double* D, char* err, int errlen) {
   try {
+#ifdef CXPP_DROPIN
+    const cxpp::gpu::J2Material mat(ElasticIso{p->E, p->nu}, to_plastic(*p));
+#else
     fem::J2PlasticMaterial mat(ElasticIso{p->E, p->nu}, to_plastic(*p));
+#endif
     fem::MaterialState s;
     load_state(state, s);
     Voigt6 eps{};
@@ -194,14 +198,19 @@ int ref_element_tangent_force(int type, const double* coords, const double* ue, 
     const size_t ng = fem::gauss_rule(t).size();
     const std::vector<Vec3> x = as_vec3(coords, size_t(n)), u = as_vec3(ue, size_t(n));
     std::unique_ptr<fem::MaterialModel> mat;
+#ifdef CXPP_DROPIN
+    using J2Law = cxpp::gpu::J2Material;
+#else
+    using J2Law = fem::J2PlasticMaterial;
+#endif
     if (p)
-      mat = std::make_unique<fem::J2PlasticMaterial>(ElasticIso{E, nu}, to_plastic(*p));
+      mat = std::make_unique<J2Law>(ElasticIso{E, nu}, to_plastic(*p));
     else
       mat = std::make_unique<fem::ElasticIsoMaterial>(ElasticIso{E, nu});
     std::vector<fem::MaterialState> st(ng);
     if (state)
       for (size_t q = 0; q < ng; ++q) load_state(state + q * ORC_STATE, st[q]);
-    const fem::ElementResponse r = fem::element_tangent_force(t, x, u, *mat, st);
+    const fem::ElementResponse r = impl::element_tangent_force(t, x, u, *mat, st);
     if (state)
       for (size_t q = 0; q < ng; ++q) store_state(st[q], state + q * ORC_STATE);
     std::memcpy(Ke, r.Ke.data(), r.Ke.size() * sizeof(double));
@@ -453,19 +462,37 @@ int ref_gpu_array_mass(const orc_model* m, int lumped, ref_system** out, char* e
   }
 }
 
-// COO system -> cxpp::gpu::solve_sparse (drop-in build only).
+// COO system -> cxpp::gpu::hip_backend().solve_sparse (drop-in build only); direct != 0
+// asks for SolverKind::Direct, else SolverKind::CG. *kind receives backend.kind().
 int ref_solve_sparse(int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
-                     const double* vals, const double* rhs, double* x, char* err, int errlen) {
+                     const double* vals, const double* rhs, int direct, double* x, int* kind,
+                     char* err, int errlen) {
   try {
     const std::vector<Index> r(rows, rows + nnz), c(cols, cols + nnz);
     const std::vector<Real> v(vals, vals + nnz), b(rhs, rhs + n);
-    const std::vector<Real> u = cxpp::gpu::solve_sparse(r, c, v, n, b, compute::SolverKind::CG);
+    const compute::ComputeBackend& be = cxpp::gpu::hip_backend();
+    if (kind) *kind = static_cast<int>(be.kind());
+    const std::vector<Real> u = be.solve_sparse(
+        r, c, v, n, b, direct ? compute::SolverKind::Direct : compute::SolverKind::CG);
     std::memcpy(x, u.data(), u.size() * sizeof(double));
     return 0;
   } catch (const std::exception& e) {
     set_err(err, errlen, e.what());
     return 1;
   }
+}
+
+// The calling thread's residency counters of cxpp::gpu (calls, mesh uploads, pattern
+// builds, DOF maps, history uploads, history commits); reset != 0 drops the cache first.
+void ref_gpu_resident_stats(int reset, int64_t* out) {
+  if (reset) cxpp::gpu::resident_reset();
+  const cxpp::gpu::ResidentStats s = cxpp::gpu::resident_stats();
+  out[0] = s.calls;
+  out[1] = s.mesh_uploads;
+  out[2] = s.pattern_builds;
+  out[3] = s.dof_maps;
+  out[4] = s.history_uploads;
+  out[5] = s.history_commits;
 }
 #endif
 

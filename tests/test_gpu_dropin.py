@@ -120,22 +120,17 @@ def test_element_stiffness(pair, fam_i):
 
 
 def test_solve_sparse_config1(pair):
-    """cxpp::gpu::solve_sparse on the reference's own COO system of config 1 against a
+    """cxpp::gpu::HipBackend::solve_sparse (the ComputeBackend seam, backend.hpp:40-57) on
+    the reference's own COO system of config 1, for SolverKind CG and Direct, against a
     dense direct solve (SciPP's spsolve is absent here)."""
-    import ctypes as C
     from calculixpp_amd import meshgen
     ref, gpu = pair
     s = ref.assemble_linear_static(meshgen.cantilever())
-    rows, cols = s.rows.astype(np.int32), s.cols.astype(np.int32)
-    vals, rhs = np.ascontiguousarray(s.vals), np.ascontiguousarray(s.rhs)
-    x = np.zeros(s.n_free)
-    err = C.create_string_buffer(512)
-    f = gpu.L.ref_solve_sparse
-    f.argtypes = [C.c_int32, C.c_int64] + [C.c_void_p] * 5 + [C.c_char_p, C.c_int]
-    p = lambda a: a.ctypes.data_as(C.c_void_p)
-    assert f(s.n_free, len(vals), p(rows), p(cols), p(vals), p(rhs), p(x), err, 512) == 0, err.value
-    exact = np.linalg.solve(s.dense(), rhs)
-    assert rel(x, exact) <= 1e-10
+    exact = np.linalg.solve(s.dense(), s.rhs)
+    for direct in (False, True):
+        x, kind = gpu.solve_sparse(s.n_free, s.rows, s.cols, s.vals, s.rhs, direct)
+        assert kind == 4  # the enumerator BackendKind::HIP takes when added
+        assert rel(x, exact) <= 1e-10
 
 
 def test_inverted_element_message(pair):
@@ -264,3 +259,109 @@ def test_array_model_errors(pair):
     m.blocks = [(_lib.C3D8, bad)]
     with pytest.raises(OracleError, match="element references unknown node"):
         gpu.array_assemble_linear_static(m)
+
+
+def test_newton_calls_pay_setup_once(pair):
+    """Newton-style use of cxpp::gpu::assemble_material_tangent on one cxpp::Model (as
+    solve_nonlinear_static, nonlinear_static.cpp:94): four calls with new displacements and
+    a commit between the second and the third. Every call agrees with the reference core;
+    the mesh, pattern and DOF map are built on the first call only, the committed history
+    is copied once and the commit is replayed on the device. A second Model is uploaded
+    afresh."""
+    import time
+    from calculixpp_amd import _lib, meshgen
+    from oracle.oracle import aos_state
+    ref, gpu = pair
+    m = meshgen.j2_block(4, 3, 3, (1.0, 0.8, 0.6), family=_lib.C3D8, elastic_every=4,
+                         jitter_frac=0.1)
+    st = aos_state(meshgen.seeded_history(m.n_points, 3))
+    gpu.resident_stats(reset=True)
+    times = []
+    with ref.open(m) as hr, gpu.open(m) as hg:
+        for it, amp in enumerate((1.0e-4, 1.5e-4, 1.8e-4, 2.0e-4)):
+            u = meshgen.smooth_field(m.coords, 7, amp)
+            s_e, f_e, st_e = ref.assemble_material_tangent(hr, u, st)
+            t0 = time.perf_counter()
+            s_g, f_g, st_g = gpu.assemble_material_tangent(hg, u, st)
+            times.append(time.perf_counter() - t0)
+            same_system(s_g, s_e)
+            assert rel(f_g, f_e) <= REL
+            for c in range(13, 26):
+                assert rel(st_g[:, c], st_e[:, c]) <= 1e-10, (it, c)
+            if it == 1:  # MaterialPoints::commit of the trial state the drop-in returned
+                st = st_g.copy()
+                st[:, :13] = st_g[:, 13:]
+        stats = gpu.resident_stats()
+        assert stats["calls"] == 4
+        assert stats["mesh_uploads"] == 1 and stats["pattern_builds"] == 1
+        assert stats["dof_maps"] == 1
+        assert stats["history_uploads"] == 1 and stats["history_commits"] == 1, stats
+        gpu.assemble_linear_static(hg)  # same model, linear path: still resident
+        assert gpu.resident_stats()["mesh_uploads"] == 1
+    m2 = meshgen.block(2, _lib.C3D20, seed_geo=61, seed_E=62)
+    same_system(gpu.assemble_linear_static(m2), ref.assemble_linear_static(m2))
+    assert gpu.resident_stats()["mesh_uploads"] == 2
+    print("drop-in call seconds:", ["%.4f" % t for t in times])
+
+
+@pytest.mark.parametrize("law", ["elastic", "iso4", "kin4", "comb4_hk"])
+@pytest.mark.parametrize("fam_i", range(4))
+def test_element_tangent_force(pair, fam_i, law):
+    """cxpp::gpu::element_tangent_force (element.hpp:160) against the reference's on one
+    element with seeded history: Ke, fe rel-L2 <= 1e-12, trial history <= 1e-10, the
+    elastic law leaving the state alone."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import J2Params
+    ref, gpu = pair
+    fam = [_lib.C3D8, _lib.C3D20, _lib.C3D20R, _lib.C3D6][fam_i]
+    b = meshgen.block(1, fam, seed_geo=71, jitter_frac=0.0)
+    conn = b.blocks[0][1][0]
+    rng = np.random.default_rng(72 + fam_i)
+    x = b.coords[conn] + rng.uniform(-0.05, 0.05, (len(conn), 3))
+    ue = rng.normal(scale=2.5e-3 if fam == _lib.C3D6 else 2e-4, size=(len(conn), 3))
+    params = None
+    if law != "elastic":
+        kinds = {"iso4": _lib.HARD_ISOTROPIC, "kin4": _lib.HARD_KINEMATIC,
+                 "comb4_hk": _lib.HARD_COMBINED}
+        hk = 5000.0 if law == "comb4_hk" else 0.0
+        params = J2Params(2.1e5, 0.3, [0.0, 4e-4, 1e-3, 1.6e-3], [250.0, 290.0, 320.0, 335.0],
+                          kinds[law], hk)
+    ng = _lib.lib().cxg_gauss_points(fam)
+    st = np.zeros((ng, 26))
+    h = meshgen.seeded_history(ng, 73).T
+    st[:, :13] = h
+    st[:, 13:] = rng.normal(size=(ng, 13))  # trial slots: stale values from elsewhere
+    ke_e, fe_e, st_e = ref.element_tangent_force(fam, x, ue, 2.1e5, 0.3, params, st)
+    ke_g, fe_g, st_g = gpu.element_tangent_force(fam, x, ue, 2.1e5, 0.3, params, st)
+    assert rel(ke_g, ke_e) <= REL and rel(fe_g, fe_e) <= REL
+    assert np.array_equal(st_g[:, :13], st_e[:, :13])
+    if params is None:
+        assert np.array_equal(st_g, st)
+    else:
+        assert (st_e[:, 19] > st_e[:, 6]).any()
+        for c in range(13, 26):
+            assert rel(st_g[:, c], st_e[:, c]) <= 1e-10, c
+
+
+def test_j2_material_evaluate(pair):
+    """cxpp::gpu::J2Material::evaluate (MaterialModel::evaluate on the device) against the
+    reference's J2PlasticMaterial::evaluate at seeded points of every hardening kind."""
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.model import J2Params
+    ref, gpu = pair
+    rng = np.random.default_rng(81)
+    hist = meshgen.seeded_history(40, 82)
+    for kind, hk in ((_lib.HARD_ISOTROPIC, 0.0), (_lib.HARD_KINEMATIC, 0.0),
+                     (_lib.HARD_COMBINED, 3000.0)):
+        p = J2Params(2.1e5, 0.3, [0.0, 4e-4, 1e-3, 1.6e-3], [250.0, 290.0, 320.0, 335.0], kind, hk)
+        n_yield = 0
+        for i in range(40):
+            eps = rng.normal(scale=1.2e-3, size=6)
+            st = np.zeros(26)
+            st[:13] = hist[:, i]
+            s_e, d_e, t_e = ref.j2_evaluate(p, eps, st)
+            s_g, d_g, t_g = gpu.j2_evaluate(p, eps, st)
+            assert rel(s_g, s_e) <= REL and rel(d_g, d_e) <= REL
+            assert rel(t_g[13:], t_e[13:]) <= 1e-10
+            n_yield += t_e[19] > t_e[6]
+        assert n_yield >= 5
