@@ -5,11 +5,13 @@ Workload (default): the 64^3 C3D20 block — 262,144 elements, 1,085,825 nodes,
 3,257,475 DOF, 557,286,921 CSR nonzeros — interior nodes jittered +-10% h (seed 0),
 E_e ~ U[150000, 250000] (seed 1), nu = 0.3, smooth seeded u (seed 2). One step is
 one full assembly of the CSR values of K (pattern reused, built once outside the
-clock) plus the internal force f_int = K u, all FP64, inputs resident in HBM.
+clock) plus the internal force f_int = K u, all FP64, inputs resident in HBM. The
+same block of C3D20R elements (8-point rule) is a second assembly leg.
 
 Prints one JSON line: throughput (elements/s), the dominant kernel's roofline figure
 from HIP events recorded on the library's own stream, and the reference CPU assembly
-timed on this host over a bounded sample.
+timed on this host over bounded samples (one pinned core, median of 3, plus an
+all-cores upper bound), all CPU timings taken before the GPU is first used.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 64] [--family C3D20]
 """
@@ -28,6 +30,7 @@ PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 (vector = matrix)
 PEAK_HBM = 8000.0   # GB/s, MI355X HBM3E
 J2_FLOP_PER_POINT = 300  # radial return, stress and compact tangent of a yielding point
 ADJ_FWD_FLOP_C3D8 = 5586  # matrix-free r = K(x, rho) u per C3D8 element
+J2_AMPLITUDE_40 = 1.2864e-4  # smooth-field amplitude at which ~40 % of the config-3 points yield
 
 
 def flops_per_element(nn: int, ngp: int, residual: bool) -> float:
@@ -46,10 +49,25 @@ def bytes_per_assembly(nnz: int, n_nodes: int, n_elem: int, nn: int, residual: b
     return b + (48 * n_nodes if residual else 0)
 
 
-def load_traffic(workload: str, variant: int, key: str = "hbm_bytes_per_step"):
-    """HBM bytes per assembly (or Newton iteration) measured by rocprofv3 --pmc
-    (FETCH_SIZE, WRITE_SIZE), copy-calibrated, as committed in profiles/traffic.json;
-    None if not measured for this variant."""
+def kernel_sources_sha() -> str:
+    """Hash of the library's kernel and launch sources (and its C header): traffic
+    counted on other sources is not reported."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "calculixpp_amd", "csrc")
+    files = sorted(os.path.join(csrc, f) for f in os.listdir(csrc)
+                   if f.endswith((".hip", ".cpp", ".hpp")))
+    for path in files + [os.path.join(ROOT, "include", "cxg.h")]:
+        with open(path, "rb") as f:
+            h.update(os.path.basename(path).encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def load_traffic(workload: str, variant: int):
+    """HBM bytes per kernel launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    copy-calibrated, as committed in profiles/traffic.json: {kernel: bytes}, plus
+    "step". None when not measured for this variant or when the record was counted on
+    kernel sources other than these (its sources_sha16 differs)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -58,7 +76,200 @@ def load_traffic(workload: str, variant: int, key: str = "hbm_bytes_per_step"):
         return None
     if not rec or rec.get("asm_variant") != variant:
         return None
-    return rec.get(key)
+    if rec.get("sources_sha16") != kernel_sources_sha():
+        return None
+    return rec
+
+
+# ---- CPU baselines: run in forked children before the GPU is first touched ----------
+
+_BATCH_RECS = None  # the batch workload's meshes, generated once, shared with the forks
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _allowed_cpus() -> list:
+    try:
+        return sorted(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover - non-Linux
+        return list(range(os.cpu_count() or 1))
+
+
+def _cpu_job(job: dict) -> dict:
+    """One timed CPU run of the reference core (or the oracle restatement when the
+    reference build is absent), in the calling process."""
+    from calculixpp_amd import _lib, meshgen
+    from oracle.oracle import Oracle, aos_state, ref_available
+    kind = "reference" if ref_available() else "port"
+    o = Oracle(kind)
+    what = job["what"]
+    if what in ("assembly", "mass", "recover"):
+        fam = _lib.TYPE_NAMES[job["family"]]
+        m = meshgen.block(job["n"], fam, seed_geo=job.get("seed", 0), seed_E=job.get("seed", 0) + 1)
+        u = meshgen.smooth_field(m.coords, 2)
+        if what == "assembly":
+            fn = lambda: o.time_linear_static(m, u)
+        elif what == "recover":
+            def fn():
+                t0 = time.perf_counter()
+                o.recover_fields(m, u)
+                return time.perf_counter() - t0
+        else:
+            from calculixpp_amd.model import Model
+            mm = Model(m.coords, m.blocks, m.E, m.nu, rho=7.85e-9)
+
+            def fn():
+                t0 = time.perf_counter()
+                o.assemble_mass(mm)
+                return time.perf_counter() - t0
+        units = m.n_elem
+    elif what == "j2":
+        m = meshgen.j2_block(*job["shape"])
+        h = aos_state(meshgen.seeded_history(m.n_points, 3))
+        u = meshgen.smooth_field(m.coords, 7, job["amplitude"])
+        fn = lambda: o.time_material_tangent(m, u, h)
+        units = m.n_elem
+    elif what == "batch":
+        recs = _BATCH_RECS if _BATCH_RECS is not None else meshgen.batch_meshes(job["meshes"])
+        sec, ev, used = 0.0, 0, 0
+        for r in recs:  # the batch's meshes in order, until about job["seconds"] of work,
+            if sec >= job["seconds"]:  # leaving out meshes too large for the budget
+                break
+            mdl = r["model"]
+            if mdl.n_elem > job["max_elements"]:
+                continue
+            used += 1
+            if r["j2"]:
+                for uu in r["u"]:
+                    sec += o.time_material_tangent(mdl, uu, aos_state(r["history"]))
+                ev += len(r["u"]) * mdl.n_elem
+            else:
+                sec += o.time_linear_static(mdl, r["u"][0])
+                ev += mdl.n_elem
+        return {"kind": kind, "units": ev, "times": [sec], "meshes": used}
+    else:
+        raise ValueError(what)
+    if job.get("barrier") is not None:
+        job["barrier"].wait()
+    times = [fn() for _ in range(job.get("reps", 1))]
+    return {"kind": kind, "units": units, "times": times}
+
+
+def _pinned(job: dict, cpu: int, out):
+    os.sched_setaffinity(0, {cpu})
+    try:
+        out.put(_cpu_job(job))
+    except Exception as e:  # reported, never fatal to the GPU legs
+        out.put({"error": f"{type(e).__name__}: {e}"})
+
+
+def cpu_one_core(job: dict) -> dict:
+    """job on one pinned host core (the first this process may use), in a fork."""
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    p = ctx.Process(target=_pinned, args=(job, _allowed_cpus()[0], q))
+    p.start()
+    res = q.get()
+    p.join()
+    return res
+
+
+def cpu_all_cores(job: dict, max_procs: int = 16) -> dict:
+    """P = min(cores this process may use, max_procs) independent processes, one per
+    core, each timing the job on its own slab (its own seeds) after a common barrier;
+    throughput = total units / the slowest process's time (no merge: an upper bound)."""
+    import multiprocessing as mp
+    cpus = _allowed_cpus()[:max_procs]
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    bar = ctx.Barrier(len(cpus))
+    procs = []
+    for k, c in enumerate(cpus):
+        j = dict(job, seed=100 + 7 * k, barrier=bar, reps=1)
+        procs.append(ctx.Process(target=_pinned, args=(j, c, q)))
+    for p in procs:
+        p.start()
+    res = [q.get() for _ in procs]
+    for p in procs:
+        p.join()
+    errs = [r["error"] for r in res if "error" in r]
+    if errs:
+        return {"error": errs[0]}
+    units = sum(r["units"] for r in res)
+    wall = max(r["times"][0] for r in res)
+    return {"value": units / wall, "processes": len(cpus), "units": units, "seconds": wall,
+            "kind": res[0]["kind"]}
+
+
+def baseline_record(res: dict, unit: str, sample: str, all_cores=None) -> dict:
+    """cpu_baseline object: one pinned core, median of the runs, plus the all-cores
+    upper bound when measured."""
+    import statistics
+    if "error" in res:
+        return {"value": None, "unit": unit, "cores": 1, "kind": None, "error": res["error"]}
+    t = statistics.median(res["times"])
+    out = {"value": res["units"] / t, "unit": unit, "cores": 1, "kind": res["kind"],
+           "sample": f"{sample}; one pinned host core, median of {len(res['times'])} runs "
+                     f"({', '.join(f'{x:.2f}' for x in res['times'])} s)",
+           "cpu_model": _cpu_model()}
+    if all_cores is not None:
+        out["all_cores"] = dict(all_cores, unit=unit, cpu_model=out["cpu_model"],
+                                note="P independent processes on disjoint samples, no merge: "
+                                     "an upper bound")
+    return out
+
+
+def cpu_phase(args) -> dict:
+    """Every CPU baseline of the run, before any GPU work."""
+    out = {}
+    if args.no_cpu:
+        return out
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    n = args.cpu_n
+    for fam in ("C3D20", "C3D20R"):
+        job = {"what": "assembly", "family": fam, "n": n, "reps": 3}
+        one = cpu_one_core(job)
+        allc = cpu_all_cores(job) if not args.no_all_cores else None
+        out[fam] = baseline_record(one, "elements/s",
+                                   f"{n}^3 {fam} block of the same seeds ({n ** 3} elements, "
+                                   "assemble_linear_static + internal_force)", allc)
+        log(f"[bench] cpu {fam}: {out[fam]['value']} elements/s on one core")
+    if not args.no_j2:
+        shape = (32, 32, 16)
+        one = cpu_one_core({"what": "j2", "shape": shape, "amplitude": J2_AMPLITUDE_40, "reps": 3})
+        out["j2"] = baseline_record(one, "elements/s",
+                                    f"{shape[0]}x{shape[1]}x{shape[2]} C3D8 J2 block, same seeds, "
+                                    "amplitude of 40 % yield, assemble_material_tangent")
+    if not args.no_recover:
+        one = cpu_one_core({"what": "recover", "family": args.family, "n": n, "reps": 3})
+        out["recover"] = baseline_record(one, "elements/s",
+                                         f"{n}^3 {args.family} block, recover_fields")
+    if not args.no_mass:
+        one = cpu_one_core({"what": "mass", "family": args.family, "n": n, "reps": 3})
+        out["mass"] = baseline_record(one, "elements/s", f"{n}^3 {args.family} block, assemble_mass")
+    if not args.no_batch:
+        global _BATCH_RECS
+        if _BATCH_RECS is None:
+            from calculixpp_amd import meshgen
+            _BATCH_RECS = meshgen.batch_meshes(args.batch_meshes)
+        one = cpu_one_core({"what": "batch", "meshes": args.batch_meshes, "seconds": 10.0,
+                            "max_elements": 5000})
+        if "error" not in one:
+            out["batch"] = baseline_record(one, "element evaluations/s",
+                                           f"the batch's first {one['meshes']} meshes of at most "
+                                           f"5,000 elements ({one['units']} element evaluations, "
+                                           "linear and J2 steps as in the batch)")
+    return out
 
 
 def adjoint_ratio(reps: int = 20) -> dict:
@@ -106,8 +317,7 @@ def adjoint_ratio(reps: int = 20) -> dict:
                                  "frac": round(t_att * 1e3 / f_ms, 4)}}
 
 
-def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
-              cpu_shape=(32, 32, 16), cpu: bool = True) -> dict:
+def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40, cpu_rec=None) -> dict:
     """One J2 Newton iteration (radial return at every Gauss point, consistent tangent
     K_t into the CSR values, internal force) on the 128x128x64 C3D8 block with seeded
     committed history (seed 3). The smooth displacement (seed 7) is scaled by bisection
@@ -183,46 +393,68 @@ def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40,
                        else "hbm", "flop": flop, "algorithmic_bytes": byt,
                        "t_attainable_ms": round(t_att * 1e3, 4),
                        "frac": round(t_att * 1e3 / out["kernel_ms"], 4),
-                       "traffic": load_traffic("J2-C3D8-128x128x64",
-                                               ctx.options.get("asm_variant"),
-                                               "hbm_bytes_per_iteration")}
-    if cpu:
-        from oracle.oracle import Oracle, aos_state, ref_available
-        kind = "reference" if ref_available() else "port"
-        cm = meshgen.j2_block(*cpu_shape)
-        ch = meshgen.seeded_history(cm.n_points, 3)
-        cu = meshgen.smooth_field(cm.coords, 7, amp)
-        sec = Oracle(kind).time_material_tangent(cm, cu, aos_state(ch))
-        out["cpu_baseline"] = {"value": cm.n_elem / sec, "unit": "elements/s", "cores": 1,
-                               "kind": kind,
-                               "sample": f"{cpu_shape[0]}x{cpu_shape[1]}x{cpu_shape[2]} C3D8 J2 "
-                                         f"block, same seeds and amplitude, {sec:.2f} s"}
-        out["gpu_over_cpu"] = out["elements_per_s"] / out["cpu_baseline"]["value"]
+                       "traffic": (load_traffic("J2-C3D8-128x128x64",
+                                                ctx.options.get("asm_variant")) or {}).get("step")}
+    if cpu_rec and cpu_rec.get("value"):
+        out["cpu_baseline"] = cpu_rec
+        out["gpu_over_cpu"] = out["elements_per_s"] / cpu_rec["value"]
     return out
 
 
-def batch_step(n_meshes: int = 512, reps: int = 3, cpu: bool = True, cpu_seconds: float = 15.0,
-               log=None) -> dict:
-    """The batched workload: n_meshes mixed C3D8/C3D20/C3D6 boxes with log-uniform
-    element counts in [100, 50000] (seed 4), half of them J2, 4 load cases each. One
-    step = the linear meshes' K plus f_int for the 4 cases, and for the J2 meshes per
-    case the return map, K_t and f_int; elements of every mesh and family go through one
-    persistent launch per assembly. Throughput counts element evaluations (a linear
-    element once, a J2 element once per case)."""
+def batch_roofline(step, ms: float) -> dict:
+    """Algorithmic FLOP and bytes of one batch step, summed per family: every linear
+    element once (K) plus its residual per load case, every J2 element per load case
+    (K_t, f_int and 300 FLOP per Gauss point of return map); bytes as the assembly formula
+    per mesh set (K values, coordinates, connectivity, E; u read and f_int written per
+    case) plus the J2 history read and written (208 bytes per point per case)."""
+    from calculixpp_amd.model import GAUSS, NODES
+    flop = byt = 0.0
+    nc = step.n_cases
+    for kind, b, asm in (("lin", step.lin, getattr(step, "lin_asm", None)),
+                         ("j2", step.j2, getattr(step, "j2_asm", None))):
+        if b is None:
+            continue
+        m = b.model
+        for t, c in m.blocks:
+            nn, ng, ne = NODES[t], GAUSS[t], c.shape[0]
+            k = flops_per_element(nn, ng, False) * ne
+            r = (flops_per_element(nn, ng, True) - flops_per_element(nn, ng, False)) * ne
+            if kind == "lin":
+                flop += k + nc * r
+                byt += (4 * nn + 8) * ne
+            else:
+                flop += nc * (k + r + J2_FLOP_PER_POINT * ng * ne)
+                byt += nc * (4 * nn + 8) * ne
+        reps = 1 if kind == "lin" else nc
+        byt += reps * (8 * asm.pattern.nnz + 24 * m.n_nodes) + nc * 48 * m.n_nodes
+        if kind == "j2":
+            byt += nc * 208 * m.n_points
+    t_f, t_b = flop / (PEAK_FP64 * 1e12), byt / (PEAK_HBM * 1e9)
+    return {"bound": "mfma" if t_f >= t_b else "hbm", "flop_per_step": flop,
+            "algorithmic_bytes_per_step": byt, "t_attainable_ms": round(max(t_f, t_b) * 1e3, 4),
+            "frac": round(max(t_f, t_b) * 1e3 / ms, 4),
+            "achieved_TFLOPs": round(flop / (ms * 1e-3) / 1e12, 2),
+            "achieved_GBps": round(byt / (ms * 1e-3) / 1e9, 1)}
+
+
+def batch_step(recs, reps: int = 3, cpu_rec=None, log=None) -> dict:
+    """The batched workload: 512 mixed C3D8/C3D20/C3D6 boxes with log-uniform element
+    counts in [100, 50000] (seed 4), half of them J2, 4 load cases each. One step = the
+    linear meshes' K plus f_int for the 4 cases, and for the J2 meshes per case the return
+    map, K_t and f_int; elements of every mesh and family go through one persistent launch
+    per assembly. Throughput counts element evaluations (a linear element once, a J2
+    element once per case)."""
     import torch
-    from calculixpp_amd import autotune, meshgen
+    from calculixpp_amd import autotune
     from calculixpp_amd.batch import BatchStep
     from calculixpp_amd import fem
-    t0 = time.perf_counter()
-    recs = meshgen.batch_meshes(n_meshes)
     t1 = time.perf_counter()
     ctx = fem.context()
     step = BatchStep(recs, ctx, variant=5)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     if log:
-        log(f"[bench] batch: {n_meshes} meshes generated in {t1 - t0:.1f}s, uploaded + "
-            f"patterns in {t2 - t1:.1f}s")
+        log(f"[bench] batch: {len(recs)} meshes uploaded + patterns in {t2 - t1:.1f}s")
     tune = autotune.tune_persistent(step.run, ctx, log=log)
     n_lin = step.lin.model.n_elem if step.lin else 0
     n_j2 = step.j2.model.n_elem if step.j2 else 0
@@ -250,41 +482,25 @@ def batch_step(n_meshes: int = 512, reps: int = 3, cpu: bool = True, cpu_seconds
     kern = {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in ctx.profile_stats().items()}
     nnz = sum(a.pattern.nnz for a in [getattr(step, "lin_asm", None), getattr(step, "j2_asm", None)]
               if a is not None)
-    out = {"workload": f"{n_meshes} mixed C3D8/C3D20/C3D6 meshes, log-uniform 100..50000 "
+    best = min(eager_ms, graph_ms)
+    out = {"workload": f"{len(recs)} mixed C3D8/C3D20/C3D6 meshes, log-uniform 100..50000 "
                        f"elements, half J2, {step.n_cases} load cases",
-           "meshes": n_meshes, "elements": n_lin + n_j2, "linear_elements": n_lin,
+           "meshes": len(recs), "elements": n_lin + n_j2, "linear_elements": n_lin,
            "j2_elements": n_j2, "element_evaluations": evals, "nnz": nnz,
            "ms_per_step": eager_ms, "graph_ms_per_step": graph_ms,
-           "evaluations_per_s": evals / (min(eager_ms, graph_ms) * 1e-3),
+           "evaluations_per_s": evals / (best * 1e-3),
+           "roofline": batch_roofline(step, best),
            "autotune": tune, "kernels_one_graph_plus_one_eager_step": kern}
     ctx.set_option("asm_variant", 7)
     del step
     torch.cuda.empty_cache()
-    if cpu:
-        from oracle.oracle import Oracle, aos_state, ref_available
-        kind = "reference" if ref_available() else "port"
-        o = Oracle(kind)
-        sec, ev, used = 0.0, 0, 0
-        for r in recs:  # the batch's meshes in order, until about cpu_seconds of work
-            if sec >= cpu_seconds:
-                break
-            used += 1
-            m = r["model"]
-            if r["j2"]:
-                for u in r["u"]:
-                    sec += o.time_material_tangent(m, u, aos_state(r["history"]))
-                ev += len(r["u"]) * m.n_elem
-            else:
-                sec += o.time_linear_static(m, r["u"][0])
-                ev += m.n_elem
-        out["cpu_baseline"] = {"value": ev / sec, "unit": "element evaluations/s", "cores": 1,
-                               "kind": kind, "sample": f"the batch's first {used} meshes "
-                               f"({ev} element evaluations, {sec:.1f} s on one host thread)"}
-        out["gpu_over_cpu"] = out["evaluations_per_s"] / out["cpu_baseline"]["value"]
+    if cpu_rec and cpu_rec.get("value"):
+        out["cpu_baseline"] = cpu_rec
+        out["gpu_over_cpu"] = out["evaluations_per_s"] / cpu_rec["value"]
     return out
 
 
-def recover_leg(model, u, family: int, reps: int = 5, cpu_n: int = 16, cpu: bool = True) -> dict:
+def recover_leg(model, u, family: int, reps: int = 5, cpu_rec=None) -> dict:
     """Field recovery (recover_fields: nodal stress and strain, f_int, reactions) of the
     displacement u on the assembly workload, from HIP events on the library's stream.
     Algorithmic counts per element: n_gp (72 n + 80) FLOP (Jacobian, gradients,
@@ -322,21 +538,9 @@ def recover_leg(model, u, family: int, reps: int = 5, cpu_n: int = 16, cpu: bool
                         "frac": round(t_att / (ms * 1e-3), 4)}}
     del r
     torch.cuda.synchronize()
-    if cpu:
-        from calculixpp_amd import meshgen
-        from oracle.oracle import Oracle, ref_available
-        kind = "reference" if ref_available() else "port"
-        cm = meshgen.block(cpu_n, family)
-        cu = meshgen.smooth_field(cm.coords, 2)
-        o = Oracle(kind)
-        t0 = time.perf_counter()
-        o.recover_fields(cm, cu)
-        sec = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": cm.n_elem / sec, "unit": "elements/s", "cores": 1,
-                               "kind": kind, "sample": f"{cpu_n}^3 block of the same family "
-                               f"and seeds ({cm.n_elem} elements, recover_fields, {sec:.1f} s "
-                               "on one host thread)"}
-        out["gpu_over_cpu"] = out["elements_per_s"] / out["cpu_baseline"]["value"]
+    if cpu_rec and cpu_rec.get("value"):
+        out["cpu_baseline"] = cpu_rec
+        out["gpu_over_cpu"] = out["elements_per_s"] / cpu_rec["value"]
     return out
 
 
@@ -393,7 +597,7 @@ def pcg_leg(n: int = 64, iters: int = 96) -> dict:
     return out
 
 
-def mass_geometric_leg(asm, family: int, reps: int = 3, cpu_n: int = 16, cpu: bool = True) -> dict:
+def mass_geometric_leg(asm, family: int, reps: int = 3, cpu_rec=None) -> dict:
     """Consistent mass, lumped mass and geometric stiffness into the CSR values of the
     assembly workload's pattern (density 7.85e-9, seeded Gauss-point stress), from HIP
     events on the library's stream. Both matrices are k_ab I3 blocks written through
@@ -436,154 +640,59 @@ def mass_geometric_leg(asm, family: int, reps: int = 3, cpu_n: int = 16, cpu: bo
                                  "frac": round(t_att / (ms * 1e-3), 4)}}
     del vals
     torch.cuda.synchronize()
-    if cpu:
-        from calculixpp_amd import meshgen
-        from calculixpp_amd.model import Model
-        from oracle.oracle import Oracle, ref_available
-        kind = "reference" if ref_available() else "port"
-        b = meshgen.block(cpu_n, family)
-        cm = Model(b.coords, b.blocks, b.E, b.nu, rho=7.85e-9)
-        o = Oracle(kind)
-        t0 = time.perf_counter()
-        o.assemble_mass(cm)
-        sec = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": cm.n_elem / sec, "unit": "elements/s", "cores": 1,
-                               "kind": kind, "sample": f"{cpu_n}^3 block of the same family "
-                               f"({cm.n_elem} elements, assemble_mass, {sec:.1f} s on one "
-                               "host thread)"}
-        out["gpu_over_cpu"] = out["mass"]["elements_per_s"] / out["cpu_baseline"]["value"]
+    if cpu_rec and cpu_rec.get("value"):
+        out["cpu_baseline"] = cpu_rec
+        out["gpu_over_cpu"] = out["mass"]["elements_per_s"] / cpu_rec["value"]
     return out
 
 
-def cpu_baseline(family: int, n: int, with_residual: bool) -> dict:
-    """Reference CPU assembly (+ internal force) on an n^3 sample of the same
-    workload, one thread (the reference is single-threaded)."""
-    from calculixpp_amd import meshgen
-    from oracle.oracle import Oracle, ref_available
-    kind = "reference" if ref_available() else "port"
-    model = meshgen.block(n, family)
-    u = meshgen.smooth_field(model.coords, 2) if with_residual else None
-    sec = Oracle(kind).time_linear_static(model, u)
-    return {"value": model.n_elem / sec, "unit": "elements/s", "cores": 1, "kind": kind,
-            "sample": f"{n}^3 block of the same family and seeds ({model.n_elem} elements, "
-                      f"assemble_linear_static{' + internal_force' if with_residual else ''}, "
-                      f"{sec:.1f} s on one host thread)"}
-
-
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--family", default="C3D20")
-    ap.add_argument("--cpu-n", type=int, default=16, help="edge of the CPU-baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-residual", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adj-reps", type=int, default=20)
-    ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
-    ap.add_argument("--no-recover", action="store_true", help="skip the field-recovery leg")
-    ap.add_argument("--no-mass", action="store_true",
-                    help="skip the mass / geometric-stiffness leg")
-    ap.add_argument("--no-pcg", action="store_true", help="skip the device-solve leg")
-    ap.add_argument("--batch-meshes", type=int, default=512)
-    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
-                    help="extra cxg_set_option for the library context (repeatable)")
-    ap.add_argument("--only", choices=["j2", "batch", "adjoint", "pcg"], default=None,
-                    help="run one secondary leg alone and print its JSON (profiling)")
-    ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
-    ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        n = args.copy_calibration * (1 << 20) // 8
-        src = torch.rand(n, dtype=torch.float64, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_each_way": n * 8, "launches": 3}), flush=True)
-        return 0
-    if args.only:
-        import torch
-        torch.cuda.set_device(0)
-        if args.option or args.variant:
-            from calculixpp_amd import fem
-            if args.variant:
-                fem.context().set_option("asm_variant", args.variant)
-            for opt in args.option:
-                key, val = opt.split("=", 1)
-                fem.context().set_option(key, int(val))
-        if args.only == "j2":
-            res = j2_newton(reps=args.steps, cpu=not args.no_cpu)
-        elif args.only == "pcg":
-            res = pcg_leg(args.n)
-        elif args.only == "batch":
-            res = batch_step(args.batch_meshes, reps=args.steps, cpu=not args.no_cpu,
-                             log=lambda m: print(m, file=sys.stderr, flush=True))
-        else:
-            res = adjoint_ratio(args.adj_reps)
-        print(json.dumps(res), flush=True)
-        return 0
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding of this path is a design, not a build", file=sys.stderr)
-        return 2
-
-    import numpy as np
+def assembly_leg(family_name: str, n: int, residual: bool, steps: int, warmup: int,
+                 variant: int = 0, patch: int = 0, options=(), log=None) -> dict:
+    """One assembly workload: the n^3 block of family_name (seeds 0, 1, 2), K values
+    (+ internal force), inputs resident in HBM. Wall-clock throughput over `steps`
+    synchronized steps, then per-kernel durations from HIP events on the library's
+    stream. Roofline figures: per kernel against its own bound (the element kernel's
+    algorithmic FP64 count over its time, the row gather's algorithmic bytes -- K values
+    written, u read and f_int written -- over its time), the dominant (longest) kernel
+    being the headline, and over the step as max(F/peak_fp64, B/peak_hbm) / t."""
     import torch
-    from calculixpp_amd import _lib, fem, meshgen
+    from calculixpp_amd import _lib, autotune, fem, meshgen
     from calculixpp_amd.model import GAUSS, NODES
-
-    family = _lib.TYPE_NAMES[args.family]
-    residual = not args.no_residual
+    family = _lib.TYPE_NAMES[family_name]
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     t0 = time.perf_counter()
-    model = meshgen.block(args.n, family)
-    u_host = meshgen.smooth_field(model.coords, 2)
+    model = meshgen.block(n, family)
+    u = torch.from_numpy(meshgen.smooth_field(model.coords, 2)).to(dev) if residual else None
     t1 = time.perf_counter()
-    u = torch.from_numpy(u_host).to(dev) if residual else None
-    from calculixpp_amd import autotune
-    if args.variant and args.patch:
-        choice = {"asm_variant": args.variant, "patch_size": args.patch, "trials": []}
+    if variant and patch:
+        choice = {"asm_variant": variant, "patch_size": patch, "trials": []}
     else:
-        choice = autotune.tune_linear(model, u, cache_path=None, log=lambda m: print(
-            m, file=sys.stderr, flush=True))
+        choice = autotune.tune_linear(model, u, cache_path=None, log=log)
     t2 = time.perf_counter()
-    for opt in args.option:
+    for opt in options:
         key, val = opt.split("=", 1)
         fem.context().set_option(key, int(val))
     asm = autotune.make_assembler(model, choice["asm_variant"], choice["patch_size"])
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    print(f"[bench] mesh {t1 - t0:.1f}s, tune {t2 - t1:.1f}s, upload+colour+pattern "
-          f"{t3 - t2:.1f}s: {model.n_elem} elements, {model.n_nodes} nodes, nnz "
-          f"{asm.pattern.nnz}, {asm.mesh.n_colors} colours, variant {choice['asm_variant']}, "
-          f"patch {choice['patch_size']}", file=sys.stderr, flush=True)
+    if log:
+        log(f"[bench] {family_name}: mesh {t1 - t0:.1f}s, tune {t2 - t1:.1f}s, "
+            f"upload+colour+pattern {t3 - t2:.1f}s: {model.n_elem} elements, {model.n_nodes} "
+            f"nodes, nnz {asm.pattern.nnz}, variant {choice['asm_variant']}")
     vals = asm.new_values()
     f_int = torch.empty(3 * model.n_nodes, dtype=torch.float64, device=dev) if residual else None
     asm.assemble(vals, u, f_int)  # set-up: arenas allocated, gather order built
-
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         asm.assemble(vals, u, f_int)
     torch.cuda.synchronize()
     ts = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         asm.assemble(vals, u, f_int)
     torch.cuda.synchronize()
-    te = time.perf_counter()
-    ms_step = (te - ts) * 1e3 / args.steps
+    ms_step = (time.perf_counter() - ts) * 1e3 / steps
     value = model.n_elem / (ms_step * 1e-3)
-    print(f"[bench] {ms_step:.3f} ms/step, {value:.4g} elements/s", file=sys.stderr, flush=True)
-
-    # Per-kernel durations from HIP events on the library's stream. One assembly is
-    # the element kernel followed by the row-gather kernel (two launches, one step);
-    # the roofline fraction is max(F/peak_fp64, B/peak_hbm) / t over both launches.
+    if log:
+        log(f"[bench] {family_name}: {ms_step:.3f} ms/step, {value:.4g} elements/s")
     ctx = asm.ctx
     ctx.profile_reset()
     ctx.profile(True)
@@ -596,92 +705,203 @@ def main() -> int:
                    "ms_per_step": round(v[1] / reps, 4), "elements": v[2]}
                for k, v in stats.items()}
     step_ms = sum(v[1] for v in stats.values()) / reps
-    name = max(stats, key=lambda k: stats[k][1])
     nn, ngp = NODES[family], GAUSS[family]
     fpe = flops_per_element(nn, ngp, residual)
     flop = fpe * model.n_elem
     bpa = bytes_per_assembly(asm.pattern.nnz, model.n_nodes, model.n_elem, nn, residual)
+    gather_bytes = 8 * asm.pattern.nnz + (48 * model.n_nodes if residual else 0)
     t_flop, t_byte = flop / (PEAK_FP64 * 1e12), bpa / (PEAK_HBM * 1e9)
-    bound = "mfma" if t_flop >= t_byte else "hbm"
-    achieved_tf = flop / (step_ms * 1e-3) / 1e12
-    achieved_gb = bpa / (step_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"{args.family}-{args.n}", choice["asm_variant"])
-    roofline = {"bound": bound,
-                "achieved": round(achieved_tf if bound == "mfma" else achieved_gb, 3),
-                "peak": PEAK_FP64 if bound == "mfma" else PEAK_HBM,
-                "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
-                "frac": round(max(t_flop, t_byte) / (step_ms * 1e-3), 4),
-                "traffic": traffic,
-                "kernel": " + ".join(sorted(stats)), "dominant": name,
-                "step_kernel_ms": round(step_ms, 4),
-                "flop_per_element": fpe, "flop_per_step": flop,
-                "algorithmic_bytes_per_step": bpa,
-                "hbm_algorithmic_GBps": round(achieved_gb, 1),
-                "t_attainable_ms": round(max(t_flop, t_byte) * 1e3, 4)}
+    traffic = load_traffic(f"{family_name}-{n}", choice["asm_variant"])
+    per_kernel = {}
+    for k, v in stats.items():
+        avg = v[1] / v[0] * 1e-3
+        if k.startswith("row_gather"):
+            per_kernel[k] = {"bound": "hbm", "algorithmic_bytes": gather_bytes,
+                             "achieved": round(gather_bytes / avg / 1e9, 1), "peak": PEAK_HBM,
+                             "unit": "GB/s", "frac": round(gather_bytes / avg / 1e9 / PEAK_HBM, 4)}
+        else:
+            per_kernel[k] = {"bound": "mfma", "algorithmic_flop": flop,
+                             "achieved": round(flop / avg / 1e12, 2), "peak": PEAK_FP64,
+                             "unit": "TFLOP/s", "frac": round(flop / avg / 1e12 / PEAK_FP64, 4)}
+        per_kernel[k]["avg_ms"] = round(v[1] / v[0], 4)
+        per_kernel[k]["traffic"] = traffic["kernels"].get(k) if traffic else None
+    dominant = max(stats, key=lambda k: stats[k][1])
+    head = dict(per_kernel[dominant])
+    head.update({"kernel": dominant,
+                 "traffic_note": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE + "
+                                  "WRITE_SIZE (copy-calibrated), profiles/traffic.json, counted "
+                                  "on these kernel sources") if traffic else
+                 "no counter record for these kernel sources",
+                 "kernels": per_kernel,
+                 "step": {"frac": round(max(t_flop, t_byte) / (step_ms * 1e-3), 4),
+                          "bound": "mfma" if t_flop >= t_byte else "hbm",
+                          "step_kernel_ms": round(step_ms, 4),
+                          "t_attainable_ms": round(max(t_flop, t_byte) * 1e3, 4),
+                          "flop_per_element": fpe, "flop_per_step": flop,
+                          "algorithmic_bytes_per_step": bpa,
+                          "achieved_TFLOPs": round(flop / (step_ms * 1e-3) / 1e12, 2),
+                          "hbm_algorithmic_GBps": round(bpa / (step_ms * 1e-3) / 1e9, 1),
+                          "traffic": traffic.get("step") if traffic else None}})
+    return {"model": model, "u": u, "asm": asm, "family": family, "value": value,
+            "ms_per_step": ms_step, "choice": choice, "kernels": kernels, "roofline": head,
+            "config": {"workload": f"{n}^3 {family_name} jittered block: CSR K values"
+                                   f"{' + internal force' if residual else ''}",
+                       "elements": model.n_elem, "nodes": model.n_nodes,
+                       "dof": 3 * model.n_nodes, "nnz": asm.pattern.nnz,
+                       "colours": asm.mesh.n_colors, "asm_variant": choice["asm_variant"],
+                       "patch_size": choice["patch_size"]}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--family", default="C3D20")
+    ap.add_argument("--cpu-n", type=int, default=10, help="edge of the CPU-baseline samples")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-all-cores", action="store_true",
+                    help="skip the all-cores upper bound of the CPU baseline")
+    ap.add_argument("--no-residual", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-reps", type=int, default=20)
+    ap.add_argument("--no-second", action="store_true",
+                    help="skip the second assembly family (C3D20R, or C3D20 for --family C3D20R)")
+    ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton-iteration leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-meshes leg")
+    ap.add_argument("--no-recover", action="store_true", help="skip the field-recovery leg")
+    ap.add_argument("--no-mass", action="store_true",
+                    help="skip the mass / geometric-stiffness leg")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the device-solve leg")
+    ap.add_argument("--batch-meshes", type=int, default=512)
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="extra cxg_set_option for the library context (repeatable)")
+    ap.add_argument("--only", choices=["j2", "batch", "adjoint", "pcg", "assembly"], default=None,
+                    help="run one leg alone and print its JSON (profiling)")
+    ap.add_argument("--variant", type=int, default=0, help="force asm_variant (0 = autotune)")
+    ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 8, "launches": 3}), flush=True)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "sharding of this path is a design, not a build", file=sys.stderr)
+        return 2
+    if args.only:
+        import torch
+        torch.cuda.set_device(0)
+        if args.option or args.variant:
+            from calculixpp_amd import fem
+            if args.variant:
+                fem.context().set_option("asm_variant", args.variant)
+            for opt in args.option:
+                key, val = opt.split("=", 1)
+                fem.context().set_option(key, int(val))
+        if args.only == "j2":
+            res = j2_newton(reps=args.steps)
+        elif args.only == "pcg":
+            res = pcg_leg(args.n)
+        elif args.only == "batch":
+            from calculixpp_amd import meshgen
+            res = batch_step(meshgen.batch_meshes(args.batch_meshes), reps=args.steps, log=log)
+        elif args.only == "assembly":
+            leg = assembly_leg(args.family, args.n, not args.no_residual, args.steps, args.warmup,
+                               args.variant, args.patch or (8 if args.variant else 0), args.option,
+                               log)
+            res = {k: leg[k] for k in ("value", "ms_per_step", "kernels", "roofline", "config")}
+        else:
+            res = adjoint_ratio(args.adj_reps)
+        print(json.dumps(res), flush=True)
+        return 0
+
+    # CPU baselines first, in forked children, before anything touches the GPU
+    cpu = cpu_phase(args)
+    import torch
+    torch.cuda.set_device(0)
+    residual = not args.no_residual
+    leg = assembly_leg(args.family, args.n, residual, args.steps, args.warmup, args.variant,
+                       args.patch, args.option, log)
+    model, u, asm, family = leg["model"], leg["u"], leg["asm"], leg["family"]
+    head = {k: leg[k] for k in ("value", "ms_per_step", "config", "choice", "roofline", "kernels")}
+    main_cpu = cpu.get(args.family)
+
+    second = None
+    if not args.no_second:
+        fam2 = "C3D20R" if args.family != "C3D20R" else "C3D20"
+        # the first leg's device buffers stay for the recovery and mass legs below
+        leg2 = assembly_leg(fam2, args.n, residual, args.steps, args.warmup, args.variant,
+                            args.patch, args.option, log)
+        second = {"family": fam2, "value": leg2["value"], "unit": "elements/s",
+                  "ms_per_step": leg2["ms_per_step"], "config": leg2["config"],
+                  "roofline": leg2["roofline"], "kernels": leg2["kernels"],
+                  "cpu_baseline": cpu.get(fam2)}
+        if second["cpu_baseline"] and second["cpu_baseline"].get("value"):
+            second["gpu_over_cpu"] = leg2["value"] / second["cpu_baseline"]["value"]
+        del leg2
+        torch.cuda.empty_cache()
 
     adjoint = None if args.no_adjoint else adjoint_ratio(args.adj_reps)
     if adjoint:
-        print(f"[bench] adjoint/forward {adjoint['ratio']:.3f} ({adjoint['vjp_ms']:.4f} / "
-              f"{adjoint['forward_ms']:.4f} ms)", file=sys.stderr, flush=True)
-
-    j2 = None if args.no_j2 else j2_newton(cpu=not args.no_cpu)
+        log(f"[bench] adjoint/forward {adjoint['ratio']:.3f} ({adjoint['vjp_ms']:.4f} / "
+            f"{adjoint['forward_ms']:.4f} ms)")
+    j2 = None if args.no_j2 else j2_newton(cpu_rec=cpu.get("j2"))
     if j2:
-        print(f"[bench] J2 Newton iteration {j2['ms_per_iteration']:.3f} ms "
-              f"({j2['elements_per_s']:.4g} elements/s, yield {j2['yield_fraction']:.3f})",
-              file=sys.stderr, flush=True)
-
+        log(f"[bench] J2 Newton iteration {j2['ms_per_iteration']:.3f} ms "
+            f"({j2['elements_per_s']:.4g} elements/s, yield {j2['yield_fraction']:.3f})")
     rec = None
     if not args.no_recover and u is not None:
-        rec = recover_leg(model, u.reshape(-1, 3), family, cpu_n=args.cpu_n, cpu=not args.no_cpu)
-        print(f"[bench] recover_fields {rec['ms']:.3f} ms ({rec['elements_per_s']:.4g} "
-              f"elements/s)", file=sys.stderr, flush=True)
-
-    mass = None if args.no_mass else mass_geometric_leg(asm, family, cpu_n=args.cpu_n,
-                                                        cpu=not args.no_cpu)
+        rec = recover_leg(model, u.reshape(-1, 3), family, cpu_rec=cpu.get("recover"))
+        log(f"[bench] recover_fields {rec['ms']:.3f} ms ({rec['elements_per_s']:.4g} elements/s)")
+    mass = None if args.no_mass else mass_geometric_leg(asm, family, cpu_rec=cpu.get("mass"))
     if mass:
-        print(f"[bench] mass {mass['mass']['ms']:.3f} ms, lumped {mass['mass_lumped']['ms']:.3f} "
-              f"ms, geometric {mass['geometric']['ms']:.3f} ms", file=sys.stderr, flush=True)
-
+        log(f"[bench] mass {mass['mass']['ms']:.3f} ms, lumped {mass['mass_lumped']['ms']:.3f} "
+            f"ms, geometric {mass['geometric']['ms']:.3f} ms")
+    del asm, leg
+    torch.cuda.empty_cache()
     pcg = None if args.no_pcg else pcg_leg(args.n)
     if pcg:
-        print(f"[bench] pcg {pcg['ms_per_iteration']:.3f} ms per iteration "
-              f"({pcg['roofline']['achieved_GBps']:.0f} GB/s)", file=sys.stderr, flush=True)
-
-    batch = None if args.no_batch else batch_step(
-        args.batch_meshes, cpu=not args.no_cpu, log=lambda m: print(m, file=sys.stderr, flush=True))
-    if batch:
-        print(f"[bench] batch step {batch['ms_per_step']:.2f} ms eager, "
-              f"{batch['graph_ms_per_step']:.2f} ms graph ({batch['evaluations_per_s']:.4g} "
-              "element evaluations/s)", file=sys.stderr, flush=True)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(family, args.cpu_n, residual)
-        print(f"[bench] cpu baseline {cpu['value']:.4g} elements/s ({cpu['kind']})",
-              file=sys.stderr, flush=True)
-
+        log(f"[bench] pcg {pcg['ms_per_iteration']:.3f} ms per iteration "
+            f"({pcg['roofline']['achieved_GBps']:.0f} GB/s)")
+    batch = None
+    if not args.no_batch:
+        from calculixpp_amd import meshgen
+        recs = _BATCH_RECS if _BATCH_RECS is not None else meshgen.batch_meshes(args.batch_meshes)
+        batch = batch_step(recs, cpu_rec=cpu.get("batch"), log=log)
+        log(f"[bench] batch step {batch['ms_per_step']:.2f} ms eager, "
+            f"{batch['graph_ms_per_step']:.2f} ms graph ({batch['evaluations_per_s']:.4g} "
+            "element evaluations/s)")
     out = {
         "metric": "C3D20 K-assembly elements/s + % FP64/HBM roofline; adjoint/forward cost",
-        "value": value, "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "value": head["value"], "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.n}^3 {args.family} jittered block: CSR K values"
-                               f"{' + internal force' if residual else ''}",
-                   "elements": model.n_elem, "nodes": model.n_nodes,
-                   "dof": 3 * model.n_nodes, "nnz": asm.pattern.nnz,
-                   "colours": asm.mesh.n_colors, "asm_variant": choice["asm_variant"],
-                   "patch_size": choice["patch_size"]},
-        "autotune": choice["trials"],
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": kernels,
+        "config": head["config"],
+        "autotune": head["choice"]["trials"],
+        "roofline": head["roofline"],
+        "cpu_baseline": main_cpu,
+        "kernels": head["kernels"],
+        "second_family": second,
         "adjoint": adjoint,
         "j2_newton": j2,
         "recover": rec,
         "mass_geometric": mass,
         "pcg": pcg,
         "batch": batch,
-        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        "gpu_over_cpu": (head["value"] / main_cpu["value"]) if main_cpu and main_cpu.get("value")
+        else None,
+        "kernel_sources_sha16": kernel_sources_sha(),
     }
     print(json.dumps(out), flush=True)
     return 0
