@@ -26,18 +26,21 @@ static hipEvent_t take_event(cxg_context* ctx) {
 }
 
 LaunchTimer::LaunchTimer(cxg_context* c, std::string name, int64_t elements)
-    : ctx(c), on(c->timing) {
+    : LaunchTimer(c, std::move(name), elements, c->stream) {}
+
+LaunchTimer::LaunchTimer(cxg_context* c, std::string name, int64_t elements, hipStream_t s)
+    : ctx(c), on(c->timing), stream(s) {
   if (!on) return;
   p.name = std::move(name);
   p.elements = elements;
   p.a = take_event(ctx);
   p.b = take_event(ctx);
-  (void)hipEventRecord(p.a, ctx->stream);
+  (void)hipEventRecord(p.a, stream);
 }
 
 LaunchTimer::~LaunchTimer() {
   if (!on) return;
-  (void)hipEventRecord(p.b, ctx->stream);
+  (void)hipEventRecord(p.b, stream);
   ctx->pending.push_back(p);
 }
 
@@ -357,7 +360,7 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->patch_size = int(value);
     return CXG_OK;
   }
-  if (k == "asm_variant" && value >= 1 && value <= 7) {
+  if (k == "asm_variant" && value >= 1 && value <= 8) {
     ctx->asm_variant = int(value);
     return CXG_OK;
   }
@@ -383,6 +386,14 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
   }
   if (k == "gather_tile" && value >= 0 && value <= 4096) {
     ctx->gather_tile = int(value);
+    return CXG_OK;
+  }
+  if (k == "stage_overlap" && (value == 0 || value == 1)) {
+    ctx->stage_overlap = int(value);
+    return CXG_OK;
+  }
+  if (k == "stage_mb" && value >= 1 && value <= 4096) {
+    ctx->stage_mb = int(value);
     return CXG_OK;
   }
   if (k == "gather_lds_cols" && value >= 8 && value <= 256) {

@@ -676,7 +676,9 @@ __host__ __device__ __forceinline__ int up_block(int a, int b, int n) {
 // (up_block(g0, g0) is even for g0 % 4 == 0, so the run starts 16-byte aligned). stage
 // holds kStageNodes * NN blocks, 16-byte aligned. Every lane of the wave calls it; it
 // ends with a barrier.
-template <int NN, int PPL>
+// NTS: streaming (non-temporal) stores, for an arena read back only after the whole
+// element pass; false: cached stores, for a ring read back while still cache-resident.
+template <int NN, int PPL, bool NTS = true>
 __device__ __forceinline__ void store_upper_blocks(double* __restrict__ stage, const int* pa,
                                                    const int* pb, const double (*K)[9],
                                                    double* __restrict__ kel, const int lane) {
@@ -697,8 +699,13 @@ __device__ __forceinline__ void store_upper_blocks(double* __restrict__ stage, c
     const int cnt = 9 * (k1 - k0);
     f64x2* dst = reinterpret_cast<f64x2*>(kel + 9 * k0);
     const f64x2* src = reinterpret_cast<const f64x2*>(stage);
-    for (int idx = lane; idx < cnt / 2; idx += kWave) __builtin_nontemporal_store(src[idx], dst + idx);
-    if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(stage[cnt - 1], kel + 9 * k0 + cnt - 1);
+    if constexpr (NTS) {
+      for (int idx = lane; idx < cnt / 2; idx += kWave) __builtin_nontemporal_store(src[idx], dst + idx);
+      if ((cnt & 1) && lane == 0) __builtin_nontemporal_store(stage[cnt - 1], kel + 9 * k0 + cnt - 1);
+    } else {
+      for (int idx = lane; idx < cnt / 2; idx += kWave) dst[idx] = src[idx];
+      if ((cnt & 1) && lane == 0) kel[9 * k0 + cnt - 1] = stage[cnt - 1];
+    }
     __syncthreads();
   }
 }
@@ -713,7 +720,7 @@ __device__ __forceinline__ void store_upper_blocks(double* __restrict__ stage, c
 // upper pair of node couples s <= t (of the lower pair (2s+1, 2s) of a diagonal couple
 // nothing is stored), and loads the two couples' gradients with 16-byte LDS reads: 6
 // reads of 96 bytes per Gauss point against 15 of 120 bytes for TL = 0.
-template <int NN, int NGP, int QC, bool UP = false, int TL = 0>
+template <int NN, int NGP, int QC, bool UP = false, int TL = 0, bool NTS = true>
 __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int lane,
                                               const int32_t* __restrict__ ce, const double Ee,
                                               const double nue, const double* __restrict__ dN,
@@ -850,7 +857,7 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
     for (int i = 0; i < 9; ++i) m[i] = K[i];
   }
   if constexpr (UP) {
-    store_upper_blocks<NN, PPL>(L.stage, pa, pb, M, kel, lane);
+    store_upper_blocks<NN, PPL, NTS>(L.stage, pa, pb, M, kel, lane);
     return;
   }
   // Staged store: rows of kStageNodes nodes at a time. Block (a, b) goes to the rows of
@@ -891,7 +898,7 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
 // With ue given, the element's nodal displacements of n_cases fields (u + c * n_dof) are
 // copied to ue + c * ue_stride + 3 * (connectivity offset), where the row gather reads
 // them by its incidence records without a connectivity load.
-template <int NN, int NGP, int QC, int WAVES, bool UP = false, int TL = 0>
+template <int NN, int NGP, int QC, int WAVES, bool UP = false, int TL = 0, bool NTS = true>
 __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
     const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ coords,
@@ -908,7 +915,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
 #pragma unroll
       for (int j = 0; j < 3; ++j) ue[c * ue_stride + 3 * k + j] = u[c * n_dof + 3 * nd + j];
   }
-  elem_iso_body<NN, NGP, QC, UP, TL>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
+  elem_iso_body<NN, NGP, QC, UP, TL, NTS>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
 }
 
@@ -1065,11 +1072,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // transposed, and the full matrix of the others.
 template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false>
 __device__ __forceinline__ void row_gather_node(
-    int32_t a, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
-    const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
-    const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
-    const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
-    const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
+    const GatherDesc& d, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
+    const IncRecord* __restrict__ rec, const int32_t* __restrict__ conn,
+    const int32_t* __restrict__ slot_all, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
     const double* __restrict__ ue, int64_t ue_stride, int up_min) {
@@ -1078,14 +1083,13 @@ __device__ __forceinline__ void row_gather_node(
   const int h = HALVES == 2 ? lane / 32 : 0;
   const int sub = HALVES == 2 ? lane % 32 : lane;
   const int bl = sub / 3, jl = sub - 3 * (sub / 3);
+  const int32_t a = d.node;
+  // the node's free rows are consecutive equations sharing one column layout
   int64_t rs[3];
-  int64_t Lrow = 0;
+  const int64_t Lrow = d.len;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const int32_t eq = dof_eq[3 * int64_t(a) + i];
-    rs[i] = eq >= 0 ? row_ptr[eq] : -1;
-    if (eq >= 0) Lrow = row_ptr[eq + 1] - row_ptr[eq];
-  }
+  for (int i = 0; i < 3; ++i)
+    rs[i] = (d.flags >> i) & 1 ? d.row0 + int64_t(__popc(d.flags & ((1 << i) - 1))) * Lrow : -1;
   const bool in_lds = Lrow <= cap;
   for (int i = 0; i < 3; ++i) {
     if (rs[i] < 0) continue;
@@ -1107,7 +1111,7 @@ __device__ __forceinline__ void row_gather_node(
   // element force vectors (fe mode): lanes sub < 3 of each half sum their elements'
   // component sub in ascending element order; the halves are added at the end
   double fsum = 0.0;
-  const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
+  const int64_t i0 = d.inc0, i1 = d.inc0 + d.inc_n;
   for (int64_t ib = i0; ib < i1; ib += kBatch * halves) {
     // issue every load of the batch first ...
     double v[kBatch][3], uv[kBatch][NU], fv[kBatch];
@@ -1267,13 +1271,11 @@ __device__ __forceinline__ void row_gather_node(
 // so that the two nodes reading one block do so close together in time).
 template <int NC, bool LIVE, bool SCALAR = false, bool UP = false>
 __global__ __launch_bounds__(kWave) void k_row_gather(
-    const int64_t* __restrict__ inc_ptr, const IncRecord* __restrict__ rec,
+    const GatherDesc* __restrict__ desc, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
-    const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
-    const int64_t* __restrict__ row_ptr, const double* __restrict__ kp,
-    const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
-    double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    int32_t n_nodes, int32_t slab, int32_t cap, const int32_t* __restrict__ order,
+    const double* __restrict__ kp, const double* __restrict__ u, int64_t n_dof,
+    const double* __restrict__ fe, double* __restrict__ vals, double* __restrict__ f_int,
+    uint8_t* __restrict__ live, int32_t n_nodes, int32_t slab, int32_t cap,
     const double* __restrict__ ue, int64_t ue_stride, int32_t up_min) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
   // rows accumulate in vals) and, for LIVE, their flags
@@ -1284,16 +1286,43 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // data again
   const int64_t ai = xcd_index(blockIdx.x, n_nodes, slab);
   if (ai < 0) return;
-  const int32_t a = order ? __builtin_amdgcn_readfirstlane(order[ai]) : int32_t(ai);
-  // nfm bit 3: every incident element has at most 10 nodes (wave-uniform)
-  if (__builtin_amdgcn_readfirstlane(int(nfm[a])) & 8)
-    row_gather_node<NC, LIVE, 2, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
-                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live,
-                                             ue, ue_stride, up_min);
+  // the position's descriptor (wave-uniform: scalar loads); flags bit 3: every incident
+  // element has at most 10 nodes
+  const GatherDesc d = desc[ai];
+  if (d.flags & 8)
+    row_gather_node<NC, LIVE, 2, SCALAR, UP>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u, n_dof,
+                                             fe, vals, f_int, live, ue, ue_stride, up_min);
   else
-    row_gather_node<NC, LIVE, 1, SCALAR, UP>(a, cap, buf, lbuf, inc_ptr, rec, conn, slot_all, nfm,
-                                             dof_eq, row_ptr, kp, u, n_dof, fe, vals, f_int, live,
-                                             ue, ue_stride, up_min);
+    row_gather_node<NC, LIVE, 1, SCALAR, UP>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u, n_dof,
+                                             fe, vals, f_int, live, ue, ue_stride, up_min);
+}
+
+// One thread per gather position: the node's descriptor (see GatherDesc). order: node
+// per position (null: index order).
+__global__ void k_gather_desc(int32_t n_nodes, const int32_t* __restrict__ order,
+                              const int8_t* __restrict__ nfm, const int32_t* __restrict__ dof_eq,
+                              const int64_t* __restrict__ row_ptr,
+                              const int64_t* __restrict__ inc_ptr, GatherDesc* __restrict__ desc) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < n_nodes; p += stride) {
+    const int32_t a = order ? order[p] : int32_t(p);
+    GatherDesc d;
+    d.node = a;
+    d.flags = 0;
+    d.row0 = 0;
+    d.len = 0;
+    for (int i = 2; i >= 0; --i) {
+      const int32_t eq = dof_eq[3 * int64_t(a) + i];
+      if (eq < 0) continue;
+      d.flags |= 1 << i;
+      d.row0 = row_ptr[eq];
+      d.len = int32_t(row_ptr[eq + 1] - row_ptr[eq]);
+    }
+    if (nfm[a] & 8) d.flags |= 8;
+    d.inc0 = inc_ptr[a];
+    d.inc_n = int32_t(inc_ptr[a + 1] - inc_ptr[a]);
+    desc[p] = d;
+  }
 }
 
 // ---- K2: J2 radial return + consistent tangent + internal force --------------
@@ -2485,24 +2514,25 @@ unsigned xcd_grid(const cxg_context* ctx, int64_t n) {
 
 // Phase B over every node of the pattern's mesh (see k_row_gather).
 // up: the arena holds upper blocks (asm_variant 7), nodes in the pattern's locality order.
-void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
-                   double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false,
-                   bool up = false) {
+// The row gather over n_pos gather positions (descriptors desc[0..n_pos)) on `stream`,
+// reading element matrices at the records' offsets into kp.
+void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream,
+                      const GatherDesc* desc, int32_t n_pos, const IncRecord* rec,
+                      const double* kp, const double* u, const double* fe, double* vals,
+                      double* f_int, uint8_t* live, int n_cases, bool scalar, bool up) {
+  if (n_pos <= 0) return;
   const cxg_mesh* m = p->mesh;
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
-  const int32_t slab = int32_t(xcd_slab(ctx, m->n_nodes));
-  const unsigned grid = xcd_grid(ctx, m->n_nodes);
+  const int32_t slab = int32_t(xcd_slab(ctx, n_pos));
+  const unsigned grid = xcd_grid(ctx, n_pos);
   const int32_t cap = int32_t(std::min<int64_t>((std::max<int64_t>(p->max_row, 1) + 7) / 8 * 8,
                                                 std::min(kRowCap, ctx->gather_lds_cols)));
   const size_t lds = size_t(3 * cap) * (sizeof(double) + (live ? 1 : 0));
 #define GATHER(NC, LV, ...)                                                                     \
-  hipLaunchKernelGGL((k_row_gather<NC, LV, ##__VA_ARGS__>), dim3(grid), dim3(kWave), lds,        \
-                     ctx->stream,                                                               \
-                     m->inc_ptr.p, p->inc_rec.p, m->conn.p, p->slot_all.p, p->nfm.p,            \
-                     p->dof_eq.p, p->row_ptr.p, p->ke_arena.p, u, n_dof, fe, vals, f_int,       \
-                     live, m->n_nodes, slab, cap, up ? p->gorder.p : nullptr,                  \
-                     up ? p->ue_arena.p : nullptr, int64_t(3) * int64_t(m->conn.n),            \
-                     int32_t(ctx->up_min_nodes))
+  hipLaunchKernelGGL((k_row_gather<NC, LV, ##__VA_ARGS__>), dim3(grid), dim3(kWave), lds, stream, \
+                     desc, rec, m->conn.p, p->slot_all.p, kp, u, n_dof, fe, vals, f_int, live,  \
+                     n_pos, slab, cap, up ? p->ue_arena.p : nullptr,                            \
+                     int64_t(3) * int64_t(m->conn.n), int32_t(ctx->up_min_nodes))
   if (up) {
     if (scalar) throw Error(CXG_ERR_UNSUPPORTED, "upper-block arena: stiffness gathers only");
     if (fe) {
@@ -2526,6 +2556,14 @@ void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, cons
     if (live) GATHER(1, true); else GATHER(1, false);
   }
 #undef GATHER
+}
+
+void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
+                   double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false,
+                   bool up = false) {
+  prepare_desc(ctx, p);
+  launch_gather_on(ctx, p, ctx->stream, p->gdesc.p, p->mesh->n_nodes, p->inc_rec.p,
+                   p->ke_arena.p, u, fe, vals, f_int, live, n_cases, scalar, up);
 }
 
 // Resident workgroups of a persistent launch: waves per CU times CUs, at most one per
@@ -2591,12 +2629,96 @@ void fill_f64(hipStream_t s, double* p, int64_t n, double v) {
   CXG_HIP(hipGetLastError());
 }
 
+// asm_variant 8 (see StagePlan): per stage s, on the context's stream A the element
+// matrices of the stage's elements into ring slot s % R (cached stores), then on the
+// plan's stream B the rows of the nodes that stage completes. A stage's elements wait
+// for the gathers that still read the slot they overwrite (those of stages up to
+// s - R + W - 1); each gather waits for its stage's elements; A finally waits for B.
+void launch_staged_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                          const double* E, const double* nu, double* vals, const double* u,
+                          double* f_int, uint8_t* live, int n_cases) {
+  const cxg_mesh* m = p->mesh;
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  StagePlan& P = *pm->stages;
+  double* ue = nullptr;
+  if (u && f_int) {
+    prepare_displacement_arena(p, n_cases);
+    ue = pm->ue_arena.p;
+  }
+  const int64_t n_dof = 3 * int64_t(m->n_nodes);
+  const int64_t ue_stride = int64_t(3) * int64_t(m->conn.n);
+  const bool two = ctx->stage_overlap != 0;
+  hipStream_t A = ctx->stream, B = two ? P.stream : ctx->stream;
+  LaunchTimer total(ctx, "staged_assembly", m->n_elem);
+  for (int s = 0; s < P.S; ++s) {
+    if (two && s >= P.R) CXG_HIP(hipStreamWaitEvent(A, P.ev_gather[size_t(s - P.R + P.W - 1)], 0));
+    const int64_t e0 = P.elem_first[size_t(s)], e1 = P.elem_first[size_t(s) + 1];
+    double* slot = P.ring.p + int64_t(s % P.R) * P.slot_cap;
+    for (size_t bi = 0; bi < m->blocks.size(); ++bi) {
+      const cxg_block_dev& Bk = m->blocks[bi];
+      const int64_t lo = std::max(e0, Bk.elem_offset), hi = std::min(e1, Bk.elem_offset + Bk.n_elem);
+      if (lo >= hi) continue;
+      const int64_t first = lo - Bk.elem_offset, cnt = hi - lo;
+      const ElemTable& T = ctx->tables[Bk.type];
+      const int32_t* conn = m->conn.p + Bk.conn_offset + first * Bk.nn;
+      double* kp = slot + (P.kp_host[size_t(lo)] - P.kp_host[size_t(e0)]);
+      double* ueb = ue ? ue + 3 * (Bk.conn_offset + first * Bk.nn) : nullptr;
+      LaunchTimer timer(ctx, std::string("elem_iso_staged_") + type_name(Bk.type), cnt, A);
+#define ISO(NN, NGP, QC, TLV)                                                                   \
+  hipLaunchKernelGGL((k_elem_iso<NN, NGP, QC, 4, true, TLV, false>), dim3(xcd_grid(ctx, cnt)), \
+                     dim3(kWave), 0, A, conn, lo, T.dN, T.w, coords, E, nu, kp, ctx->flag_dev, \
+                     cnt, xcd_slab(ctx, cnt), u, n_dof, n_cases, ueb, ue_stride)
+#define LAUNCH(NN, NGP, UPPER, QC)                                                              \
+  if constexpr (NN == 20) {                                                                     \
+    if (ctx->elem_tiling == 1) ISO(NN, NGP, QC, 1); else ISO(NN, NGP, QC, 0);                   \
+  } else {                                                                                      \
+    ISO(NN, NGP, QC, 0);                                                                        \
+  }
+      CXG_DISPATCH(Bk.type, LAUNCH)
+#undef LAUNCH
+#undef ISO
+      CXG_HIP(hipGetLastError());
+    }
+    if (two) {
+      CXG_HIP(hipEventRecord(P.ev_elem[size_t(s)], A));
+      CXG_HIP(hipStreamWaitEvent(B, P.ev_elem[size_t(s)], 0));
+    }
+    const int64_t n0 = P.node_off[size_t(s)], n1 = P.node_off[size_t(s) + 1];
+    if (n1 > n0) {
+      LaunchTimer timer(ctx, "row_gather_staged", n1 - n0, B);
+      launch_gather_on(ctx, p, B, P.desc.p + n0, int32_t(n1 - n0), P.rec.p, P.ring.p,
+                       f_int ? u : nullptr, nullptr, vals, f_int, live, n_cases, false, true);
+      CXG_HIP(hipGetLastError());
+    }
+    if (two) CXG_HIP(hipEventRecord(P.ev_gather[size_t(s)], B));
+  }
+  if (two) CXG_HIP(hipStreamWaitEvent(A, P.ev_gather[size_t(P.S - 1)], 0));
+}
+
+// Whether this assembly takes the staged path: asm_variant 8, every family in the
+// upper-block layout, not under capture (graphs replay the one-pass kernels), and a
+// stage plan that pays for the mesh.
+bool use_stages(cxg_context* ctx, const cxg_pattern* p, const double* coords) {
+  if (ctx->asm_variant != 8) return false;
+  for (const cxg_block_dev& B : p->mesh->blocks)
+    if (B.n_elem > 0 && B.nn < ctx->up_min_nodes) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CXG_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return false;
+  prepare_order(ctx, p, coords);
+  return prepare_stages(ctx, p);
+}
+
 void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             const double* E, const double* nu, double* vals, const double* u,
                             double* f_int, uint8_t* live, int n_cases) {
   const cxg_mesh* m = p->mesh;
   if (n_cases != 1 && (ctx->asm_variant < 4 || (n_cases != 2 && n_cases != 4)))
     throw Error(CXG_ERR_UNSUPPORTED, "load cases: 1, or 2 or 4 with the two-phase assembly");
+  if (m->n_nodes > 0 && use_stages(ctx, p, coords)) {
+    launch_staged_linear(ctx, p, coords, E, nu, vals, u, f_int, live, n_cases);
+    return;
+  }
   if (ctx->asm_variant >= 4) {
     // Two-phase: element pair blocks to the arena, then one owner wave per node row.
     cxg_pattern* pm = const_cast<cxg_pattern*>(p);
@@ -2625,7 +2747,7 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 #undef LAUNCH
       CXG_HIP(hipGetLastError());
     }
-    const bool up = ctx->asm_variant == 7 && uses_upper_blocks(ctx, m);
+    const bool up = ctx->asm_variant >= 7 && uses_upper_blocks(ctx, m);
     if (up) prepare_order(ctx, p, coords);
     // variant 7 with f_int: the element kernels copy u per element for the gather
     double* ue = nullptr;
@@ -2635,7 +2757,7 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
     }
     const int64_t n_dof = 3 * int64_t(m->n_nodes);
     const int64_t ue_stride = int64_t(3) * int64_t(m->conn.n);
-    for (size_t bi = 0; (ctx->asm_variant == 4 || ctx->asm_variant == 7) && bi < m->blocks.size(); ++bi) {
+    for (size_t bi = 0; (ctx->asm_variant == 4 || ctx->asm_variant >= 7) && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];
       if (B.n_elem == 0) continue;
       const ElemTable& T = ctx->tables[B.type];
@@ -2942,6 +3064,138 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
   pm->gorder_tile = tile;
 }
 
+void prepare_desc(cxg_context* ctx, const cxg_pattern* p) {
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  const bool ordered = p->gorder.p && p->gorder_tile > 0;
+  const int key = ordered ? p->gorder_tile : -1;
+  if (pm->gdesc_key == key) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CXG_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    throw Error(CXG_ERR_STATE, "the gather descriptors are built before capture (assemble once first)");
+  guard_realloc(p, pm->gdesc.p != nullptr, "gather descriptors (gather order changed)");
+  const int32_t n = p->mesh->n_nodes;
+  pm->gdesc.alloc(size_t(std::max(n, 1)));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_gather_desc, dim3(grid_for(n)), dim3(256), 0, ctx->stream, n,
+                       ordered ? p->gorder.p : nullptr, p->nfm.p, p->dof_eq.p, p->row_ptr.p,
+                       p->mesh->inc_ptr.p, pm->gdesc.p);
+    CXG_HIP(hipGetLastError());
+  }
+  pm->gdesc_key = key;
+}
+
+bool prepare_stages(cxg_context* ctx, const cxg_pattern* p) {
+  // A ring beyond this is no longer cache-resident: the one-pass kernels run instead.
+  constexpr int64_t kRingBytesMax = int64_t(1) << 30;
+  constexpr int kMaxSpan = 8;
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  const int tile = p->gorder.p ? p->gorder_tile : -1;
+  if (pm->stages_tile == tile && pm->stages_mb == ctx->stage_mb)
+    return !pm->stages_off && pm->stages != nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CXG_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    throw Error(CXG_ERR_STATE, "the stage plan is built before capture (assemble once first)");
+  guard_realloc(p, pm->stages != nullptr, "stage plan (gather order or stage_mb changed)");
+  pm->stages.reset();
+  pm->stages_off = true;
+  pm->stages_tile = tile;
+  pm->stages_mb = ctx->stage_mb;
+  const cxg_mesh* m = p->mesh;
+  const int64_t ne = m->n_elem;
+  const int32_t nn = m->n_nodes;
+  const int64_t n_inc = int64_t(m->inc_elem.n);
+  if (ne < 2 || nn < 1) return false;
+  auto P = std::make_unique<StagePlan>();
+  // host copies of the element slots, the incidence and the gather order
+  CXG_HIP(hipStreamSynchronize(ctx->stream));
+  P->kp_host.resize(size_t(ne) + 1);
+  std::vector<int64_t> iptr(size_t(nn) + 1);
+  std::vector<int32_t> ielem(static_cast<size_t>(n_inc));
+  std::vector<IncRecord> rec(static_cast<size_t>(n_inc));
+  std::vector<int32_t> order(static_cast<size_t>(nn));
+  CXG_HIP(hipMemcpy(P->kp_host.data(), p->kpptr.p, P->kp_host.size() * sizeof(int64_t),
+                    hipMemcpyDeviceToHost));
+  CXG_HIP(hipMemcpy(iptr.data(), m->inc_ptr.p, iptr.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (n_inc > 0) {
+    CXG_HIP(hipMemcpy(ielem.data(), m->inc_elem.p, ielem.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    CXG_HIP(hipMemcpy(rec.data(), p->inc_rec.p, rec.size() * sizeof(IncRecord), hipMemcpyDeviceToHost));
+  }
+  if (tile > 0)
+    CXG_HIP(hipMemcpy(order.data(), p->gorder.p, order.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  else
+    for (int32_t i = 0; i < nn; ++i) order[size_t(i)] = i;
+  const std::vector<int64_t>& kp = P->kp_host;
+  // stages of consecutive elements, each at most stage_mb of element matrices
+  const int64_t target = std::max<int64_t>(1, int64_t(ctx->stage_mb) * (int64_t(1) << 20) / 8);
+  std::vector<int32_t> estage(static_cast<size_t>(ne));
+  P->elem_first.push_back(0);
+  for (int64_t e = 0; e < ne; ++e) {
+    if (e > P->elem_first.back() && kp[size_t(e) + 1] - kp[size_t(P->elem_first.back())] > target)
+      P->elem_first.push_back(e);
+    estage[size_t(e)] = int32_t(P->elem_first.size()) - 1;
+  }
+  P->elem_first.push_back(ne);
+  P->S = int(P->elem_first.size()) - 1;
+  for (int s = 0; s < P->S; ++s)
+    P->slot_cap = std::max(P->slot_cap, kp[size_t(P->elem_first[size_t(s) + 1])] -
+                                            kp[size_t(P->elem_first[size_t(s)])]);
+  // completion stage of every node (its last incident element's; nodes of no element go
+  // with the first stage) and the widest span of stages one node reads
+  std::vector<int32_t> cstage(static_cast<size_t>(nn), 0);
+  P->W = 1;
+  for (int32_t a = 0; a < nn; ++a) {
+    if (iptr[size_t(a)] == iptr[size_t(a) + 1]) continue;
+    const int32_t f = estage[size_t(ielem[size_t(iptr[size_t(a)])])];
+    const int32_t c = estage[size_t(ielem[size_t(iptr[size_t(a) + 1] - 1)])];
+    cstage[size_t(a)] = c;
+    P->W = std::max(P->W, c - f + 1);
+  }
+  P->R = std::min(P->S, P->W + 1);
+  if (P->S < 2 || P->W > kMaxSpan || int64_t(P->R) * P->slot_cap * 8 > kRingBytesMax) return false;
+  // gather positions grouped by completion stage, the locality order kept within a stage
+  P->node_off.assign(size_t(P->S) + 1, 0);
+  for (int32_t a = 0; a < nn; ++a) ++P->node_off[size_t(cstage[size_t(a)]) + 1];
+  for (int s = 0; s < P->S; ++s) P->node_off[size_t(s) + 1] += P->node_off[size_t(s)];
+  std::vector<int32_t> sorder(static_cast<size_t>(nn));
+  {
+    std::vector<int64_t> fill(P->node_off.begin(), P->node_off.end() - 1);
+    for (int32_t i = 0; i < nn; ++i) {
+      const int32_t a = order[size_t(i)];
+      sorder[size_t(fill[size_t(cstage[size_t(a)])]++)] = a;
+    }
+  }
+  // incidence records addressing the ring: element e of stage s sits at slot s % R,
+  // kp[e] - kp[first element of s] into it
+  for (int64_t i = 0; i < n_inc; ++i) {
+    const int32_t e = ielem[size_t(i)];
+    const int32_t st = estage[size_t(e)];
+    rec[size_t(i)].rows += int64_t(st % P->R) * P->slot_cap - kp[size_t(P->elem_first[size_t(st)])];
+  }
+  P->rec.alloc(size_t(std::max<int64_t>(n_inc, 1)));
+  if (n_inc > 0)
+    CXG_HIP(hipMemcpy(P->rec.p, rec.data(), rec.size() * sizeof(IncRecord), hipMemcpyHostToDevice));
+  DevBuf<int32_t> sord(static_cast<size_t>(nn));
+  CXG_HIP(hipMemcpy(sord.p, sorder.data(), sorder.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  P->desc.alloc(static_cast<size_t>(nn));
+  hipLaunchKernelGGL(k_gather_desc, dim3(grid_for(nn)), dim3(256), 0, ctx->stream, nn, sord.p,
+                     p->nfm.p, p->dof_eq.p, p->row_ptr.p, m->inc_ptr.p, P->desc.p);
+  CXG_HIP(hipGetLastError());
+  CXG_HIP(hipStreamSynchronize(ctx->stream));  // sord is released on return
+  P->ring.alloc(size_t(int64_t(P->R) * P->slot_cap));
+  CXG_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+  P->ev_elem.assign(size_t(P->S), nullptr);
+  P->ev_gather.assign(size_t(P->S), nullptr);
+  for (int s = 0; s < P->S; ++s) {
+    CXG_HIP(hipEventCreateWithFlags(&P->ev_elem[size_t(s)], hipEventDisableTiming));
+    CXG_HIP(hipEventCreateWithFlags(&P->ev_gather[size_t(s)], hipEventDisableTiming));
+  }
+  pm->stages = std::move(P);
+  pm->stages_off = false;
+  return true;
+}
+
 void prepare_work(cxg_context* ctx, const cxg_pattern* p) {
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
   if (pm->work_chunk == ctx->chunk_elems && pm->queue.p) return;
@@ -3003,7 +3257,7 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
                          pm->ke_arena.p, pm->fe_arena.p, ctx->flag_dev);
       CXG_HIP(hipGetLastError());
     }
-    const bool up = ctx->asm_variant == 7 && uses_upper_blocks(ctx, m);
+    const bool up = ctx->asm_variant >= 7 && uses_upper_blocks(ctx, m);
     if (up) prepare_order(ctx, p, coords);
     for (size_t bi = 0; ctx->asm_variant != 5 && bi < m->blocks.size(); ++bi) {
       const cxg_block_dev& B = m->blocks[bi];

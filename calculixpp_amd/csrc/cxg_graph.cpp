@@ -10,7 +10,8 @@
 // the committed history, after a commit) in place.
 //
 // Capture rules kept here: everything that allocates, uploads or synchronizes (the
-// two-phase arenas, the J2 material table) happens before capture begins; the graph
+// two-phase arenas, the gather order and descriptors, the J2 material table) happens
+// before capture begins; the graph
 // holds kernel nodes only (buffers are cleared by fill kernels, never by memset
 // nodes); capture runs on a private non-blocking stream of default priority; the
 // device status word is read outside the graph, by cxg_check.
@@ -105,10 +106,11 @@ cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, cons
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
+    if (ctx->asm_variant >= 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
       cxg::prepare_order(ctx, p, coords);
       if (with_f) cxg::prepare_displacement_arena(p, 1);
     }
+    if (ctx->asm_variant >= 4) cxg::prepare_desc(ctx, p);
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, with_f ? u : nullptr,
                                   with_f ? f_int : nullptr, nullptr);
@@ -133,10 +135,11 @@ cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
+    if (ctx->asm_variant >= 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
       cxg::prepare_order(ctx, p, coords);
       cxg::prepare_displacement_arena(p, int(n_cases));
     }
+    if (ctx->asm_variant >= 4) cxg::prepare_desc(ctx, p);
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_linear(ctx, p, coords, E, nu, vals, u, f_int, nullptr, int(n_cases));
     });
@@ -168,8 +171,9 @@ cxg_status cxg_graph_capture_j2(cxg_context* ctx, const cxg_pattern* p, const do
     }
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, true);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant == 7 && cxg::uses_upper_blocks(ctx, p->mesh))
+    if (ctx->asm_variant >= 7 && cxg::uses_upper_blocks(ctx, p->mesh))
       cxg::prepare_order(ctx, p, coords);
+    if (ctx->asm_variant >= 4) cxg::prepare_desc(ctx, p);
     const int32_t* mid = n_mats > 0 ? mat_id : nullptr;
     cxg::capture(ctx, g.get(), [&] {
       cxg::launch_assemble_j2(ctx, p, coords, E, nu, mid, g->j2_table.p, u, committed, trial,

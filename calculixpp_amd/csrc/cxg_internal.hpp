@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <unordered_set>
@@ -135,6 +136,11 @@ struct cxg_context {
   // C3D20/C3D20R element kernels (asm_variant 4, 7): lane -> node-pair tiling (0: a node
   // and four consecutive partners, 1: 2x2 pairs of two node couples)
   int elem_tiling = 1;
+  // asm_variant 8: target size of one ring slot of the staged assembly, in MiB
+  int stage_mb = 64;
+  // asm_variant 8: 1 = gathers on a second stream, overlapping later stages' element
+  // kernels; 0 = every launch in order on the context's stream
+  int stage_overlap = 1;
   std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
   double* scratch = nullptr;  // device address of the fixed scratch array on `device`
 };
@@ -146,7 +152,10 @@ struct LaunchTimer {
   cxg_context* ctx;
   cxg_context::Pending p{};
   bool on;
+  hipStream_t stream;
   LaunchTimer(cxg_context* c, std::string name, int64_t elements);
+  // the same on another stream of the context's device
+  LaunchTimer(cxg_context* c, std::string name, int64_t elements, hipStream_t s);
   ~LaunchTimer();
 };
 void collect_timing(cxg_context* ctx);  // after a stream synchronization
@@ -215,6 +224,44 @@ struct IncRecord {
   int32_t n;      // nodes of the element
   int32_t la;     // local index of the node inside the element
 };
+// What the row gather needs of one node, per gather position (built once per pattern
+// and gather order by k_gather_desc), so that a wave starts from one scalar load
+// instead of a chain order -> dof_eq -> row_ptr / inc_ptr.
+struct GatherDesc {
+  int64_t row0;   // CSR offset of the node's first free row (its rows are consecutive
+                  // equations of len columns each); unused when no DOF is free
+  int64_t inc0;   // first incidence record of the node
+  int32_t len;    // columns per row
+  int32_t node;   // node index
+  int32_t inc_n;  // incident elements
+  int32_t flags;  // bits 0..2: DOF c free; bit 3: every incident element has <= 10 nodes
+};
+// Staged two-phase assembly (asm_variant 8). The elements are cut into S stages of
+// consecutive element numbers; a node is complete once the stage of its last incident
+// element has run, and its rows are gathered right after that stage, on a second stream,
+// while the next stages' element matrices are formed. The element matrices live in a
+// ring of R slots (R = W + 1, W the most stages one node's incident elements span), so
+// the arena that a gather reads was written moments before and is still cache-resident.
+struct StagePlan {
+  int S = 0, R = 0, W = 0;
+  int64_t slot_cap = 0;             // doubles per ring slot
+  std::vector<int64_t> elem_first;  // S+1 element boundaries (model element numbers)
+  std::vector<int64_t> kp_host;     // the pattern's kpptr (n_elem+1), on the host
+  std::vector<int64_t> node_off;    // S+1 offsets into desc: the nodes completed by stage s
+  DevBuf<IncRecord> rec;            // incidence records addressing the ring
+  DevBuf<GatherDesc> desc;          // gather positions grouped by completion stage
+  DevBuf<double> ring;              // R * slot_cap
+  hipStream_t stream = nullptr;     // the gathers' stream
+  std::vector<hipEvent_t> ev_elem, ev_gather;
+  StagePlan() = default;
+  StagePlan(const StagePlan&) = delete;
+  StagePlan& operator=(const StagePlan&) = delete;
+  ~StagePlan() {
+    for (hipEvent_t e : ev_elem) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_gather) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
 }  // namespace cxg
 
 struct cxg_pattern {
@@ -258,6 +305,16 @@ struct cxg_pattern {
   // built; empty for tile 0, node index order).
   cxg::DevBuf<int32_t> gorder;
   int gorder_tile = -1;
+  // Gather descriptors in the gather order: gorder when built with a tile > 0, else
+  // node index order (gdesc_key: that tile, -1 for index order, -2 not built).
+  cxg::DevBuf<cxg::GatherDesc> gdesc;
+  int gdesc_key = -2;
+  // asm_variant 8: the stage plan (null: not built), built for the gather order of
+  // tile stages_tile and a ring slot of stages_mb MiB; stages_off: staging does not pay
+  // for this mesh (the ring would exceed its budget), variant 7 runs instead
+  std::unique_ptr<cxg::StagePlan> stages;
+  int stages_tile = -2, stages_mb = 0;
+  bool stages_off = false;
   // Captured graphs (cxg_graph_capture_*) holding the addresses of the buffers above;
   // while any is alive those buffers are never freed or moved (cxg::guard_realloc).
   mutable int live_graphs = 0;
@@ -300,6 +357,12 @@ void prepare_displacement_arena(const cxg_pattern* p, int n_cases);
 // The upper-block gather's node order from the node coordinates (asm_variant 7; a
 // no-op when built for the context's gather_tile). Synchronizes the stream.
 void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords);
+// The gather descriptors for the pattern's current gather order (a no-op when built for
+// it); enqueues one kernel. Before capture, as it allocates.
+void prepare_desc(cxg_context* ctx, const cxg_pattern* p);
+// asm_variant 8: the stage plan for the current gather order and stage_mb (a no-op when
+// built for them); false when staging does not pay for this mesh. Synchronizes.
+bool prepare_stages(cxg_context* ctx, const cxg_pattern* p);
 void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                         const double* E, const double* nu, const int32_t* mat_id,
                         const void* j2_table, const double* u, const double* committed,

@@ -407,3 +407,67 @@ def test_j2_upper_arena_equals_full_arena_bitwise(gpu, fam_i, up_min):
         ctx.set_option("up_min_nodes", 0)
     for x, y in zip(out[4], out[7]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("stage_mb", [1, 3])
+@pytest.mark.parametrize("mesh", ["mixed", "c3d20", "c3d8", "c3d20r"])
+def test_staged_assembly_equals_one_pass_bitwise(gpu, port, mesh, stage_mb):
+    """asm_variant 8 forms the element matrices stage by stage into a ring of arena slots
+    and gathers each stage's completed nodes on a second stream. A node still sums its
+    incident elements in element order from the same upper blocks, so K, the live flags
+    and f_int (1, 2 and 4 load cases) equal asm_variant 7's bit for bit. A small ring slot
+    (stage_mb) cuts these meshes into several stages (checked by the staged timer entry)."""
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.fem import _ptr
+    if mesh == "mixed":
+        m = _spc_model(*meshgen.mixed_box(900, 61), 62)
+    else:
+        fam = {"c3d20": _lib.C3D20, "c3d8": _lib.C3D8, "c3d20r": _lib.C3D20R}[mesh]
+        n = 10 if fam == _lib.C3D8 else 8
+        b = meshgen.block(n, fam, seed_geo=63, seed_E=64, ny=n - 1, nz=n - 2)
+        m = _spc_model(b.coords, b.blocks, 65)
+    ctx = gpu.context()
+    a = gpu.Assembler(m, ctx)
+    dev = ctx.device
+    n3 = 3 * m.n_nodes
+    u = torch.from_numpy(np.stack([meshgen.smooth_field(m.coords, 66 + c).ravel()
+                                   for c in range(4)])).to(dev)
+    nnz = a.pattern.nnz
+    out = {}
+    try:
+        ctx.set_option("stage_mb", stage_mb)
+        for variant in (7, 8, 8):  # the second staged pass reuses the ring and its events
+            ctx.set_option("asm_variant", variant)
+            for nc in (1, 2, 4):
+                v, lv = a.new_values(), a.new_live()
+                f = torch.empty(nc * n3, dtype=torch.float64, device=dev)
+                ctx.profile_reset()
+                ctx.profile(True)
+                ctx.check(ctx.lib.cxg_assemble_linear_cases(
+                    ctx.h, a.pattern.h, _ptr(a.mesh.coords), _ptr(a.mesh.E), _ptr(a.mesh.nu),
+                    _ptr(v), nc, _ptr(u), _ptr(f), _ptr(lv)))
+                ctx.profile(False)
+                stats = ctx.profile_stats()
+                if variant == 8:
+                    assert "staged_assembly" in stats, stats
+                    assert stats["row_gather_staged"][0] >= 2  # several stages gathered
+                out.setdefault((variant, nc), []).append(
+                    (v[:nnz].cpu().numpy(), lv[:nnz].cpu().numpy(), f.cpu().numpy()))
+        sys_8 = gpu.assemble_linear_static(m)  # one-shot key-set path, staged
+    finally:
+        ctx.set_option("asm_variant", 7)
+        ctx.set_option("stage_mb", 64)
+    for nc in (1, 2, 4):
+        ref = out[7, nc][0]
+        for res in out[8, nc]:
+            for x, y in zip(ref, res):
+                assert np.array_equal(x, y), (mesh, nc)
+    exp = port.assemble_linear_static(m)
+    rp, ci, vals = exp.csr()
+    assert np.array_equal(sys_8.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(sys_8.col_idx.cpu().numpy(), ci)
+    assert rel_l2(sys_8.vals.cpu().numpy(), vals) <= REL_K
+    for c in range(4):
+        f_exp = port.internal_force(m, u[c].cpu().numpy())
+        assert rel_l2(out[8, 4][0][2][c * n3:(c + 1) * n3], f_exp) <= REL_K
