@@ -723,6 +723,13 @@ def assembly_leg(family_name: str, n: int, residual: bool, steps: int, warmup: i
             per_kernel[k] = {"bound": "hbm", "algorithmic_bytes": gather_bytes,
                              "achieved": round(gather_bytes / avg / 1e9, 1), "peak": PEAK_HBM,
                              "unit": "GB/s", "frac": round(gather_bytes / avg / 1e9 / PEAK_HBM, 4)}
+        elif k.startswith("elem_hform"):
+            # gradient form: the slot (lambda, mu, 3n gradients per point) written, the
+            # connectivity and (u given) the per-element displacement copy
+            hb = model.n_elem * (8 * (2 + 3 * nn * ngp) + 4 * nn + (48 * nn if residual else 0))
+            per_kernel[k] = {"bound": "hbm", "algorithmic_bytes": hb,
+                             "achieved": round(hb / avg / 1e9, 1), "peak": PEAK_HBM,
+                             "unit": "GB/s", "frac": round(hb / avg / 1e9 / PEAK_HBM, 4)}
         else:
             per_kernel[k] = {"bound": "mfma", "algorithmic_flop": flop,
                              "achieved": round(flop / avg / 1e12, 2), "peak": PEAK_FP64,

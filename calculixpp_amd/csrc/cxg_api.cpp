@@ -388,6 +388,10 @@ cxg_status cxg_set_option(cxg_context* ctx, const char* key, int64_t value) {
     ctx->gather_tile = int(value);
     return CXG_OK;
   }
+  if (k == "gradient_form" && (value == 0 || value == 1)) {
+    ctx->gradient_form = int(value);
+    return CXG_OK;
+  }
   if (k == "stage_overlap" && (value == 0 || value == 1)) {
     ctx->stage_overlap = int(value);
     return CXG_OK;

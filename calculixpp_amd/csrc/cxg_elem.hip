@@ -653,6 +653,16 @@ __device__ __forceinline__ void jacobian_inverse(const double* x, const double* 
   rs = sqrt(jacobian_detw<NN, true>(x, d, w, A, flag));
 }
 
+// h = sqrt(det J w) J^-T dN of one node at one Gauss point (A = J^-1 row-major, d = the
+// reference gradient), with the multiply-adds written out, so that every kernel forming
+// h gets the same bits.
+__device__ __forceinline__ void scaled_gradient(const double* A, const double s, const double* d,
+                                                double h[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    h[i] = s * __builtin_fma(A[6 + i], d[2], __builtin_fma(A[3 + i], d[1], A[i] * d[0]));
+}
+
 template <int NN, int NGP, int QC>
 struct ElemLds {
   double x[NN * 3];
@@ -792,17 +802,11 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
   for (int q0 = 0; q0 < NGP; q0 += QC) {
     if (lane < QC * NN) {
       const int q = q0 + cq;
-      double h0 = 0.0, h1 = 0.0, h2 = 0.0;
-      if (q < NGP) {
-        const double* A = L.Ai + 9 * q;
-        const double s = L.rs[q];
-        h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
-        h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
-        h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
-      }
-      L.h[3 * lane + 0] = h0;
-      L.h[3 * lane + 1] = h1;
-      L.h[3 * lane + 2] = h2;
+      double hv[3] = {0.0, 0.0, 0.0};
+      if (q < NGP) scaled_gradient(L.Ai + 9 * q, L.rs[q], d, hv);
+      L.h[3 * lane + 0] = hv[0];
+      L.h[3 * lane + 1] = hv[1];
+      L.h[3 * lane + 2] = hv[2];
     }
     __syncthreads();
     load_dn(q0 + QC, d);  // in flight during this chunk's products
@@ -917,6 +921,70 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
   }
   elem_iso_body<NN, NGP, QC, UP, TL, NTS>(L, int(threadIdx.x), conn + le * NN, E[elem_offset + le],
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
+}
+
+// ---- K1, gradient form (C3D20R) -----------------------------------------------------
+// Instead of its upper node-pair blocks the element's slot holds what they are made of:
+// slot[0..1] = lambda, mu and the scaled gradients h_{k,c}(q) = sqrt(det J w)
+// (J^-T dN_k)_c at 2 + 3n q + 3k + c (point-major: at each point the 3n values of the
+// element are consecutive, as the row gather's lanes read them). The gather forms
+// K_ab = lambda M_ab + mu M_ab^T + mu tr(M_ab) I, M_ab = sum_q h_a h_b^T, with the
+// operations of k_elem_iso, so the values are the same bits; the slot is 2 + 3 n NGP
+// doubles (3.9 KB for C3D20R against 15.1 KB of upper blocks).
+// 64 / NGP elements per wave: lane (e, q) forms J^-1 and sqrt(det J w) of its point into
+// LDS, then the wave forms the gradients of one element at a time, lane t the point
+// t / n and node t % n, so that the stores of a wave are consecutive.
+template <int NN, int NGP>
+__global__ __launch_bounds__(kWave) void k_elem_hform(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
+    int* __restrict__ flag, int64_t n_elem, int64_t slab, const double* __restrict__ u,
+    int64_t n_dof, int32_t n_cases, double* __restrict__ ue, int64_t ue_stride) {
+  static_assert(kWave % NGP == 0, "one lane per point");
+  constexpr int EPW = kWave / NGP;
+  __shared__ double X[EPW][NN * 3];
+  __shared__ double Ai[EPW][NGP][9];
+  __shared__ double R[EPW][NGP];
+  const int lane = threadIdx.x;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  for (int idx = lane; idx < ne * NN; idx += kWave) {
+    const int e = idx / NN, k = idx - e * NN;
+    const int64_t ck = (le0 + e) * NN + k;
+    const int64_t nd = conn[ck];
+    for (int c = 0; c < 3; ++c) X[e][3 * k + c] = coords[3 * nd + c];
+    if (ue)
+      for (int cs = 0; cs < n_cases; ++cs)
+        for (int j = 0; j < 3; ++j) ue[cs * ue_stride + 3 * ck + j] = u[cs * n_dof + 3 * nd + j];
+  }
+  __syncthreads();
+  {
+    const int e = lane / NGP, q = lane - NGP * (lane / NGP);
+    if (e < ne) {
+      jacobian_inverse<NN>(X[e], dN + q * NN * 3, wq[q], Ai[e][q], R[e][q], flag);
+      if (q == 0) {
+        double lam, mu;
+        lame(E[elem_offset + le0 + e], nu[elem_offset + le0 + e], lam, mu);
+        double* slot = kp + (le0 + e) * (9 * NN * NN);
+        slot[0] = lam;
+        slot[1] = mu;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = 0; e < ne; ++e) {
+    double* slot = kp + (le0 + e) * (9 * NN * NN) + 2;
+    for (int t = lane; t < NGP * NN; t += kWave) {
+      const int q = t / NN;
+      double h[3];
+      scaled_gradient(Ai[e][q], R[e][q], dN + 3 * t, h);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) slot[3 * t + c] = h[c];
+    }
+  }
 }
 
 // ---- K1 on the matrix cores (asm_variant 6): M = H^T H by FP64 MFMA -------------------
@@ -1051,6 +1119,14 @@ __global__ __launch_bounds__(kWave) void k_elem_iso_mfma(
 constexpr int kRowCap = 256;
 constexpr int kBatch = 4;  // incident elements whose loads are issued together
 
+// v of lane l (l wave-uniform) to every lane.
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(b), l);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(b >> 32), l);
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
 // buf[0] += v in LDS without a read-back (DS_ADD_F64). The rows belong to one wave, whose
 // LDS operations execute in issue order, so every entry is still summed in element order.
 __device__ __forceinline__ void lds_add(double* p, double v) {
@@ -1070,7 +1146,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 // UP: the arena holds the upper node-pair blocks (elem_iso_body with UP = true) of the
 // elements of at least up_min nodes at their slots, the blocks of partners b < a read
 // transposed, and the full matrix of the others.
-template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false>
+// HF > 0 (with UP, HALVES = 1): the 20-node elements hold their gradient form
+// (k_elem_hform, HF Gauss points); the lane forms its column of M_ab from it, takes the
+// rest of the 3x3 block from the two other lanes of its node, then K_ab as k_elem_iso
+// does. Other elements are read as with UP alone.
+template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false, int HF = 0>
 __device__ __forceinline__ void row_gather_node(
     const GatherDesc& d, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
     const IncRecord* __restrict__ rec, const int32_t* __restrict__ conn,
@@ -1112,16 +1192,32 @@ __device__ __forceinline__ void row_gather_node(
   // component sub in ascending element order; the halves are added at the end
   double fsum = 0.0;
   const int64_t i0 = d.inc0, i1 = d.inc0 + d.inc_n;
-  for (int64_t ib = i0; ib < i1; ib += kBatch * halves) {
+  // gradient form only where one element is taken per step (every incidence is then
+  // wave-uniform)
+  constexpr int HFE = HALVES == 1 ? HF : 0;
+  // one incidence per step in the gradient form: its partner gradients take 16 VGPRs per
+  // incidence, and a batch of 2 (4) cut the waves per SIMD so far that the gather took
+  // 8% (44%) longer at 64^3 C3D20R
+  constexpr int KB = HFE > 0 ? 1 : kBatch;
+  constexpr int HB = HFE > 0 ? HFE : 1;
+  for (int64_t ib = i0; ib < i1; ib += KB * halves) {
     // issue every load of the batch first ...
-    double v[kBatch][3], uv[kBatch][NU], fv[kBatch];
-    int32_t colp[kBatch];
-    int fm[kBatch];
-    bool ok[kBatch];
+    double v[KB][3], uv[KB][NU], fv[KB];
+    // gradient form: partner gradients h_{b,jl}(q), the lane's share of the row node's
+    // gradients / lambda / mu, and whether incidence k is in that form (wave-uniform)
+    double hb[KB][HB], cv[KB];
+    bool hk[KB];
+    int32_t colp[KB];
+    int fm[KB];
+    bool ok[KB];
 #pragma unroll
-    for (int k = 0; k < kBatch; ++k) {
+    for (int k = 0; k < KB; ++k) {
       ok[k] = false;
+      hk[k] = false;
       v[k][0] = v[k][1] = v[k][2] = 0.0;
+      cv[k] = 0.0;
+#pragma unroll
+      for (int q = 0; q < HB; ++q) hb[k][q] = 0.0;
 #pragma unroll
       for (int c = 0; c < NU; ++c) uv[k][c] = 0.0;
       colp[k] = 0;
@@ -1137,7 +1233,22 @@ __device__ __forceinline__ void row_gather_node(
           const int32_t sl = slot_all[r.slot + bl];
           colp[k] = sl & kSlotMask;
           fm[k] = sl >> kSlotFreeShift;
-          if constexpr (SCALAR) {
+          if (HFE > 0 && r.n == 20) {
+            // slot: lambda, mu, then h_{k,c}(q) at 2 + 3n q + 3k + c. Lane 3q + i < 3 HF
+            // loads the row node's h_{la,i}(q), lanes 3 HF and 3 HF + 1 lambda and mu
+            // (the compute step below reads them across the wave); every lane its
+            // partner's h_{bl,jl}(q), the wave's lanes consecutive at each q.
+            hk[k] = true;
+            const int la = r.la, n = r.n;
+            const int64_t base = r.rows - int64_t(9) * la * n;
+            const int qa = lane / 3, ia = lane - 3 * (lane / 3);
+            const int64_t oa = lane < 3 * HFE ? 2 + int64_t(3) * n * qa + 3 * la + ia
+                                              : (lane == 3 * HFE + 1 ? 1 : 0);
+            cv[k] = kp[base + oa];
+            const double* hp = kp + base + 2 + 3 * bl + jl;
+#pragma unroll
+            for (int q = 0; q < HB; ++q) hb[k][q] = hp[int64_t(3) * n * q];
+          } else if constexpr (SCALAR) {
             const double s = __builtin_nontemporal_load(kp + r.rows / 9 + bl);
             v[k][0] = jl == 0 ? s : 0.0;
             v[k][1] = jl == 1 ? s : 0.0;
@@ -1180,11 +1291,41 @@ __device__ __forceinline__ void row_gather_node(
         }
       }
     }
+    if constexpr (HFE > 0) {
+      // Gradient form, the whole wave (wave-uniform condition): the lane's column
+      // M_ab[.][jl] = sum_q h_a(q) h_{b,jl}(q) in ascending q with the row node's
+      // gradients read across the wave, then from the node's two other lanes the row
+      // M_ab[jl][.] and the diagonal (5 shuffles), then K_ab[i][jl] as k_elem_iso forms it.
+      const int g0 = lane - jl;
+      const int j1 = jl == 2 ? 0 : jl + 1, j2 = jl == 0 ? 2 : jl - 1;
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        if (!__builtin_amdgcn_readfirstlane(int(hk[k]))) continue;
+        double mc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < HFE; ++q)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) mc[i] = __builtin_fma(read_lane(cv[k], 3 * q + i), hb[k][q], mc[i]);
+        const double lam = read_lane(cv[k], 3 * HFE), mu = read_lane(cv[k], 3 * HFE + 1);
+        const double dg = jl == 0 ? mc[0] : (jl == 1 ? mc[1] : mc[2]);  // M_ab[jl][jl]
+        const double p1 = jl == 0 ? mc[2] : (jl == 1 ? mc[0] : mc[1]);  // M_ab[jl - 1][jl]
+        const double p2 = jl == 0 ? mc[1] : (jl == 1 ? mc[2] : mc[0]);  // M_ab[jl + 1][jl]
+        const double t0 = __shfl(dg, g0), t1 = __shfl(dg, g0 + 1), t2 = __shfl(dg, g0 + 2);
+        const double r1 = __shfl(p1, g0 + j1);  // M_ab[jl][jl + 1]
+        const double r2 = __shfl(p2, g0 + j2);  // M_ab[jl][jl - 1]
+        const double tr = mu * (t0 + t1 + t2);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double mt = i == jl ? dg : (i == j1 ? r1 : r2);  // M_ab[jl][i]
+          v[k][i] = __builtin_fma(lam, mc[i], __builtin_fma(mu, mt, i == jl ? tr : 0.0));
+        }
+      }
+    }
     // ... then accumulate in ascending element order: element (k, half 0) before
     // (k, half 1) before (k + 1, half 0). LDS updates of one wave are ordered by program
     // order; the global read-modify-writes of long rows are separated by barriers.
 #pragma unroll
-    for (int k = 0; k < kBatch; ++k)
+    for (int k = 0; k < KB; ++k)
     for (int hh = 0; hh < halves; ++hh) {
       if (!in_lds && (k > 0 || hh > 0)) __syncthreads();
       if (!ok[k] || h != hh) continue;
@@ -1269,7 +1410,7 @@ __device__ __forceinline__ void row_gather_node(
 // the mass and geometric stiffness. One wave per node.
 // UP: upper-block arena, nodes taken in the order given by `order` (a locality order,
 // so that the two nodes reading one block do so close together in time).
-template <int NC, bool LIVE, bool SCALAR = false, bool UP = false>
+template <int NC, bool LIVE, bool SCALAR = false, bool UP = false, int HF = 0>
 __global__ __launch_bounds__(kWave) void k_row_gather(
     const GatherDesc* __restrict__ desc, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
@@ -1290,11 +1431,11 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // element has at most 10 nodes
   const GatherDesc d = desc[ai];
   if (d.flags & 8)
-    row_gather_node<NC, LIVE, 2, SCALAR, UP>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u, n_dof,
-                                             fe, vals, f_int, live, ue, ue_stride, up_min);
+    row_gather_node<NC, LIVE, 2, SCALAR, UP, HF>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u,
+                                                 n_dof, fe, vals, f_int, live, ue, ue_stride, up_min);
   else
-    row_gather_node<NC, LIVE, 1, SCALAR, UP>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u, n_dof,
-                                             fe, vals, f_int, live, ue, ue_stride, up_min);
+    row_gather_node<NC, LIVE, 1, SCALAR, UP, HF>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u,
+                                                 n_dof, fe, vals, f_int, live, ue, ue_stride, up_min);
 }
 
 // One thread per gather position: the node's descriptor (see GatherDesc). order: node
@@ -2519,7 +2660,8 @@ unsigned xcd_grid(const cxg_context* ctx, int64_t n) {
 void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream,
                       const GatherDesc* desc, int32_t n_pos, const IncRecord* rec,
                       const double* kp, const double* u, const double* fe, double* vals,
-                      double* f_int, uint8_t* live, int n_cases, bool scalar, bool up) {
+                      double* f_int, uint8_t* live, int n_cases, bool scalar, bool up,
+                      int hf = 0) {
   if (n_pos <= 0) return;
   const cxg_mesh* m = p->mesh;
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
@@ -2533,7 +2675,18 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
                      desc, rec, m->conn.p, p->slot_all.p, kp, u, n_dof, fe, vals, f_int, live,  \
                      n_pos, slab, cap, up ? p->ue_arena.p : nullptr,                            \
                      int64_t(3) * int64_t(m->conn.n), int32_t(ctx->up_min_nodes))
-  if (up) {
+  if (hf > 0) {
+    // gradient-form arena (k_elem_hform): linear stiffness, with or without f_int
+    if (!up || scalar || fe || hf != 8)
+      throw Error(CXG_ERR_UNSUPPORTED, "gradient-form arena: linear stiffness of 8-point elements only");
+    if (n_cases == 4) {
+      if (live) GATHER(4, true, false, true, 8); else GATHER(4, false, false, true, 8);
+    } else if (n_cases == 2) {
+      if (live) GATHER(2, true, false, true, 8); else GATHER(2, false, false, true, 8);
+    } else {
+      if (live) GATHER(1, true, false, true, 8); else GATHER(1, false, false, true, 8);
+    }
+  } else if (up) {
     if (scalar) throw Error(CXG_ERR_UNSUPPORTED, "upper-block arena: stiffness gathers only");
     if (fe) {
       if (live) GATHER(0, true, false, true); else GATHER(0, false, false, true);
@@ -2560,10 +2713,10 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
 
 void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
                    double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false,
-                   bool up = false) {
+                   bool up = false, int hf = 0) {
   prepare_desc(ctx, p);
   launch_gather_on(ctx, p, ctx->stream, p->gdesc.p, p->mesh->n_nodes, p->inc_rec.p,
-                   p->ke_arena.p, u, fe, vals, f_int, live, n_cases, scalar, up);
+                   p->ke_arena.p, u, fe, vals, f_int, live, n_cases, scalar, up, hf);
 }
 
 // Resident workgroups of a persistent launch: waves per CU times CUs, at most one per
@@ -2747,7 +2900,8 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
 #undef LAUNCH
       CXG_HIP(hipGetLastError());
     }
-    const bool up = ctx->asm_variant >= 7 && uses_upper_blocks(ctx, m);
+    const int hs = hform_stride(ctx, m);
+    const bool up = ctx->asm_variant >= 7 && (hs > 0 || uses_upper_blocks(ctx, m));
     if (up) prepare_order(ctx, p, coords);
     // variant 7 with f_int: the element kernels copy u per element for the gather
     double* ue = nullptr;
@@ -2763,6 +2917,17 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
       const ElemTable& T = ctx->tables[B.type];
       const int32_t* conn = m->conn.p + B.conn_offset;
       double* kp = pm->ke_arena.p + p->kp_base[bi];
+      if (hs > 0 && B.type == CXG_C3D20R) {
+        LaunchTimer timer(ctx, "elem_hform_C3D20R", B.n_elem);
+        constexpr int EPW = kWave / 8;
+        const int64_t groups = (B.n_elem + EPW - 1) / EPW;
+        hipLaunchKernelGGL((k_elem_hform<20, 8>), dim3(xcd_grid(ctx, groups)), dim3(kWave), 0,
+                           ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E, nu, kp,
+                           ctx->flag_dev, B.n_elem, xcd_slab(ctx, groups), u, n_dof, n_cases,
+                           ue ? ue + 3 * B.conn_offset : nullptr, ue_stride);
+        CXG_HIP(hipGetLastError());
+        continue;
+      }
       const bool bup = up && B.nn >= ctx->up_min_nodes;  // this family's arena layout
       LaunchTimer timer(ctx, std::string(bup ? "elem_iso_up_" : "elem_iso_") + type_name(B.type), B.n_elem);
 #define ISO(NN, NGP, QC, UPV, TLV)                                                              \
@@ -2786,8 +2951,8 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
       CXG_HIP(hipGetLastError());
     }
     if (m->n_nodes > 0) {
-      LaunchTimer timer(ctx, up ? "row_gather_up" : "row_gather", m->n_elem);
-      launch_gather(ctx, p, f_int ? u : nullptr, nullptr, vals, f_int, live, n_cases, false, up);
+      LaunchTimer timer(ctx, hs > 0 ? "row_gather_hform" : (up ? "row_gather_up" : "row_gather"), m->n_elem);
+      launch_gather(ctx, p, f_int ? u : nullptr, nullptr, vals, f_int, live, n_cases, false, up, hs);
       CXG_HIP(hipGetLastError());
     }
     return;
@@ -2923,6 +3088,21 @@ bool uses_upper_blocks(const cxg_context* ctx, const cxg_mesh* m) {
   for (const cxg_block_dev& B : m->blocks)
     if (B.n_elem > 0 && B.nn >= ctx->up_min_nodes) return true;
   return false;
+}
+
+int hform_stride(const cxg_context* ctx, const cxg_mesh* m) {
+  if (ctx->asm_variant != 7 || !ctx->gradient_form) return 0;
+  bool c3d20r = false;
+  for (const cxg_block_dev& B : m->blocks) {
+    if (B.n_elem == 0) continue;
+    if (B.type == CXG_C3D20) return 0;  // 20 nodes too: the gather tells forms apart by n
+    c3d20r = c3d20r || B.type == CXG_C3D20R;
+  }
+  return c3d20r ? gauss_points(CXG_C3D20R) : 0;
+}
+
+bool linear_upper_path(const cxg_context* ctx, const cxg_mesh* m) {
+  return ctx->asm_variant >= 7 && (hform_stride(ctx, m) > 0 || uses_upper_blocks(ctx, m));
 }
 
 void prepare_displacement_arena(const cxg_pattern* p, int n_cases) {

@@ -106,7 +106,7 @@ cxg_status cxg_graph_capture_linear(cxg_context* ctx, const cxg_pattern* p, cons
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant >= 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
+    if (cxg::linear_upper_path(ctx, p->mesh)) {
       cxg::prepare_order(ctx, p, coords);
       if (with_f) cxg::prepare_displacement_arena(p, 1);
     }
@@ -135,7 +135,7 @@ cxg_status cxg_graph_capture_linear_cases(cxg_context* ctx, const cxg_pattern* p
     g->elements = p->mesh->n_elem;
     if (ctx->asm_variant >= 4) cxg::prepare_arenas(p, false);
     if (ctx->asm_variant == 5) cxg::prepare_work(ctx, p);
-    if (ctx->asm_variant >= 7 && cxg::uses_upper_blocks(ctx, p->mesh)) {
+    if (cxg::linear_upper_path(ctx, p->mesh)) {
       cxg::prepare_order(ctx, p, coords);
       cxg::prepare_displacement_arena(p, int(n_cases));
     }

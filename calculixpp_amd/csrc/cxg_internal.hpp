@@ -141,6 +141,9 @@ struct cxg_context {
   // asm_variant 8: 1 = gathers on a second stream, overlapping later stages' element
   // kernels; 0 = every launch in order on the context's stream
   int stage_overlap = 1;
+  // asm_variant 7: 1 = the C3D20R elements of a mesh without C3D20 keep their scaled
+  // gradients instead of their upper blocks, and the gather forms K_ab
+  int gradient_form = 1;
   std::unordered_set<void*> allocs;  // cxg_malloc blocks, released by cxg_destroy
   double* scratch = nullptr;  // device address of the fixed scratch array on `device`
 };
@@ -351,6 +354,13 @@ void prepare_work(cxg_context* ctx, const cxg_pattern* p);
 // asm_variant 7 stores upper blocks for the families of at least up_min_nodes nodes; a
 // mesh without such a family runs the full-matrix kernels and gather of asm_variant 4.
 bool uses_upper_blocks(const cxg_context* ctx, const cxg_mesh* m);
+// The gradient form of asm_variant 7 (option gradient_form): the Gauss points of the
+// C3D20R elements (8) when the mesh has C3D20R and no C3D20 elements, else 0. The
+// C3D20R elements then keep their scaled gradients, the others their usual layout.
+int hform_stride(const cxg_context* ctx, const cxg_mesh* m);
+// Whether the linear assembly of asm_variant 7 runs the ordered gather (upper blocks or
+// gradient form): its node order and displacement arena are then prepared.
+bool linear_upper_path(const cxg_context* ctx, const cxg_mesh* m);
 // The per-element displacement arena of asm_variant 7 for n_cases fields (no-op when
 // large enough); before capture, as it allocates.
 void prepare_displacement_arena(const cxg_pattern* p, int n_cases);

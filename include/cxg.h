@@ -109,6 +109,11 @@ const char* cxg_last_error(const cxg_context* ctx);
  *   "up_min_nodes" asm_variant 7: element families of at least this many nodes store
  *                 the upper node-pair blocks, the others their full matrix; the gather
  *                 reads each element in its own layout (0..64, default 0: every family)
+ *   "gradient_form" asm_variant 7, meshes with C3D20R and without C3D20 elements: 1 =
+ *                 each C3D20R element keeps lambda, mu and its scaled shape-function
+ *                 gradients at its 8 points (482 doubles) instead of its upper blocks
+ *                 (1890), and the row gather forms K_ab from them; the other families
+ *                 keep their layout; same values bit for bit (0/1, default 1)
  *   "elem_tiling" 20-node element kernels of asm_variant 4 and 7: 0 = a lane takes a
  *                 node and four consecutive partners, 1 = the 2x2 pairs of two node
  *                 couples with 16-byte LDS reads (default 1; same values bit for bit)

@@ -31,13 +31,17 @@ def _same(a, b):
     return np.array_equal(a.cpu().numpy(), b.cpu().numpy())
 
 
-@pytest.mark.parametrize("variant", [4, 1, 7])
-def test_linear_graph_replays_match_eager(gpu, port, variant):
+@pytest.mark.parametrize("variant,gradient_form", [(4, 0), (1, 0), (7, 0), (7, 1)])
+def test_linear_graph_replays_match_eager(gpu, port, variant, gradient_form):
+    """gradient_form 1 runs on a C3D20R block (8 Gauss points): the captured element
+    kernels store the scaled gradients and the captured gather forms K from them."""
     import torch
     from calculixpp_amd import _lib, meshgen
-    m = meshgen.block(4, _lib.C3D20, seed_geo=41, seed_E=42, ny=3, nz=3)
+    fam = _lib.C3D20R if gradient_form else _lib.C3D20
+    m = meshgen.block(4, fam, seed_geo=41, seed_E=42, ny=3, nz=3)
     ctx = gpu.context()
     ctx.set_option("asm_variant", variant)
+    ctx.set_option("gradient_form", gradient_form)
     try:
         a = gpu.Assembler(m, ctx)
         dev = ctx.device
@@ -66,6 +70,7 @@ def test_linear_graph_replays_match_eager(gpu, port, variant):
             _ = (ve * 2.0).sum().item()  # unrelated torch work between replays
     finally:
         ctx.set_option("asm_variant", 7)
+        ctx.set_option("gradient_form", 1)
 
 
 @pytest.mark.parametrize("fam_i", [0, 1])

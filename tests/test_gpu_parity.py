@@ -471,3 +471,78 @@ def test_staged_assembly_equals_one_pass_bitwise(gpu, port, mesh, stage_mb):
     for c in range(4):
         f_exp = port.internal_force(m, u[c].cpu().numpy())
         assert rel_l2(out[8, 4][0][2][c * n3:(c + 1) * n3], f_exp) <= REL_K
+
+
+@pytest.mark.parametrize("lds_cols", [256, 8])
+@pytest.mark.parametrize("mesh", ["c3d20r", "c3d20r_mixed", "c3d8", "c3d20_c3d20r"])
+def test_gradient_form_equals_upper_blocks_bitwise(gpu, port, mesh, lds_cols):
+    """With gradient_form on, the C3D20R element kernel stores lambda, mu and the scaled
+    gradients h(q) per element, and the row gather forms each K_ab from them with the
+    operations of the element kernel: K, the live flags and f_int (1, 2 and 4 load cases)
+    equal the upper-block arena's bit for bit. In c3d20r_mixed (C3D20R, C3D8 and C3D6
+    cells) the gather meets both layouts at one node; lds_cols 8 sends the rows through
+    the global accumulation. Meshes without C3D20R, or with C3D20 beside it, keep the
+    upper blocks (no gradient-form launch in the profile)."""
+    import torch
+    from calculixpp_amd import _lib, meshgen
+    from calculixpp_amd.fem import _ptr
+    if mesh in ("c3d20r_mixed", "c3d20_c3d20r"):
+        coords, blocks = meshgen.mixed_box(900, 71)
+        out_b = []
+        for t, c in blocks:
+            if t == _lib.C3D20:
+                if mesh == "c3d20_c3d20r":
+                    out_b += [(_lib.C3D20, c[0::2]), (_lib.C3D20R, c[1::2])]
+                else:
+                    out_b.append((_lib.C3D20R, c))
+            else:
+                out_b.append((t, c))
+        m = _spc_model(coords, out_b, 72)
+    else:
+        fam = {"c3d8": _lib.C3D8, "c3d20r": _lib.C3D20R}[mesh]
+        b = meshgen.block(6, fam, seed_geo=75, seed_E=76, ny=5, nz=4)
+        m = _spc_model(b.coords, b.blocks, 77)
+    hform = mesh in ("c3d20r", "c3d20r_mixed")
+    ctx = gpu.context()
+    a = gpu.Assembler(m, ctx)
+    dev = ctx.device
+    n3 = 3 * m.n_nodes
+    u = torch.from_numpy(np.stack([meshgen.smooth_field(m.coords, 78 + c).ravel()
+                                   for c in range(4)])).to(dev)
+    nnz = a.pattern.nnz
+    out = {}
+    try:
+        ctx.set_option("gather_lds_cols", lds_cols)
+        for gf in (0, 1, 1):  # the second pass reuses the arena
+            ctx.set_option("gradient_form", gf)
+            for nc in (1, 2, 4):
+                v, lv = a.new_values(), a.new_live()
+                f = torch.empty(nc * n3, dtype=torch.float64, device=dev)
+                ctx.profile_reset()
+                ctx.profile(True)
+                ctx.check(ctx.lib.cxg_assemble_linear_cases(
+                    ctx.h, a.pattern.h, _ptr(a.mesh.coords), _ptr(a.mesh.E), _ptr(a.mesh.nu),
+                    _ptr(v), nc, _ptr(u), _ptr(f), _ptr(lv)))
+                ctx.profile(False)
+                stats = ctx.profile_stats()
+                assert ("row_gather_hform" in stats) == (gf == 1 and hform), stats
+                out.setdefault((gf, nc), []).append(
+                    (v[:nnz].cpu().numpy(), lv[:nnz].cpu().numpy(), f.cpu().numpy()))
+            if gf == 1:
+                sys_h = gpu.assemble_linear_static(m)  # one-shot key-set path
+    finally:
+        ctx.set_option("gradient_form", 1)
+        ctx.set_option("gather_lds_cols", 256)
+    for nc in (1, 2, 4):
+        ref = out[0, nc][0]
+        for res in out[1, nc]:
+            for x, y in zip(ref, res):
+                assert np.array_equal(x, y), (mesh, nc)
+    exp = port.assemble_linear_static(m)
+    rp, ci, vals = exp.csr()
+    assert np.array_equal(sys_h.row_ptr.cpu().numpy(), rp)
+    assert np.array_equal(sys_h.col_idx.cpu().numpy(), ci)
+    assert rel_l2(sys_h.vals.cpu().numpy(), vals) <= REL_K
+    for c in range(4):
+        f_exp = port.internal_force(m, u[c].cpu().numpy())
+        assert rel_l2(out[1, 4][0][2][c * n3:(c + 1) * n3], f_exp) <= REL_K
