@@ -274,6 +274,13 @@ cxg_status cxg_create(int device, void* stream, cxg_context** out) {
     for (int t : {CXG_C3D8, CXG_C3D20, CXG_C3D20R, CXG_C3D6}) {
       std::vector<double> dN, w, N;
       cxg::host_tables(t, dN, w, N);
+      // the point-major table, then its transpose ((3k + c) n_gp + q), read by kernels
+      // that give each lane one point
+      const int nn = cxg::nodes_per_element(t), ng = cxg::gauss_points(t);
+      const size_t n1 = dN.size();
+      dN.resize(2 * n1);
+      for (int q = 0; q < ng; ++q)
+        for (int r = 0; r < 3 * nn; ++r) dN[n1 + size_t(r) * ng + q] = dN[size_t(q) * 3 * nn + r];
       ctx->tab_dN[t].alloc(dN.size());
       ctx->tab_w[t].alloc(w.size());
       ctx->tab_N[t].alloc(N.size());

@@ -611,16 +611,17 @@ constexpr int kStageNodes = 4;  // element-matrix node rows staged in LDS per st
 typedef double f64x2 __attribute__((ext_vector_type(2)));  // one 16-byte store
 
 // det J * w of one Gauss point from the node coordinates x[3n] and the natural derivatives
-// d[n][3], and with INV also J^-1 (row-major, cofactors) into A; raises kFlagJacobian when
-// det J <= 0 (element.cpp:86-87).
-template <int NN, bool INV>
+// d[n][3] (at d[(3k + c) S]: S = 1 for a point's row of the dN table, S = n_gp for the
+// point's column of its transpose), and with INV also J^-1 (row-major, cofactors) into
+// A; raises kFlagJacobian when det J <= 0 (element.cpp:86-87).
+template <int NN, bool INV, int S = 1>
 __device__ __forceinline__ double jacobian_detw(const double* x, const double* __restrict__ d,
                                                 const double w, double* A,
                                                 int* __restrict__ flag) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (int k = 0; k < NN; ++k) {
     const double x0 = x[3 * k], x1 = x[3 * k + 1], x2 = x[3 * k + 2];
-    const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+    const double d0 = d[(3 * k) * S], d1 = d[(3 * k + 1) * S], d2 = d[(3 * k + 2) * S];
     J[0][0] += x0 * d0; J[0][1] += x0 * d1; J[0][2] += x0 * d2;
     J[1][0] += x1 * d0; J[1][1] += x1 * d1; J[1][2] += x1 * d2;
     J[2][0] += x2 * d0; J[2][1] += x2 * d1; J[2][2] += x2 * d2;
@@ -646,11 +647,24 @@ __device__ __forceinline__ double jacobian_detw(const double* x, const double* _
 }
 
 // J^-1 into A and sqrt(det J * w) of one Gauss point (see jacobian_detw).
-template <int NN>
+template <int NN, int S = 1>
 __device__ __forceinline__ void jacobian_inverse(const double* x, const double* __restrict__ d,
                                                  const double w, double* A, double& rs,
                                                  int* __restrict__ flag) {
-  rs = sqrt(jacobian_detw<NN, true>(x, d, w, A, flag));
+  rs = sqrt(jacobian_detw<NN, true, S>(x, d, w, A, flag));
+}
+
+// The natural derivatives of point q for a lane that takes one point: families of at
+// most 8 points read the transposed table that the device table of every family holds
+// after the point-major one ((3k + c) n_gp + q: the lanes' reads consecutive); the
+// 27-point families read their point-major row (3 n consecutive doubles per lane, in
+// 16-byte loads), which measured faster for them (recovery 0.76 against 1.03 ms at 64^3
+// C3D20). Element k component c is at dn_point(dN, q)[(3k + c) dn_stride].
+template <int NGP>
+constexpr int dn_stride() { return NGP <= 8 ? NGP : 1; }
+template <int NN, int NGP>
+__device__ __forceinline__ const double* dn_point(const double* dN, int q) {
+  return NGP <= 8 ? dN + NGP * NN * 3 + q : dN + q * NN * 3;
 }
 
 // h = sqrt(det J w) J^-T dN of one node at one Gauss point (A = J^-1 row-major, d = the
@@ -775,7 +789,8 @@ __device__ __forceinline__ void elem_iso_body(ElemLds<NN, NGP, QC>& L, const int
     L.x[3 * lane + 2] = xp[2];
   }
   __syncthreads();
-  for (int q = lane; q < NGP; q += kWave) jacobian_inverse<NN>(L.x, dN + q * NN * 3, wq[q], L.Ai + 9 * q, L.rs[q], flag);
+  for (int q = lane; q < NGP; q += kWave)
+    jacobian_inverse<NN, dn_stride<NGP>()>(L.x, dn_point<NN, NGP>(dN, q), wq[q], L.Ai + 9 * q, L.rs[q], flag);
   double M[PPL][9];
 #pragma unroll
   for (int t = 0; t < PPL; ++t)
@@ -964,7 +979,7 @@ __global__ __launch_bounds__(kWave) void k_elem_hform(
   {
     const int e = lane / NGP, q = lane - NGP * (lane / NGP);
     if (e < ne) {
-      jacobian_inverse<NN>(X[e], dN + q * NN * 3, wq[q], Ai[e][q], R[e][q], flag);
+      jacobian_inverse<NN, dn_stride<NGP>()>(X[e], dn_point<NN, NGP>(dN, q), wq[q], Ai[e][q], R[e][q], flag);
       if (q == 0) {
         double lam, mu;
         lame(E[elem_offset + le0 + e], nu[elem_offset + le0 + e], lam, mu);
@@ -1033,7 +1048,7 @@ __global__ __launch_bounds__(kWave) void k_elem_iso_mfma(
   }
   __syncthreads();
   for (int q = lane; q < NGP; q += kWave)
-    jacobian_inverse<NN>(L.x, dN + q * NN * 3, wq[q], L.Ai + 9 * q, L.rs[q], flag);
+    jacobian_inverse<NN, dn_stride<NGP>()>(L.x, dn_point<NN, NGP>(dN, q), wq[q], L.Ai + 9 * q, L.rs[q], flag);
   __syncthreads();
   // this lane's H entries: h[s][t] = H[4s + lane/16][16t + lane%16]
   const int qs = lane / 16, cs = lane % 16;
@@ -1767,14 +1782,15 @@ __device__ __forceinline__ void j2_gauss_point(
     const int64_t pt, const int64_t n_points, const double* __restrict__ dN,
     const double* __restrict__ wq, const double* __restrict__ committed,
     double* __restrict__ trial, int* __restrict__ flag) {
-  const double* d = dN + q * NN * 3;
+  const double* d = dn_point<NN, NGP>(dN, q);
   double* A = P.Ai + 9 * q;
   double rs;
-  jacobian_inverse<NN>(G.x, d, wq[q], A, rs, flag);
+  jacobian_inverse<NN, dn_stride<NGP>()>(G.x, d, wq[q], A, rs, flag);
   P.rs[q] = rs;
   double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (int k = 0; k < NN; ++k) {
-    const double d0 = d[3 * k], d1 = d[3 * k + 1], d2 = d[3 * k + 2];
+    constexpr int S = dn_stride<NGP>();
+    const double d0 = d[(3 * k) * S], d1 = d[(3 * k + 1) * S], d2 = d[(3 * k + 2) * S];
     const double g0 = A[0] * d0 + A[3] * d1 + A[6] * d2;
     const double g1 = A[1] * d0 + A[4] * d1 + A[7] * d2;
     const double g2 = A[2] * d0 + A[5] * d1 + A[8] * d2;
@@ -1820,6 +1836,44 @@ __device__ __forceinline__ void j2_gauss_point(
   }
 }
 
+// Phase 2's per-point pieces, with the multiply-adds written out so that every kernel
+// forming them gets the same bits. h = rs J^-T dN_k and t = B(h)^T n of node k at point q.
+template <int NN, int NGP>
+__device__ __forceinline__ void j2_gradients(const J2Elem<NN, NGP>& P, const int q,
+                                             const double* __restrict__ d, double h[3],
+                                             double t[3]) {
+  scaled_gradient(P.Ai + 9 * q, P.rs[q], d, h);
+  const double* n = P.T + 9 * q + 3;
+  t[0] = __builtin_fma(h[2], n[4], __builtin_fma(h[1], n[3], h[0] * n[0]));
+  t[1] = __builtin_fma(h[2], n[5], __builtin_fma(h[0], n[3], h[1] * n[1]));
+  t[2] = __builtin_fma(h[1], n[5], __builtin_fma(h[0], n[4], h[2] * n[2]));
+}
+
+// f_a += B(h_a)^T (rs sigma) at one point (sq = rs sigma in Voigt order).
+__device__ __forceinline__ void j2_force_add(const double* g, const double* sq, double f[3]) {
+  f[0] += __builtin_fma(g[2], sq[4], __builtin_fma(g[1], sq[3], g[0] * sq[0]));
+  f[1] += __builtin_fma(g[2], sq[5], __builtin_fma(g[0], sq[3], g[1] * sq[1]));
+  f[2] += __builtin_fma(g[1], sq[5], __builtin_fma(g[0], sq[4], g[2] * sq[2]));
+}
+
+// K_ab += lam h_a h_b^T + mu h_b h_a^T + mu (h_a . h_b) I - c t_a t_b^T at one point.
+__device__ __forceinline__ void j2_pair_add(double K[9], const double av[3], const double bv[3],
+                                            const double ta[3], const double tb[3],
+                                            const double lam, const double mu, const double c) {
+  const double la[3] = {lam * av[0], lam * av[1], lam * av[2]};
+  const double mb[3] = {mu * bv[0], mu * bv[1], mu * bv[2]};
+  const double ct[3] = {c * ta[0], c * ta[1], c * ta[2]};
+  const double dg = mu * __builtin_fma(av[2], bv[2], __builtin_fma(av[1], bv[1], av[0] * bv[0]));
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      K[3 * i + j] = __builtin_fma(-ct[i], tb[j], __builtin_fma(mb[i], av[j], __builtin_fma(la[i], bv[j], K[3 * i + j])));
+  K[0] += dg;
+  K[4] += dg;
+  K[8] += dg;
+}
+
 // Phase 2 for one element: every lane of the wave calls it; ends with a barrier. UP: K_t
 // (symmetric) as its packed upper node-pair blocks (store_upper_blocks) instead of full
 // rows.
@@ -1845,77 +1899,44 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
 #pragma unroll
     for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
   }
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  double f[3] = {0.0, 0.0, 0.0};
   for (int q0 = 0; q0 < NGP; q0 += QC) {
     for (int idx = lane; idx < QC * NN; idx += kWave) {
       const int q = q0 + idx / NN;
-      double h0 = 0.0, h1 = 0.0, h2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
-      if (q < NGP) {
-        const double* A = P.Ai + 9 * q;
-        const double* d = dN + (q * NN + idx % NN) * 3;
-        const double s = P.rs[q];
-        h0 = s * (A[0] * d[0] + A[3] * d[1] + A[6] * d[2]);
-        h1 = s * (A[1] * d[0] + A[4] * d[1] + A[7] * d[2]);
-        h2 = s * (A[2] * d[0] + A[5] * d[1] + A[8] * d[2]);
-        const double* n = P.T + 9 * q + 3;
-        t0 = h0 * n[0] + h1 * n[3] + h2 * n[4];
-        t1 = h1 * n[1] + h0 * n[3] + h2 * n[5];
-        t2 = h2 * n[2] + h0 * n[4] + h1 * n[5];
+      double h[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+      if (q < NGP) j2_gradients(P, q, dN + (q * NN + idx % NN) * 3, h, t);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        w.c.h[3 * idx + i] = h[i];
+        w.c.t[3 * idx + i] = t[i];
       }
-      w.c.h[3 * idx + 0] = h0;
-      w.c.h[3 * idx + 1] = h1;
-      w.c.h[3 * idx + 2] = h2;
-      w.c.t[3 * idx + 0] = t0;
-      w.c.t[3 * idx + 1] = t1;
-      w.c.t[3 * idx + 2] = t2;
     }
     __syncthreads();
     const int qn = NGP - q0 < QC ? NGP - q0 : QC;
     if (lane < NN)
-      for (int qq = 0; qq < qn; ++qq) {
-        const double* g = w.c.h + (qq * NN + lane) * 3;
-        const double* sq = P.S + 6 * (q0 + qq);
-        f0 += g[0] * sq[0] + g[1] * sq[3] + g[2] * sq[4];
-        f1 += g[1] * sq[1] + g[0] * sq[3] + g[2] * sq[5];
-        f2 += g[2] * sq[2] + g[0] * sq[4] + g[1] * sq[5];
-      }
+      for (int qq = 0; qq < qn; ++qq) j2_force_add(w.c.h + (qq * NN + lane) * 3, P.S + 6 * (q0 + qq), f);
 #pragma unroll 1
     for (int qq = 0; qq < qn; ++qq) {
       const double* Tq = P.T + 9 * (q0 + qq);
-      const double lam = Tq[0], mu = Tq[1], c = Tq[2];
       const double* ha = w.c.h + (qq * NN + ga_c) * 3;
       const double* ta = w.c.t + (qq * NN + ga_c) * 3;
-      const double a0 = ha[0], a1 = ha[1], a2 = ha[2];
-      const double av[3] = {a0, a1, a2};
-      const double la[3] = {lam * a0, lam * a1, lam * a2};
-      const double ct[3] = {c * ta[0], c * ta[1], c * ta[2]};
+      const double av[3] = {ha[0], ha[1], ha[2]}, tav[3] = {ta[0], ta[1], ta[2]};
 #pragma unroll
       for (int t = 0; t < PPL; ++t) {
         const int b = gb0 + t < NN ? (gb0 + t >= 0 ? gb0 + t : 0) : NN - 1;
         const double* hb = w.c.h + (qq * NN + b) * 3;
         const double* tb = w.c.t + (qq * NN + b) * 3;
-        const double b0 = hb[0], b1 = hb[1], b2 = hb[2];
-        const double bv[3] = {b0, b1, b2};
-        const double mb[3] = {mu * b0, mu * b1, mu * b2};
-        const double tv[3] = {tb[0], tb[1], tb[2]};
-        const double dg = mu * (a0 * b0 + a1 * b1 + a2 * b2);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            K[t][3 * i + j] += la[i] * bv[j] + mb[i] * av[j] - ct[i] * tv[j];
-        K[t][0] += dg;
-        K[t][4] += dg;
-        K[t][8] += dg;
+        const double bv[3] = {hb[0], hb[1], hb[2]}, tbv[3] = {tb[0], tb[1], tb[2]};
+        j2_pair_add(K[t], av, bv, tav, tbv, Tq[0], Tq[1], Tq[2]);
       }
     }
     __syncthreads();
   }
   if (lane < NN) {
     double* fp = fel + 3 * lane;
-    fp[0] = f0;
-    fp[1] = f1;
-    fp[2] = f2;
+    fp[0] = f[0];
+    fp[1] = f[1];
+    fp[2] = f[2];
   }
   if constexpr (UP) {
     store_upper_blocks<NN, PPL>(w.stage, pa, pb, K, kel, lane);
@@ -2195,11 +2216,14 @@ struct RecoverLds {
   int32_t node[NN];
 };
 
-// Physical gradient g_k = J^-T dN_k at one Gauss point (A = J^-1 row-major).
+// Physical gradient g_k = J^-T dN_k at one Gauss point (A = J^-1 row-major; the natural
+// derivatives at d[c S], as in jacobian_detw).
+template <int S = 1>
 __device__ __forceinline__ void grad_at(const double* A, const double* d, double g[3]) {
-  g[0] = A[0] * d[0] + A[3] * d[1] + A[6] * d[2];
-  g[1] = A[1] * d[0] + A[4] * d[1] + A[7] * d[2];
-  g[2] = A[2] * d[0] + A[5] * d[1] + A[8] * d[2];
+  const double d0 = d[0], d1 = d[S], d2 = d[2 * S];
+  g[0] = A[0] * d0 + A[3] * d1 + A[6] * d2;
+  g[1] = A[1] * d0 + A[4] * d1 + A[7] * d2;
+  g[2] = A[2] * d0 + A[5] * d1 + A[8] * d2;
 }
 
 // Elements per wave of the recovery kernel: one lane per (element, Gauss point) and per
@@ -2246,12 +2270,12 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
   double A[9], e6[6], sg[6], sc = 0.0;
   if (pt_lane) {
     const int64_t le = le0 + e, eg = elem_offset + le;
-    const double* d = dN + q * NN * 3;
-    sc = jacobian_detw<NN, true>(R.x, d, wq[q], A, flag);
+    const double* d = dn_point<NN, NGP>(dN, q);
+    sc = jacobian_detw<NN, true, dn_stride<NGP>()>(R.x, d, wq[q], A, flag);
     double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < NN; ++k) {
       double g[3];
-      grad_at(A, d + 3 * k, g);
+      grad_at<dn_stride<NGP>()>(A, d + 3 * k * dn_stride<NGP>(), g);
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) H[i][j] += R.u[3 * k + i] * g[j];
     }
@@ -2436,13 +2460,13 @@ __global__ __launch_bounds__(kWave) void k_vjp(
     const int e = lane / NGP, q = lane - e * NGP;
     VjpLds<NN, NGP>& R = L[e];
     const int64_t ge = elem_offset + le0 + e;
-    const double* d = dN + q * NN * 3;
+    const double* d = dn_point<NN, NGP>(dN, q);
     double A[9];
-    const double s = jacobian_detw<NN, true>(R.x, d, wq[q], A, flag);
+    const double s = jacobian_detw<NN, true, dn_stride<NGP>()>(R.x, d, wq[q], A, flag);
     double Hu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Hw[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < NN; ++k) {
       double g[3];
-      grad_at(A, d + 3 * k, g);
+      grad_at<dn_stride<NGP>()>(A, d + 3 * k * dn_stride<NGP>(), g);
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
           Hu[i][j] += R.u[3 * k + i] * g[j];
@@ -2555,7 +2579,8 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
     for (int idx = lane; idx < NGP * NN; idx += kWave) L.n[idx] = Nq[idx];
   __syncthreads();
   for (int q = lane; q < NGP; q += kWave)
-    L.sc[q] = jacobian_detw<NN, KIND == kGeometric>(L.x, dN + q * NN * 3, wq[q], L.Ai + 9 * q, flag);
+    L.sc[q] = jacobian_detw<NN, KIND == kGeometric, dn_stride<NGP>()>(L.x, dn_point<NN, NGP>(dN, q), wq[q],
+                                                        L.Ai + 9 * q, flag);
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
   int ga, gb0;

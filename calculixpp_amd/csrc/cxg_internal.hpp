@@ -47,7 +47,7 @@ constexpr int kSlotFreeShift = 28;
 // shape-function values N[q][k].
 struct ElemTable {
   int nn = 0, ngp = 0;
-  const double* dN = nullptr;  // device, ngp*nn*3
+  const double* dN = nullptr;  // device, ngp*nn*3 point-major, then its transpose (nn*3*ngp)
   const double* w = nullptr;   // device, ngp
   const double* N = nullptr;   // device, ngp*nn
 };
