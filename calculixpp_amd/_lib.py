@@ -83,6 +83,11 @@ _SIGS = {
     "cxg_vjp": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_assemble_mass": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P]),
     "cxg_assemble_geometric": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "cxg_pattern_same_direction": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_int64)]),
+    "cxg_assemble_mass_sd": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P]),
+    "cxg_assemble_geometric_sd": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "cxg_compact_csr": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P,
+                                  C.POINTER(C.c_int64)]),
     "cxg_recover_fields": (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(J2Material), C.c_int32,
                                      _P, _P, _P, _P, _P, _P, _P, _P]),
     "cxg_graph_capture_linear": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_P)]),

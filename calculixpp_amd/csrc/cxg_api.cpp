@@ -618,6 +618,60 @@ cxg_status cxg_assemble_mass(cxg_context* ctx, const cxg_pattern* p, const doubl
   });
 }
 
+cxg_status cxg_pattern_same_direction(cxg_context* ctx, const cxg_pattern* p, int64_t* row_ptr,
+                                      int32_t* col_idx, int64_t* nnz_out) {
+  if (!p || !nnz_out) return cxg::fail(ctx, CXG_ERR_ARG, "bad pattern_same_direction arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  if (!cxg::same_direction_fits(ctx, p))
+    return cxg::fail(ctx, CXG_ERR_UNSUPPORTED,
+                     "same-direction rows: a CSR row is longer than the gather's LDS columns");
+  return cxg::guarded(ctx, [&] {
+    cxg::prepare_same_direction(ctx, p);
+    *nnz_out = p->sd_nnz;
+    if (row_ptr)
+      CXG_HIP(hipMemcpyAsync(row_ptr, p->sd_row_ptr.p, size_t(p->n_free + 1) * sizeof(int64_t),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    if (col_idx && p->sd_nnz > 0)
+      CXG_HIP(hipMemcpyAsync(col_idx, p->sd_col_idx.p, size_t(p->sd_nnz) * sizeof(int32_t),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+cxg_status cxg_assemble_mass_sd(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                const double* rho, int32_t lumped, double* vals, uint8_t* live) {
+  if (!p || !coords || !rho || !vals || !live)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_mass_sd arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_scalar(ctx, p, coords, lumped ? 1 : 0, rho, nullptr, vals, live, true);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_assemble_geometric_sd(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                     const double* gp_stress, double* vals, uint8_t* live) {
+  if (!p || !coords || !gp_stress || !vals || !live)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad assemble_geometric_sd arguments");
+  if (p->mesh->ctx != ctx) return cxg::fail(ctx, CXG_ERR_STATE, "pattern belongs to another context");
+  return cxg::guarded(ctx, [&] {
+    cxg::launch_assemble_scalar(ctx, p, coords, 2, nullptr, gp_stress, vals, live, true);
+    cxg::check_flags(ctx);
+  });
+}
+
+cxg_status cxg_compact_csr(cxg_context* ctx, int32_t n_rows, const int64_t* row_ptr,
+                           const int32_t* col_idx, const uint8_t* live, const double* vals,
+                           int64_t* row_ptr_out, int32_t* col_idx_out, double* vals_out,
+                           int64_t* nnz_out) {
+  if (n_rows < 0 || !row_ptr || !col_idx || !live || !vals || !row_ptr_out || !col_idx_out ||
+      !vals_out || !nnz_out)
+    return cxg::fail(ctx, CXG_ERR_ARG, "bad compact_csr arguments");
+  return cxg::guarded(ctx, [&] {
+    *nnz_out = cxg::compact_csr(ctx, n_rows, row_ptr, col_idx, live, vals, row_ptr_out,
+                                col_idx_out, vals_out);
+  });
+}
+
 cxg_status cxg_assemble_geometric(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                   const double* gp_stress, double* vals, uint8_t* live) {
   if (!p || !coords || !gp_stress || !vals)

@@ -1165,14 +1165,27 @@ __device__ __forceinline__ double wave_sum(double v) {
 // (k_elem_hform, HF Gauss points); the lane forms its column of M_ab from it, takes the
 // rest of the 3x3 block from the two other lanes of its node, then K_ab as k_elem_iso
 // does. Other elements are read as with UP alone.
-template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false, int HF = 0>
+// SD (with SCALAR and LIVE; every row in LDS): the rows go out on the same-direction
+// pattern (row (a, i): the columns (b, i), sd_row_ptr per free equation): a position of
+// row i received a value exactly when its column has direction i (tbuf marks them), and
+// the marked positions keep their order.
+template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false, int HF = 0, bool SD = false>
 __device__ __forceinline__ void row_gather_node(
     const GatherDesc& d, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
     const IncRecord* __restrict__ rec, const int32_t* __restrict__ conn,
     const int32_t* __restrict__ slot_all, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    const double* __restrict__ ue, int64_t ue_stride, int up_min) {
+    const double* __restrict__ ue, int64_t ue_stride, int up_min,
+    const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq) {
+  static_assert(!SD || (SCALAR && LIVE), "same-direction rows: scalar gathers with flags");
+  uint8_t* tbuf = lbuf + 3 * cap;
+  // same-direction row starts, loaded now so that they arrive during the accumulation
+  int64_t sd0[3] = {0, 0, 0};
+  if constexpr (SD)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if ((d.flags >> i) & 1) sd0[i] = sd_row_ptr[dof_eq[3 * int64_t(d.node) + i]];
   const int lane = threadIdx.x;
   constexpr int halves = HALVES;
   const int h = HALVES == 2 ? lane / 32 : 0;
@@ -1192,6 +1205,7 @@ __device__ __forceinline__ void row_gather_node(
       if (in_lds) {
         buf[i * cap + c] = 0.0;
         if constexpr (LIVE) lbuf[i * cap + c] = 0;
+        if constexpr (SD) tbuf[i * cap + c] = 0;
       } else {
         vals[rs[i] + c] = 0.0;
         if constexpr (LIVE) live[rs[i] + c] = 0;
@@ -1362,6 +1376,7 @@ __device__ __forceinline__ void row_gather_node(
             if (rsj >= 0) lds_add(buf + jl * cap + pos, sv);
             if constexpr (LIVE)
               if (sv != 0.0) lbuf[jl * cap + pos] = 1;
+            if constexpr (SD) tbuf[jl * cap + pos] = 1;
           } else {
             if (rsj >= 0) vals[rsj + pos] += sv;
             if constexpr (LIVE)
@@ -1391,7 +1406,24 @@ __device__ __forceinline__ void row_gather_node(
     if (!in_lds) __syncthreads();  // global RMWs of the next batch must see these
   }
   __syncthreads();
-  if (in_lds)
+  if constexpr (SD) {
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    for (int i = 0; i < 3; ++i) {
+      if (rs[i] < 0) continue;
+      int64_t o = sd0[i];
+      for (int64_t c0 = 0; c0 < Lrow; c0 += kWave) {
+        const int64_t c = c0 + lane;
+        const bool on = c < Lrow && tbuf[i * cap + c];
+        const uint64_t m = __ballot(on);
+        if (on) {
+          const int64_t at = o + __popcll(m & below);
+          __builtin_nontemporal_store(buf[i * cap + c], vals + at);
+          live[at] = lbuf[i * cap + c];
+        }
+        o += __popcll(m);
+      }
+    }
+  } else if (in_lds)
     for (int i = 0; i < 3; ++i) {
       if (rs[i] < 0) continue;
       for (int64_t c = lane; c < Lrow; c += kWave)
@@ -1425,16 +1457,17 @@ __device__ __forceinline__ void row_gather_node(
 // the mass and geometric stiffness. One wave per node.
 // UP: upper-block arena, nodes taken in the order given by `order` (a locality order,
 // so that the two nodes reading one block do so close together in time).
-template <int NC, bool LIVE, bool SCALAR = false, bool UP = false, int HF = 0>
+template <int NC, bool LIVE, bool SCALAR = false, bool UP = false, int HF = 0, bool SD = false>
 __global__ __launch_bounds__(kWave) void k_row_gather(
     const GatherDesc* __restrict__ desc, const IncRecord* __restrict__ rec,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ slot_all,
     const double* __restrict__ kp, const double* __restrict__ u, int64_t n_dof,
     const double* __restrict__ fe, double* __restrict__ vals, double* __restrict__ f_int,
     uint8_t* __restrict__ live, int32_t n_nodes, int32_t slab, int32_t cap,
-    const double* __restrict__ ue, int64_t ue_stride, int32_t up_min) {
+    const double* __restrict__ ue, int64_t ue_stride, int32_t up_min,
+    const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
-  // rows accumulate in vals) and, for LIVE, their flags
+  // rows accumulate in vals) and, for LIVE, their flags (SD: then the marks of tbuf)
   extern __shared__ double row_lds[];
   double* buf = row_lds;
   uint8_t* lbuf = reinterpret_cast<uint8_t*>(row_lds + 3 * cap);
@@ -1446,11 +1479,13 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // element has at most 10 nodes
   const GatherDesc d = desc[ai];
   if (d.flags & 8)
-    row_gather_node<NC, LIVE, 2, SCALAR, UP, HF>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u,
-                                                 n_dof, fe, vals, f_int, live, ue, ue_stride, up_min);
+    row_gather_node<NC, LIVE, 2, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
+                                                     u, n_dof, fe, vals, f_int, live, ue, ue_stride,
+                                                     up_min, sd_row_ptr, dof_eq);
   else
-    row_gather_node<NC, LIVE, 1, SCALAR, UP, HF>(d, cap, buf, lbuf, rec, conn, slot_all, kp, u,
-                                                 n_dof, fe, vals, f_int, live, ue, ue_stride, up_min);
+    row_gather_node<NC, LIVE, 1, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
+                                                     u, n_dof, fe, vals, f_int, live, ue, ue_stride,
+                                                     up_min, sd_row_ptr, dof_eq);
 }
 
 // One thread per gather position: the node's descriptor (see GatherDesc). order: node
@@ -2686,7 +2721,7 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
                       const GatherDesc* desc, int32_t n_pos, const IncRecord* rec,
                       const double* kp, const double* u, const double* fe, double* vals,
                       double* f_int, uint8_t* live, int n_cases, bool scalar, bool up,
-                      int hf = 0) {
+                      int hf = 0, bool sd = false) {
   if (n_pos <= 0) return;
   const cxg_mesh* m = p->mesh;
   const int64_t n_dof = 3 * int64_t(m->n_nodes);
@@ -2694,13 +2729,19 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
   const unsigned grid = xcd_grid(ctx, n_pos);
   const int32_t cap = int32_t(std::min<int64_t>((std::max<int64_t>(p->max_row, 1) + 7) / 8 * 8,
                                                 std::min(kRowCap, ctx->gather_lds_cols)));
-  const size_t lds = size_t(3 * cap) * (sizeof(double) + (live ? 1 : 0));
+  const size_t lds = size_t(3 * cap) * (sizeof(double) + (live ? 1 : 0) + (sd ? 1 : 0));
 #define GATHER(NC, LV, ...)                                                                     \
   hipLaunchKernelGGL((k_row_gather<NC, LV, ##__VA_ARGS__>), dim3(grid), dim3(kWave), lds, stream, \
                      desc, rec, m->conn.p, p->slot_all.p, kp, u, n_dof, fe, vals, f_int, live,  \
                      n_pos, slab, cap, up ? p->ue_arena.p : nullptr,                            \
-                     int64_t(3) * int64_t(m->conn.n), int32_t(ctx->up_min_nodes))
-  if (hf > 0) {
+                     int64_t(3) * int64_t(m->conn.n), int32_t(ctx->up_min_nodes),               \
+                     sd ? p->sd_row_ptr.p : nullptr, p->dof_eq.p)
+  if (sd) {
+    // same-direction rows: every row must fit the LDS (checked by the caller)
+    if (!scalar || !live || up || p->max_row > cap)
+      throw Error(CXG_ERR_UNSUPPORTED, "same-direction rows: scalar gathers of rows held in LDS");
+    GATHER(1, true, true, false, 0, true);
+  } else if (hf > 0) {
     // gradient-form arena (k_elem_hform): linear stiffness, with or without f_int
     if (!up || scalar || fe || hf != 8)
       throw Error(CXG_ERR_UNSUPPORTED, "gradient-form arena: linear stiffness of 8-point elements only");
@@ -2738,10 +2779,10 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
 
 void launch_gather(cxg_context* ctx, const cxg_pattern* p, const double* u, const double* fe,
                    double* vals, double* f_int, uint8_t* live, int n_cases, bool scalar = false,
-                   bool up = false, int hf = 0) {
+                   bool up = false, int hf = 0, bool sd = false) {
   prepare_desc(ctx, p);
   launch_gather_on(ctx, p, ctx->stream, p->gdesc.p, p->mesh->n_nodes, p->inc_rec.p,
-                   p->ke_arena.p, u, fe, vals, f_int, live, n_cases, scalar, up, hf);
+                   p->ke_arena.p, u, fe, vals, f_int, live, n_cases, scalar, up, hf, sd);
 }
 
 // Resident workgroups of a persistent launch: waves per CU times CUs, at most one per
@@ -3526,9 +3567,20 @@ void launch_commit(cxg_context* ctx, int64_t n_points, double* committed, const 
   CXG_HIP(hipGetLastError());
 }
 
+bool same_direction_fits(const cxg_context* ctx, const cxg_pattern* p) {
+  return p->max_row <= std::min(kRowCap, ctx->gather_lds_cols);
+}
+
 void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             int kind, const double* rho, const double* sig, double* vals,
-                            uint8_t* live) {
+                            uint8_t* live, bool same_direction) {
+  if (same_direction) {
+    if (!live || !same_direction_fits(ctx, p))
+      throw Error(CXG_ERR_UNSUPPORTED,
+                  "same-direction rows need live flags and every row within the gather's LDS "
+                  "columns (option gather_lds_cols)");
+    prepare_same_direction(ctx, p);
+  }
   const cxg_mesh* m = p->mesh;
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
   prepare_arenas(p, false);
@@ -3555,9 +3607,9 @@ void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double
     CXG_HIP(hipGetLastError());
   }
   if (m->n_nodes > 0) {
-    LaunchTimer timer(ctx, kind == kGeometric ? "row_gather_geometric" : "row_gather_mass",
-                      m->n_elem);
-    launch_gather(ctx, p, nullptr, nullptr, vals, nullptr, live, 1, true);
+    LaunchTimer timer(ctx, std::string(kind == kGeometric ? "row_gather_geometric" : "row_gather_mass") +
+                               (same_direction ? "_sd" : ""), m->n_elem);
+    launch_gather(ctx, p, nullptr, nullptr, vals, nullptr, live, 1, true, false, 0, same_direction);
     CXG_HIP(hipGetLastError());
   }
 }

@@ -281,6 +281,12 @@ struct cxg_pattern {
   // first element.
   cxg::DevBuf<int32_t> slot_all;
   cxg::DevBuf<int64_t> sptr;                  // n_elem+1
+  // The same-direction pattern (mass, geometric stiffness: k_ab I3 blocks): row (a, i)
+  // keeps the structural columns (b, i), in the same order; built on first use
+  // (sd_nnz -1 before).
+  cxg::DevBuf<int64_t> sd_row_ptr;            // n_free+1
+  cxg::DevBuf<int32_t> sd_col_idx;            // sd_nnz
+  int64_t sd_nnz = -1;
   std::vector<int64_t> slot_base;
   int64_t slot_total = 0;                     // sum over elements of n^2
   // Two-phase assembly arena, allocated on first use: a slot of 9n^2 doubles per element
@@ -384,10 +390,14 @@ void launch_material_points(cxg_context* ctx, int64_t n_points, const int32_t* m
                             const double* strain, const double* committed, double* trial,
                             double* stress, double* tangent);
 // Mass (kind 0 consistent, 1 lumped; rho per element) or geometric stiffness (kind 2;
-// sig per Gauss point) into the CSR values (K8 + the row gather).
+// sig per Gauss point) into the CSR values (K8 + the row gather). same_direction: vals
+// and live (both required) on the pattern's same-direction pattern (prepared), which
+// needs every row in LDS (max_row <= the gather's LDS columns).
 void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                             int kind, const double* rho, const double* sig, double* vals,
-                            uint8_t* live);
+                            uint8_t* live, bool same_direction = false);
+// Whether the same-direction gather fits this pattern (every row in LDS).
+bool same_direction_fits(const cxg_context* ctx, const cxg_pattern* p);
 // Field recovery (K7): Gauss-point strain/stress, node means, f_int and reactions.
 void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, const double* E,
                     const double* nu, const double* u, const int32_t* mat_id, const void* j2_table,
@@ -406,6 +416,12 @@ void pcg(cxg_context* ctx, int32_t n, const int64_t* rp, const int32_t* ci, cons
 int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
                         const double* vals, int64_t* row_ptr_out, int32_t* col_idx_out,
                         double* vals_out);
+// The same, for any CSR of n_rows rows.
+int64_t compact_csr(cxg_context* ctx, int32_t n_rows, const int64_t* row_ptr,
+                    const int32_t* col_idx, const uint8_t* live, const double* vals,
+                    int64_t* row_ptr_out, int32_t* col_idx_out, double* vals_out);
+// Builds the pattern's same-direction pattern (no-op when built); synchronizes.
+void prepare_same_direction(cxg_context* ctx, const cxg_pattern* p);
 // Read back and clear the device status word; throws on a raised flag.
 void check_flags(cxg_context* ctx);
 

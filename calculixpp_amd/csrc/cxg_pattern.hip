@@ -322,7 +322,120 @@ __global__ __launch_bounds__(256) void k_compact(int32_t n_free, const int64_t* 
   }
 }
 
+// The same-direction pattern: row (a, i) keeps the structural columns (b, i). comp[e] is
+// the direction of free equation e; one wave per row, columns placed with a ballot.
+__global__ void k_eq_comp(int64_t n_dof, const int32_t* __restrict__ dof_eq,
+                          int8_t* __restrict__ comp) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g < n_dof && dof_eq[g] >= 0) comp[dof_eq[g]] = int8_t(g % 3);
+}
+
+__global__ __launch_bounds__(256) void k_sd_count(int32_t n_free,
+                                                  const int64_t* __restrict__ row_ptr,
+                                                  const int32_t* __restrict__ col_idx,
+                                                  const int8_t* __restrict__ comp,
+                                                  int64_t* __restrict__ count) {
+  const int64_t r = int64_t(blockIdx.x) * kRowsPerBlock + threadIdx.x / kWave;
+  if (r >= n_free) return;
+  const int lane = threadIdx.x % kWave;
+  const int8_t cr = comp[r];
+  const int64_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  int64_t c = 0;
+  for (int64_t k = k0; k < k1; k += kWave) {
+    const bool on = k + lane < k1 && comp[col_idx[k + lane]] == cr;
+    c += __popcll(__ballot(on));
+  }
+  if (lane == 0) count[r] = c;
+}
+
+__global__ __launch_bounds__(256) void k_sd_fill(int32_t n_free,
+                                                 const int64_t* __restrict__ row_ptr,
+                                                 const int32_t* __restrict__ col_idx,
+                                                 const int8_t* __restrict__ comp,
+                                                 const int64_t* __restrict__ row_out,
+                                                 int32_t* __restrict__ col_out) {
+  const int64_t r = int64_t(blockIdx.x) * kRowsPerBlock + threadIdx.x / kWave;
+  if (r >= n_free) return;
+  const int lane = threadIdx.x % kWave;
+  const int8_t cr = comp[r];
+  const int64_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  int64_t o = row_out[r];
+  for (int64_t k = k0; k < k1; k += kWave) {
+    const int32_t col = k + lane < k1 ? col_idx[k + lane] : 0;
+    const bool on = k + lane < k1 && comp[col] == cr;
+    const uint64_t m = __ballot(on);
+    if (on) col_out[o + __popcll(m & lanes_below(lane))] = col;
+    o += __popcll(m);
+  }
+}
+
+// Exclusive scan of count[0..n) into out[0..n], out[n] = the total (stream order).
+void exclusive_scan(hipStream_t s, int64_t n, const int64_t* count, int64_t* out) {
+  const int64_t sblocks = (n + kScanItems - 1) / kScanItems;
+  DevBuf<int64_t> bsum(static_cast<size_t>(sblocks));
+  hipLaunchKernelGGL(k_scan_blocks, dim3(unsigned(sblocks)), dim3(kScanThreads), 0, s, n,
+                     count, out, bsum.p);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, s, sblocks, bsum.p, out, n);
+  hipLaunchKernelGGL(k_scan_add, dim3(unsigned(sblocks)), dim3(kScanThreads), 0, s, n, bsum.p,
+                     out);
+  CXG_HIP(hipGetLastError());
+  CXG_HIP(hipStreamSynchronize(s));  // bsum is released on return
+}
+
 }  // namespace
+
+void prepare_same_direction(cxg_context* ctx, const cxg_pattern* p) {
+  if (p->sd_nnz >= 0) return;
+  cxg_pattern* pm = const_cast<cxg_pattern*>(p);
+  hipStream_t s = ctx->stream;
+  const int32_t n = p->n_free;
+  const int64_t n_dof = 3 * int64_t(p->mesh->n_nodes);
+  pm->sd_row_ptr.alloc(size_t(n) + 1);
+  int64_t total = 0;
+  if (n > 0) {
+    DevBuf<int8_t> comp(static_cast<size_t>(n));
+    DevBuf<int64_t> count(static_cast<size_t>(n));
+    hipLaunchKernelGGL(k_eq_comp, dim3(unsigned((n_dof + 255) / 256)), dim3(256), 0, s, n_dof,
+                       p->dof_eq.p, comp.p);
+    const unsigned rblocks = unsigned((int64_t(n) + kRowsPerBlock - 1) / kRowsPerBlock);
+    hipLaunchKernelGGL(k_sd_count, dim3(rblocks), dim3(256), 0, s, n, p->row_ptr.p,
+                       p->col_idx.p, comp.p, count.p);
+    CXG_HIP(hipGetLastError());
+    exclusive_scan(s, n, count.p, pm->sd_row_ptr.p);
+    CXG_HIP(hipMemcpy(&total, pm->sd_row_ptr.p + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    pm->sd_col_idx.alloc(size_t(std::max<int64_t>(total, 1)));
+    hipLaunchKernelGGL(k_sd_fill, dim3(rblocks), dim3(256), 0, s, n, p->row_ptr.p,
+                       p->col_idx.p, comp.p, pm->sd_row_ptr.p, pm->sd_col_idx.p);
+    CXG_HIP(hipGetLastError());
+    CXG_HIP(hipStreamSynchronize(s));  // comp and count are released on return
+  } else {
+    CXG_HIP(hipMemcpy(pm->sd_row_ptr.p, &total, sizeof(int64_t), hipMemcpyHostToDevice));
+    pm->sd_col_idx.alloc(1);
+  }
+  pm->sd_nnz = total;
+}
+
+int64_t compact_csr(cxg_context* ctx, int32_t n, const int64_t* row_ptr, const int32_t* col_idx,
+                    const uint8_t* live, const double* vals, int64_t* row_ptr_out,
+                    int32_t* col_idx_out, double* vals_out) {
+  hipStream_t s = ctx->stream;
+  int64_t kept = 0;
+  if (n > 0) {
+    DevBuf<int64_t> count(static_cast<size_t>(n));
+    const unsigned rblocks = unsigned((int64_t(n) + kRowsPerBlock - 1) / kRowsPerBlock);
+    hipLaunchKernelGGL(k_live_count, dim3(rblocks), dim3(256), 0, s, n, row_ptr, live, count.p);
+    CXG_HIP(hipGetLastError());
+    exclusive_scan(s, n, count.p, row_ptr_out);
+    hipLaunchKernelGGL(k_compact, dim3(rblocks), dim3(256), 0, s, n, row_ptr, col_idx, vals,
+                       live, row_ptr_out, col_idx_out, vals_out);
+    CXG_HIP(hipGetLastError());
+    CXG_HIP(hipMemcpyAsync(&kept, row_ptr_out + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  } else {
+    CXG_HIP(hipMemcpyAsync(row_ptr_out, &kept, sizeof(int64_t), hipMemcpyHostToDevice, s));
+  }
+  CXG_HIP(hipStreamSynchronize(s));
+  return kept;
+}
 
 // The reference's key set: scatter_block (src/fem/assembly.cpp:90) skips element
 // contributions equal to 0.0, so an entry all of whose contributions are 0.0 never
@@ -330,33 +443,8 @@ __global__ __launch_bounds__(256) void k_compact(int32_t n_free, const int64_t* 
 int64_t compact_pattern(cxg_context* ctx, const cxg_pattern* p, const uint8_t* live,
                         const double* vals, int64_t* row_ptr_out, int32_t* col_idx_out,
                         double* vals_out) {
-  hipStream_t s = ctx->stream;
-  const int32_t n = p->n_free;
-  int64_t kept = 0;
-  if (n > 0) {
-    DevBuf<int64_t> count(static_cast<size_t>(n));
-    const unsigned rblocks = unsigned((int64_t(n) + kRowsPerBlock - 1) / kRowsPerBlock);
-    hipLaunchKernelGGL(k_live_count, dim3(rblocks), dim3(256), 0, s, n, p->row_ptr.p, live,
-                       count.p);
-    CXG_HIP(hipGetLastError());
-    const int64_t sblocks = (int64_t(n) + kScanItems - 1) / kScanItems;
-    DevBuf<int64_t> bsum(static_cast<size_t>(sblocks));
-    hipLaunchKernelGGL(k_scan_blocks, dim3(unsigned(sblocks)), dim3(kScanThreads), 0, s,
-                       int64_t(n), count.p, row_ptr_out, bsum.p);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, s, sblocks, bsum.p,
-                       row_ptr_out, int64_t(n));
-    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(sblocks)), dim3(kScanThreads), 0, s, int64_t(n),
-                       bsum.p, row_ptr_out);
-    CXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_compact, dim3(rblocks), dim3(256), 0, s, n, p->row_ptr.p, p->col_idx.p,
-                       vals, live, row_ptr_out, col_idx_out, vals_out);
-    CXG_HIP(hipGetLastError());
-    CXG_HIP(hipMemcpyAsync(&kept, row_ptr_out + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  } else {
-    CXG_HIP(hipMemcpyAsync(row_ptr_out, &kept, sizeof(int64_t), hipMemcpyHostToDevice, s));
-  }
-  CXG_HIP(hipStreamSynchronize(s));  // kept is read, count and bsum may be released
-  return kept;
+  return compact_csr(ctx, p->n_free, p->row_ptr.p, p->col_idx.p, live, vals, row_ptr_out,
+                     col_idx_out, vals_out);
 }
 
 void build_pattern(cxg_context* ctx, const cxg_mesh* m, const int32_t* dof_eq_dev,

@@ -268,6 +268,73 @@ class Assembler:
                                                            _ptr(m.coords), _ptr(g), _ptr(vals),
                                                            _ptr(live)))
 
+    def same_direction(self):
+        """(row_ptr, col_idx) of the same-direction pattern (row (a, i): the structural
+        columns (b, i)), which holds the key set of k_ab I3 block matrices; built once."""
+        if getattr(self, "_sd", None) is None:
+            n = C.c_int64()
+            self.ctx.check(self.ctx.lib.cxg_pattern_same_direction(
+                self.ctx.h, self.pattern.h, None, None, C.byref(n)))
+            dev = self.ctx.device
+            rp = torch.empty(self.n_free + 1, dtype=torch.int64, device=dev)
+            ci = torch.empty(max(n.value, 1), dtype=torch.int32, device=dev)
+            self.ctx.check(self.ctx.lib.cxg_pattern_same_direction(
+                self.ctx.h, self.pattern.h, _ptr(rp), _ptr(ci), C.byref(n)))
+            self._sd = (rp, ci[: n.value])
+        return self._sd
+
+    def sd_fits(self) -> bool:
+        """Whether the same-direction gather fits this pattern (every row within the
+        gather's LDS columns; else cxg_pattern_same_direction reports UNSUPPORTED)."""
+        try:
+            self.same_direction()
+        except _lib.CxgError as e:
+            if e.status != _lib.ERR_UNSUPPORTED:
+                raise
+            return False
+        return True
+
+    def assemble_mass_sd(self, vals: torch.Tensor, live: torch.Tensor, lumped: bool = False,
+                         rho: Optional[torch.Tensor] = None) -> None:
+        """assemble_mass with vals and live on the same-direction pattern
+        (cxg_assemble_mass_sd)."""
+        m = self.mesh
+        if rho is None:
+            rho = torch.from_numpy(self.model.rho).to(self.ctx.device)
+        self.same_direction()
+        self.ctx.check(self.ctx.lib.cxg_assemble_mass_sd(
+            self.ctx.h, self.pattern.h, _ptr(m.coords), _ptr(rho), int(bool(lumped)), _ptr(vals),
+            _ptr(live)))
+
+    def assemble_geometric_sd(self, vals: torch.Tensor, live: torch.Tensor,
+                              gp_stress: torch.Tensor) -> None:
+        """assemble_geometric on the same-direction pattern (cxg_assemble_geometric_sd)."""
+        m = self.mesh
+        g = gp_stress.to(self.ctx.device, torch.float64).contiguous()
+        if g.shape != (m.n_points, 6):
+            raise ValueError(f"gp_stress must be ({m.n_points}, 6)")
+        self.same_direction()
+        self.ctx.check(self.ctx.lib.cxg_assemble_geometric_sd(
+            self.ctx.h, self.pattern.h, _ptr(m.coords), _ptr(g), _ptr(vals), _ptr(live)))
+
+    def sd_system(self, vals: torch.Tensor, live: torch.Tensor):
+        """(row_ptr, col_idx, vals) of the key set from same-direction values: the pattern
+        itself when every entry is live, else its live entries (cxg_compact_csr)."""
+        rp, ci = self.same_direction()
+        nnz = ci.numel()
+        va = vals[:nnz]
+        if nnz == 0 or bool(live[:nnz].all()):
+            return rp, ci, va
+        dev = self.ctx.device
+        rp_o = torch.empty_like(rp)
+        ci_o = torch.empty(nnz, dtype=torch.int32, device=dev)
+        va_o = torch.empty(nnz, dtype=torch.float64, device=dev)
+        kept = C.c_int64()
+        self.ctx.check(self.ctx.lib.cxg_compact_csr(self.ctx.h, self.n_free, _ptr(rp), _ptr(ci),
+                                                    _ptr(live), _ptr(va), _ptr(rp_o), _ptr(ci_o),
+                                                    _ptr(va_o), C.byref(kept)))
+        return rp_o, ci_o[: kept.value], va_o[: kept.value]
+
     def capture(self, vals: torch.Tensor, u: Optional[torch.Tensor] = None,
                 f_int: Optional[torch.Tensor] = None) -> Graph:
         """One assembly (as assemble with these buffers) captured as a HIP graph."""
@@ -365,7 +432,10 @@ def assemble_linear_static(model: Model, device=None) -> LinearSystem:
 def _load_free_system(a: Assembler, vals: torch.Tensor, live: torch.Tensor) -> LinearSystem:
     """The compacted system of a matrix that drives no load: assemble_mass and
     assemble_geometric_stiffness discard the eliminated-column side effect."""
-    rp, ci, va = a.compact(vals, live)
+    return _key_set_system(a, *a.compact(vals, live))
+
+
+def _key_set_system(a: Assembler, rp, ci, va) -> LinearSystem:
     rhs = torch.zeros(a.n_free, dtype=torch.float64, device=a.ctx.device)
     return LinearSystem(a.n_free, rp, ci, va, rhs, a.dof_eq, a.prescribed)
 
@@ -375,6 +445,12 @@ def assemble_mass(model: Model, lumped: bool = False, device=None) -> LinearSyst
     consistent element_mass or row-sum element_mass_lumped (fem/element.cpp:786-831).
     *MASS point masses are outside this path."""
     a = Assembler(model, context(device))
+    if a.sd_fits():
+        nnz = max(a.same_direction()[1].numel(), 1)
+        vals = torch.empty(nnz, dtype=torch.float64, device=a.ctx.device)
+        live = torch.empty(nnz, dtype=torch.uint8, device=a.ctx.device)
+        a.assemble_mass_sd(vals, live, lumped)
+        return _key_set_system(a, *a.sd_system(vals, live))
     vals, live = a.new_values(), a.new_live()
     a.assemble_mass(vals, lumped, live=live)
     return _load_free_system(a, vals, live)
@@ -396,8 +472,15 @@ def assemble_geometric_stiffness(model: Model, gp_stress, device=None) -> Linear
         if model.active is not None and not np.all(model.active):
             ng = np.concatenate([np.full(c.shape[0], GAUSS[t]) for t, c in model.blocks])
             g = g[np.repeat(np.asarray(model.active, bool), ng)]
+    gt = torch.from_numpy(np.ascontiguousarray(g))
+    if a.sd_fits():
+        nnz = max(a.same_direction()[1].numel(), 1)
+        vals = torch.empty(nnz, dtype=torch.float64, device=a.ctx.device)
+        live = torch.empty(nnz, dtype=torch.uint8, device=a.ctx.device)
+        a.assemble_geometric_sd(vals, live, gt)
+        return _key_set_system(a, *a.sd_system(vals, live))
     vals, live = a.new_values(), a.new_live()
-    a.assemble_geometric(vals, torch.from_numpy(np.ascontiguousarray(g)), live=live)
+    a.assemble_geometric(vals, gt, live=live)
     return _load_free_system(a, vals, live)
 
 

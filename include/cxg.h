@@ -298,6 +298,29 @@ cxg_status cxg_assemble_mass(cxg_context* ctx, const cxg_pattern* p, const doubl
 cxg_status cxg_assemble_geometric(cxg_context* ctx, const cxg_pattern* p, const double* coords,
                                   const double* gp_stress, double* vals, uint8_t* live);
 
+/* Both matrices are k_ab I3 blocks: an entry couples two DOFs of one direction or is
+ * never touched, so the reference's key set lies inside the same-direction pattern, the
+ * structural pattern's entries (a, i)-(b, i) (a third of it). cxg_pattern_same_direction
+ * builds that pattern once per pattern (synchronizes) and returns its nnz in *nnz_out
+ * (host); row_ptr[n_free+1] / col_idx[nnz] (device, optional) receive a copy.
+ * cxg_assemble_mass_sd / cxg_assemble_geometric_sd write vals[nnz] and live[nnz] (both
+ * required) on it directly, a third of the structural writes and no compaction when
+ * every entry is live; cxg_compact_csr keeps the live entries of any CSR (as
+ * cxg_pattern_compact; synchronizes). They need every CSR row within the gather's LDS
+ * columns (option "gather_lds_cols"; rows of 20-node hexahedra fit the default) and
+ * return CXG_ERR_UNSUPPORTED otherwise (cxg_pattern_same_direction first of all): use
+ * cxg_assemble_mass + cxg_pattern_compact. */
+cxg_status cxg_pattern_same_direction(cxg_context* ctx, const cxg_pattern* p, int64_t* row_ptr,
+                                      int32_t* col_idx, int64_t* nnz_out);
+cxg_status cxg_assemble_mass_sd(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                const double* rho, int32_t lumped, double* vals, uint8_t* live);
+cxg_status cxg_assemble_geometric_sd(cxg_context* ctx, const cxg_pattern* p, const double* coords,
+                                     const double* gp_stress, double* vals, uint8_t* live);
+cxg_status cxg_compact_csr(cxg_context* ctx, int32_t n_rows, const int64_t* row_ptr,
+                           const int32_t* col_idx, const uint8_t* live, const double* vals,
+                           int64_t* row_ptr_out, int32_t* col_idx_out, double* vals_out,
+                           int64_t* nnz_out);
+
 /* ---- K7: field recovery -----------------------------------------------------
  * Replaces recover_fields (fem/stress.cpp:189-273), its material-point form
  * (fem/stress.cpp:275-353) and recover_gauss_stress (fem/stress.cpp:147-187) for the
