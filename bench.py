@@ -81,6 +81,92 @@ def load_traffic(workload: str, variant: int):
     return rec
 
 
+# Kernels of each counted workload: the library's timer name -> the kernel's name in
+# rocprofv3's records (default options: asm_variant 7, gradient form for C3D20R).
+_TRAFFIC_KERNELS = {
+    "C3D20-64": {"elem_iso_up_C3D20": "k_elem_iso<20, 27, 3, 4, true, 1, true>",
+                 "row_gather_up": "k_row_gather<1, false, false, true, 0, false>"},
+    "C3D20R-64": {"elem_hform_C3D20R": "k_elem_hform<20, 8>",
+                  "row_gather_hform": "k_row_gather<1, false, false, true, 8, false>"},
+    "J2-C3D8-128x128x64": {"elem_j2_up_C3D8": "k_elem_j2<8, 8, 8, true>",
+                           "row_gather_j2_up": "k_row_gather<0, false, false, true, 0, false>"},
+}
+
+
+def _short_kernel(name: str) -> str:
+    """k_name<template args> of a rocprofv3 kernel name."""
+    i = name.find("::k_")
+    i = i + 2 if i >= 0 else 0
+    depth, j = 0, i
+    while j < len(name):
+        if name[j] == "<":
+            depth += 1
+        elif name[j] == ">":
+            depth -= 1
+        elif name[j] == "(" and depth == 0:
+            break
+        j += 1
+    return name[i:j].strip()
+
+
+def _pmc_per_dispatch(folder: str, counter: str) -> dict:
+    """{short kernel name: mean counter value per dispatch} of a rocprofv3 --pmc run."""
+    import csv
+    import glob
+    sums, counts = {}, {}
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                k = _short_kernel(r["Kernel_Name"])
+                sums[k] = sums.get(k, 0.0) + float(r["Counter_Value"])
+                counts[k] = counts.get(k, 0) + 1
+    return {k: sums[k] / counts[k] for k in sums}
+
+
+def ingest_traffic(root: str, copy_mib: int) -> dict:
+    """profiles/traffic.json records from rocprofv3 --pmc runs under root: calib_fetch /
+    calib_write (bench.py --copy-calibration copy_mib, one pass per counter) and, per
+    workload W of _TRAFFIC_KERNELS, W_fetch / W_write. FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) are scaled by what they read for the plain copy (bytes copied / counted)."""
+    copy_bytes = copy_mib * (1 << 20)
+    cf = max(_pmc_per_dispatch(os.path.join(root, "calib_fetch"), "FETCH_SIZE").values())
+    cw = max(_pmc_per_dispatch(os.path.join(root, "calib_write"), "WRITE_SIZE").values())
+    f_fetch, f_write = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {}
+    sha = kernel_sources_sha()
+    for wl, names in _TRAFFIC_KERNELS.items():
+        fd, wd = os.path.join(root, wl + "_fetch"), os.path.join(root, wl + "_write")
+        if not (os.path.isdir(fd) and os.path.isdir(wd)):
+            continue
+        fetch = _pmc_per_dispatch(fd, "FETCH_SIZE")
+        write = _pmc_per_dispatch(wd, "WRITE_SIZE")
+        kern, detail = {}, {}
+        for timer, rk in names.items():
+            if rk not in fetch or rk not in write:
+                raise SystemExit(f"{wl}: no counter record for {rk} (have {sorted(fetch)})")
+            fb, wb = fetch[rk] * 1024 * f_fetch, write[rk] * 1024 * f_write
+            kern[timer] = int(round(fb + wb))
+            detail[timer] = {"kernel": rk, "fetch_bytes": int(round(fb)),
+                             "write_bytes": int(round(wb)),
+                             "fetch_kib_counted": round(fetch[rk], 1),
+                             "write_kib_counted": round(write[rk], 1)}
+        data[wl] = {"asm_variant": 7, "sources_sha16": sha, "kernels": kern,
+                    "step": int(sum(kern.values())), "detail": detail,
+                    "calibration": {"copy_bytes": copy_bytes, "fetch_factor": round(f_fetch, 4),
+                                    "write_factor": round(f_write, 4)}}
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+        f.write("\n")
+    return data
+
+
 # ---- CPU baselines: run in forked children before the GPU is first touched ----------
 
 _BATCH_RECS = None  # the batch workload's meshes, generated once, shared with the forks
@@ -394,7 +480,7 @@ def j2_newton(shape=(128, 128, 64), reps: int = 5, target: float = 0.40, cpu_rec
                        "t_attainable_ms": round(t_att * 1e3, 4),
                        "frac": round(t_att * 1e3 / out["kernel_ms"], 4),
                        "traffic": (load_traffic("J2-C3D8-128x128x64",
-                                                ctx.options.get("asm_variant")) or {}).get("step")}
+                                                ctx.options.get("asm_variant", 7)) or {}).get("step")}
     if cpu_rec and cpu_rec.get("value"):
         out["cpu_baseline"] = cpu_rec
         out["gpu_over_cpu"] = out["elements_per_s"] / cpu_rec["value"]
@@ -821,8 +907,15 @@ def main() -> int:
     ap.add_argument("--patch", type=int, default=0, help="force patch_size (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--ingest-traffic", default=None, metavar="DIR",
+                    help="write profiles/traffic.json from rocprofv3 --pmc runs under DIR "
+                         "(no GPU use; see ingest_traffic)")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.ingest_traffic:
+        data = ingest_traffic(args.ingest_traffic, args.copy_calibration or 1024)
+        print(json.dumps({k: v.get("step") for k, v in data.items()}), flush=True)
+        return 0
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 8
