@@ -1154,6 +1154,30 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// The first kPreRec incidence records of a node, one dword per lane: lane 8k + w holds
+// dword w of record k (a single coalesced load of 256 bytes into one VGPR, read across
+// the wave by the one-element-per-step path).
+constexpr int kPreRec = 8;
+static_assert(sizeof(IncRecord) == 32, "8 dwords per incidence record");
+
+__device__ __forceinline__ uint32_t load_records(const IncRecord* __restrict__ rec,
+                                                 const GatherDesc& d, int lane) {
+  const int n = d.inc_n < kPreRec ? d.inc_n : kPreRec;
+  return lane < 8 * n ? reinterpret_cast<const uint32_t*>(rec + d.inc0)[lane] : 0u;
+}
+
+// Record `it` (< i1) of the node whose records start at i0, the same in every lane: from
+// the lanes of rv for the first kPreRec (lane reads), else from memory.
+__device__ __forceinline__ IncRecord record_uniform(const IncRecord* __restrict__ rec, int64_t it,
+                                                    int64_t i0, uint32_t rv) {
+  const int64_t k = it - i0;
+  if (k >= kPreRec) return rec[it];
+  uint32_t w[8];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) w[f] = __builtin_amdgcn_readlane(rv, int(8 * k) + f);
+  return __builtin_bit_cast(IncRecord, w);
+}
+
 // One node's three rows. HALVES = 2: two elements per step, one per half-wave, for
 // nodes whose incident elements all have <= 10 nodes (3n <= 30 columns fit in 32 lanes).
 // SCALAR: the arena holds k_ab per node pair (n^2 per element, at a ninth of the full
@@ -1177,7 +1201,8 @@ __device__ __forceinline__ void row_gather_node(
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
     const double* __restrict__ ue, int64_t ue_stride, int up_min,
-    const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq) {
+    const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq, uint32_t rv,
+    int lane) {
   static_assert(!SD || (SCALAR && LIVE), "same-direction rows: scalar gathers with flags");
   uint8_t* tbuf = lbuf + 3 * cap;
   // same-direction row starts, loaded now so that they arrive during the accumulation
@@ -1186,7 +1211,6 @@ __device__ __forceinline__ void row_gather_node(
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       if ((d.flags >> i) & 1) sd0[i] = sd_row_ptr[dof_eq[3 * int64_t(d.node) + i]];
-  const int lane = threadIdx.x;
   constexpr int halves = HALVES;
   const int h = HALVES == 2 ? lane / 32 : 0;
   const int sub = HALVES == 2 ? lane % 32 : lane;
@@ -1254,7 +1278,7 @@ __device__ __forceinline__ void row_gather_node(
       fv[k] = 0.0;
       const int64_t it = ib + int64_t(k) * halves + h;
       if (it < i1) {
-        const IncRecord r = rec[it];
+        const IncRecord r = HALVES == 1 ? record_uniform(rec, it, i0, rv) : rec[it];
         if constexpr (NC == 0)
           if (sub < 3) fv[k] = fe[3 * (r.conn + r.la) + sub];
         if (bl < r.n) {
@@ -1471,21 +1495,24 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   extern __shared__ double row_lds[];
   double* buf = row_lds;
   uint8_t* lbuf = reinterpret_cast<uint8_t*>(row_lds + 3 * cap);
+  const int lane = threadIdx.x;
   // one contiguous node range per XCD (slab > 0): its L2 sees the neighbouring nodes'
   // data again
   const int64_t ai = xcd_index(blockIdx.x, n_nodes, slab);
   if (ai < 0) return;
-  // the position's descriptor (wave-uniform: scalar loads); flags bit 3: every incident
-  // element has at most 10 nodes
+  // the position's descriptor (wave-uniform: scalar loads; flags bit 3: every incident
+  // element has at most 10 nodes) and its first incidence records, one dword per lane
+  // (one coalesced load instead of a scalar load per record and batch)
   const GatherDesc d = desc[ai];
+  const uint32_t rv = d.flags & 8 ? 0u : load_records(rec, d, lane);
   if (d.flags & 8)
     row_gather_node<NC, LIVE, 2, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
                                                      u, n_dof, fe, vals, f_int, live, ue, ue_stride,
-                                                     up_min, sd_row_ptr, dof_eq);
+                                                     up_min, sd_row_ptr, dof_eq, rv, lane);
   else
     row_gather_node<NC, LIVE, 1, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
                                                      u, n_dof, fe, vals, f_int, live, ue, ue_stride,
-                                                     up_min, sd_row_ptr, dof_eq);
+                                                     up_min, sd_row_ptr, dof_eq, rv, lane);
 }
 
 // One thread per gather position: the node's descriptor (see GatherDesc). order: node
