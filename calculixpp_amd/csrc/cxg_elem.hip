@@ -1474,6 +1474,105 @@ __device__ __forceinline__ void row_gather_node(
   }
 }
 
+// One node's three rows from incident elements of at most 8 nodes each, all with
+// upper-block arena slots, and f_int from the element forces fe (the J2 path): lane 8s + b
+// takes partner b of incidence s of a group of 8, loading the whole 3x3 block (four
+// 16-byte loads and one 8-byte load, whatever the block's alignment) with one slot word,
+// so that every load of the group issues in one step. The LDS adds go incidence by
+// incidence (ascending element order for every entry, as row_gather_node sums them) and
+// f_int sums the element forces in row_gather_node's two-half order: the values are the
+// same bits. The first group's records come from rv (kPreRec = 8 of them).
+template <bool LIVE>
+__device__ __forceinline__ void row_gather_pb(
+    const GatherDesc& d, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
+    const IncRecord* __restrict__ rec, const int32_t* __restrict__ slot_all,
+    const double* __restrict__ kp, const double* __restrict__ fe, double* __restrict__ vals,
+    double* __restrict__ f_int, uint8_t* __restrict__ live, uint32_t rv, int lane) {
+  const int s = lane >> 3, b = lane & 7;
+  const int n_inc = d.inc_n;
+  int64_t rs[3];
+  const int64_t Lrow = d.len;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    rs[i] = (d.flags >> i) & 1 ? d.row0 + int64_t(__popc(d.flags & ((1 << i) - 1))) * Lrow : -1;
+  for (int i = 0; i < 3; ++i) {
+    if (rs[i] < 0) continue;
+    for (int64_t c = lane; c < Lrow; c += kWave) {
+      buf[i * cap + c] = 0.0;
+      if constexpr (LIVE) lbuf[i * cap + c] = 0;
+    }
+  }
+  double h0 = 0.0, h1 = 0.0;  // f_int: the even incidences in order, the odd ones in order
+  for (int g0 = 0; g0 < n_inc; g0 += 8) {
+    const int sg = g0 + s;
+    // record sg (zeros past the node's incidences); the permutes run in every lane
+    IncRecord r;
+    if (g0 == 0) {
+      uint32_t w[8];
+#pragma unroll
+      for (int f = 0; f < 8; ++f) w[f] = uint32_t(__shfl(int(rv), 8 * s + f));
+      r = __builtin_bit_cast(IncRecord, w);
+    } else {
+      r = sg < n_inc ? rec[d.inc0 + sg] : IncRecord{};
+    }
+    const bool ok = sg < n_inc && b < r.n;
+    double blk[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) blk[k] = 0.0;
+    int32_t sl = 0;
+    double fv = 0.0;
+    const bool dir = b >= r.la;
+    if (ok) {
+      sl = slot_all[r.slot + b];
+      const int64_t base = r.rows - int64_t(9) * r.la * r.n;
+      const int64_t o = base + 9 * int64_t(dir ? up_block(r.la, b, r.n) : up_block(b, r.la, r.n));
+      const int lo = int(o & 1);
+      const f64x2* q = reinterpret_cast<const f64x2*>(kp + o + lo);
+      const f64x2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+      const double one = kp[lo ? o : o + 8];
+      const double t[8] = {q0.x, q0.y, q1.x, q1.y, q2.x, q2.y, q3.x, q3.y};
+      blk[0] = lo ? one : t[0];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) blk[k] = lo ? t[k - 1] : t[k];
+      blk[8] = lo ? t[7] : one;
+    }
+    if (sg < n_inc && b < 3) fv = fe[3 * (r.conn + r.la) + b];
+    const int colp = sl & kSlotMask, fm = sl >> kSlotFreeShift;
+    // K_ab[i][j]: block (a, b) row-major for b >= a, block (b, a) transposed for b < a
+    const int ng = n_inc - g0 < 8 ? n_inc - g0 : 8;
+    for (int e = 0; e < ng; ++e) {
+      if (s == e && ok) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (!((fm >> j) & 1)) continue;
+          const int64_t pos = int64_t(colp) + __popc(fm & ((1 << j) - 1));
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const double v = dir ? blk[3 * i + j] : blk[3 * j + i];
+            if (rs[i] >= 0) lds_add(buf + i * cap + pos, v);
+            if constexpr (LIVE)
+              if (v != 0.0) lbuf[i * cap + pos] = 1;
+          }
+        }
+      }
+    }
+    for (int e = 0; e < ng; ++e) {
+      const double x = __shfl(fv, 8 * e + (lane < 3 ? lane : 0));
+      if (e & 1) h1 += x;  // g0 is a multiple of 8: e's parity is the incidence's
+      else h0 += x;
+    }
+  }
+  __syncthreads();
+  for (int i = 0; i < 3; ++i) {
+    if (rs[i] < 0) continue;
+    for (int64_t c = lane; c < Lrow; c += kWave)
+      __builtin_nontemporal_store(buf[i * cap + c], vals + rs[i] + c);
+    if constexpr (LIVE)
+      for (int64_t c = lane; c < Lrow; c += kWave) live[rs[i] + c] = lbuf[i * cap + c];
+  }
+  if (lane < 3) f_int[3 * int64_t(d.node) + lane] = h0 + h1;
+}
+
 // NC displacement fields (load cases) at u + c*n_dof give NC internal forces at
 // f_int + c*n_dof from the same pass over the element rows (u may be null: K only).
 // NC = 0: f_int is the sum of the element force vectors fe instead. LIVE: also flag the
@@ -1504,8 +1603,15 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // element has at most 10 nodes) and its first incidence records, one dword per lane
   // (one coalesced load instead of a scalar load per record and batch)
   const GatherDesc d = desc[ai];
-  const uint32_t rv = d.flags & 8 ? 0u : load_records(rec, d, lane);
-  if (d.flags & 8)
+  // Nodes whose elements all have at most 8 nodes (flags bit 3: at most 10, which among
+  // the supported families means C3D8 and C3D6), every element with upper blocks, rows in
+  // LDS, element forces (the J2 path): one partner block per lane (row_gather_pb).
+  const bool pb = NC == 0 && UP && !SCALAR && HF == 0 && (d.flags & 8) && up_min <= 6 &&
+                  d.len <= cap && fe != nullptr;
+  const uint32_t rv = (d.flags & 8) && !pb ? 0u : load_records(rec, d, lane);
+  if (pb)
+    row_gather_pb<LIVE>(d, cap, buf, lbuf, rec, slot_all, kp, fe, vals, f_int, live, rv, lane);
+  else if (d.flags & 8)
     row_gather_node<NC, LIVE, 2, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
                                                      u, n_dof, fe, vals, f_int, live, ue, ue_stride,
                                                      up_min, sd_row_ptr, dof_eq, rv, lane);
