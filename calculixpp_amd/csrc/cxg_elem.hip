@@ -2504,6 +2504,111 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
   }
 }
 
+// Element forces alone (the matrix-free internal force: no means, no point stresses, no
+// J2): k_gauss_fields' arithmetic for them with a leaner LDS, the point phase's
+// coordinates and displacements overlaid by the node phase's J^-1 and det J w sigma, so
+// that more waves reside (7.7 KB per wave of 8 C3D8 elements against 11.5 KB). The force
+// of (element, node) goes node-major, to 3 * (its incidence), so that k_node_sum_seq
+// reads each node's vectors as one contiguous run.
+template <int NN, int NGP>
+struct ForceLds {
+  union {
+    struct {
+      double x[NN * 3];
+      double u[NN * 3];
+    };
+    struct {
+      double Ai[NGP * 9];  // J^-1 per Gauss point (row-major)
+      double S[NGP * 6];   // det J w sigma per Gauss point
+    };
+  };
+};
+
+// The natural-derivative tables of a family of at most 8 points (the point-major table
+// and its transpose, 2 NGP NN 3 doubles: 3 KB for C3D8) are staged in the LDS, so that the
+// point and node lanes read them there instead of by vector loads from the cache.
+template <int NN, int NGP>
+constexpr bool forces_dn_lds() { return NGP <= 8 && 2 * NGP * NN * 3 <= 512; }
+
+template <int NN, int NGP, int WAVES>
+__global__ __launch_bounds__(kWave, WAVES) void k_elem_forces(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN_g,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, const double* __restrict__ u,
+    const int32_t* __restrict__ ipos, double* __restrict__ fe, int* __restrict__ flag,
+    int64_t n_elem, int64_t slab) {
+  constexpr int EPW = rec_epw(NN, NGP);
+  constexpr bool DNL = forces_dn_lds<NN, NGP>();
+  constexpr int NDN = DNL ? 2 * NGP * NN * 3 : 1;
+  __shared__ ForceLds<NN, NGP> L[EPW];
+  __shared__ double dn_lds[NDN];
+  const int lane = threadIdx.x;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  if constexpr (DNL)
+    for (int i = lane; i < NDN; i += kWave) dn_lds[i] = dN_g[i];
+  const double* dN = DNL ? dn_lds : dN_g;
+  for (int idx = lane; idx < ne * NN; idx += kWave) {
+    const int e = idx / NN, k = idx - e * NN;
+    const int64_t nd = 3 * int64_t(conn[(le0 + e) * NN + k]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      L[e].x[3 * k + c] = coords[nd + c];
+      L[e].u[3 * k + c] = u[nd + c];
+    }
+  }
+  __syncthreads();
+  static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
+  const bool pt_lane = lane < ne * NGP;
+  const int e = pt_lane ? lane / NGP : 0, q = pt_lane ? lane - e * NGP : 0;
+  ForceLds<NN, NGP>& R = L[e];
+  double A[9], sg[6], sc = 0.0;
+  if (pt_lane) {
+    const int64_t eg = elem_offset + le0 + e;
+    const double* d = dn_point<NN, NGP>(dN, q);
+    sc = jacobian_detw<NN, true, dn_stride<NGP>()>(R.x, d, wq[q], A, flag);
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < NN; ++k) {
+      double g[3];
+      grad_at<dn_stride<NGP>()>(A, d + 3 * k * dn_stride<NGP>(), g);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) H[i][j] += R.u[3 * k + i] * g[j];
+    }
+    double e6[6], lam, mu;
+    voigt_strain(H, e6);
+    lame(E[eg], nu[eg], lam, mu);
+    iso_stress(lam, mu, e6, sg);
+  }
+  __syncthreads();  // x and u are read: J^-1 and det J w sigma take their place
+  if (pt_lane) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) R.S[6 * q + i] = sc * sg[i];
+  }
+  __syncthreads();
+  if (lane < ne * NN) {
+    const int e = lane / NN, k = lane - e * NN;
+    const ForceLds<NN, NGP>& R = L[e];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = 0; q < NGP; ++q) {
+      double g[3];
+      grad_at(R.Ai + 9 * q, dN + (q * NN + k) * 3, g);
+      const double* s = R.S + 6 * q;
+      f0 += g[0] * s[0] + g[1] * s[3] + g[2] * s[4];
+      f1 += g[1] * s[1] + g[0] * s[3] + g[2] * s[5];
+      f2 += g[2] * s[2] + g[0] * s[4] + g[1] * s[5];
+    }
+    // node-major: at the incidence of (element, node) (ipos: the block's conn offsets)
+    double* fp = fe + 3 * int64_t(ipos[(le0 + e) * NN + k]);
+    fp[0] = f0;
+    fp[1] = f1;
+    fp[2] = f2;
+  }
+}
+
 // Phase 2: 16 lanes per node (4 nodes per wave). fe holds the element forces at
 // 3*(conn offset) + component, as the J2 element-force arena; inc_at gives the node's conn
 // offset in each incident element. Lanes 0-11 sum the incident elements' Gauss-point
@@ -2551,33 +2656,33 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
 }
 
 // Sums of per-element node vectors over each node's incident elements, in ascending
-// element order: out0[3a + c] = sum_e a0[3 * inc_at + c] (and out1 from a1). 8 lanes per
-// node: lanes 0-2 the components of out0, 3-5 those of out1.
-__global__ void k_node_sum(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
-                           const int64_t* __restrict__ inc_at, const double* __restrict__ a0,
-                           const double* __restrict__ a1, double* __restrict__ out0,
-                           double* __restrict__ out1) {
+// element order, for node-major vectors (k_elem_forces, k_vjp): node a's are the run
+// 3 * [inc_ptr[a], inc_ptr[a + 1]), so out0[3a + c] = sum_it a0[3 it + c] (and out1 from
+// a1). G lanes per node: lanes 0-2 the components of out0, 3-5 those of out1 (G = 8;
+// G = 4: out0 alone).
+template <int G>
+__global__ void k_node_sum_seq(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
+                               const double* __restrict__ a0, const double* __restrict__ a1,
+                               double* __restrict__ out0, double* __restrict__ out1) {
+  static_assert(G == 4 || G == 8, "4 lanes per node for one array, 8 for two");
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int32_t a = int32_t(t / 8);
-  const int c = int(t % 8);
+  const int32_t a = int32_t(t / G);
+  const int c = int(t % G);
   if (a >= n_nodes || c >= 6 || (c >= 3 && !a1)) return;
-  const double* src = c < 3 ? a0 : a1;
-  const int cc = c < 3 ? c : c - 3;
+  const double* src = (c < 3 ? a0 : a1) + (c < 3 ? c : c - 3);
   const int64_t i0 = inc_ptr[a], i1 = inc_ptr[a + 1];
   double v = 0.0;
-  for (int64_t it = i0; it < i1; it += kNodeBatch) {  // as k_node_fields
-    int64_t at[kNodeBatch];
+  for (int64_t it = i0; it < i1; it += kNodeBatch) {
     double x[kNodeBatch];
 #pragma unroll
-    for (int k = 0; k < kNodeBatch; ++k) at[k] = it + k < i1 ? 3 * inc_at[it + k] + cc : -1;
-#pragma unroll
-    for (int k = 0; k < kNodeBatch; ++k) x[k] = at[k] >= 0 ? src[at[k]] : 0.0;
+    for (int k = 0; k < kNodeBatch; ++k) x[k] = it + k < i1 ? src[3 * (it + k)] : 0.0;
 #pragma unroll
     for (int k = 0; k < kNodeBatch; ++k)
-      if (at[k] >= 0) v += x[k];
+      if (it + k < i1) v += x[k];
   }
-  (c < 3 ? out0 : out1)[3 * int64_t(a) + cc] = v;
+  (c < 3 ? out0 : out1)[3 * int64_t(a) + (c < 3 ? c : c - 3)] = v;
 }
+
 
 // ---- K6: adjoint of r = K(x, E) u -------------------------------------------
 // Per Gauss point, with s = det J * w, sigma = D eps:
@@ -2587,33 +2692,48 @@ __global__ void k_node_sum(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
 // B is recomputed at the Gauss points (never stored), exactly as the forward pass. EPW
 // elements per wave as in the recovery kernel: one lane per (element, Gauss point), then
 // one per (element, node) writing the element's dx and du vectors to arenas (3 per
-// connectivity entry) that k_node_sum adds up per node in ascending element order.
+// connectivity entry, node-major as k_elem_forces) that k_node_sum_seq adds up per node in
+// ascending element order.
 template <int NN, int NGP>
 struct VjpLds {
-  double x[NN * 3];
-  double u[NN * 3];
-  double w[NN * 3];
-  double Ai[NGP * 9];  // J^-1 per Gauss point
-  double P[NGP * 9];
-  double S[NGP * 6];   // s sigma_w
-  double phi[NGP];
+  union {
+    struct {  // the point phase reads
+      double x[NN * 3];
+      double u[NN * 3];
+      double w[NN * 3];
+    };
+    struct {  // ... and then leaves for the node phase
+      double Ai[NGP * 9];  // J^-1 per Gauss point
+      double P[NGP * 9];
+      double S[NGP * 6];   // s sigma_w
+      double phi[NGP];
+    };
+  };
 };
 
 template <int NN, int NGP>
 __global__ __launch_bounds__(kWave) void k_vjp(
-    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN_g,
     const double* __restrict__ wq, const double* __restrict__ coords,
     const double* __restrict__ E, const double* __restrict__ nu, const double* __restrict__ u,
-    const double* __restrict__ w, double* __restrict__ dxe, double* __restrict__ dE,
-    double* __restrict__ due, int* __restrict__ flag, int64_t n_elem, int64_t slab) {
+    const double* __restrict__ w, const int32_t* __restrict__ ipos, double* __restrict__ dxe,
+    double* __restrict__ dE, double* __restrict__ due, int* __restrict__ flag, int64_t n_elem,
+    int64_t slab) {
   constexpr int EPW = rec_epw(NN, NGP);
   static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
+  // small natural-derivative tables in the LDS, as k_elem_forces
+  constexpr bool DNL = forces_dn_lds<NN, NGP>();
+  constexpr int NDN = DNL ? 2 * NGP * NN * 3 : 1;
   __shared__ VjpLds<NN, NGP> L[EPW];
+  __shared__ double dn_lds[NDN];
   const int lane = threadIdx.x;
   const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
   if (grp < 0) return;
   const int64_t le0 = grp * EPW;
   const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  if constexpr (DNL)
+    for (int i = lane; i < NDN; i += kWave) dn_lds[i] = dN_g[i];
+  const double* dN = DNL ? dn_lds : dN_g;
   for (int idx = lane; idx < ne * NN; idx += kWave) {
     const int e = idx / NN, k = idx - e * NN;
     const int64_t nd = conn[(le0 + e) * NN + k];
@@ -2624,12 +2744,15 @@ __global__ __launch_bounds__(kWave) void k_vjp(
     }
   }
   __syncthreads();
-  if (lane < ne * NGP) {
-    const int e = lane / NGP, q = lane - e * NGP;
+  // the point phase's results, held across the barrier after which they overlay x, u, w
+  const bool pt_lane = lane < ne * NGP;
+  const int pe = pt_lane ? lane / NGP : 0, pq_i = pt_lane ? lane - pe * NGP : 0;
+  double A[9], Pm[9], Sw6[6], pq = 0.0;
+  if (pt_lane) {
+    const int e = pe, q = pq_i;
     VjpLds<NN, NGP>& R = L[e];
     const int64_t ge = elem_offset + le0 + e;
     const double* d = dn_point<NN, NGP>(dN, q);
-    double A[9];
     const double s = jacobian_detw<NN, true, dn_stride<NGP>()>(R.x, d, wq[q], A, flag);
     double Hu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Hw[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < NN; ++k) {
@@ -2647,7 +2770,6 @@ __global__ __launch_bounds__(kWave) void k_vjp(
     voigt_strain(Hw, ew);
     iso_stress(lam, mu, eu, su);
     iso_stress(lam, mu, ew, sw);
-    double pq = 0.0;
     for (int i = 0; i < 6; ++i) pq += ew[i] * su[i];
     pq *= s;
     const double Su[3][3] = {{su[0], su[3], su[4]}, {su[3], su[1], su[5]}, {su[4], su[5], su[2]}};
@@ -2657,11 +2779,17 @@ __global__ __launch_bounds__(kWave) void k_vjp(
         double m = 0.0;
         for (int l = 0; l < 3; ++l) m += Su[i][l] * Hw[l][j] + Sw[i][l] * Hu[l][j];
         // P[j][i] = pq delta_ij - s M[i][j]
-        R.P[9 * q + 3 * j + i] = (i == j ? pq : 0.0) - s * m;
+        Pm[3 * j + i] = (i == j ? pq : 0.0) - s * m;
       }
-    for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
-    for (int i = 0; i < 6; ++i) R.S[6 * q + i] = s * sw[i];
-    R.phi[q] = pq;
+    for (int i = 0; i < 6; ++i) Sw6[i] = s * sw[i];
+  }
+  __syncthreads();  // x, u and w are read: the point results take their place
+  if (pt_lane) {
+    VjpLds<NN, NGP>& R = L[pe];
+    for (int i = 0; i < 9; ++i) R.P[9 * pq_i + i] = Pm[i];
+    for (int i = 0; i < 9; ++i) R.Ai[9 * pq_i + i] = A[i];
+    for (int i = 0; i < 6; ++i) R.S[6 * pq_i + i] = Sw6[i];
+    R.phi[pq_i] = pq;
   }
   __syncthreads();
   if (lane < ne) {
@@ -2686,7 +2814,7 @@ __global__ __launch_bounds__(kWave) void k_vjp(
       f1 += g[1] * S[1] + g[0] * S[3] + g[2] * S[5];
       f2 += g[2] * S[2] + g[0] * S[4] + g[1] * S[5];
     }
-    const int64_t o = 3 * ((le0 + e) * NN + k);
+    const int64_t o = 3 * int64_t(ipos[(le0 + e) * NN + k]);  // node-major, as k_elem_forces
     dxe[o] = a0;
     dxe[o + 1] = a1;
     dxe[o + 2] = a2;
@@ -3197,14 +3325,12 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
     if (B.n_elem == 0) continue;
     const ElemTable& T = ctx->tables[B.type];
     const int32_t* conn = m->conn.p + B.conn_offset;
-    double* feb = arena + 3 * B.conn_offset;
     LaunchTimer timer(ctx, std::string("internal_force_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_gauss_fields<NN, NGP, false>),                                          \
+  hipLaunchKernelGGL((k_elem_forces<NN, NGP, NGP <= 8 ? 5 : 4>),                                \
                      dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
-                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, B.gp_offset,             \
-                     m->n_points, T.dN, T.w, coords, E, nu, u, nullptr, nullptr, nullptr,       \
-                     nullptr, feb, nullptr, ctx->flag_dev, B.n_elem,                            \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
+                     nu, u, m->inc_pos.p + B.conn_offset, arena, ctx->flag_dev, B.n_elem,      \
                      xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
     CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
@@ -3212,9 +3338,9 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
   }
   if (m->n_nodes > 0) {
     LaunchTimer timer(ctx, "internal_force_nodes", m->n_nodes);
-    hipLaunchKernelGGL(k_node_sum, dim3(unsigned((int64_t(m->n_nodes) * 8 + 255) / 256)),
-                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_at.p,
-                       arena, nullptr, f_int, nullptr);
+    hipLaunchKernelGGL(k_node_sum_seq<4>, dim3(unsigned((int64_t(m->n_nodes) * 4 + 255) / 256)),
+                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, arena, nullptr, f_int,
+                       nullptr);
     CXG_HIP(hipGetLastError());
   }
 }
@@ -3800,8 +3926,8 @@ void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const
   hipLaunchKernelGGL((k_vjp<NN, NGP>),                                                          \
                      dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
                      dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
-                     nu, u, w, m->vjp_dx.p + 3 * B.conn_offset, dE,                             \
-                     m->rec_fe.p + 3 * B.conn_offset, ctx->flag_dev, B.n_elem,                  \
+                     nu, u, w, m->inc_pos.p + B.conn_offset, m->vjp_dx.p, dE, m->rec_fe.p,      \
+                     ctx->flag_dev, B.n_elem,                                                   \
                      xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
     CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
@@ -3809,9 +3935,9 @@ void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const
   }
   if (m->n_nodes > 0) {
     LaunchTimer timer(ctx, "vjp_nodes", m->n_nodes);
-    hipLaunchKernelGGL(k_node_sum, dim3(unsigned((int64_t(m->n_nodes) * 8 + 255) / 256)),
-                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_at.p,
-                       m->vjp_dx.p, m->rec_fe.p, dx, du);
+    hipLaunchKernelGGL(k_node_sum_seq<8>, dim3(unsigned((int64_t(m->n_nodes) * 8 + 255) / 256)),
+                       dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->vjp_dx.p,
+                       m->rec_fe.p, dx, du);
     CXG_HIP(hipGetLastError());
   }
 }
