@@ -3327,7 +3327,7 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
     const int32_t* conn = m->conn.p + B.conn_offset;
     LaunchTimer timer(ctx, std::string("internal_force_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_elem_forces<NN, NGP, NGP <= 8 ? 5 : 4>),                                \
+  hipLaunchKernelGGL((k_elem_forces<NN, NGP, 4>),                                               \
                      dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
                      dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
                      nu, u, m->inc_pos.p + B.conn_offset, arena, ctx->flag_dev, B.n_elem,      \
