@@ -1223,20 +1223,24 @@ __device__ __forceinline__ void row_gather_node(
   for (int i = 0; i < 3; ++i)
     rs[i] = (d.flags >> i) & 1 ? d.row0 + int64_t(__popc(d.flags & ((1 << i) - 1))) * Lrow : -1;
   const bool in_lds = Lrow <= cap;
-  for (int i = 0; i < 3; ++i) {
-    if (rs[i] < 0) continue;
-    for (int64_t c = lane; c < Lrow; c += kWave) {
-      if (in_lds) {
-        buf[i * cap + c] = 0.0;
-        if constexpr (LIVE) lbuf[i * cap + c] = 0;
-        if constexpr (SD) tbuf[i * cap + c] = 0;
-      } else {
+  if (in_lds) {
+    // the whole row area (values, then the flag bytes) with 16-byte writes: a few LDS
+    // instructions instead of one per row, chunk and array
+    constexpr int NB = (LIVE ? 1 : 0) + (SD ? 1 : 0);
+    const int nbytes = 3 * cap * (8 + NB);
+    uint4* z = reinterpret_cast<uint4*>(buf);
+    for (int c = lane; c < nbytes / 16; c += kWave) z[c] = make_uint4(0u, 0u, 0u, 0u);
+    for (int c = nbytes / 16 * 16 + lane; c < nbytes; c += kWave) reinterpret_cast<uint8_t*>(buf)[c] = 0;
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      if (rs[i] < 0) continue;
+      for (int64_t c = lane; c < Lrow; c += kWave) {
         vals[rs[i] + c] = 0.0;
         if constexpr (LIVE) live[rs[i] + c] = 0;
       }
     }
+    __syncthreads();  // the zeros in vals are seen by the lanes that add
   }
-  if (!in_lds) __syncthreads();  // the zeros in vals are seen by the lanes that add
   constexpr int NU = NC > 0 ? NC : 1;
   double f[NU][3];
 #pragma unroll
@@ -1495,12 +1499,12 @@ __device__ __forceinline__ void row_gather_pb(
 #pragma unroll
   for (int i = 0; i < 3; ++i)
     rs[i] = (d.flags >> i) & 1 ? d.row0 + int64_t(__popc(d.flags & ((1 << i) - 1))) * Lrow : -1;
-  for (int i = 0; i < 3; ++i) {
-    if (rs[i] < 0) continue;
-    for (int64_t c = lane; c < Lrow; c += kWave) {
-      buf[i * cap + c] = 0.0;
-      if constexpr (LIVE) lbuf[i * cap + c] = 0;
-    }
+  {
+    // the row area (values, then the flag bytes) with 16-byte writes, as row_gather_node
+    const int nbytes = 3 * cap * (8 + (LIVE ? 1 : 0));
+    uint4* z = reinterpret_cast<uint4*>(buf);
+    for (int c = lane; c < nbytes / 16; c += kWave) z[c] = make_uint4(0u, 0u, 0u, 0u);
+    for (int c = nbytes / 16 * 16 + lane; c < nbytes; c += kWave) reinterpret_cast<uint8_t*>(buf)[c] = 0;
   }
   double h0 = 0.0, h1 = 0.0;  // f_int: the even incidences in order, the odd ones in order
   for (int g0 = 0; g0 < n_inc; g0 += 8) {
@@ -1591,7 +1595,7 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
   // rows accumulate in vals) and, for LIVE, their flags (SD: then the marks of tbuf)
-  extern __shared__ double row_lds[];
+  extern __shared__ __attribute__((aligned(16))) double row_lds[];
   double* buf = row_lds;
   uint8_t* lbuf = reinterpret_cast<uint8_t*>(row_lds + 3 * cap);
   const int lane = threadIdx.x;
