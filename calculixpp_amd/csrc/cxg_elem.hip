@@ -1191,8 +1191,9 @@ __device__ __forceinline__ IncRecord record_uniform(const IncRecord* __restrict_
 // does. Other elements are read as with UP alone.
 // SD (with SCALAR and LIVE; every row in LDS): the rows go out on the same-direction
 // pattern (row (a, i): the columns (b, i), sd_row_ptr per free equation): a position of
-// row i received a value exactly when its column has direction i (tbuf marks them), and
-// the marked positions keep their order.
+// row i received a value exactly when its column has direction i (a bit of the
+// position's flag byte marks it, another its liveness), and the marked positions keep
+// their order.
 template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false, int HF = 0, bool SD = false>
 __device__ __forceinline__ void row_gather_node(
     const GatherDesc& d, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
@@ -1204,7 +1205,6 @@ __device__ __forceinline__ void row_gather_node(
     const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq, uint32_t rv,
     int lane) {
   static_assert(!SD || (SCALAR && LIVE), "same-direction rows: scalar gathers with flags");
-  uint8_t* tbuf = lbuf + 3 * cap;
   // same-direction row starts, loaded now so that they arrive during the accumulation
   int64_t sd0[3] = {0, 0, 0};
   if constexpr (SD)
@@ -1226,8 +1226,8 @@ __device__ __forceinline__ void row_gather_node(
   if (in_lds) {
     // the whole row area (values, then the flag bytes) with 16-byte writes: a few LDS
     // instructions instead of one per row, chunk and array
-    constexpr int NB = (LIVE ? 1 : 0) + (SD ? 1 : 0);
-    const int nbytes = 3 * cap * (8 + NB);
+    // SD: one flag byte per position (cap, a multiple of 4); else LIVE: one per entry
+    const int nbytes = 3 * cap * 8 + (SD ? (cap + 3) / 4 * 4 : (LIVE ? 3 * cap : 0));
     uint4* z = reinterpret_cast<uint4*>(buf);
     for (int c = lane; c < nbytes / 16; c += kWave) z[c] = make_uint4(0u, 0u, 0u, 0u);
     for (int c = nbytes / 16 * 16 + lane; c < nbytes; c += kWave) reinterpret_cast<uint8_t*>(buf)[c] = 0;
@@ -1402,9 +1402,15 @@ __device__ __forceinline__ void row_gather_node(
           const int64_t rsj = jl == 0 ? rs[0] : (jl == 1 ? rs[1] : rs[2]);
           if (in_lds) {
             if (rsj >= 0) lds_add(buf + jl * cap + pos, sv);
-            if constexpr (LIVE)
+            if constexpr (SD) {
+              // one flag byte per position: bit 3 + i touched in row i, bit i live in row
+              // i; one OR into the byte's 32-bit word (the incidences' marks only add)
+              const uint32_t mk = ((8u | (sv != 0.0 ? 1u : 0u)) << jl) << (8 * (pos & 3));
+              __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(lbuf) + (pos >> 2), mk,
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else if constexpr (LIVE) {
               if (sv != 0.0) lbuf[jl * cap + pos] = 1;
-            if constexpr (SD) tbuf[jl * cap + pos] = 1;
+            }
           } else {
             if (rsj >= 0) vals[rsj + pos] += sv;
             if constexpr (LIVE)
@@ -1435,20 +1441,23 @@ __device__ __forceinline__ void row_gather_node(
   }
   __syncthreads();
   if constexpr (SD) {
+    // chunk by chunk, one flag byte per position for the three rows
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-    for (int i = 0; i < 3; ++i) {
-      if (rs[i] < 0) continue;
-      int64_t o = sd0[i];
-      for (int64_t c0 = 0; c0 < Lrow; c0 += kWave) {
-        const int64_t c = c0 + lane;
-        const bool on = c < Lrow && tbuf[i * cap + c];
+    int64_t o[3] = {sd0[0], sd0[1], sd0[2]};
+    for (int64_t c0 = 0; c0 < Lrow; c0 += kWave) {
+      const int64_t c = c0 + lane;
+      const uint32_t fl = c < Lrow ? lbuf[c] : 0u;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (rs[i] < 0) continue;
+        const bool on = (fl >> (3 + i)) & 1u;
         const uint64_t m = __ballot(on);
         if (on) {
-          const int64_t at = o + __popcll(m & below);
+          const int64_t at = o[i] + __popcll(m & below);
           __builtin_nontemporal_store(buf[i * cap + c], vals + at);
-          live[at] = lbuf[i * cap + c];
+          live[at] = uint8_t((fl >> i) & 1u);
         }
-        o += __popcll(m);
+        o[i] += __popcll(m);
       }
     }
   } else if (in_lds)
@@ -1594,7 +1603,7 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const double* __restrict__ ue, int64_t ue_stride, int32_t up_min,
     const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
-  // rows accumulate in vals) and, for LIVE, their flags (SD: then the marks of tbuf)
+  // rows accumulate in vals) and, for LIVE, their flags (SD: one flag byte per position instead)
   extern __shared__ __attribute__((aligned(16))) double row_lds[];
   double* buf = row_lds;
   uint8_t* lbuf = reinterpret_cast<uint8_t*>(row_lds + 3 * cap);
@@ -2994,7 +3003,8 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
   const unsigned grid = xcd_grid(ctx, n_pos);
   const int32_t cap = int32_t(std::min<int64_t>((std::max<int64_t>(p->max_row, 1) + 7) / 8 * 8,
                                                 std::min(kRowCap, ctx->gather_lds_cols)));
-  const size_t lds = size_t(3 * cap) * (sizeof(double) + (live ? 1 : 0) + (sd ? 1 : 0));
+  const size_t lds = size_t(3 * cap) * sizeof(double) +
+                     (sd ? size_t(cap + 3) / 4 * 4 : (live ? size_t(3 * cap) : 0));
 #define GATHER(NC, LV, ...)                                                                     \
   hipLaunchKernelGGL((k_row_gather<NC, LV, ##__VA_ARGS__>), dim3(grid), dim3(kWave), lds, stream, \
                      desc, rec, m->conn.p, p->slot_all.p, kp, u, n_dof, fe, vals, f_int, live,  \
