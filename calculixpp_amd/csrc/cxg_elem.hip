@@ -2030,13 +2030,6 @@ __device__ __forceinline__ void j2_gradients(const J2Elem<NN, NGP>& P, const int
   t[2] = __builtin_fma(h[1], n[5], __builtin_fma(h[0], n[4], h[2] * n[2]));
 }
 
-// f_a += B(h_a)^T (rs sigma) at one point (sq = rs sigma in Voigt order).
-__device__ __forceinline__ void j2_force_add(const double* g, const double* sq, double f[3]) {
-  f[0] += __builtin_fma(g[2], sq[4], __builtin_fma(g[1], sq[3], g[0] * sq[0]));
-  f[1] += __builtin_fma(g[2], sq[5], __builtin_fma(g[0], sq[3], g[1] * sq[1]));
-  f[2] += __builtin_fma(g[1], sq[5], __builtin_fma(g[0], sq[4], g[2] * sq[2]));
-}
-
 // K_ab += lam h_a h_b^T + mu h_b h_a^T + mu (h_a . h_b) I - c t_a t_b^T at one point.
 __device__ __forceinline__ void j2_pair_add(double K[9], const double av[3], const double bv[3],
                                             const double ta[3], const double tb[3],
@@ -2080,7 +2073,13 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
 #pragma unroll
     for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
   }
-  double f[3] = {0.0, 0.0, 0.0};
+  // element force f_k += B(h_k)^T (rs sigma) per point (sq = rs sigma in Voigt order):
+  // lane 3k + c sums component c of node k, g[c] sq[c] first, then the two shear terms
+  // with explicit multiply-adds, its operands picked by lane
+  double f = 0.0;
+  const int fk = lane / 3, fc = lane - 3 * (lane / 3);
+  const int g1i = fc == 0 ? 1 : 0, s1i = fc == 2 ? 4 : 3;
+  const int g2i = fc == 2 ? 1 : 2, s2i = fc == 0 ? 4 : 5;
   for (int q0 = 0; q0 < NGP; q0 += QC) {
     for (int idx = lane; idx < QC * NN; idx += kWave) {
       const int q = q0 + idx / NN;
@@ -2094,8 +2093,12 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
     }
     __syncthreads();
     const int qn = NGP - q0 < QC ? NGP - q0 : QC;
-    if (lane < NN)
-      for (int qq = 0; qq < qn; ++qq) j2_force_add(w.c.h + (qq * NN + lane) * 3, P.S + 6 * (q0 + qq), f);
+    if (lane < 3 * NN)
+      for (int qq = 0; qq < qn; ++qq) {
+        const double* g = w.c.h + (qq * NN + fk) * 3;
+        const double* sq = P.S + 6 * (q0 + qq);
+        f += __builtin_fma(g[g2i], sq[s2i], __builtin_fma(g[g1i], sq[s1i], g[fc] * sq[fc]));
+      }
 #pragma unroll 1
     for (int qq = 0; qq < qn; ++qq) {
       const double* Tq = P.T + 9 * (q0 + qq);
@@ -2113,12 +2116,7 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
     }
     __syncthreads();
   }
-  if (lane < NN) {
-    double* fp = fel + 3 * lane;
-    fp[0] = f[0];
-    fp[1] = f[1];
-    fp[2] = f[2];
-  }
+  if (lane < 3 * NN) fel[lane] = f;
   if constexpr (UP) {
     store_upper_blocks<NN, PPL>(w.stage, pa, pb, K, kel, lane);
     return;
