@@ -176,7 +176,6 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
   for (int32_t v : conn) ++inc_ptr[size_t(v) + 1];
   for (int32_t a = 0; a < n_nodes; ++a) inc_ptr[size_t(a) + 1] += inc_ptr[size_t(a)];
   std::vector<int32_t> inc_elem(static_cast<size_t>(n_conn));
-  std::vector<int64_t> inc_at(static_cast<size_t>(n_conn));
   std::vector<int32_t> inc_pos(static_cast<size_t>(n_conn));
   if (n_conn > int64_t(INT32_MAX))
     throw Error(CXG_ERR_UNSUPPORTED, "more than 2^31 - 1 connectivity entries in one mesh");
@@ -185,7 +184,6 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
     for (int64_t e = 0; e < n_elem; ++e)
       for (int64_t k = eptr[size_t(e)]; k < eptr[size_t(e) + 1]; ++k) {
         const int32_t v = conn[size_t(k)];
-        inc_at[size_t(cur[size_t(v)])] = k;
         inc_pos[size_t(k)] = int32_t(cur[size_t(v)]);
         inc_elem[size_t(cur[size_t(v)]++)] = int32_t(e);
       }
@@ -234,7 +232,6 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
   M->eptr.alloc(eptr.size());
   M->inc_ptr.alloc(inc_ptr.size());
   M->inc_elem.alloc(std::max<size_t>(inc_elem.size(), 1));
-  M->inc_at.alloc(std::max<size_t>(inc_at.size(), 1));
   M->inc_pos.alloc(std::max<size_t>(inc_pos.size(), 1));
   if (n_conn)
     CXG_HIP(hipMemcpyAsync(M->conn.p, conn.data(), conn.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -243,8 +240,6 @@ void create_mesh(cxg_context* ctx, int32_t n_nodes, int32_t n_blocks, const cxg_
                          hipMemcpyHostToDevice, s));
   if (!inc_elem.empty()) {
     CXG_HIP(hipMemcpyAsync(M->inc_elem.p, inc_elem.data(), inc_elem.size() * sizeof(int32_t),
-                           hipMemcpyHostToDevice, s));
-    CXG_HIP(hipMemcpyAsync(M->inc_at.p, inc_at.data(), inc_at.size() * sizeof(int64_t),
                            hipMemcpyHostToDevice, s));
     CXG_HIP(hipMemcpyAsync(M->inc_pos.p, inc_pos.data(), inc_pos.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, s));

@@ -2423,8 +2423,8 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
     const double* __restrict__ nu, const double* __restrict__ u,
     const int32_t* __restrict__ mat_id, const J2Dev* __restrict__ mats,
     const double* __restrict__ committed, double* __restrict__ emean,
-    double* __restrict__ fe, double* __restrict__ gauss_stress, int* __restrict__ flag,
-    int64_t n_elem, int64_t slab) {
+    const int32_t* __restrict__ ipos, double* __restrict__ fe, double* __restrict__ gauss_stress,
+    int* __restrict__ flag, int64_t n_elem, int64_t slab) {
   constexpr int EPW = rec_epw(NN, NGP);
   __shared__ RecoverLds<NN, NGP> L[EPW];
   const int lane = threadIdx.x;
@@ -2508,7 +2508,8 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
       f1 += g[1] * s1 + g[0] * s3 + g[2] * s5;
       f2 += g[2] * s2 + g[0] * s4 + g[1] * s5;
     }
-    double* fp = fe + 3 * ((le0 + e) * NN + k);
+    // node-major, at the incidence of (element, node), as k_elem_forces
+    double* fp = fe + 3 * int64_t(ipos[(le0 + e) * NN + k]);
     fp[0] = f0;
     fp[1] = f1;
     fp[2] = f2;
@@ -2620,16 +2621,15 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_forces(
   }
 }
 
-// Phase 2: 16 lanes per node (4 nodes per wave). fe holds the element forces at
-// 3*(conn offset) + component, as the J2 element-force arena; inc_at gives the node's conn
-// offset in each incident element. Lanes 0-11 sum the incident elements' Gauss-point
-// means of one component (strain, then stress), lanes 12-14 one component of the force, in
-// ascending element order; the means are divided by the number of incident elements.
+// Phase 2: 16 lanes per node (4 nodes per wave). fe holds the element forces node-major,
+// at 3 * (incidence) + component, so a node's are one contiguous run. Lanes 0-11 sum the
+// incident elements' Gauss-point means of one component (strain, then stress), lanes
+// 12-14 one component of the force, in ascending element order; the means are divided by
+// the number of incident elements.
 constexpr int kNodeBatch = 8;
 
 __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_ptr,
                               const int32_t* __restrict__ inc_elem,
-                              const int64_t* __restrict__ inc_at,
                               const double* __restrict__ emean, const double* __restrict__ fe,
                               const double* __restrict__ f_ext, double* __restrict__ strain,
                               double* __restrict__ stress, double* __restrict__ reaction,
@@ -2647,7 +2647,7 @@ __global__ void k_node_fields(int32_t n_nodes, const int64_t* __restrict__ inc_p
     double x[kNodeBatch];
 #pragma unroll
     for (int k = 0; k < kNodeBatch; ++k)
-      at[k] = it + k < i1 ? (c < 12 ? 12 * int64_t(inc_elem[it + k]) + c : 3 * inc_at[it + k] + (c - 12)) : -1;
+      at[k] = it + k < i1 ? (c < 12 ? 12 * int64_t(inc_elem[it + k]) + c : 3 * (it + k) + (c - 12)) : -1;
 #pragma unroll
     for (int k = 0; k < kNodeBatch; ++k) x[k] = at[k] >= 0 ? (c < 12 ? emean : fe)[at[k]] : 0.0;
 #pragma unroll
@@ -3897,14 +3897,14 @@ void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, c
     if (B.n_elem == 0) continue;
     const ElemTable& T = ctx->tables[B.type];
     const int32_t* conn = m->conn.p + B.conn_offset;
-    double* feb = m->rec_fe.p + 3 * B.conn_offset;
     LaunchTimer timer(ctx, "gauss_fields", B.n_elem);
 #define LAUNCH_R(NN, NGP, J2)                                                                   \
   hipLaunchKernelGGL((k_gauss_fields<NN, NGP, J2>),                                             \
                      dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
                      dim3(kWave), 0, ctx->stream, conn, B.elem_offset, B.gp_offset,             \
                      m->n_points, T.dN, T.w, coords, E, nu, u, mat_id, dmats, committed,        \
-                     m->rec_mean.p, feb, gauss_stress, ctx->flag_dev, B.n_elem,                 \
+                     m->rec_mean.p, m->inc_pos.p + B.conn_offset, m->rec_fe.p, gauss_stress,    \
+                     ctx->flag_dev, B.n_elem,                                                   \
                      xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
 #define LAUNCH(NN, NGP, UP, QC)                                  \
   if (dmats && mat_id) LAUNCH_R(NN, NGP, true);                  \
@@ -3918,8 +3918,7 @@ void launch_recover(cxg_context* ctx, const cxg_mesh* m, const double* coords, c
     LaunchTimer timer(ctx, "node_fields", m->n_nodes);
     hipLaunchKernelGGL(k_node_fields, dim3(unsigned((int64_t(m->n_nodes) * 16 + 255) / 256)),
                        dim3(256), 0, ctx->stream, m->n_nodes, m->inc_ptr.p, m->inc_elem.p,
-                       m->inc_at.p, m->rec_mean.p, m->rec_fe.p, f_ext, strain, stress, reaction,
-                       f_int);
+                       m->rec_mean.p, m->rec_fe.p, f_ext, strain, stress, reaction, f_int);
     CXG_HIP(hipGetLastError());
   }
 }

@@ -189,13 +189,12 @@ struct cxg_mesh {
   cxg::DevBuf<int64_t> eptr;      // model element -> offset into conn (n_elem+1)
   cxg::DevBuf<int64_t> inc_ptr;   // node -> incident elements (n_nodes+1)
   cxg::DevBuf<int32_t> inc_elem;  // incident model element numbers, ascending
-  cxg::DevBuf<int64_t> inc_at;    // per incidence: conn offset of the node in the element
+  cxg::DevBuf<int32_t> inc_pos;   // per conn offset: its place in the node's incidence list
                                   // (an element lists a node once; create_mesh checks)
-  cxg::DevBuf<int32_t> inc_pos;   // per conn offset: its incidence (inverse of inc_at)
   int32_t max_candidates = 0;     // max over nodes of sum of incident element sizes
   // Field recovery arenas, allocated on first use: per element the Gauss-point means of
-  // strain and stress (12), per element node its force vector (3, at 3 * conn offset; the
-  // matrix-free internal force and the adjoint put theirs node-major, at 3 * incidence).
+  // strain and stress (12), per element node its force vector (3, node-major at
+  // 3 * incidence, as the matrix-free internal force and the adjoint put theirs).
   mutable cxg::DevBuf<double> rec_mean;
   mutable cxg::DevBuf<double> rec_fe;
   // The adjoint's per-element dx vectors (3 per connectivity entry, node-major; du goes to
