@@ -527,21 +527,22 @@ def batch_step(recs, reps: int = 3, cpu_rec=None, log=None) -> dict:
     """The batched workload: 512 mixed C3D8/C3D20/C3D6 boxes with log-uniform element
     counts in [100, 50000] (seed 4), half of them J2, 4 load cases each. One step = the
     linear meshes' K plus f_int for the 4 cases, and for the J2 meshes per case the return
-    map, K_t and f_int; elements of every mesh and family go through one persistent launch
-    per assembly. Throughput counts element evaluations (a linear element once, a J2
-    element once per case)."""
+    map, K_t and f_int; the elements of every mesh go through one launch per family
+    (asm_variant 7) or one persistent launch for all families (asm_variant 5), whichever
+    autotune.tune_batch measures faster. Throughput counts element evaluations (a linear
+    element once, a J2 element once per case)."""
     import torch
     from calculixpp_amd import autotune
     from calculixpp_amd.batch import BatchStep
     from calculixpp_amd import fem
     t1 = time.perf_counter()
     ctx = fem.context()
-    step = BatchStep(recs, ctx, variant=5)
+    step = BatchStep(recs, ctx, variant=7)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     if log:
         log(f"[bench] batch: {len(recs)} meshes uploaded + patterns in {t2 - t1:.1f}s")
-    tune = autotune.tune_persistent(step.run, ctx, log=log)
+    tune = autotune.tune_batch(step.run, ctx, log=log)
     n_lin = step.lin.model.n_elem if step.lin else 0
     n_j2 = step.j2.model.n_elem if step.j2 else 0
     evals = n_lin + step.n_cases * n_j2

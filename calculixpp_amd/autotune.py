@@ -17,11 +17,15 @@ The tuner times each candidate on the actual mesh (HIP events on the library's
 stream) and caches the winner in a JSON file keyed by element family, element
 count and nonzero count, so later runs on the same workload reuse it.
 
-The persistent launch (asm_variant 5, batches of uneven meshes) has two more knobs,
-tuned by tune_persistent on the actual batch step:
+A batch of uneven meshes (batch.BatchStep) runs either as per-family launches with
+upper blocks (asm_variant 7: the batch concatenates each family over its meshes, so
+one family is one launch) or as one persistent launch (asm_variant 5) with two more
+knobs:
   chunk_elems       elements per work item (the unit a workgroup takes from the
                     queue: small items balance better, large ones cost fewer atomics)
   persistent_waves  resident waves per compute unit of the persistent grid.
+tune_batch times variant 7 and every persistent candidate on the actual batch step
+and keeps the fastest; tune_persistent sweeps the persistent knobs alone.
 """
 from __future__ import annotations
 
@@ -137,5 +141,35 @@ def tune_persistent(run, ctx: fem.Context, candidates: Iterable = PERSISTENT_CAN
             best = {"chunk_elems": chunk, "persistent_waves": waves, "ms": ms}
     ctx.set_option("chunk_elems", best["chunk_elems"])
     ctx.set_option("persistent_waves", best["persistent_waves"])
+    best["trials"] = trials
+    return best
+
+
+def tune_batch(run, ctx: fem.Context, candidates: Iterable = PERSISTENT_CANDIDATES,
+               steps: int = 2, log=None) -> dict:
+    """Pick how a batch step runs (`run` enqueues one step on ctx's stream, e.g.
+    batch.BatchStep.run): per-family launches (asm_variant 7) or the persistent launch
+    (asm_variant 5) at its best (chunk_elems, persistent_waves). The winner is left set
+    on ctx. Returns {"asm_variant", "ms", "trials"} plus chunk_elems and
+    persistent_waves for variant 5."""
+    ctx.set_option("asm_variant", 7)
+    run()  # arenas, gather order and descriptors for this variant
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms7 = (time.perf_counter() - t0) * 1e3 / steps
+    if log:
+        log(f"[autotune] per-family launches (variant 7): {ms7:.3f} ms")
+    ctx.set_option("asm_variant", 5)
+    pers = tune_persistent(run, ctx, candidates, steps, log)
+    trials = [{"asm_variant": 7, "ms": ms7}] + [dict(asm_variant=5, **t) for t in pers["trials"]]
+    if ms7 <= pers["ms"]:
+        ctx.set_option("asm_variant", 7)
+        best = {"asm_variant": 7, "ms": ms7}
+    else:
+        best = {"asm_variant": 5, "chunk_elems": pers["chunk_elems"],
+                "persistent_waves": pers["persistent_waves"], "ms": pers["ms"]}
     best["trials"] = trials
     return best

@@ -3,9 +3,12 @@
 The reference assembles one Model per call (fem::assemble_linear_static,
 src/fem/assembly.cpp:225-257; fem::assemble_material_tangent, :385-423). A batch of
 independent meshes is concatenated here into one block-diagonal model: node ranges
-are disjoint, each family's elements of every mesh form one block, so one persistent
-launch (asm_variant 5) covers every element of every mesh and one gather launch every
-row. Each mesh's blocks keep their order inside the family runs, so every node still
+are disjoint, each family's elements of every mesh form one block, so one element
+launch per family (asm_variant 7, upper node-pair blocks; the gather order goes mesh by
+mesh, each mesh's nodes in its own locality order) or one persistent launch for all
+families (asm_variant 5) covers every element of every mesh, and one gather launch every
+row (autotune.tune_batch picks between them by measurement). Each mesh's blocks keep
+their order inside the family runs, so every node still
 sums its incident elements in its own mesh's element order (the order of the
 reference's duplicate summation) when the mesh lists its blocks in family order
 C3D8, C3D20, C3D20R, C3D6 (meshgen.mixed_box does). Per-mesh results are views:
@@ -123,7 +126,7 @@ class BatchStep:
     records: dicts with "model", "j2" (bool), "u" (n_cases, n_nodes, 3) and, for J2
     meshes, "history" (13, n_points) — the shape meshgen.batch_meshes returns."""
 
-    def __init__(self, records, ctx: Optional[fem.Context] = None, variant: int = 5):
+    def __init__(self, records, ctx: Optional[fem.Context] = None, variant: int = 7):
         self.ctx = ctx or fem.context()
         self.ctx.set_option("asm_variant", variant)
         dev = self.ctx.device

@@ -22,6 +22,7 @@
 //   scatter                src/fem/assembly.cpp:76-110 (SPC-only transform)
 #include <algorithm>
 #include <climits>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -3468,8 +3469,11 @@ void prepare_arenas(const cxg_pattern* p, bool element_forces) {
 // box, each swept along the longest extent in layers a quarter element thick: the nodes
 // of one element, which read each other's upper blocks, come close together in the
 // order, so the second read of a block finds it in L2. The element size is estimated as
-// the edge of the bounding box volume shared out over the elements. Any order gives the
-// same values (each node's rows are summed by one wave in element order).
+// the edge of the bounding box volume shared out over the elements. A mesh of several
+// bodies (connected components of the node graph, such as the independent meshes of a
+// batch, which may share one region of space) is ordered body by body, in the order of
+// each body's smallest node, each body by its own box and element size. Any order gives
+// the same values (each node's rows are summed by one wave in element order).
 void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords) {
   cxg_pattern* pm = const_cast<cxg_pattern*>(p);
   const int tile = ctx->gather_tile;
@@ -3479,65 +3483,125 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
   CXG_HIP(hipStreamIsCapturing(ctx->stream, &cs));
   if (cs != hipStreamCaptureStatusNone)
     throw Error(CXG_ERR_STATE, "the gather order is built before capture (assemble once first)");
-  const int32_t n = p->mesh->n_nodes;
+  const cxg_mesh* m = p->mesh;
+  const int32_t n = m->n_nodes;
   if (tile <= 0 || n == 0) {
     pm->gorder.reset();
     pm->gorder_tile = tile;
     return;
   }
   std::vector<double> x(size_t(3) * size_t(n));
+  std::vector<int32_t> conn(size_t(m->conn.n));
   CXG_HIP(hipStreamSynchronize(ctx->stream));
   CXG_HIP(hipMemcpy(x.data(), coords, x.size() * sizeof(double), hipMemcpyDeviceToHost));
-  double lo[3], hi[3];
-  for (int c = 0; c < 3; ++c) lo[c] = hi[c] = x[size_t(c)];
-  for (int32_t i = 0; i < n; ++i)
+  if (!conn.empty())
+    CXG_HIP(hipMemcpy(conn.data(), m->conn.p, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+
+  // Components: union-find with the smaller root kept, so each root is its component's
+  // smallest node; numbered in ascending root order.
+  std::vector<int32_t> comp(static_cast<size_t>(n));
+  for (int32_t i = 0; i < n; ++i) comp[size_t(i)] = i;
+  const auto find = [&](int32_t a) {
+    while (comp[size_t(a)] != a) {
+      comp[size_t(a)] = comp[size_t(comp[size_t(a)])];
+      a = comp[size_t(a)];
+    }
+    return a;
+  };
+  for (const cxg_block_dev& B : m->blocks)
+    for (int64_t e = 0; e < B.n_elem; ++e) {
+      const int32_t* c = conn.data() + B.conn_offset + e * B.nn;
+      int32_t r0 = find(c[0]);
+      for (int k = 1; k < B.nn; ++k) {
+        const int32_t r = find(c[k]);
+        if (r < r0) {
+          comp[size_t(r0)] = r;
+          r0 = r;
+        } else if (r > r0) {
+          comp[size_t(r)] = r0;
+        }
+      }
+    }
+  for (int32_t i = 0; i < n; ++i) comp[size_t(i)] = find(i);  // every node to its root
+  int32_t n_comp = 0;
+  for (int32_t i = 0; i < n; ++i)  // a root precedes its nodes, so comp[root] is its number
+    comp[size_t(i)] = comp[size_t(i)] == i ? n_comp++ : comp[size_t(comp[size_t(i)])];
+
+  // Each body's frame: sweep axis last, tile width and layer thickness from its own
+  // bounding box and element count.
+  struct Frame {
+    double lo[3], hi[3];
+    int64_t ne = 0;
+    int ax[3] = {0, 1, 2};
+    double w = 1.0, q = 1.0;
+    int64_t ntv = 1, ns = 1, nv = 1, nu = 1;
+  };
+  std::vector<Frame> F(static_cast<size_t>(n_comp));
+  for (Frame& f : F)
     for (int c = 0; c < 3; ++c) {
-      lo[c] = std::min(lo[c], x[3 * size_t(i) + c]);
-      hi[c] = std::max(hi[c], x[3 * size_t(i) + c]);
+      f.lo[c] = std::numeric_limits<double>::infinity();
+      f.hi[c] = -std::numeric_limits<double>::infinity();
     }
-  int ax[3] = {0, 1, 2};  // the sweep axis last
-  std::sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] < hi[b] - lo[b]; });
-  const double ext = hi[ax[2]] - lo[ax[2]];
-  double vol = 1.0;
-  int dims = 0;
-  for (int c = 0; c < 3; ++c)
-    if (hi[c] - lo[c] > 1e-12 * ext) {
-      vol *= hi[c] - lo[c];
-      ++dims;
+  for (int32_t i = 0; i < n; ++i) {
+    Frame& f = F[size_t(comp[size_t(i)])];
+    for (int c = 0; c < 3; ++c) {
+      f.lo[c] = std::min(f.lo[c], x[3 * size_t(i) + c]);
+      f.hi[c] = std::max(f.hi[c], x[3 * size_t(i) + c]);
     }
-  const double ne = double(std::max<int64_t>(p->mesh->n_elem, 1));
-  double h = dims > 0 ? std::pow(vol / ne, 1.0 / dims) : 1.0;
-  if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
-  const double w = tile * h, q = 0.25 * h;
+  }
+  for (const cxg_block_dev& B : m->blocks)
+    for (int64_t e = 0; e < B.n_elem; ++e) ++F[size_t(comp[size_t(conn[size_t(B.conn_offset + e * B.nn)])])].ne;
   const auto cell = [](double d, double step) { return int64_t(std::floor(d / step)); };
-  // key (tile column, layer, v, u) of every node; the cells are >= 0 and at most those
-  // of the bounding box's far corner
-  const int64_t ntv = cell(hi[ax[1]] - lo[ax[1]], w) + 1;
-  const int64_t nt = (cell(hi[ax[0]] - lo[ax[0]], w) + 1) * ntv;
-  const int64_t ns = cell(hi[ax[2]] - lo[ax[2]], q) + 1;
-  const int64_t nv = cell(hi[ax[1]] - lo[ax[1]], q) + 1;
-  const int64_t nu = cell(hi[ax[0]] - lo[ax[0]], q) + 1;
+  unsigned __int128 span_max = 0;
+  for (Frame& f : F) {
+    std::sort(f.ax, f.ax + 3, [&](int a, int b) { return f.hi[a] - f.lo[a] < f.hi[b] - f.lo[b]; });
+    const double ext = f.hi[f.ax[2]] - f.lo[f.ax[2]];
+    double vol = 1.0;
+    int dims = 0;
+    for (int c = 0; c < 3; ++c)
+      if (f.hi[c] - f.lo[c] > 1e-12 * ext) {
+        vol *= f.hi[c] - f.lo[c];
+        ++dims;
+      }
+    const double ne = double(std::max<int64_t>(f.ne, 1));
+    double h = dims > 0 ? std::pow(vol / ne, 1.0 / dims) : 1.0;
+    if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
+    f.w = tile * h;
+    f.q = 0.25 * h;
+    // key (tile column, layer, v, u) of every node; the cells are >= 0 and at most those
+    // of the bounding box's far corner
+    f.ntv = cell(f.hi[f.ax[1]] - f.lo[f.ax[1]], f.w) + 1;
+    const int64_t nt = (cell(f.hi[f.ax[0]] - f.lo[f.ax[0]], f.w) + 1) * f.ntv;
+    f.ns = cell(f.hi[f.ax[2]] - f.lo[f.ax[2]], f.q) + 1;
+    f.nv = cell(f.hi[f.ax[1]] - f.lo[f.ax[1]], f.q) + 1;
+    f.nu = cell(f.hi[f.ax[0]] - f.lo[f.ax[0]], f.q) + 1;
+    const unsigned __int128 span =
+        static_cast<unsigned __int128>(nt) * uint64_t(f.ns) * uint64_t(f.nv) * uint64_t(f.nu);
+    span_max = std::max(span_max, span);
+  }
   const auto cells = [&](int32_t i, int64_t c[4]) {
+    const Frame& f = F[size_t(comp[size_t(i)])];
     const double* xi = &x[3 * size_t(i)];
-    const double du = xi[ax[0]] - lo[ax[0]], dv = xi[ax[1]] - lo[ax[1]], ds = xi[ax[2]] - lo[ax[2]];
-    c[0] = cell(du, w) * ntv + cell(dv, w);
-    c[1] = cell(ds, q);
-    c[2] = cell(dv, q);
-    c[3] = cell(du, q);
+    const double du = xi[f.ax[0]] - f.lo[f.ax[0]], dv = xi[f.ax[1]] - f.lo[f.ax[1]],
+                 ds = xi[f.ax[2]] - f.lo[f.ax[2]];
+    c[0] = cell(du, f.w) * f.ntv + cell(dv, f.w);
+    c[1] = cell(ds, f.q);
+    c[2] = cell(dv, f.q);
+    c[3] = cell(du, f.q);
   };
   std::vector<int32_t> ord(static_cast<size_t>(n));
-  const unsigned __int128 span = static_cast<unsigned __int128>(nt) * uint64_t(ns) * uint64_t(nv) * uint64_t(nu);
-  if (span < (static_cast<unsigned __int128>(1) << 63)) {
-    // one 64-bit key per node, ordered by a stable LSD radix sort over 16-bit digits: ties
-    // keep ascending node order
+  if (span_max < (static_cast<unsigned __int128>(1) << 63)) {
+    // one 64-bit key per node within its body, ordered by a stable LSD radix sort over
+    // 16-bit digits (ties keep ascending node order), then a stable counting sort by body
     std::vector<uint64_t> k(static_cast<size_t>(n)), k2(static_cast<size_t>(n));
     std::vector<int32_t> o2(static_cast<size_t>(n));
     uint64_t kmax = 0;
     for (int32_t i = 0; i < n; ++i) {
+      const Frame& f = F[size_t(comp[size_t(i)])];
       int64_t c[4];
       cells(i, c);
-      k[size_t(i)] = ((uint64_t(c[0]) * uint64_t(ns) + uint64_t(c[1])) * uint64_t(nv) + uint64_t(c[2])) *
-                         uint64_t(nu) + uint64_t(c[3]);
+      k[size_t(i)] = ((uint64_t(c[0]) * uint64_t(f.ns) + uint64_t(c[1])) * uint64_t(f.nv) + uint64_t(c[2])) *
+                         uint64_t(f.nu) + uint64_t(c[3]);
       kmax = std::max(kmax, k[size_t(i)]);
       ord[size_t(i)] = i;
     }
@@ -3559,22 +3623,39 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
       k.swap(k2);
       ord.swap(o2);
     }
+    if (n_comp > 1) {
+      std::vector<size_t> start(size_t(n_comp) + 1, 0);
+      for (int32_t i = 0; i < n; ++i) ++start[size_t(comp[size_t(i)]) + 1];
+      for (int32_t c = 0; c < n_comp; ++c) start[size_t(c) + 1] += start[size_t(c)];
+      for (int32_t j = 0; j < n; ++j) {
+        const int32_t i = ord[size_t(j)];
+        o2[start[size_t(comp[size_t(i)])]++] = i;
+      }
+      ord.swap(o2);
+    }
   } else {
     struct Key {
-      int64_t c[4];
+      int64_t c[5];
       int32_t i;
     };
     std::vector<Key> key(static_cast<size_t>(n));
     for (int32_t i = 0; i < n; ++i) {
-      cells(i, key[size_t(i)].c);
+      key[size_t(i)].c[0] = comp[size_t(i)];
+      cells(i, key[size_t(i)].c + 1);
       key[size_t(i)].i = i;
     }
     std::sort(key.begin(), key.end(), [](const Key& a, const Key& b) {
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 5; ++j)
         if (a.c[j] != b.c[j]) return a.c[j] < b.c[j];
       return a.i < b.i;
     });
     for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = key[size_t(i)].i;
+  }
+  // the gather indexes node data by this order: it must be a permutation of the nodes
+  std::vector<uint8_t> seen(static_cast<size_t>(n), 0);
+  for (const int32_t v : ord) {
+    if (v < 0 || v >= n || seen[size_t(v)]) throw Error(CXG_ERR_STATE, "gather order is not a permutation of the nodes");
+    seen[size_t(v)] = 1;
   }
   pm->gorder.alloc(size_t(n));
   CXG_HIP(hipMemcpy(pm->gorder.p, ord.data(), ord.size() * sizeof(int32_t), hipMemcpyHostToDevice));

@@ -1,4 +1,5 @@
-"""Batches of uneven meshes with mixed families in one persistent launch (asm_variant 5).
+"""Batches of uneven meshes with mixed families: one persistent launch (asm_variant 5) or
+one launch per family (asm_variant 4 full rows, 7 upper blocks in the per-mesh gather order).
 
 A small instance of the batched workload (meshgen.batch_meshes: mixed C3D8 / C3D20 /
 C3D6 boxes, per-mesh E and nu, half of the meshes J2 with seeded history, 4 load
@@ -9,7 +10,8 @@ on that mesh alone:
   K, f_int                rel-L2 <= 1e-12
   trial history           rel-L2 <= 1e-10
 The work-item size and the persistent grid size are swept; graph replays of the whole
-step are compared bitwise with the eager step.
+step are compared bitwise with the eager step, and the variants bitwise with each other
+(every entry is summed in the same element order whatever the launch shape).
 """
 import numpy as np
 import pytest
@@ -102,23 +104,70 @@ def test_persistent_batch_matches_oracle(gpu, records, expected, chunk, waves):
         ctx.set_option("persistent_waves", 8)
 
 
-def test_batch_per_family_kernels_match_oracle(gpu, records, expected):
+@pytest.mark.parametrize("variant", [4, 7])
+def test_batch_per_family_kernels_match_oracle(gpu, records, expected, variant):
     from calculixpp_amd.batch import BatchStep
     ctx = gpu.context()
     try:
-        step = BatchStep(records, ctx, variant=4)
+        step = BatchStep(records, ctx, variant=variant)
         step.run()
         _check(step, records, expected)
     finally:
         ctx.set_option("asm_variant", 7)
 
 
-def test_batch_graph_replays_match_eager(gpu, records):
+def _outputs(step):
+    import torch
+    torch.cuda.synchronize()
+    return ([step.lin_vals.clone(), step.lin_f.clone()] +
+            [x.clone() for v, f, mp in zip(step.j2_vals, step.j2_f, step.j2_mp) for x in (v, f, mp.trial)])
+
+
+def test_batch_variants_same_bits(gpu, records):
+    """Persistent (5), per-family full rows (4) and per-family upper blocks in the
+    per-mesh gather order (7, also in node order with gather_tile 0) give the same bits."""
     import torch
     from calculixpp_amd.batch import BatchStep
     ctx = gpu.context()
     try:
-        step = BatchStep(records, ctx, variant=5)
+        outs = []
+        for variant, tile in [(5, 8), (4, 8), (7, 8), (7, 0)]:
+            ctx.set_option("gather_tile", tile)
+            step = BatchStep(records, ctx, variant=variant)
+            step.run()
+            outs.append(_outputs(step))
+            del step
+        for o in outs[1:]:
+            assert all(torch.equal(a, b) for a, b in zip(outs[0], o))
+    finally:
+        ctx.set_option("asm_variant", 7)
+        ctx.set_option("gather_tile", 8)
+
+
+def test_tune_batch_choice_matches_oracle(gpu, records, expected):
+    from calculixpp_amd import autotune
+    from calculixpp_amd.batch import BatchStep
+    ctx = gpu.context()
+    try:
+        step = BatchStep(records, ctx)
+        best = autotune.tune_batch(step.run, ctx, candidates=[(4, 8), (16, 2)], steps=1)
+        assert best["asm_variant"] in (5, 7) and len(best["trials"]) == 3
+        assert ctx.options["asm_variant"] == best["asm_variant"]
+        step.run()
+        _check(step, records, expected)
+    finally:
+        ctx.set_option("asm_variant", 7)
+        ctx.set_option("chunk_elems", 4)
+        ctx.set_option("persistent_waves", 8)
+
+
+@pytest.mark.parametrize("variant", [5, 7])
+def test_batch_graph_replays_match_eager(gpu, records, variant):
+    import torch
+    from calculixpp_amd.batch import BatchStep
+    ctx = gpu.context()
+    try:
+        step = BatchStep(records, ctx, variant=variant)
         step.run()
         torch.cuda.synchronize()
         ref_lin = (step.lin_vals.clone(), step.lin_f.clone())

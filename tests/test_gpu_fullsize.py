@@ -252,13 +252,16 @@ def batch_records():
     return meshgen.batch_meshes(512)
 
 
+@pytest.mark.parametrize("variant", [5, 7])
 @pytest.mark.parametrize("mix", ["bench", "all_linear"])
-def test_batch_meshes_past_2_31(fem, port, batch_records, mix):
+def test_batch_meshes_past_2_31(fem, port, batch_records, mix, variant):
     """The 512-mesh batch: per-mesh CSR and internal forces of the meshes stored past
     2^31 entries of the arena (int64 IncRecord offsets) and of the CSR values. "bench" is
     the benchmark's mix (half the meshes J2: 2.0 G linear nonzeros, so only arena offsets
     pass 2^31); "all_linear" takes every mesh's linear step (3.9 G nonzeros), whose later
-    meshes' CSR values start past 2^31."""
+    meshes' CSR values start past 2^31. Variant 5 is the persistent launch, 7 the
+    per-family launches with upper blocks, which sit at the start of the same per-element
+    slots (9 n^2 doubles), so the arena offsets are those of variant 5."""
     import torch
     from calculixpp_amd.batch import BatchStep
     from calculixpp_amd.model import NODES
@@ -269,7 +272,7 @@ def test_batch_meshes_past_2_31(fem, port, batch_records, mix):
             m = r["model"]
             if m.j2_id is not None:
                 r["model"] = type(m)(m.coords, m.blocks, m.E, m.nu)
-    step = BatchStep(recs, fem.context(), variant=5)
+    step = BatchStep(recs, fem.context(), variant=variant)
     try:
         step.run()
         torch.cuda.synchronize()
@@ -330,4 +333,5 @@ def test_batch_meshes_past_2_31(fem, port, batch_records, mix):
         print("batch checks:", checked, "linear nnz", lin_nnz)
     finally:
         del step
+        fem.context().set_option("asm_variant", 7)
         _free_device()
