@@ -2060,20 +2060,44 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
   constexpr int NP = NN * (NN + 1) / 2;
   constexpr int PPL = (NP + kWave - 1) / kWave;
   constexpr int ND = 3 * NN;
-  // a lane owns node a and up to PPL consecutive partners b (see decode_group)
+  // TC (upper-block arena, four pairs per lane, even node count): a lane owns a 2x2 tile
+  // of node couples (2s, 2s+1) x (2t, 2t+1), s <= t (of a diagonal tile the lower pair
+  // (2s+1, 2s) is computed and not stored), and reads each couple's h and t with three
+  // 16-byte LDS reads: 24 doubles per Gauss point against 30 for (a, b0 .. b0+3).
+  // Otherwise (and in the full-row persistent kernel, where the tiles' registers spill) a
+  // lane owns node a and up to PPL consecutive partners b (see decode_group). Either way
+  // every pair is summed by j2_pair_add in the same order with the same operands.
+  constexpr bool TC = UP && PPL == 4 && NN % 2 == 0;
   int pa[PPL], pb[PPL];
-  int ga, gb0;
-  decode_group<NN, PPL>(lane, ga, gb0);
+  int ga = 0, gb0 = 0;
+  int cs = 0, ct = 0;  // TC: the lane's couples (clamped for idle lanes)
+  if constexpr (TC) {
+    int s_ = -1, t_ = -1;
+    decode_group<NN / 2, 1>(lane, s_, t_);
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      const int a = 2 * s_ + t / 2, b = 2 * t_ + t % 2;
+      const bool ok = s_ >= 0 && a <= b;
+      pa[t] = ok ? a : -1;
+      pb[t] = ok ? b : -1;
+    }
+    cs = s_ >= 0 ? s_ : 0;
+    ct = s_ >= 0 ? t_ : 0;
+  } else {
+    decode_group<NN, PPL>(lane, ga, gb0);
+#pragma unroll
+    for (int t = 0; t < PPL; ++t) {
+      const bool ok = ga >= 0 && gb0 + t < NN;
+      pa[t] = ok ? ga : -1;
+      pb[t] = ok ? gb0 + t : -1;
+    }
+  }
   const int ga_c = ga >= 0 ? ga : 0;
   double K[PPL][9];
 #pragma unroll
-  for (int t = 0; t < PPL; ++t) {
-    const bool ok = ga >= 0 && gb0 + t < NN;
-    pa[t] = ok ? ga : -1;
-    pb[t] = ok ? gb0 + t : -1;
+  for (int t = 0; t < PPL; ++t)
 #pragma unroll
     for (int i = 0; i < 9; ++i) K[t][i] = 0.0;
-  }
   // element force f_k += B(h_k)^T (rs sigma) per point (sq = rs sigma in Voigt order):
   // lane 3k + c sums component c of node k, g[c] sq[c] first, then the two shear terms
   // with explicit multiply-adds, its operands picked by lane
@@ -2101,7 +2125,26 @@ __device__ __forceinline__ void j2_element_matrix(const J2Elem<NN, NGP>& P, W& w
         f += __builtin_fma(g[g2i], sq[s2i], __builtin_fma(g[g1i], sq[s1i], g[fc] * sq[fc]));
       }
 #pragma unroll 1
-    for (int qq = 0; qq < qn; ++qq) {
+    for (int qq = 0; qq < qn && TC; ++qq) {
+      const double* Tq = P.T + 9 * (q0 + qq);
+      // couples: 6 contiguous doubles, 16-byte aligned (h and t are, and 2 nodes are 48
+      // bytes)
+      const f64x2* hs = reinterpret_cast<const f64x2*>(w.c.h + (qq * NN + 2 * cs) * 3);
+      const f64x2* hc = reinterpret_cast<const f64x2*>(w.c.h + (qq * NN + 2 * ct) * 3);
+      const f64x2* ts = reinterpret_cast<const f64x2*>(w.c.t + (qq * NN + 2 * cs) * 3);
+      const f64x2* tc = reinterpret_cast<const f64x2*>(w.c.t + (qq * NN + 2 * ct) * 3);
+      const f64x2 s0 = hs[0], s1 = hs[1], s2 = hs[2], c0 = hc[0], c1 = hc[1], c2 = hc[2];
+      const f64x2 u0 = ts[0], u1 = ts[1], u2 = ts[2], v0 = tc[0], v1 = tc[1], v2 = tc[2];
+      const double A[2][3] = {{s0.x, s0.y, s1.x}, {s1.y, s2.x, s2.y}};
+      const double B[2][3] = {{c0.x, c0.y, c1.x}, {c1.y, c2.x, c2.y}};
+      const double TA[2][3] = {{u0.x, u0.y, u1.x}, {u1.y, u2.x, u2.y}};
+      const double TB[2][3] = {{v0.x, v0.y, v1.x}, {v1.y, v2.x, v2.y}};
+#pragma unroll
+      for (int t = 0; t < PPL; ++t)
+        j2_pair_add(K[t], A[t / 2], B[t % 2], TA[t / 2], TB[t % 2], Tq[0], Tq[1], Tq[2]);
+    }
+#pragma unroll 1
+    for (int qq = 0; qq < qn && !TC; ++qq) {
       const double* Tq = P.T + 9 * (q0 + qq);
       const double* ha = w.c.h + (qq * NN + ga_c) * 3;
       const double* ta = w.c.t + (qq * NN + ga_c) * 3;
@@ -3867,7 +3910,7 @@ void launch_assemble_j2(cxg_context* ctx, const cxg_pattern* p, const double* co
       const bool bup = up && B.nn >= ctx->up_min_nodes;  // this family's arena layout
       LaunchTimer timer(ctx, std::string(bup ? "elem_j2_up_" : "elem_j2_") + type_name(B.type), B.n_elem);
 #define J2K(NN, NGP, QC, UPV)                                                                   \
-  hipLaunchKernelGGL((k_elem_j2<NN, NGP, QC, UPV>),                                             \
+  hipLaunchKernelGGL((k_elem_j2<NN, NGP, (NN == 20 && NGP == 27 ? 9 : QC), UPV>),                                             \
                      dim3(xcd_grid(ctx, (B.n_elem + j2_epw(NN, NGP) - 1) / j2_epw(NN, NGP))), dim3(kWave), \
                      0, ctx->stream, conn, B.elem_offset, B.n_elem, B.gp_offset, m->n_points,   \
                      T.dN, T.w, coords, E, nu, mat_id, dmats, u, committed, trial, kp, fe,      \
