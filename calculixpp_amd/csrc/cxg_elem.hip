@@ -2586,8 +2586,19 @@ struct ForceLds {
 template <int NN, int NGP>
 constexpr bool forces_dn_lds() { return NGP <= 8 && 2 * NGP * NN * 3 <= 512; }
 
-template <int NN, int NGP, int WAVES>
-__global__ __launch_bounds__(kWave, WAVES) void k_elem_forces(
+// Orders this wave's LDS accesses (the wave's lanes exchange data through the LDS; the
+// other waves of the workgroup work on other elements and are not waited for).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// WPB waves per workgroup, each on its own EPW elements: the staged natural-derivative
+// table is loaded once per workgroup and shared; past that load the waves synchronise
+// only with themselves.
+template <int NN, int NGP, int WAVES, int WPB>
+__global__ __launch_bounds__(kWave * WPB, WAVES) void k_elem_forces(
     const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN_g,
     const double* __restrict__ wq, const double* __restrict__ coords,
     const double* __restrict__ E, const double* __restrict__ nu, const double* __restrict__ u,
@@ -2596,15 +2607,21 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_forces(
   constexpr int EPW = rec_epw(NN, NGP);
   constexpr bool DNL = forces_dn_lds<NN, NGP>();
   constexpr int NDN = DNL ? 2 * NGP * NN * 3 : 1;
-  __shared__ ForceLds<NN, NGP> L[EPW];
+  __shared__ ForceLds<NN, NGP> LW[WPB][EPW];
   __shared__ double dn_lds[NDN];
-  const int lane = threadIdx.x;
-  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
-  if (grp < 0) return;
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int64_t n_grp = (n_elem + EPW - 1) / EPW;
+  const int64_t blk = xcd_index(blockIdx.x, (n_grp + WPB - 1) / WPB, slab);
+  if (blk < 0) return;  // the whole workgroup
+  const int64_t grp = blk * WPB + wv;
   const int64_t le0 = grp * EPW;
-  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
-  if constexpr (DNL)
-    for (int i = lane; i < NDN; i += kWave) dn_lds[i] = dN_g[i];
+  // a wave past the last group keeps no element but takes part in the workgroup barrier
+  const int ne = grp < n_grp ? int(n_elem - le0 < EPW ? n_elem - le0 : EPW) : 0;
+  ForceLds<NN, NGP>* L = LW[wv];
+  if constexpr (DNL) {
+    for (int i = threadIdx.x; i < NDN; i += kWave * WPB) dn_lds[i] = dN_g[i];
+    __syncthreads();
+  }
   const double* dN = DNL ? dn_lds : dN_g;
   for (int idx = lane; idx < ne * NN; idx += kWave) {
     const int e = idx / NN, k = idx - e * NN;
@@ -2615,7 +2632,7 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_forces(
       L[e].u[3 * k + c] = u[nd + c];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
   const bool pt_lane = lane < ne * NGP;
   const int e = pt_lane ? lane / NGP : 0, q = pt_lane ? lane - e * NGP : 0;
@@ -2637,14 +2654,14 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_forces(
     lame(E[eg], nu[eg], lam, mu);
     iso_stress(lam, mu, e6, sg);
   }
-  __syncthreads();  // x and u are read: J^-1 and det J w sigma take their place
+  wave_lds_sync();  // x and u are read: J^-1 and det J w sigma take their place
   if (pt_lane) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
 #pragma unroll
     for (int i = 0; i < 6; ++i) R.S[6 * q + i] = sc * sg[i];
   }
-  __syncthreads();
+  wave_lds_sync();
   if (lane < ne * NN) {
     const int e = lane / NN, k = lane - e * NN;
     const ForceLds<NN, NGP>& R = L[e];
@@ -2766,8 +2783,11 @@ struct VjpLds {
   };
 };
 
-template <int NN, int NGP>
-__global__ __launch_bounds__(kWave) void k_vjp(
+// WPB waves per workgroup, each on its own EPW elements, sharing the staged table (as
+// k_elem_forces; 2 waves keep the residency of single-wave workgroups, whose LDS each
+// held a copy of the table).
+template <int NN, int NGP, int WPB>
+__global__ __launch_bounds__(kWave * WPB) void k_vjp(
     const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN_g,
     const double* __restrict__ wq, const double* __restrict__ coords,
     const double* __restrict__ E, const double* __restrict__ nu, const double* __restrict__ u,
@@ -2779,15 +2799,21 @@ __global__ __launch_bounds__(kWave) void k_vjp(
   // small natural-derivative tables in the LDS, as k_elem_forces
   constexpr bool DNL = forces_dn_lds<NN, NGP>();
   constexpr int NDN = DNL ? 2 * NGP * NN * 3 : 1;
-  __shared__ VjpLds<NN, NGP> L[EPW];
+  __shared__ VjpLds<NN, NGP> LW[WPB][EPW];
   __shared__ double dn_lds[NDN];
-  const int lane = threadIdx.x;
-  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
-  if (grp < 0) return;
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int64_t n_grp = (n_elem + EPW - 1) / EPW;
+  const int64_t blk = xcd_index(blockIdx.x, (n_grp + WPB - 1) / WPB, slab);
+  if (blk < 0) return;  // the whole workgroup
+  const int64_t grp = blk * WPB + wv;
   const int64_t le0 = grp * EPW;
-  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
-  if constexpr (DNL)
-    for (int i = lane; i < NDN; i += kWave) dn_lds[i] = dN_g[i];
+  // a wave past the last group keeps no element but takes part in the workgroup barrier
+  const int ne = grp < n_grp ? int(n_elem - le0 < EPW ? n_elem - le0 : EPW) : 0;
+  VjpLds<NN, NGP>* L = LW[wv];
+  if constexpr (DNL) {
+    for (int i = threadIdx.x; i < NDN; i += kWave * WPB) dn_lds[i] = dN_g[i];
+    __syncthreads();
+  }
   const double* dN = DNL ? dn_lds : dN_g;
   for (int idx = lane; idx < ne * NN; idx += kWave) {
     const int e = idx / NN, k = idx - e * NN;
@@ -2798,7 +2824,7 @@ __global__ __launch_bounds__(kWave) void k_vjp(
       L[e].w[3 * k + c] = w[3 * nd + c];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // the point phase's results, held across the barrier after which they overlay x, u, w
   const bool pt_lane = lane < ne * NGP;
   const int pe = pt_lane ? lane / NGP : 0, pq_i = pt_lane ? lane - pe * NGP : 0;
@@ -2838,7 +2864,7 @@ __global__ __launch_bounds__(kWave) void k_vjp(
       }
     for (int i = 0; i < 6; ++i) Sw6[i] = s * sw[i];
   }
-  __syncthreads();  // x, u and w are read: the point results take their place
+  wave_lds_sync();  // x, u and w are read: the point results take their place
   if (pt_lane) {
     VjpLds<NN, NGP>& R = L[pe];
     for (int i = 0; i < 9; ++i) R.P[9 * pq_i + i] = Pm[i];
@@ -2846,7 +2872,7 @@ __global__ __launch_bounds__(kWave) void k_vjp(
     for (int i = 0; i < 6; ++i) R.S[6 * pq_i + i] = Sw6[i];
     R.phi[pq_i] = pq;
   }
-  __syncthreads();
+  wave_lds_sync();
   if (lane < ne) {
     double phi = 0.0;
     for (int q = 0; q < NGP; ++q) phi += L[lane].phi[q];
@@ -3367,6 +3393,14 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
   });
 }
 
+// Waves per workgroup of k_elem_forces and k_vjp, and the workgroups for n elements of
+// epw per wave.
+constexpr int kForceWaves = 4;
+constexpr int kVjpWaves = 2;
+int64_t wave_blocks(int64_t n_elem, int epw, int wpb) {
+  return ((n_elem + epw - 1) / epw + wpb - 1) / wpb;
+}
+
 // Matrix-free f_int = sum_e Ke ue (stress.cpp:125-145): the element forces
 // sum_q det J w B^T sigma of k_gauss_fields (means off) into the element-force arena,
 // then summed per node in ascending element order.
@@ -3383,11 +3417,11 @@ void launch_internal_force(cxg_context* ctx, const cxg_mesh* m, const double* co
     const int32_t* conn = m->conn.p + B.conn_offset;
     LaunchTimer timer(ctx, std::string("internal_force_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_elem_forces<NN, NGP, 4>),                                               \
-                     dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
-                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
-                     nu, u, m->inc_pos.p + B.conn_offset, arena, ctx->flag_dev, B.n_elem,      \
-                     xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
+  hipLaunchKernelGGL((k_elem_forces<NN, NGP, 4, kForceWaves>),                                  \
+                     dim3(xcd_grid(ctx, wave_blocks(B.n_elem, rec_epw(NN, NGP), kForceWaves))), \
+                     dim3(kWave * kForceWaves), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, \
+                     coords, E, nu, u, m->inc_pos.p + B.conn_offset, arena, ctx->flag_dev,      \
+                     B.n_elem, xcd_slab(ctx, wave_blocks(B.n_elem, rec_epw(NN, NGP), kForceWaves)))
     CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
     CXG_HIP(hipGetLastError());
@@ -4058,12 +4092,12 @@ void launch_vjp(cxg_context* ctx, const cxg_mesh* m, const double* coords, const
     const int32_t* conn = m->conn.p + B.conn_offset;
     LaunchTimer timer(ctx, std::string("vjp_") + type_name(B.type), B.n_elem);
 #define LAUNCH(NN, NGP, UP, QC)                                                                 \
-  hipLaunchKernelGGL((k_vjp<NN, NGP>),                                                          \
-                     dim3(xcd_grid(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP))), \
-                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
-                     nu, u, w, m->inc_pos.p + B.conn_offset, m->vjp_dx.p, dE, m->rec_fe.p,      \
-                     ctx->flag_dev, B.n_elem,                                                   \
-                     xcd_slab(ctx, (B.n_elem + rec_epw(NN, NGP) - 1) / rec_epw(NN, NGP)))
+  hipLaunchKernelGGL((k_vjp<NN, NGP, kVjpWaves>),                                               \
+                     dim3(xcd_grid(ctx, wave_blocks(B.n_elem, rec_epw(NN, NGP), kVjpWaves))),    \
+                     dim3(kWave * kVjpWaves), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w,   \
+                     coords, E, nu, u, w, m->inc_pos.p + B.conn_offset, m->vjp_dx.p, dE,        \
+                     m->rec_fe.p, ctx->flag_dev, B.n_elem,                                      \
+                     xcd_slab(ctx, wave_blocks(B.n_elem, rec_epw(NN, NGP), kVjpWaves)))
     CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
     CXG_HIP(hipGetLastError());
