@@ -939,6 +939,114 @@ __global__ __launch_bounds__(kWave, WAVES) void k_elem_iso(
                              nu[elem_offset + le], dN, wq, coords, kp + le * (9 * NN * NN), flag);
 }
 
+// Elements of at most 8 nodes and Gauss points (C3D8, C3D6), upper blocks: EPW elements
+// per wave. The Jacobians of all their points are formed at once (one lane per element
+// and point; one element per wave leaves 56 of 64 lanes idle there for C3D8), then each
+// element's products, K_ab and staged store run as in elem_iso_body (TL = 0, one chunk
+// of points), with the same device code in the same order: the values are the same bits.
+constexpr int iso_multi_epw(int ngp) { return ngp >= 8 ? kWave / ngp : 16; }
+
+template <int NN, int NGP, int QC, int EPW>
+struct IsoMultiLds {
+  double x[EPW][NN * 3];
+  double Ai[EPW][NGP * 9];
+  double rs[EPW][NGP];
+  alignas(16) double h[QC * NN * 3];
+  alignas(16) double stage[kStageNodes * 3 * 3 * NN];
+};
+
+template <int NN, int NGP, int QC>
+__global__ __launch_bounds__(kWave) void k_elem_iso_multi(
+    const int32_t* __restrict__ conn, int64_t elem_offset, const double* __restrict__ dN,
+    const double* __restrict__ wq, const double* __restrict__ coords,
+    const double* __restrict__ E, const double* __restrict__ nu, double* __restrict__ kp,
+    int* __restrict__ flag, int64_t n_elem, int64_t slab, const double* __restrict__ u,
+    int64_t n_dof, int32_t n_cases, double* __restrict__ ue, int64_t ue_stride) {
+  constexpr int EPW = iso_multi_epw(NGP);
+  constexpr int NP = NN * (NN + 1) / 2;
+  static_assert(NN <= 8 && NGP <= QC && NP <= kWave && QC * NN <= kWave && EPW * NGP <= kWave,
+                "one pair and one (point, node) per lane, one chunk of points");
+  __shared__ IsoMultiLds<NN, NGP, QC, EPW> L;
+  const int lane = threadIdx.x;
+  const int64_t grp = xcd_index(blockIdx.x, (n_elem + EPW - 1) / EPW, slab);
+  if (grp < 0) return;
+  const int64_t le0 = grp * EPW;
+  const int ne = int(n_elem - le0 < EPW ? n_elem - le0 : EPW);
+  for (int idx = lane; idx < ne * NN; idx += kWave) {
+    const int e = idx / NN, k = idx - e * NN;
+    const int64_t ck = (le0 + e) * NN + k;
+    const int64_t nd = conn[ck];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) L.x[e][3 * k + c] = coords[3 * nd + c];
+    if (ue)
+      for (int cs = 0; cs < n_cases; ++cs)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ue[cs * ue_stride + 3 * ck + j] = u[cs * n_dof + 3 * nd + j];
+  }
+  __syncthreads();
+  if (lane < ne * NGP) {
+    const int e = lane / NGP, q = lane - NGP * (lane / NGP);
+    jacobian_inverse<NN, dn_stride<NGP>()>(L.x[e], dn_point<NN, NGP>(dN, q), wq[q],
+                                           L.Ai[e] + 9 * q, L.rs[e][q], flag);
+  }
+  __syncthreads();
+  // the lane's pair (a, b) and (point, node) of the gradient pass, as elem_iso_body
+  int ga, gb0;
+  decode_group<NN, 1>(lane, ga, gb0);
+  int pa[1], pb[1];
+  pa[0] = ga >= 0 && gb0 < NN ? ga : -1;
+  pb[0] = ga >= 0 && gb0 < NN ? gb0 : -1;
+  const int ga_c = ga >= 0 ? ga : 0;
+  const int b_c = gb0 < NN ? (gb0 >= 0 ? gb0 : 0) : NN - 1;
+  const int cq = lane / NN, cn = lane - NN * (lane / NN);
+  double d[3] = {0.0, 0.0, 0.0};
+  if (lane < QC * NN && cq < NGP) {
+    const double* p = dN + (cq * NN + cn) * 3;
+    d[0] = p[0];
+    d[1] = p[1];
+    d[2] = p[2];
+  }
+  for (int e = 0; e < ne; ++e) {
+    if (lane < QC * NN) {
+      double hv[3] = {0.0, 0.0, 0.0};
+      if (cq < NGP) scaled_gradient(L.Ai[e] + 9 * cq, L.rs[e][cq], d, hv);
+      L.h[3 * lane + 0] = hv[0];
+      L.h[3 * lane + 1] = hv[1];
+      L.h[3 * lane + 2] = hv[2];
+    }
+    __syncthreads();
+    double M[1][9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M[0][i] = 0.0;
+#pragma unroll 1
+    for (int qq = 0; qq < QC; ++qq) {
+      const double* ha = L.h + (qq * NN + ga_c) * 3;
+      const double* hb = L.h + (qq * NN + b_c) * 3;
+      const double av[3] = {ha[0], ha[1], ha[2]}, bv[3] = {hb[0], hb[1], hb[2]};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[0][3 * i + j] = __builtin_fma(av[i], bv[j], M[0][3 * i + j]);
+    }
+    double lam, mu;
+    lame(E[elem_offset + le0 + e], nu[elem_offset + le0 + e], lam, mu);
+    {
+      double* m = M[0];
+      const double tr = mu * (m[0] + m[4] + m[8]);
+      double K[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          K[3 * i + j] = __builtin_fma(lam, m[3 * i + j], __builtin_fma(mu, m[3 * j + i], i == j ? tr : 0.0));
+#pragma unroll
+      for (int i = 0; i < 9; ++i) m[i] = K[i];
+    }
+    // ends with a barrier: the next element's gradients may overwrite h
+    store_upper_blocks<NN, 1>(L.stage, pa, pb, M, kp + (le0 + e) * (9 * NN * NN), lane);
+  }
+}
+
 // ---- K1, gradient form (C3D20R) -----------------------------------------------------
 // Instead of its upper node-pair blocks the element's slot holds what they are made of:
 // slot[0..1] = lambda, mu and the scaled gradients h_{k,c}(q) = sqrt(det J w)
@@ -3344,6 +3452,13 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
                      dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
                      nu, kp, ctx->flag_dev, B.n_elem, xcd_slab(ctx, B.n_elem), u, n_dof,        \
                      n_cases, ue ? ue + 3 * B.conn_offset : nullptr, ue_stride)
+#define MULTI(NN, NGP, QC)                                                                      \
+  hipLaunchKernelGGL((k_elem_iso_multi<NN, NGP, QC>),                                            \
+                     dim3(xcd_grid(ctx, (B.n_elem + iso_multi_epw(NGP) - 1) / iso_multi_epw(NGP))), \
+                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, T.dN, T.w, coords, E,    \
+                     nu, kp, ctx->flag_dev, B.n_elem,                                           \
+                     xcd_slab(ctx, (B.n_elem + iso_multi_epw(NGP) - 1) / iso_multi_epw(NGP)),  \
+                     u, n_dof, n_cases, ue ? ue + 3 * B.conn_offset : nullptr, ue_stride)
 #define LAUNCH(NN, NGP, UPPER, QC)                                                              \
   if constexpr (NN == 20) {                                                                     \
     if (ctx->elem_tiling == 1) {                                                                \
@@ -3351,11 +3466,14 @@ void launch_assemble_linear(cxg_context* ctx, const cxg_pattern* p, const double
     } else {                                                                                    \
       if (bup) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                     \
     }                                                                                           \
+  } else if constexpr (NN <= 8 && NGP <= QC) {                                                  \
+    if (bup) MULTI(NN, NGP, QC); else ISO(NN, NGP, QC, false, 0);                              \
   } else {                                                                                      \
     if (bup) ISO(NN, NGP, QC, true, 0); else ISO(NN, NGP, QC, false, 0);                       \
   }
       CXG_DISPATCH(B.type, LAUNCH)
 #undef LAUNCH
+#undef MULTI
 #undef ISO
       CXG_HIP(hipGetLastError());
     }
