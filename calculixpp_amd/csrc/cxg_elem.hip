@@ -1298,28 +1298,15 @@ __device__ __forceinline__ IncRecord record_uniform(const IncRecord* __restrict_
 // (k_elem_hform, HF Gauss points); the lane forms its column of M_ab from it, takes the
 // rest of the 3x3 block from the two other lanes of its node, then K_ab as k_elem_iso
 // does. Other elements are read as with UP alone.
-// SD (with SCALAR and LIVE; every row in LDS): the rows go out on the same-direction
-// pattern (row (a, i): the columns (b, i), sd_row_ptr per free equation): a position of
-// row i received a value exactly when its column has direction i (a bit of the
-// position's flag byte marks it, another its liveness), and the marked positions keep
-// their order.
-template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false, int HF = 0, bool SD = false>
+// (Same-direction rows of the scalar gathers go through row_gather_sd.)
+template <int NC, bool LIVE, int HALVES, bool SCALAR, bool UP = false, int HF = 0>
 __device__ __forceinline__ void row_gather_node(
     const GatherDesc& d, int cap, double* __restrict__ buf, uint8_t* __restrict__ lbuf,
     const IncRecord* __restrict__ rec, const int32_t* __restrict__ conn,
     const int32_t* __restrict__ slot_all, const double* __restrict__ kp,
     const double* __restrict__ u, int64_t n_dof, const double* __restrict__ fe,
     double* __restrict__ vals, double* __restrict__ f_int, uint8_t* __restrict__ live,
-    const double* __restrict__ ue, int64_t ue_stride, int up_min,
-    const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq, uint32_t rv,
-    int lane) {
-  static_assert(!SD || (SCALAR && LIVE), "same-direction rows: scalar gathers with flags");
-  // same-direction row starts, loaded now so that they arrive during the accumulation
-  int64_t sd0[3] = {0, 0, 0};
-  if constexpr (SD)
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      if ((d.flags >> i) & 1) sd0[i] = sd_row_ptr[dof_eq[3 * int64_t(d.node) + i]];
+    const double* __restrict__ ue, int64_t ue_stride, int up_min, uint32_t rv, int lane) {
   constexpr int halves = HALVES;
   const int h = HALVES == 2 ? lane / 32 : 0;
   const int sub = HALVES == 2 ? lane % 32 : lane;
@@ -1335,8 +1322,8 @@ __device__ __forceinline__ void row_gather_node(
   if (in_lds) {
     // the whole row area (values, then the flag bytes) with 16-byte writes: a few LDS
     // instructions instead of one per row, chunk and array
-    // SD: one flag byte per position (cap, a multiple of 4); else LIVE: one per entry
-    const int nbytes = 3 * cap * 8 + (SD ? (cap + 3) / 4 * 4 : (LIVE ? 3 * cap : 0));
+    // LIVE: one flag byte per entry
+    const int nbytes = 3 * cap * 8 + (LIVE ? 3 * cap : 0);
     uint4* z = reinterpret_cast<uint4*>(buf);
     for (int c = lane; c < nbytes / 16; c += kWave) z[c] = make_uint4(0u, 0u, 0u, 0u);
     for (int c = nbytes / 16 * 16 + lane; c < nbytes; c += kWave) reinterpret_cast<uint8_t*>(buf)[c] = 0;
@@ -1511,15 +1498,8 @@ __device__ __forceinline__ void row_gather_node(
           const int64_t rsj = jl == 0 ? rs[0] : (jl == 1 ? rs[1] : rs[2]);
           if (in_lds) {
             if (rsj >= 0) lds_add(buf + jl * cap + pos, sv);
-            if constexpr (SD) {
-              // one flag byte per position: bit 3 + i touched in row i, bit i live in row
-              // i; one OR into the byte's 32-bit word (the incidences' marks only add)
-              const uint32_t mk = ((8u | (sv != 0.0 ? 1u : 0u)) << jl) << (8 * (pos & 3));
-              __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(lbuf) + (pos >> 2), mk,
-                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else if constexpr (LIVE) {
+            if constexpr (LIVE)
               if (sv != 0.0) lbuf[jl * cap + pos] = 1;
-            }
           } else {
             if (rsj >= 0) vals[rsj + pos] += sv;
             if constexpr (LIVE)
@@ -1549,27 +1529,7 @@ __device__ __forceinline__ void row_gather_node(
     if (!in_lds) __syncthreads();  // global RMWs of the next batch must see these
   }
   __syncthreads();
-  if constexpr (SD) {
-    // chunk by chunk, one flag byte per position for the three rows
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-    int64_t o[3] = {sd0[0], sd0[1], sd0[2]};
-    for (int64_t c0 = 0; c0 < Lrow; c0 += kWave) {
-      const int64_t c = c0 + lane;
-      const uint32_t fl = c < Lrow ? lbuf[c] : 0u;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        if (rs[i] < 0) continue;
-        const bool on = (fl >> (3 + i)) & 1u;
-        const uint64_t m = __ballot(on);
-        if (on) {
-          const int64_t at = o[i] + __popcll(m & below);
-          __builtin_nontemporal_store(buf[i * cap + c], vals + at);
-          live[at] = uint8_t((fl >> i) & 1u);
-        }
-        o[i] += __popcll(m);
-      }
-    }
-  } else if (in_lds)
+  if (in_lds)
     for (int i = 0; i < 3; ++i) {
       if (rs[i] < 0) continue;
       for (int64_t c = lane; c < Lrow; c += kWave)
@@ -1695,6 +1655,104 @@ __device__ __forceinline__ void row_gather_pb(
   if (lane < 3) f_int[3 * int64_t(d.node) + lane] = h0 + h1;
 }
 
+// Same-direction rows of a k_ab I3 matrix (mass, geometric stiffness; every row in LDS).
+// Row (a, i) takes the columns (b, i), and the three rows of a node carry the same sums
+// k_ab, so each partner's sum is accumulated once, at the position of its first free DOF
+// in the structural row, by one lane per (incidence, partner): L lanes per incidence, G
+// incidences per step (10 and 6 where every incident element has at most 10 nodes, else
+// 20 and 3), the adds of a step issued incidence by incidence, so every sum keeps
+// ascending element order as in row_gather_node. Each position's flag byte holds, per
+// direction i, touched (bit 3 + i: the position is a column of direction i that received
+// a value) and live (bit i: a value other than 0.0), and in bits 6-7 the position's rank
+// among its node's free DOFs (the offset back to the value). Row i then goes out chunk by
+// chunk: its touched positions, in order, ballot-ranked onto sd_row_ptr's row. One value
+// row per node where three rows each took one of every partner's three equal sums (1.80
+// -> 1.13 ms for the 64^3 C3D20 mass) gives the same bits.
+template <bool LIVE>
+__device__ __forceinline__ void row_gather_sd(
+    const GatherDesc& d, int cap, double* __restrict__ buf, const IncRecord* __restrict__ rec,
+    const int32_t* __restrict__ slot_all, const double* __restrict__ kp,
+    double* __restrict__ vals, uint8_t* __restrict__ live,
+    const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq, uint32_t rv,
+    int lane) {
+  static_assert(LIVE, "same-direction rows carry live flags");
+  int64_t o[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if ((d.flags >> i) & 1) o[i] = sd_row_ptr[dof_eq[3 * int64_t(d.node) + i]];
+  uint8_t* lb = reinterpret_cast<uint8_t*>(buf + cap);
+  {
+    // the value row and the flag bytes with 16-byte writes
+    const int nbytes = cap * 8 + (cap + 3) / 4 * 4;
+    uint4* z = reinterpret_cast<uint4*>(buf);
+    for (int c = lane; c < nbytes / 16; c += kWave) z[c] = make_uint4(0u, 0u, 0u, 0u);
+    for (int c = nbytes / 16 * 16 + lane; c < nbytes; c += kWave) reinterpret_cast<uint8_t*>(buf)[c] = 0;
+  }
+  const bool small = (d.flags & 8) != 0;
+  const int L = small ? 10 : 20, G = small ? 6 : 3;
+  const int s = lane / L, b = lane - L * (lane / L);
+  const int n_inc = d.inc_n;
+  for (int g0 = 0; g0 < n_inc; g0 += G) {
+    const int sg = g0 + s;
+    const bool inc_ok = s < G && sg < n_inc;
+    // record sg: from rv while the group lies in the first kPreRec (the permutes run in
+    // every lane), else from memory
+    IncRecord r{};
+    if (g0 + G <= kPreRec) {
+      const int src = inc_ok ? sg : 0;
+      uint32_t w[8];
+#pragma unroll
+      for (int f = 0; f < 8; ++f) w[f] = uint32_t(__shfl(int(rv), 8 * src + f));
+      r = __builtin_bit_cast(IncRecord, w);
+    } else if (inc_ok) {
+      r = rec[d.inc0 + sg];
+    }
+    const bool ok = inc_ok && b < r.n;
+    int32_t sl = 0;
+    double k = 0.0;
+    if (ok) {
+      sl = slot_all[r.slot + b];
+      k = __builtin_nontemporal_load(kp + r.rows / 9 + b);
+    }
+    const int colp = sl & kSlotMask, fm = sl >> kSlotFreeShift;
+    const int ng = n_inc - g0 < G ? n_inc - g0 : G;
+    for (int e = 0; e < ng; ++e) {
+      if (s != e || !ok || fm == 0) continue;
+      lds_add(buf + colp, k);
+      const uint32_t lv = k != 0.0 ? 1u : 0u;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        if (!((fm >> j) & 1)) continue;
+        const int rank = __popc(fm & ((1 << j) - 1));
+        const int pos = colp + rank;
+        const uint32_t mk = (((8u | lv) << j) | (uint32_t(rank) << 6)) << (8 * (pos & 3));
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(lb) + (pos >> 2), mk, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+  }
+  __syncthreads();
+  // chunk by chunk, one flag byte per position for the three rows
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  const int64_t Lrow = d.len;
+  for (int64_t c0 = 0; c0 < Lrow; c0 += kWave) {
+    const int64_t c = c0 + lane;
+    const uint32_t fl = c < Lrow ? lb[c] : 0u;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (!((d.flags >> i) & 1)) continue;
+      const bool on = (fl >> (3 + i)) & 1u;
+      const uint64_t m = __ballot(on);
+      if (on) {
+        const int64_t at = o[i] + __popcll(m & below);
+        __builtin_nontemporal_store(buf[c - (fl >> 6)], vals + at);
+        live[at] = uint8_t((fl >> i) & 1u);
+      }
+      o[i] += __popcll(m);
+    }
+  }
+}
+
 // NC displacement fields (load cases) at u + c*n_dof give NC internal forces at
 // f_int + c*n_dof from the same pass over the element rows (u may be null: K only).
 // NC = 0: f_int is the sum of the element force vectors fe instead. LIVE: also flag the
@@ -1712,7 +1770,8 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
     const double* __restrict__ ue, int64_t ue_stride, int32_t up_min,
     const int64_t* __restrict__ sd_row_ptr, const int32_t* __restrict__ dof_eq) {
   // three rows of cap columns (cap: the pattern's longest row, at most kRowCap; longer
-  // rows accumulate in vals) and, for LIVE, their flags (SD: one flag byte per position instead)
+  // rows accumulate in vals) and, for LIVE, their flags (SD: one value row and a flag byte
+  // per position instead, see row_gather_sd)
   extern __shared__ __attribute__((aligned(16))) double row_lds[];
   double* buf = row_lds;
   uint8_t* lbuf = reinterpret_cast<uint8_t*>(row_lds + 3 * cap);
@@ -1725,22 +1784,28 @@ __global__ __launch_bounds__(kWave) void k_row_gather(
   // element has at most 10 nodes) and its first incidence records, one dword per lane
   // (one coalesced load instead of a scalar load per record and batch)
   const GatherDesc d = desc[ai];
-  // Nodes whose elements all have at most 8 nodes (flags bit 3: at most 10, which among
-  // the supported families means C3D8 and C3D6), every element with upper blocks, rows in
-  // LDS, element forces (the J2 path): one partner block per lane (row_gather_pb).
-  const bool pb = NC == 0 && UP && !SCALAR && HF == 0 && (d.flags & 8) && up_min <= 6 &&
-                  d.len <= cap && fe != nullptr;
-  const uint32_t rv = (d.flags & 8) && !pb ? 0u : load_records(rec, d, lane);
-  if (pb)
-    row_gather_pb<LIVE>(d, cap, buf, lbuf, rec, slot_all, kp, fe, vals, f_int, live, rv, lane);
-  else if (d.flags & 8)
-    row_gather_node<NC, LIVE, 2, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
-                                                     u, n_dof, fe, vals, f_int, live, ue, ue_stride,
-                                                     up_min, sd_row_ptr, dof_eq, rv, lane);
-  else
-    row_gather_node<NC, LIVE, 1, SCALAR, UP, HF, SD>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
-                                                     u, n_dof, fe, vals, f_int, live, ue, ue_stride,
-                                                     up_min, sd_row_ptr, dof_eq, rv, lane);
+  if constexpr (SD) {
+    // same-direction rows (scalar gathers with flags, every row in LDS)
+    row_gather_sd<LIVE>(d, cap, buf, rec, slot_all, kp, vals, live, sd_row_ptr, dof_eq,
+                        load_records(rec, d, lane), lane);
+  } else {
+    // Nodes whose elements all have at most 8 nodes (flags bit 3: at most 10, which among
+    // the supported families means C3D8 and C3D6), every element with upper blocks, rows
+    // in LDS, element forces (the J2 path): one partner block per lane (row_gather_pb).
+    const bool pb = NC == 0 && UP && !SCALAR && HF == 0 && (d.flags & 8) && up_min <= 6 &&
+                    d.len <= cap && fe != nullptr;
+    const uint32_t rv = (d.flags & 8) && !pb ? 0u : load_records(rec, d, lane);
+    if (pb)
+      row_gather_pb<LIVE>(d, cap, buf, lbuf, rec, slot_all, kp, fe, vals, f_int, live, rv, lane);
+    else if (d.flags & 8)
+      row_gather_node<NC, LIVE, 2, SCALAR, UP, HF>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
+                                                   u, n_dof, fe, vals, f_int, live, ue, ue_stride,
+                                                   up_min, rv, lane);
+    else
+      row_gather_node<NC, LIVE, 1, SCALAR, UP, HF>(d, cap, buf, lbuf, rec, conn, slot_all, kp,
+                                                   u, n_dof, fe, vals, f_int, live, ue, ue_stride,
+                                                   up_min, rv, lane);
+  }
 }
 
 // One thread per gather position: the node's descriptor (see GatherDesc). order: node
@@ -3179,8 +3244,9 @@ void launch_gather_on(cxg_context* ctx, const cxg_pattern* p, hipStream_t stream
   const unsigned grid = xcd_grid(ctx, n_pos);
   const int32_t cap = int32_t(std::min<int64_t>((std::max<int64_t>(p->max_row, 1) + 7) / 8 * 8,
                                                 std::min(kRowCap, ctx->gather_lds_cols)));
-  const size_t lds = size_t(3 * cap) * sizeof(double) +
-                     (sd ? size_t(cap + 3) / 4 * 4 : (live ? size_t(3 * cap) : 0));
+  // same-direction rows: one value row and a flag byte per position (row_gather_sd)
+  const size_t lds = sd ? size_t(cap) * sizeof(double) + size_t(cap + 3) / 4 * 4
+                        : size_t(3 * cap) * sizeof(double) + (live ? size_t(3 * cap) : 0);
 #define GATHER(NC, LV, ...)                                                                     \
   hipLaunchKernelGGL((k_row_gather<NC, LV, ##__VA_ARGS__>), dim3(grid), dim3(kWave), lds, stream, \
                      desc, rec, m->conn.p, p->slot_all.p, kp, u, n_dof, fe, vals, f_int, live,  \
