@@ -3088,46 +3088,73 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
 // liveness flags give the reference's key set.
 enum : int { kMassConsistent = 0, kMassLumped = 1, kGeometric = 2 };
 
-// LDS of k_elem_scalar: node coordinates, det J * w per point, for the mass the shape-value
-// table (read 5 times per point and lane: from LDS, not through the vector memory path)
-// and, for the geometric stiffness, J^-1 per point and the gradients of QC points at a
-// time; the element's n x n scalars.
+// LDS of k_elem_scalar, per wave: node coordinates, det J * w per point, for the
+// geometric stiffness J^-1 per point and the gradients of QC points at a time; the
+// element's n x n scalars. The mass's shape-value table (read 5 times per point and lane:
+// from LDS, not through the vector memory path) is one per workgroup.
 template <int NN, int NGP, int QC, int KIND>
 struct ScalarLds {
   double x[NN * 3];
   double sc[NGP];
-  double n[KIND == kGeometric ? 1 : NGP * NN];  // shape values N[q][k] (mass)
   double Ai[KIND == kGeometric ? NGP * 9 : 1];
   double g[KIND == kGeometric ? QC * NN * 3 : 1];
   double k[NN * NN];
   double row[NN];
 };
 
-// One element per wave (XCD-aware like k_elem_iso). A lane owns node a and up to four
-// consecutive upper partners b (decode_group): k_ab = rho sum_q det J w N_a N_b (mass) or
-// sum_q det J w g_a^T sigma_q g_b (geometric, with t_a = sigma_q g_a formed once per point
-// for the lane's partners); the upper values are mirrored in LDS, so k is bitwise
-// symmetric, and written row-major with coalesced streaming stores.
+// Waves per workgroup of k_elem_scalar: the mass kinds share the shape-value table
+// between 4 waves (one element each); the geometric stiffness, which has none, runs one.
+constexpr int scalar_waves(int kind) { return kind == kGeometric ? 1 : 4; }
+
+// A wave's LDS exchange point: the workgroup barrier where the workgroup is one wave,
+// else wave_lds_sync (the other waves work on other elements).
+template <int WPB>
+__device__ __forceinline__ void wave_exchange() {
+  if constexpr (WPB == 1)
+    __syncthreads();
+  else
+    wave_lds_sync();
+}
+
+// One element per wave (XCD-aware like k_elem_iso), WPB waves per workgroup. A lane owns
+// node a and up to four consecutive upper partners b (decode_group): k_ab = rho sum_q
+// det J w N_a N_b (mass) or sum_q det J w g_a^T sigma_q g_b (geometric, with t_a =
+// sigma_q g_a formed once per point for the lane's partners); the upper values are
+// mirrored in LDS, so k is bitwise symmetric, and written row-major with coalesced
+// streaming stores. Past the shared table's load each wave synchronises only with itself.
 template <int NN, int NGP, int QC, int KIND>
-__global__ __launch_bounds__(kWave) void k_elem_scalar(
+__global__ __launch_bounds__(kWave * scalar_waves(KIND)) void k_elem_scalar(
     const int32_t* __restrict__ conn, int64_t elem_offset, int64_t gp_offset,
     const double* __restrict__ dN, const double* __restrict__ wq, const double* __restrict__ Nq,
     const double* __restrict__ coords, const double* __restrict__ rho,
     const double* __restrict__ sig, double* __restrict__ kp, int* __restrict__ flag,
     int64_t n_elem, int64_t slab) {
-  __shared__ ScalarLds<NN, NGP, QC, KIND> L;
-  const int lane = threadIdx.x;
-  const int64_t le = xcd_index(blockIdx.x, n_elem, slab);
-  if (le < 0) return;
-  if (lane < NN) {
+  constexpr int WPB = scalar_waves(KIND);
+  __shared__ ScalarLds<NN, NGP, QC, KIND> LW[WPB];
+  __shared__ double ntab[KIND == kGeometric ? 1 : NGP * NN];  // shape values N[q][k] (mass)
+  // a single wave keeps constant LDS addresses (a base indexed by the wave took the
+  // geometric kernel from 80 to 100 VGPRs, 6 to 4 waves per SIMD)
+  const int lane = WPB == 1 ? int(threadIdx.x) : int(threadIdx.x % kWave);
+  const int wv = WPB == 1 ? 0 : int(threadIdx.x / kWave);
+  const int64_t blk = xcd_index(blockIdx.x, (n_elem + WPB - 1) / WPB, slab);
+  if (blk < 0) return;  // the whole workgroup
+  const int64_t le = blk * WPB + wv;
+  ScalarLds<NN, NGP, QC, KIND>& L = LW[wv];
+  // (a single-wave workgroup's element exists: xcd_index checked it)
+  const bool has_elem = WPB == 1 || le < n_elem;
+  if (has_elem && lane < NN) {
     const double* xp = coords + 3 * int64_t(conn[le * NN + lane]);
     L.x[3 * lane + 0] = xp[0];
     L.x[3 * lane + 1] = xp[1];
     L.x[3 * lane + 2] = xp[2];
   }
-  if constexpr (KIND != kGeometric)
-    for (int idx = lane; idx < NGP * NN; idx += kWave) L.n[idx] = Nq[idx];
-  __syncthreads();
+  if constexpr (KIND != kGeometric) {
+    for (int idx = threadIdx.x; idx < NGP * NN; idx += kWave * WPB) ntab[idx] = Nq[idx];
+    __syncthreads();
+  } else {
+    wave_exchange<WPB>();
+  }
+  if (!has_elem) return;  // a wave past the last element (after the workgroup barrier)
   for (int q = lane; q < NGP; q += kWave)
     L.sc[q] = jacobian_detw<NN, KIND == kGeometric, dn_stride<NGP>()>(L.x, dn_point<NN, NGP>(dN, q), wq[q],
                                                         L.Ai + 9 * q, flag);
@@ -3143,12 +3170,12 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
     b[t] = gb0 + t < NN ? (gb0 + t >= 0 ? gb0 + t : 0) : NN - 1;
     k[t] = 0.0;
   }
-  __syncthreads();
+  wave_exchange<WPB>();
   if constexpr (KIND != kGeometric) {
     const double r = rho[elem_offset + le];
 #pragma unroll 3
     for (int q = 0; q < NGP; ++q) {
-      const double* n = L.n + q * NN;
+      const double* n = ntab + q * NN;
       const double c = r * L.sc[q] * n[a];
 #pragma unroll
       for (int t = 0; t < PPL; ++t) k[t] += c * n[b[t]];
@@ -3170,7 +3197,7 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
         L.g[3 * idx + 1] = g1;
         L.g[3 * idx + 2] = g2;
       }
-      __syncthreads();
+      wave_exchange<WPB>();
       const int qn = NGP - q0 < QC ? NGP - q0 : QC;
 #pragma unroll 1
       for (int qq = 0; qq < qn; ++qq) {
@@ -3188,7 +3215,7 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
           k[t] += t0 * gB[0] + t1 * gB[1] + t2 * gB[2];
         }
       }
-      __syncthreads();
+      wave_exchange<WPB>();
     }
   }
 #pragma unroll
@@ -3197,14 +3224,14 @@ __global__ __launch_bounds__(kWave) void k_elem_scalar(
       L.k[a * NN + gb0 + t] = k[t];
       L.k[(gb0 + t) * NN + a] = k[t];
     }
-  __syncthreads();
+  wave_exchange<WPB>();
   if constexpr (KIND == kMassLumped) {
     for (int r = lane; r < NN; r += kWave) {
       double sum = 0.0;
       for (int c = 0; c < NN; ++c) sum += L.k[r * NN + c];
       L.row[r] = sum;
     }
-    __syncthreads();
+    wave_exchange<WPB>();
   }
   // the element's n x n scalars, row-major, at a ninth of its full-matrix arena offset
   double* ks = kp + le * NN * NN;
@@ -4206,10 +4233,11 @@ void launch_assemble_scalar(cxg_context* ctx, const cxg_pattern* p, const double
     double* kp = pm->ke_arena.p + p->kp_base[bi] / 9;  // scalar arena: n^2 per element
     LaunchTimer timer(ctx, std::string(names[kind]) + type_name(B.type), B.n_elem);
 #define LAUNCH_K(NN, NGP, QC, KIND)                                                             \
-  hipLaunchKernelGGL((k_elem_scalar<NN, NGP, QC, KIND>), dim3(xcd_grid(ctx, B.n_elem)),         \
-                     dim3(kWave), 0, ctx->stream, conn, B.elem_offset, B.gp_offset, T.dN, T.w,  \
-                     T.N, coords, rho, sig, kp, ctx->flag_dev, B.n_elem,                        \
-                     xcd_slab(ctx, B.n_elem))
+  hipLaunchKernelGGL((k_elem_scalar<NN, NGP, QC, KIND>),                                        \
+                     dim3(xcd_grid(ctx, wave_blocks(B.n_elem, 1, scalar_waves(KIND)))),         \
+                     dim3(kWave * scalar_waves(KIND)), 0, ctx->stream, conn, B.elem_offset,     \
+                     B.gp_offset, T.dN, T.w, T.N, coords, rho, sig, kp, ctx->flag_dev,          \
+                     B.n_elem, xcd_slab(ctx, wave_blocks(B.n_elem, 1, scalar_waves(KIND))))
 #define LAUNCH(NN, NGP, UP, QC)                                          \
   if (kind == kMassConsistent) LAUNCH_K(NN, NGP, QC, kMassConsistent);   \
   else if (kind == kMassLumped) LAUNCH_K(NN, NGP, QC, kMassLumped);      \
