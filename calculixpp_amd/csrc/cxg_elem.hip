@@ -3786,8 +3786,11 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
     CXG_HIP(hipMemcpy(conn.data(), m->conn.p, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
 
   // Components: union-find with the smaller root kept, so each root is its component's
-  // smallest node; numbered in ascending root order.
+  // smallest node; numbered in ascending root order. Nodes that no element lists form
+  // one last component together (not one each).
   std::vector<int32_t> comp(static_cast<size_t>(n));
+  std::vector<uint8_t> used(static_cast<size_t>(n), 0);
+  for (const int32_t v : conn) used[size_t(v)] = 1;
   for (int32_t i = 0; i < n; ++i) comp[size_t(i)] = i;
   const auto find = [&](int32_t a) {
     while (comp[size_t(a)] != a) {
@@ -3812,8 +3815,17 @@ void prepare_order(cxg_context* ctx, const cxg_pattern* p, const double* coords)
     }
   for (int32_t i = 0; i < n; ++i) comp[size_t(i)] = find(i);  // every node to its root
   int32_t n_comp = 0;
-  for (int32_t i = 0; i < n; ++i)  // a root precedes its nodes, so comp[root] is its number
+  for (int32_t i = 0; i < n; ++i) {  // a root precedes its nodes, so comp[root] is its number
+    if (!used[size_t(i)]) continue;  // (an unused node is its own root: no element joins it)
     comp[size_t(i)] = comp[size_t(i)] == i ? n_comp++ : comp[size_t(comp[size_t(i)])];
+  }
+  bool any_unused = false;
+  for (int32_t i = 0; i < n; ++i)
+    if (!used[size_t(i)]) {
+      comp[size_t(i)] = n_comp;
+      any_unused = true;
+    }
+  if (any_unused) ++n_comp;
 
   // Each body's frame: sweep axis last, tile width and layer thickness from its own
   // bounding box and element count.
