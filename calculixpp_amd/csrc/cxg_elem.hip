@@ -2734,9 +2734,11 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
 }
 
 // Element forces alone (the matrix-free internal force: no means, no point stresses, no
-// J2): k_gauss_fields' arithmetic for them with a leaner LDS, the point phase's
-// coordinates and displacements overlaid by the node phase's J^-1 and det J w sigma, so
-// that more waves reside (7.7 KB per wave of 8 C3D8 elements against 11.5 KB). The force
+// J2). With s = det J w sigma (symmetric 3x3) and g_k = J^-T d_k, the force of node k is
+// sum_q s g_k = sum_q P d_k with P = s J^-1 (P_cj = sum_i s_ci (J^-1)_ji), so the point
+// phase leaves 9 values per point (P) where J^-1 and s took 15, and the node phase takes 9
+// multiply-adds per point where forming g and then s g took 18; P overlays the point
+// phase's coordinates and displacements (4.6 KB per wave of 8 C3D8 elements). The force
 // of (element, node) goes node-major, to 3 * (its incidence), so that k_node_sum_seq
 // reads each node's vectors as one contiguous run.
 template <int NN, int NGP>
@@ -2746,10 +2748,7 @@ struct ForceLds {
       double x[NN * 3];
       double u[NN * 3];
     };
-    struct {
-      double Ai[NGP * 9];  // J^-1 per Gauss point (row-major)
-      double S[NGP * 6];   // det J w sigma per Gauss point
-    };
+    double P[NGP * 9];  // det J w sigma J^-1 per Gauss point (row-major)
   };
 };
 
@@ -2827,12 +2826,17 @@ __global__ __launch_bounds__(kWave * WPB, WAVES) void k_elem_forces(
     lame(E[eg], nu[eg], lam, mu);
     iso_stress(lam, mu, e6, sg);
   }
-  wave_lds_sync();  // x and u are read: J^-1 and det J w sigma take their place
+  wave_lds_sync();  // x and u are read: P takes their place
   if (pt_lane) {
+    // s in Voigt order (xx yy zz xy xz yz) as a symmetric 3x3, A = J^-1 row-major
+    const double s0 = sc * sg[0], s1 = sc * sg[1], s2 = sc * sg[2];
+    const double s3 = sc * sg[3], s4 = sc * sg[4], s5 = sc * sg[5];
+    const double S[3][3] = {{s0, s3, s4}, {s3, s1, s5}, {s4, s5, s2}};
 #pragma unroll
-    for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int i = 0; i < 6; ++i) R.S[6 * q + i] = sc * sg[i];
+      for (int j = 0; j < 3; ++j)
+        R.P[9 * q + 3 * c + j] = S[c][0] * A[3 * j] + S[c][1] * A[3 * j + 1] + S[c][2] * A[3 * j + 2];
   }
   wave_lds_sync();
   if (lane < ne * NN) {
@@ -2840,12 +2844,12 @@ __global__ __launch_bounds__(kWave * WPB, WAVES) void k_elem_forces(
     const ForceLds<NN, NGP>& R = L[e];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = 0; q < NGP; ++q) {
-      double g[3];
-      grad_at(R.Ai + 9 * q, dN + (q * NN + k) * 3, g);
-      const double* s = R.S + 6 * q;
-      f0 += g[0] * s[0] + g[1] * s[3] + g[2] * s[4];
-      f1 += g[1] * s[1] + g[0] * s[3] + g[2] * s[5];
-      f2 += g[2] * s[2] + g[0] * s[4] + g[1] * s[5];
+      const double* d = dN + (q * NN + k) * 3;
+      const double d0 = d[0], d1 = d[1], d2 = d[2];
+      const double* P = R.P + 9 * q;
+      f0 += P[0] * d0 + P[1] * d1 + P[2] * d2;
+      f1 += P[3] * d0 + P[4] * d1 + P[5] * d2;
+      f2 += P[6] * d0 + P[7] * d1 + P[8] * d2;
     }
     // node-major: at the incidence of (element, node) (ipos: the block's conn offsets)
     double* fp = fe + 3 * int64_t(ipos[(le0 + e) * NN + k]);
@@ -2938,7 +2942,9 @@ __global__ void k_node_sum_seq(int32_t n_nodes, const int64_t* __restrict__ inc_
 // elements per wave as in the recovery kernel: one lane per (element, Gauss point), then
 // one per (element, node) writing the element's dx and du vectors to arenas (3 per
 // connectivity entry, node-major as k_elem_forces) that k_node_sum_seq adds up per node in
-// ascending element order.
+// ascending element order. With g_m = J^-T d_m both node sums are sum_q X_q d_m, X = P_q
+// J^-1 for dx and s sigma_w J^-1 for du (as k_elem_forces), so the point phase leaves two
+// 3x3 matrices per point where J^-1, P and s sigma_w took 24 values.
 template <int NN, int NGP>
 struct VjpLds {
   union {
@@ -2948,9 +2954,8 @@ struct VjpLds {
       double w[NN * 3];
     };
     struct {  // ... and then leaves for the node phase
-      double Ai[NGP * 9];  // J^-1 per Gauss point
-      double P[NGP * 9];
-      double S[NGP * 6];   // s sigma_w
+      double Q[NGP * 9];  // P J^-1 per Gauss point (row-major)
+      double F[NGP * 9];  // s sigma_w J^-1
       double phi[NGP];
     };
   };
@@ -3040,9 +3045,16 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
   wave_lds_sync();  // x, u and w are read: the point results take their place
   if (pt_lane) {
     VjpLds<NN, NGP>& R = L[pe];
-    for (int i = 0; i < 9; ++i) R.P[9 * pq_i + i] = Pm[i];
-    for (int i = 0; i < 9; ++i) R.Ai[9 * pq_i + i] = A[i];
-    for (int i = 0; i < 6; ++i) R.S[6 * pq_i + i] = Sw6[i];
+    const double Sw[3][3] = {{Sw6[0], Sw6[3], Sw6[4]}, {Sw6[3], Sw6[1], Sw6[5]}, {Sw6[4], Sw6[5], Sw6[2]}};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        R.Q[9 * pq_i + 3 * c + j] =
+            Pm[3 * c] * A[3 * j] + Pm[3 * c + 1] * A[3 * j + 1] + Pm[3 * c + 2] * A[3 * j + 2];
+        R.F[9 * pq_i + 3 * c + j] =
+            Sw[c][0] * A[3 * j] + Sw[c][1] * A[3 * j + 1] + Sw[c][2] * A[3 * j + 2];
+      }
     R.phi[pq_i] = pq;
   }
   wave_lds_sync();
@@ -3057,16 +3069,16 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
     const VjpLds<NN, NGP>& R = L[e];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = 0; q < NGP; ++q) {
-      double g[3];
-      grad_at(R.Ai + 9 * q, dN + (q * NN + k) * 3, g);
-      const double* P = R.P + 9 * q;
-      a0 += P[0] * g[0] + P[1] * g[1] + P[2] * g[2];
-      a1 += P[3] * g[0] + P[4] * g[1] + P[5] * g[2];
-      a2 += P[6] * g[0] + P[7] * g[1] + P[8] * g[2];
-      const double* S = R.S + 6 * q;
-      f0 += g[0] * S[0] + g[1] * S[3] + g[2] * S[4];
-      f1 += g[1] * S[1] + g[0] * S[3] + g[2] * S[5];
-      f2 += g[2] * S[2] + g[0] * S[4] + g[1] * S[5];
+      const double* d = dN + (q * NN + k) * 3;
+      const double d0 = d[0], d1 = d[1], d2 = d[2];
+      const double* Q = R.Q + 9 * q;
+      a0 += Q[0] * d0 + Q[1] * d1 + Q[2] * d2;
+      a1 += Q[3] * d0 + Q[4] * d1 + Q[5] * d2;
+      a2 += Q[6] * d0 + Q[7] * d1 + Q[8] * d2;
+      const double* F = R.F + 9 * q;
+      f0 += F[0] * d0 + F[1] * d1 + F[2] * d2;
+      f1 += F[3] * d0 + F[4] * d1 + F[5] * d2;
+      f2 += F[6] * d0 + F[7] * d1 + F[8] * d2;
     }
     const int64_t o = 3 * int64_t(ipos[(le0 + e) * NN + k]);  // node-major, as k_elem_forces
     dxe[o] = a0;
