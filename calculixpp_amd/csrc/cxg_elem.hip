@@ -2606,9 +2606,8 @@ struct RecoverLds {
     };
     double eps[NGP * 6];  // written once the points' x and u reads are done
   };
-  double Ai[NGP * 9];   // J^-1 per Gauss point (row-major)
+  double P[NGP * 9];    // det J w sigma J^-1 per Gauss point (row-major; see ForceLds)
   double sig[NGP * 6];
-  double sc[NGP];       // det J * w
   int32_t node[NN];
 };
 
@@ -2694,12 +2693,19 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
   }
   __syncthreads();  // x and u are read: eps takes their place
   if (pt_lane) {
-    for (int i = 0; i < 9; ++i) R.Ai[9 * q + i] = A[i];
+    // the node phase's force is sum_q P d_k, P = det J w sigma J^-1 (as k_elem_forces)
+    const double s0 = sc * sg[0], s1 = sc * sg[1], s2 = sc * sg[2];
+    const double s3 = sc * sg[3], s4 = sc * sg[4], s5 = sc * sg[5];
+    const double S[3][3] = {{s0, s3, s4}, {s3, s1, s5}, {s4, s5, s2}};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        R.P[9 * q + 3 * c + j] = S[c][0] * A[3 * j] + S[c][1] * A[3 * j + 1] + S[c][2] * A[3 * j + 2];
     for (int i = 0; i < 6; ++i) {
       R.eps[6 * q + i] = e6[i];
       R.sig[6 * q + i] = sg[i];
     }
-    R.sc[q] = sc;
   }
   __syncthreads();
   for (int idx = lane; emean && idx < ne * 12; idx += kWave) {
@@ -2715,15 +2721,12 @@ __global__ __launch_bounds__(kWave, 4) void k_gauss_fields(
     const RecoverLds<NN, NGP>& R = L[e];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = 0; q < NGP; ++q) {
-      double g[3];
-      grad_at(R.Ai + 9 * q, dN + (q * NN + k) * 3, g);
-      const double* sg = R.sig + 6 * q;
-      const double sc = R.sc[q];
-      const double s0 = sc * sg[0], s1 = sc * sg[1], s2 = sc * sg[2];
-      const double s3 = sc * sg[3], s4 = sc * sg[4], s5 = sc * sg[5];
-      f0 += g[0] * s0 + g[1] * s3 + g[2] * s4;
-      f1 += g[1] * s1 + g[0] * s3 + g[2] * s5;
-      f2 += g[2] * s2 + g[0] * s4 + g[1] * s5;
+      const double* d = dN + (q * NN + k) * 3;
+      const double d0 = d[0], d1 = d[1], d2 = d[2];
+      const double* P = R.P + 9 * q;
+      f0 += P[0] * d0 + P[1] * d1 + P[2] * d2;
+      f1 += P[3] * d0 + P[4] * d1 + P[5] * d2;
+      f2 += P[6] * d0 + P[7] * d1 + P[8] * d2;
     }
     // node-major, at the incidence of (element, node), as k_elem_forces
     double* fp = fe + 3 * int64_t(ipos[(le0 + e) * NN + k]);
