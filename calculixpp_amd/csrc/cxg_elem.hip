@@ -3113,7 +3113,7 @@ struct ScalarLds {
   double sc[NGP];
   double Ai[KIND == kGeometric ? NGP * 9 : 1];
   double g[KIND == kGeometric ? QC * NN * 3 : 1];
-  double k[NN * NN];
+  alignas(16) double k[NN * NN];  // 16-byte aligned: paired reads and writes of the rows
   double row[NN];
 };
 
