@@ -2644,6 +2644,63 @@ __device__ __forceinline__ void displacement_gradient(const double* u, const dou
     for (int j = 0; j < 3; ++j) H[i][j] = G[i][0] * A[j] + G[i][1] * A[3 + j] + G[i][2] * A[6 + j];
 }
 
+// The C3D8 natural derivatives in registers (hex8_dn; no table reads). A point lane's, at
+// d[3k + c]: 12 distinct products of the factors 1 + g and 1 - g, their signs those of the
+// node corners.
+__device__ __forceinline__ void hex8_dn_point(int q, double d[24]) {
+  constexpr double gp = 1.0 + kGauss2, gm = 1.0 - kGauss2;
+  double f[3][2];  // f[a][s]: the factor along axis a of a node on side s (1: +) of the point
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const bool p = (q >> a) & 1;
+    f[a][1] = p ? gp : gm;
+    f[a][0] = p ? gm : gp;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int a = c == 0 ? 1 : 0, b = c == 2 ? 1 : 2;
+      const double v = (0.125 * f[a][hex_corner_pos(k, a)]) * f[b][hex_corner_pos(k, b)];
+      d[3 * k + c] = hex_corner_pos(k, c) ? v : -v;
+    }
+}
+
+// Calls fn(q, d0, d1, d2) for the NGP points in order with node k's natural derivatives:
+// for C3D8 from the closed form (the 12 values of node k, by the point's sides along the
+// other two axes; the loop unrolled), else from the point-major table dN.
+template <int NN, int NGP, class Fn>
+__device__ __forceinline__ void for_node_points(const double* dN, int k, Fn&& fn) {
+  if constexpr (NN == 8 && NGP == 8) {
+    constexpr double gp = 1.0 + kGauss2, gm = 1.0 - kGauss2;
+    double f[3][2];  // f[a][s]: the factor along axis a at the points on side s (1: +)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool p = hex_corner_pos(k, a);
+      f[a][1] = p ? gp : gm;
+      f[a][0] = p ? gm : gp;
+    }
+    double e[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double s = hex_corner_pos(k, c) ? 0.125 : -0.125;
+      const int a = c == 0 ? 1 : 0, b = c == 2 ? 1 : 2;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) e[c][i][j] = (s * f[a][i]) * f[b][j];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      fn(q, e[0][(q >> 1) & 1][(q >> 2) & 1], e[1][q & 1][(q >> 2) & 1], e[2][q & 1][(q >> 1) & 1]);
+  } else {
+    for (int q = 0; q < NGP; ++q) {
+      const double* d = dN + (q * NN + k) * 3;
+      fn(q, d[0], d[1], d[2]);
+    }
+  }
+}
+
 // Elements per wave of the recovery kernel: one lane per (element, Gauss point) and per
 // (element, node) (2 C3D20, 3 C3D20R, 8 C3D8, 10 C3D6).
 constexpr int rec_epw(int nn, int ngp) {
@@ -2798,7 +2855,8 @@ __global__ __launch_bounds__(kWave * WPB, WAVES) void k_elem_forces(
     const int32_t* __restrict__ ipos, double* __restrict__ fe, int* __restrict__ flag,
     int64_t n_elem, int64_t slab) {
   constexpr int EPW = rec_epw(NN, NGP);
-  constexpr bool DNL = forces_dn_lds<NN, NGP>();
+  constexpr bool HEX8 = NN == 8 && NGP == 8;  // natural derivatives in closed form
+  constexpr bool DNL = forces_dn_lds<NN, NGP>() && !HEX8;
   constexpr int NDN = DNL ? 2 * NGP * NN * 3 : 1;
   __shared__ ForceLds<NN, NGP> LW[WPB][EPW];
   __shared__ double dn_lds[NDN];
@@ -2833,10 +2891,13 @@ __global__ __launch_bounds__(kWave * WPB, WAVES) void k_elem_forces(
   double A[9], sg[6], sc = 0.0;
   if (pt_lane) {
     const int64_t eg = elem_offset + le0 + e;
-    const double* d = dn_point<NN, NGP>(dN, q);
-    sc = jacobian_detw<NN, true, dn_stride<NGP>()>(R.x, d, wq[q], A, flag);
+    double dh[24];
+    if constexpr (HEX8) hex8_dn_point(q, dh);
+    const double* d = HEX8 ? dh : dn_point<NN, NGP>(dN, q);
+    constexpr int DS = HEX8 ? 1 : dn_stride<NGP>();
+    sc = jacobian_detw<NN, true, DS>(R.x, d, wq[q], A, flag);
     double H[3][3];
-    displacement_gradient<NN, dn_stride<NGP>()>(R.u, d, A, H);
+    displacement_gradient<NN, DS>(R.u, d, A, H);
     double e6[6], lam, mu;
     voigt_strain(H, e6);
     lame(E[eg], nu[eg], lam, mu);
@@ -2859,14 +2920,12 @@ __global__ __launch_bounds__(kWave * WPB, WAVES) void k_elem_forces(
     const int e = lane / NN, k = lane - e * NN;
     const ForceLds<NN, NGP>& R = L[e];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int q = 0; q < NGP; ++q) {
-      const double* d = dN + (q * NN + k) * 3;
-      const double d0 = d[0], d1 = d[1], d2 = d[2];
+    for_node_points<NN, NGP>(dN, k, [&](int q, double d0, double d1, double d2) {
       const double* P = R.P + 9 * q;
       f0 += P[0] * d0 + P[1] * d1 + P[2] * d2;
       f1 += P[3] * d0 + P[4] * d1 + P[5] * d2;
       f2 += P[6] * d0 + P[7] * d1 + P[8] * d2;
-    }
+    });
     // node-major: at the incidence of (element, node) (ipos: the block's conn offsets)
     double* fp = fe + 3 * int64_t(ipos[(le0 + e) * NN + k]);
     fp[0] = f0;
@@ -2990,8 +3049,9 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
     int64_t slab) {
   constexpr int EPW = rec_epw(NN, NGP);
   static_assert(EPW * NGP <= kWave && EPW * NN <= kWave, "one lane per point and per node");
-  // small natural-derivative tables in the LDS, as k_elem_forces
-  constexpr bool DNL = forces_dn_lds<NN, NGP>();
+  // small natural-derivative tables in the LDS, as k_elem_forces (C3D8: closed form)
+  constexpr bool HEX8 = NN == 8 && NGP == 8;
+  constexpr bool DNL = forces_dn_lds<NN, NGP>() && !HEX8;
   constexpr int NDN = DNL ? 2 * NGP * NN * 3 : 1;
   __shared__ VjpLds<NN, NGP> LW[WPB][EPW];
   __shared__ double dn_lds[NDN];
@@ -3027,11 +3087,14 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
     const int e = pe, q = pq_i;
     VjpLds<NN, NGP>& R = L[e];
     const int64_t ge = elem_offset + le0 + e;
-    const double* d = dn_point<NN, NGP>(dN, q);
-    const double s = jacobian_detw<NN, true, dn_stride<NGP>()>(R.x, d, wq[q], A, flag);
+    double dh[24];
+    if constexpr (HEX8) hex8_dn_point(q, dh);
+    const double* d = HEX8 ? dh : dn_point<NN, NGP>(dN, q);
+    constexpr int DS = HEX8 ? 1 : dn_stride<NGP>();
+    const double s = jacobian_detw<NN, true, DS>(R.x, d, wq[q], A, flag);
     double Hu[3][3], Hw[3][3];
-    displacement_gradient<NN, dn_stride<NGP>()>(R.u, d, A, Hu);
-    displacement_gradient<NN, dn_stride<NGP>()>(R.w, d, A, Hw);
+    displacement_gradient<NN, DS>(R.u, d, A, Hu);
+    displacement_gradient<NN, DS>(R.w, d, A, Hw);
     double eu[6], ew[6], su[6], sw[6], lam, mu;
     lame(E[ge], nu[ge], lam, mu);
     voigt_strain(Hu, eu);
@@ -3077,9 +3140,7 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
     const int e = lane / NN, k = lane - e * NN;
     const VjpLds<NN, NGP>& R = L[e];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int q = 0; q < NGP; ++q) {
-      const double* d = dN + (q * NN + k) * 3;
-      const double d0 = d[0], d1 = d[1], d2 = d[2];
+    for_node_points<NN, NGP>(dN, k, [&](int q, double d0, double d1, double d2) {
       const double* Q = R.Q + 9 * q;
       a0 += Q[0] * d0 + Q[1] * d1 + Q[2] * d2;
       a1 += Q[3] * d0 + Q[4] * d1 + Q[5] * d2;
@@ -3088,7 +3149,7 @@ __global__ __launch_bounds__(kWave * WPB) void k_vjp(
       f0 += F[0] * d0 + F[1] * d1 + F[2] * d2;
       f1 += F[3] * d0 + F[4] * d1 + F[5] * d2;
       f2 += F[6] * d0 + F[7] * d1 + F[8] * d2;
-    }
+    });
     const int64_t o = 3 * int64_t(ipos[(le0 + e) * NN + k]);  // node-major, as k_elem_forces
     dxe[o] = a0;
     dxe[o + 1] = a1;

@@ -58,6 +58,25 @@ int gauss_points(int type);       // 0 for types outside the GPU path
 void host_tables(int type, std::vector<double>& dN, std::vector<double>& w,
                  std::vector<double>& N);
 
+// The C3D8 natural derivatives at its 2 x 2 x 2 Gauss points in closed form. Point q has
+// coordinate +-g along axis a (+ when bit a of q is set; g = 1.0 / std::sqrt(3.0) as the
+// Gauss rule computes it, one ulp above the nearest double to 1/sqrt(3)), node k the corner
+// sign s_ka (CalculiX order), so every factor 1 + s_ka p_a of dN_k/dxi_c = s_kc (1 + s_ka
+// p_a)(1 + s_kb p_b) / 8 is 1 + g or 1 - g. hex8_dn rounds as the table does (one rounded
+// product; the sign and the 1/8 exact), and host_tables checks the table against it, so
+// the kernels that form these values instead of reading the table get the table's bits.
+constexpr double kGauss2 = 0x1.279a74590331dp-1;
+__host__ __device__ constexpr bool hex_corner_pos(int k, int a) {
+  return a == 0 ? ((k ^ (k >> 1)) & 1) != 0 : a == 1 ? ((k >> 1) & 1) != 0 : ((k >> 2) & 1) != 0;
+}
+__host__ __device__ inline double hex8_dn(int q, int k, int c) {
+  double f[3];
+  for (int a = 0; a < 3; ++a)
+    f[a] = hex_corner_pos(k, a) == (((q >> a) & 1) != 0) ? 1.0 + kGauss2 : 1.0 - kGauss2;
+  const int a = c == 0 ? 1 : 0, b = c == 2 ? 1 : 2;
+  return ((hex_corner_pos(k, c) ? 0.125 : -0.125) * f[a]) * f[b];
+}
+
 // Device buffer owning helper (hipMalloc/hipFree), move-only.
 template <class T>
 struct DevBuf {

@@ -172,6 +172,12 @@ void host_tables(int type, std::vector<double>& dN, std::vector<double>& w,
     w[q] = g[q].w;
     n_values(type, p, N.data() + q * size_t(nn));
   }
+  if (type == CXG_C3D8)  // the kernels' closed form must give the table's bits
+    for (size_t q = 0; q < g.size(); ++q)
+      for (int k = 0; k < nn; ++k)
+        for (int c = 0; c < 3; ++c)
+          if (dN[(q * nn + k) * 3 + c] != hex8_dn(int(q), k, c))
+            throw Error(CXG_ERR_STATE, "C3D8 natural derivatives differ from their closed form");
 }
 
 }  // namespace cxg
