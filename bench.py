@@ -172,15 +172,24 @@ def ingest_traffic(root: str, copy_mib: int) -> dict:
 _BATCH_RECS = None  # the batch workload's meshes, generated once, shared with the forks
 
 def _cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it, with its family and model numbers and the
+    logical CPUs seen and allowed (a virtual machine may report only a vendor line, e.g.
+    "AMD EPYC", as its model name)."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # the first processor's block
+                key, _, val = line.partition(":")
+                fields.setdefault(key.strip(), val.strip())
     except OSError:
         pass
     import platform
-    return platform.processor() or "unknown"
+    name = fields.get("model name") or platform.processor() or "unknown"
+    ids = ", ".join(f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields)
+    return (f"{name} ({ids + '; ' if ids else ''}{os.cpu_count()} logical CPUs, "
+            f"{len(_allowed_cpus())} allowed)")
 
 
 def _allowed_cpus() -> list:
